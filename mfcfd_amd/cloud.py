@@ -1,0 +1,321 @@
+"""Synthetic NACA 4-digit O-type point clouds in the reference's point layout.
+
+The reference reads its cloud from ``point/point.h5`` (point_preprocessor.F90:215-546):
+per point ``original_id, x, y, nx, ny, min_dist, left, right, qtdepth, flag_1, flag_2,
+nbhs, conn(1:nbhs)`` with 1-based indices.  No sample cloud ships with the reference,
+so this module generates one of the same layout:
+
+* ``n_wall`` points on the airfoil (ring 0), ``n_radial`` rings, the last ring a
+  far-field circle of radius ``far_radius`` chords centred on mid-chord;
+* geometric radial spacing starting at ``first_spacing`` chords;
+* seeded uniform jitter of +-``jitter`` x local spacing (radial and tangential
+  components) on interior points only;
+* neighbours: every point within ``radius_factor`` x h_i, nearest first, capped at
+  ``max_nbhs``, where h_i = max(radial, tangential) local spacing of point i;
+* flags: wall ``flag_1=0, flag_2=1``; interior ``flag_1=1, flag_2=0``; outer ``flag_1=2``;
+* ``left``/``right`` run clockwise round both the airfoil and the far-field circle, so
+  the normal rot90(right-left) of point_normals.F90:36-48 points into the fluid at the
+  wall (wall stencils take delta_n >= 0, generate_connectivity.F90:171-174) and out of
+  the domain at the far field (outer stencils take delta_n <= 0, :241-244).
+
+Arrays are returned in ring-major order (index = ring * n_wall + i), which is also the
+``original_id`` order.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class PointCloud:
+    """A point cloud in the reference's per-point layout (0-based arrays, 1-based ids).
+
+    ``conn`` is ``(N, W)`` row-major with 0-based neighbour indices and -1 padding;
+    ``left``/``right`` are 0-based (-1 for interior points).
+    """
+
+    x: np.ndarray
+    y: np.ndarray
+    nx: np.ndarray
+    ny: np.ndarray
+    min_dist: np.ndarray
+    left: np.ndarray
+    right: np.ndarray
+    flag_1: np.ndarray
+    flag_2: np.ndarray
+    nbhs: np.ndarray
+    conn: np.ndarray
+    qtdepth: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def max_nbhs(self) -> int:
+        return int(self.conn.shape[1])
+
+    def __post_init__(self):
+        if self.qtdepth is None:
+            self.qtdepth = np.zeros(self.x.shape[0], dtype=np.int32)
+
+
+def naca4_half_thickness(xc: np.ndarray, t: float) -> np.ndarray:
+    """NACA 4-digit symmetric half-thickness with a closed trailing edge (-0.1036 x^4)."""
+    xc = np.clip(xc, 0.0, 1.0)
+    return 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2
+                      + 0.2843 * xc ** 3 - 0.1036 * xc ** 4)
+
+
+def _geometric_ratio(first: float, n_steps: int, total: float) -> float:
+    """Ratio r with first*(r^n - 1)/(r - 1) = total (bisection)."""
+    if first * n_steps >= total:
+        return 1.0
+    lo, hi = 1.0 + 1e-12, 2.0
+    while first * (hi ** n_steps - 1.0) / (hi - 1.0) < total:
+        hi *= 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        s = first * (mid ** n_steps - 1.0) / (mid - 1.0)
+        if s < total:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def _wall_normals(ax: np.ndarray, ay: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Outward normals of a clockwise closed polyline (left of the l->r direction)."""
+    dx = np.roll(ax, -1) - np.roll(ax, 1)
+    dy = np.roll(ay, -1) - np.roll(ay, 1)
+    nrm = np.hypot(dx, dy)
+    return -dy / nrm, dx / nrm
+
+
+def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
+               first_spacing: float = 2e-3, far_radius: float = 15.0,
+               jitter: float = 0.1, seed: int = 0, max_nbhs: int = 24,
+               radius_factor: float = 2.5, smooth_passes: int | None = None) -> PointCloud:
+    """Generate an O-type cloud round a NACA 00tt airfoil (see module docstring)."""
+    from scipy.spatial import cKDTree
+
+    W, R = int(n_wall), int(n_radial)
+    if W < 8 or R < 3:
+        raise ValueError("need n_wall >= 8 and n_radial >= 3")
+    # clockwise parameter starting at the trailing edge: phi decreasing from 0
+    phi = -2.0 * np.pi * np.arange(W) / W
+    xc = 0.5 + 0.5 * np.cos(phi)
+    ax = xc
+    ay = np.sign(np.sin(phi)) * naca4_half_thickness(xc, thickness)
+    anx, any_ = _wall_normals(ax, ay)
+    # marching directions: wall normals smoothed round the airfoil so the trailing-edge
+    # fan opens over several columns instead of one
+    mx_, my_ = anx.copy(), any_.copy()
+    if smooth_passes is None:
+        # binomial smoothing width ~ sqrt(passes) columns: keep it a fixed arc fraction
+        smooth_passes = int(40 * (W / 160.0) ** 2)
+    for _ in range(smooth_passes):
+        mx_ = np.roll(mx_, 1) + 2.0 * mx_ + np.roll(mx_, -1)
+        my_ = np.roll(my_, 1) + 2.0 * my_ + np.roll(my_, -1)
+        nrm = np.hypot(mx_, my_)
+        mx_, my_ = mx_ / nrm, my_ / nrm
+
+    cx, cy = 0.5, 0.0
+    D = far_radius - 0.5
+    r = _geometric_ratio(first_spacing, R - 1, D)
+    d = np.zeros(R)
+    if r == 1.0:
+        d[1:] = np.linspace(first_spacing, D, R - 1)
+    else:
+        d[1:] = first_spacing * (r ** np.arange(1, R) - 1.0) / (r - 1.0)
+    d[-1] = D
+    b = (d / D) ** 2
+    ux, uy = np.cos(phi), np.sin(phi)
+    # P_ij = (1-b_j) (A_i + d_j n_i) + b_j (c + (0.5 + d_j) u_i)
+    Px = (1.0 - b)[:, None] * (ax[None, :] + d[:, None] * mx_[None, :]) \
+        + b[:, None] * (cx + (0.5 + d)[:, None] * ux[None, :])
+    Py = (1.0 - b)[:, None] * (ay[None, :] + d[:, None] * my_[None, :]) \
+        + b[:, None] * (cy + (0.5 + d)[:, None] * uy[None, :])
+    Px[-1] = cx + far_radius * ux
+    Py[-1] = cy + far_radius * uy
+    Px[0], Py[0] = ax, ay
+
+    # local spacings on the unjittered grid
+    def dist(ax_, ay_, bx_, by_):
+        return np.hypot(ax_ - bx_, ay_ - by_)
+
+    h_tan = 0.5 * (dist(Px, Py, np.roll(Px, -1, 1), np.roll(Py, -1, 1))
+                   + dist(Px, Py, np.roll(Px, 1, 1), np.roll(Py, 1, 1)))
+    h_rad = np.empty_like(Px)
+    h_rad[0] = dist(Px[0], Py[0], Px[1], Py[1])
+    h_rad[-1] = dist(Px[-1], Py[-1], Px[-2], Py[-2])
+    h_rad[1:-1] = 0.5 * (dist(Px[1:-1], Py[1:-1], Px[2:], Py[2:])
+                         + dist(Px[1:-1], Py[1:-1], Px[:-2], Py[:-2]))
+    h = np.maximum(h_tan, h_rad)
+
+    if jitter > 0.0 and R > 2:
+        rng = np.random.default_rng(seed)
+        j1 = rng.uniform(-jitter, jitter, size=(R - 2, W))
+        j2 = rng.uniform(-jitter, jitter, size=(R - 2, W))
+        # radial unit: towards the next ring; tangential unit: towards the next point
+        erx = Px[2:] - Px[:-2]
+        ery = Py[2:] - Py[:-2]
+        en = np.hypot(erx, ery)
+        erx, ery = erx / en, ery / en
+        etx = np.roll(Px[1:-1], -1, 1) - np.roll(Px[1:-1], 1, 1)
+        ety = np.roll(Py[1:-1], -1, 1) - np.roll(Py[1:-1], 1, 1)
+        en = np.hypot(etx, ety)
+        etx, ety = etx / en, ety / en
+        Px[1:-1] = Px[1:-1] + j1 * h_rad[1:-1] * erx + j2 * h_tan[1:-1] * etx
+        Py[1:-1] = Py[1:-1] + j1 * h_rad[1:-1] * ery + j2 * h_tan[1:-1] * ety
+
+    x = Px.reshape(-1).copy()
+    y = Py.reshape(-1).copy()
+    hh = h.reshape(-1)
+    N = x.shape[0]
+
+    flag_1 = np.ones(N, dtype=np.int32)
+    flag_1[:W] = 0
+    flag_1[(R - 1) * W:] = 2
+    flag_2 = np.zeros(N, dtype=np.int32)
+    flag_2[:W] = 1
+    left = np.full(N, -1, dtype=np.int32)
+    right = np.full(N, -1, dtype=np.int32)
+    ring_i = np.arange(W)
+    for base in (0, (R - 1) * W):
+        left[base + ring_i] = base + (ring_i - 1) % W
+        right[base + ring_i] = base + (ring_i + 1) % W
+
+    nx = np.zeros(N)
+    ny = np.ones(N)
+    for base in (0, (R - 1) * W):
+        sl = slice(base, base + W)
+        lx, ly = x[left[sl]], y[left[sl]]
+        rx, ry = x[right[sl]], y[right[sl]]
+        vx, vy = 0.5 * ((y[sl] - ly) + (ry - y[sl])), 0.5 * ((x[sl] - lx) + (rx - x[sl]))
+        det = np.sqrt(vx * vx + vy * vy)
+        nx[sl], ny[sl] = -vx / det, vy / det
+
+    tree = cKDTree(np.column_stack([x, y]))
+    dd, ii = tree.query(np.column_stack([x, y]), k=max_nbhs + 1, workers=-1)
+    conn = np.full((N, max_nbhs), -1, dtype=np.int32)
+    nbhs = np.zeros(N, dtype=np.int32)
+    self_col = ii == np.arange(N)[:, None]
+    ok = (~self_col) & (dd <= radius_factor * hh[:, None]) & np.isfinite(dd)
+    # keep nearest-first order, drop self, cap at max_nbhs
+    order = np.argsort(~ok, axis=1, kind="stable")
+    ii_sorted = np.take_along_axis(ii, order, axis=1)
+    ok_sorted = np.take_along_axis(ok, order, axis=1)
+    nbhs[:] = ok_sorted.sum(axis=1).clip(max=max_nbhs)
+    keep = ok_sorted[:, :max_nbhs]
+    conn[:, :] = np.where(keep, ii_sorted[:, :max_nbhs], -1)
+
+    cloud = PointCloud(x=x, y=y, nx=nx, ny=ny, min_dist=np.zeros(N), left=left,
+                       right=right, flag_1=flag_1, flag_2=flag_2, nbhs=nbhs, conn=conn)
+    repaired = _repair_stencils(cloud, tree, max_nbhs)
+    valid = cloud.conn >= 0
+    kk = np.where(valid, cloud.conn, 0)
+    dist_k = np.where(valid, np.hypot(x[kk] - x[:, None], y[kk] - y[:, None]), np.inf)
+    cloud.min_dist = dist_k.min(axis=1)
+    if np.any(cloud.nbhs < 2):
+        raise RuntimeError("cloud has points with fewer than 2 neighbours")
+
+    meta = dict(generator="naca_ogrid", n_wall=W, n_radial=R, thickness=thickness,
+                first_spacing=first_spacing, far_radius=far_radius, jitter=jitter,
+                seed=seed, max_nbhs=max_nbhs, radius_factor=radius_factor,
+                radial_ratio=r, h_rule="max(radial, tangential) local spacing",
+                repaired_points=int(repaired))
+    cloud.meta = meta
+    return cloud
+
+
+# half-stencils each point class uses (flux_residual.F90:23-70): bit 0 xpos, 1 xneg,
+# 2 ypos, 3 yneg
+_REQUIRED_BITS = {0: (0, 1, 3), 1: (0, 1, 2, 3), 2: (0, 1, 2)}
+_MIN_HALF = 3
+
+
+def _repair_stencils(cloud: PointCloud, tree, max_nbhs: int) -> int:
+    """Re-pick neighbours of points whose half-stencils hold fewer than 3 points.
+
+    The nearest-first pick can leave a half-plane (e.g. behind a trailing-edge fan)
+    nearly empty, which makes that side's 2x2 least-squares system singular.  For such
+    points the stencil is re-chosen from the 4*max_nbhs nearest: nearest first, but
+    reserving room so every half-plane the point's class uses gets at least 3 points.
+    """
+    mask = split_stencils(cloud, cloud.nx, cloud.ny)
+    n_bad = 0
+    for cls, bits in _REQUIRED_BITS.items():
+        sel = cloud.flag_1 == cls
+        counts = np.stack([((mask & (1 << b)) != 0).sum(axis=1) for b in bits], axis=1)
+        bad = np.where(sel & (counts.min(axis=1) < _MIN_HALF))[0]
+        for i in bad:
+            _, cand = tree.query([cloud.x[i], cloud.y[i]], k=4 * max_nbhs + 1)
+            cand = [int(c) for c in cand if c != i and c < cloud.n]
+            m = _mask_row(cloud, i, np.array(cand))
+            chosen: list[int] = []
+            have = {b: 0 for b in bits}
+            for b in bits:  # reserve the nearest _MIN_HALF of each needed half-plane
+                for j, c in enumerate(cand):
+                    if have[b] >= _MIN_HALF:
+                        break
+                    if (m[j] >> b) & 1 and c not in chosen:
+                        chosen.append(c)
+                        for bb in bits:
+                            have[bb] += (m[j] >> bb) & 1
+            for c in cand:  # fill the rest nearest first
+                if len(chosen) >= max_nbhs:
+                    break
+                if c not in chosen:
+                    chosen.append(c)
+            chosen = chosen[:max_nbhs]
+            chosen.sort(key=lambda c: (cloud.x[c] - cloud.x[i]) ** 2 + (cloud.y[c] - cloud.y[i]) ** 2)
+            cloud.conn[i, :] = -1
+            cloud.conn[i, :len(chosen)] = chosen
+            cloud.nbhs[i] = len(chosen)
+            n_bad += 1
+    return n_bad
+
+
+def _mask_row(cloud: PointCloud, i: int, cand: np.ndarray) -> np.ndarray:
+    delx = cloud.x[cand] - cloud.x[i]
+    dely = cloud.y[cand] - cloud.y[i]
+    tx, ty = cloud.ny[i], -cloud.nx[i]
+    dels = delx * tx + dely * ty
+    deln = delx * cloud.nx[i] + dely * cloud.ny[i]
+    m = (dels <= 0).astype(np.uint8) | ((dels >= 0).astype(np.uint8) << 1)
+    if cloud.flag_1[i] == 0:
+        m |= 8
+    elif cloud.flag_1[i] == 2:
+        m |= 4
+    else:
+        m |= ((deln <= 0).astype(np.uint8) << 2) | ((deln >= 0).astype(np.uint8) << 3)
+    return m
+
+
+def split_stencils(cloud: PointCloud, nx: np.ndarray, ny: np.ndarray) -> np.ndarray:
+    """Direction mask per connectivity slot, as generate_connectivity.F90:39-256 splits.
+
+    bit 0 xpos (delta_s <= 0), bit 1 xneg (delta_s >= 0), bit 2 ypos (delta_n <= 0),
+    bit 3 yneg (delta_n >= 0).  Wall points put every neighbour in yneg (:171-174) and
+    outer points every neighbour in ypos (:241-244).  Returns uint8 ``(N, W)``.
+    """
+    N, W = cloud.conn.shape
+    valid = cloud.conn >= 0
+    k = np.where(valid, cloud.conn, 0)
+    delx = cloud.x[k] - cloud.x[:, None]
+    dely = cloud.y[k] - cloud.y[:, None]
+    tx, ty = ny[:, None], -nx[:, None]
+    dels = delx * tx + dely * ty
+    deln = delx * nx[:, None] + dely * ny[:, None]
+    m = (dels <= 0).astype(np.uint8) | ((dels >= 0).astype(np.uint8) << 1)
+    wall = (cloud.flag_1 == 0)[:, None]
+    outer = (cloud.flag_1 == 2)[:, None]
+    interior = ~(wall | outer)
+    ym = np.where(interior, (deln <= 0).astype(np.uint8) << 2 | (deln >= 0).astype(np.uint8) << 3,
+                  np.where(wall, np.uint8(8), np.uint8(4)))
+    m = (m | ym).astype(np.uint8)
+    return np.where(valid, m, 0).astype(np.uint8)

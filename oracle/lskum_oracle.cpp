@@ -1,0 +1,760 @@
+// CPU oracle: restatement of the reference's FP64 LSKUM fixed-point iteration.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or calls this
+// file; it is used by the test suite, by the smoke check and by the benchmark's CPU
+// baseline leg, always as the checker and never as the thing measured or shipped.
+//
+// Every routine follows one routine of Nischay-Pro/mfcfd src_mpi_primal (file:line
+// cited at each function) with the same evaluation order: left-to-right products,
+// the same summation order over neighbours, libm exp/log/erf/pow/sqrt, and no FMA
+// contraction (built with -ffp-contract=off).  Arrays use the reference's per-point
+// layout with 0-based indices:
+//   prim/prim_old/q/flux_res [i*4 + r]       (reference (4,N))
+//   dq/qm/temp               [i*8 + r*2 + d] (reference (2,4,N): d fastest, then r)
+//   conn/x*_conn             [i*W + k]       (reference (N,W) slot-major)
+//
+// The solver is templated on the scalar type so the same code runs in double (the
+// oracle) and in forward-mode dual numbers (the tangent used to check the adjoint).
+//
+// Parity status of each routine (what pins it) is listed in the test files that call
+// it; routines pinned only by tests against the reference's own Fortran build of the
+// same routine are marked there.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dual.h"
+
+namespace orc {
+
+// parameter.F90:8-11
+static const double RHO_INF = 1.0;
+static const double PR_INF = 1.0 / 1.4;
+static const double GAMMA = 1.4;
+static const double PI = 4.0 * std::atan(1.0);
+// limiters.F90:78 compares against the REAL*4 literal 10e-6, promoted to REAL*8
+static const double LIM_THRESH = (double)10e-6f;
+
+inline double val(double v) { return v; }
+using std::erf;
+using std::exp;
+using std::log;
+using std::log10;
+using std::pow;
+using std::sqrt;
+
+struct Params {
+  double cfl, vl_const, power, mach, aoa_deg, euler;
+  int32_t rks, inner_iterations, shapes, interior_points_normal_flag, format, restart;
+  double restart_res_old;
+};
+
+template <class T>
+struct Solver {
+  int n = 0, W = 0;
+  Params p{};
+  double theta = 0, q_inf[4] = {0, 0, 0, 0}, q_init[4] = {0, 0, 0, 0};
+  // geometry (x, y and the normals are differentiable inputs of the adjoint)
+  std::vector<T> x, y, nx, ny;
+  std::vector<double> min_dist;
+  std::vector<int32_t> left, right, flag1, flag2, nbhs, conn;
+  std::vector<int32_t> xpos_n, xneg_n, ypos_n, yneg_n, xpos_c, xneg_c, ypos_c, yneg_c;
+  std::vector<int32_t> wall_idx, outer_idx, interior_idx, shape_idx;
+  // state
+  std::vector<T> prim, prim_old, q, dq, qm, temp, flux_res, delta;
+  std::vector<T> res_sqr_pt;
+  T sum_res_sqr = 0, res_old = 0, res_new = 0, residue = 0;
+  std::vector<T> Cl, Cd, Cm, cp;  // per shape; cp per shape point
+
+  // parameter.F90:16-34 setup_case_parameters
+  void setup_case_parameters() {
+    theta = p.aoa_deg * PI / 180.0;
+    q_init[0] = RHO_INF;
+    q_init[1] = p.mach * std::cos(theta);
+    q_init[2] = p.mach * std::sin(theta);
+    q_init[3] = PR_INF;
+    for (int r = 0; r < 4; ++r) q_inf[r] = q_init[r];
+  }
+
+  // point_preprocessor.F90:388-429: index lists of the point classes, in point order
+  void build_index_lists() {
+    wall_idx.clear(); outer_idx.clear(); interior_idx.clear(); shape_idx.clear();
+    for (int k = 0; k < n; ++k) {
+      if (flag1[k] == 0) wall_idx.push_back(k);
+      else if (flag1[k] == 1) interior_idx.push_back(k);
+      else if (flag1[k] == 2) outer_idx.push_back(k);
+      if (flag2[k] > 0) shape_idx.push_back(k);
+    }
+  }
+
+  void allocate() {
+    prim.assign(4 * n, T(0)); prim_old.assign(4 * n, T(0)); q.assign(4 * n, T(0));
+    flux_res.assign(4 * n, T(0)); dq.assign(8 * n, T(0)); qm.assign(8 * n, T(0));
+    temp.assign(8 * n, T(0)); delta.assign(n, T(0)); res_sqr_pt.assign(n, T(0));
+    xpos_n.assign(n, 0); xneg_n.assign(n, 0); ypos_n.assign(n, 0); yneg_n.assign(n, 0);
+    xpos_c.assign((size_t)n * W, -1); xneg_c.assign((size_t)n * W, -1);
+    ypos_c.assign((size_t)n * W, -1); yneg_c.assign((size_t)n * W, -1);
+    Cl.assign(p.shapes, T(0)); Cd.assign(p.shapes, T(0)); Cm.assign(p.shapes, T(0));
+    cp.assign(shape_idx.size(), T(0));
+  }
+
+  // initial_conditions.F90:9-24 (restart == 0 branch)
+  void initial_conditions() {
+    setup_case_parameters();
+    for (int k = 0; k < n; ++k)
+      for (int r = 0; r < 4; ++r) prim[4 * k + r] = T(q_init[r]);
+  }
+
+  // point_normals.F90:8-105 compute_normals
+  void compute_normals() {
+    auto boundary = [&](const std::vector<int32_t>& idx) {
+      for (int m : idx) {
+        int l = left[m], r = right[m];
+        T lx = x[l], ly = y[l], mx = x[m], my = y[m], rx = x[r], ry = y[r];
+        T nx1 = my - ly, nx2 = ry - my, ny1 = mx - lx, ny2 = rx - mx;
+        T vx = 0.5 * (nx1 + nx2), vy = 0.5 * (ny1 + ny2);
+        T det = sqrt(vx * vx + vy * vy);
+        nx[m] = -vx / det;
+        ny[m] = vy / det;
+      }
+    };
+    boundary(wall_idx);
+    boundary(outer_idx);
+    if (p.interior_points_normal_flag == 0 && p.format != 2) {
+      for (int k : interior_idx) { nx[k] = T(0.0); ny[k] = T(1.0); }
+    } else if (p.interior_points_normal_flag == 1 && p.format != 2) {
+      for (int k : interior_idx) { nx[k] = T(1.0); ny[k] = T(0.0); }
+    }
+  }
+
+  // generate_connectivity.F90:8-256: split each stencil by the signs of delta_s and
+  // delta_n in the point's (t, n) frame; ties land in both halves.  Wall points take
+  // every neighbour in yneg (:171-174), outer points every neighbour in ypos (:241-244).
+  void generate_connectivity() {
+    for (int i = 0; i < n; ++i) {
+      xpos_n[i] = xneg_n[i] = ypos_n[i] = yneg_n[i] = 0;
+      double xi = val(x[i]), yi = val(y[i]);
+      double nxi = val(nx[i]), nyi = val(ny[i]);
+      double tx = nyi, ty = -nxi;
+      for (int r = 0; r < nbhs[i]; ++r) {
+        int nbh = conn[(size_t)i * W + r];
+        double delx = val(x[nbh]) - xi, dely = val(y[nbh]) - yi;
+        double dels = delx * tx + dely * ty;
+        double deln = delx * nxi + dely * nyi;
+        if (dels <= 0.0) xpos_c[(size_t)i * W + xpos_n[i]++] = nbh;
+        if (dels >= 0.0) xneg_c[(size_t)i * W + xneg_n[i]++] = nbh;
+        if (flag1[i] == 1) {
+          if (deln <= 0.0) ypos_c[(size_t)i * W + ypos_n[i]++] = nbh;
+          if (deln >= 0.0) yneg_c[(size_t)i * W + yneg_n[i]++] = nbh;
+        } else if (flag1[i] == 0) {
+          yneg_c[(size_t)i * W + yneg_n[i]++] = nbh;
+        } else if (flag1[i] == 2) {
+          ypos_c[(size_t)i * W + ypos_n[i]++] = nbh;
+        }
+      }
+    }
+  }
+
+  // state_update.F90:190-236 func_delta
+  void func_delta() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      T min_delt = T(1.0);
+      for (int r = 0; r < nbhs[i]; ++r) {
+        int k = conn[(size_t)i * W + r];
+        T rho = prim[4 * k + 0], u1 = prim[4 * k + 1], u2 = prim[4 * k + 2], pr = prim[4 * k + 3];
+        T dist = (x[k] - x[i]) * (x[k] - x[i]) + (y[k] - y[i]) * (y[k] - y[i]);
+        dist = sqrt(dist);
+        T mod_u = sqrt(u1 * u1 + u2 * u2);
+        T delta_t = dist / (mod_u + 3.0 * sqrt(pr / rho));
+        delta_t = p.cfl * delta_t;
+        if (val(min_delt) > val(delta_t)) min_delt = delta_t;
+      }
+      delta[i] = min_delt;
+    }
+  }
+
+  // q_variables.F90:8-37 eval_q_variables
+  void eval_q_variables() {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; ++k) {
+      T rho = prim[4 * k + 0], u1 = prim[4 * k + 1], u2 = prim[4 * k + 2], pr = prim[4 * k + 3];
+      T beta = 0.5 * rho / pr;
+      q[4 * k + 0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2);
+      T two_times_beta = 2.0 * beta;
+      q[4 * k + 1] = two_times_beta * u1;
+      q[4 * k + 2] = two_times_beta * u2;
+      q[4 * k + 3] = -two_times_beta;
+    }
+  }
+
+  // q_variables.F90:41-112 eval_q_derivatives
+  void eval_q_derivatives() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      T x_i = x[i], y_i = y[i];
+      T sxx = T(0.0), syy = T(0.0), sxy = T(0.0);
+      T sxq[4], syq[4];
+      for (int r = 0; r < 4; ++r) {
+        sxq[r] = T(0.0); syq[r] = T(0.0);
+        qm[8 * i + 2 * r + 0] = q[4 * i + r];
+        qm[8 * i + 2 * r + 1] = q[4 * i + r];
+      }
+      for (int k = 0; k < nbhs[i]; ++k) {
+        int nbh = conn[(size_t)i * W + k];
+        for (int r = 0; r < 4; ++r) {
+          if (val(q[4 * nbh + r]) > val(qm[8 * i + 2 * r + 0])) qm[8 * i + 2 * r + 0] = q[4 * nbh + r];
+          if (val(q[4 * nbh + r]) < val(qm[8 * i + 2 * r + 1])) qm[8 * i + 2 * r + 1] = q[4 * nbh + r];
+        }
+        T delx = x[nbh] - x_i, dely = y[nbh] - y_i;
+        T dist = sqrt(delx * delx + dely * dely);
+        T weights = pow(dist, p.power);
+        sxx = sxx + delx * delx * weights;
+        syy = syy + dely * dely * weights;
+        sxy = sxy + delx * dely * weights;
+        for (int r = 0; r < 4; ++r) {
+          sxq[r] = sxq[r] + weights * delx * (q[4 * nbh + r] - q[4 * i + r]);
+          syq[r] = syq[r] + weights * dely * (q[4 * nbh + r] - q[4 * i + r]);
+        }
+      }
+      T det = sxx * syy - sxy * sxy;
+      T one_by_det = 1.0 / det;
+      for (int r = 0; r < 4; ++r) {
+        dq[8 * i + 2 * r + 0] = (sxq[r] * syy - syq[r] * sxy) * one_by_det;
+        dq[8 * i + 2 * r + 1] = (syq[r] * sxx - sxq[r] * sxy) * one_by_det;
+      }
+    }
+  }
+
+  // q_variables.F90:115-212 eval_q_inner_loop
+  void eval_q_inner_loop() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      T x_i = x[i], y_i = y[i];
+      T sxx = T(0.0), syy = T(0.0), sxy = T(0.0);
+      T sxq[4], syq[4], qi[4];
+      for (int r = 0; r < 4; ++r) { sxq[r] = T(0.0); syq[r] = T(0.0); qi[r] = q[4 * i + r]; }
+      for (int k = 0; k < nbhs[i]; ++k) {
+        int nbh = conn[(size_t)i * W + k];
+        T delx = x[nbh] - x_i, dely = y[nbh] - y_i;
+        T dist = sqrt(delx * delx + dely * dely);
+        T weights = pow(dist, p.power);
+        sxx = sxx + delx * delx * weights;
+        syy = syy + dely * dely * weights;
+        sxy = sxy + delx * dely * weights;
+        for (int r = 0; r < 4; ++r) {
+          T temp1 = qi[r] - 0.5 * (delx * dq[8 * i + 2 * r + 0] + dely * dq[8 * i + 2 * r + 1]);
+          T temp2 = q[4 * nbh + r] - 0.5 * (delx * dq[8 * nbh + 2 * r + 0] + dely * dq[8 * nbh + 2 * r + 1]);
+          sxq[r] = sxq[r] + (weights * delx * (temp2 - temp1));
+          syq[r] = syq[r] + (weights * dely * (temp2 - temp1));
+        }
+      }
+      T det = sxx * syy - sxy * sxy;
+      T one_by_det = 1.0 / det;
+      for (int r = 0; r < 4; ++r) {
+        temp[8 * i + 2 * r + 0] = one_by_det * (sxq[r] * syy - syq[r] * sxy);
+        temp[8 * i + 2 * r + 1] = one_by_det * (syq[r] * sxx - sxq[r] * sxy);
+      }
+    }
+  }
+
+  // q_variables.F90:215-227 eval_update_innerloop
+  void eval_update_innerloop() {
+    for (size_t j = 0; j < dq.size(); ++j) dq[j] = temp[j];
+  }
+
+  // q_variables.F90:230-256 qtilde_to_primitive
+  static void qtilde_to_primitive(const T qt[4], T& u1, T& u2, T& rho, T& pr) {
+    T beta = -qt[3] * 0.5;
+    T tmp = 0.5 / beta;
+    u1 = qt[1] * tmp;
+    u2 = qt[2] * tmp;
+    T temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
+    T temp2 = temp1 - (log(beta) / (GAMMA - 1));
+    rho = exp(temp2);
+    pr = rho * tmp;
+  }
+
+  // limiters.F90:64-110 venkat_limiter (phi of point k's own qm and min_dist)
+  void venkat_limiter(const T qtilde[4], T phi[4], int k) const {
+    for (int r = 0; r < 4; ++r) {
+      T qq = q[4 * k + r];
+      T del_neg = qtilde[r] - qq;
+      double adn = std::fabs(val(del_neg));
+      if (adn <= LIM_THRESH) {
+        phi[r] = T(1.0);
+      } else if (adn > LIM_THRESH) {
+        T del_pos;
+        if (val(del_neg) > 0.0) del_pos = qm[8 * k + 2 * r + 0] - qq;
+        else del_pos = qm[8 * k + 2 * r + 1] - qq;
+        // epsi**3.0d0 (limiters.F90:88-89): the reference's own amdflang build lowers the
+        // integral real power to (epsi*epsi)*epsi, which the oracle follows bit for bit
+        double epsi = p.vl_const * min_dist[k];
+        epsi = epsi * epsi * epsi;
+        T num = (del_pos * del_pos) + (epsi * epsi);
+        num = num * del_neg + 2.0 * del_neg * del_neg * del_pos;
+        T den = del_pos * del_pos + 2.0 * del_neg * del_neg;
+        den = den + del_neg * del_pos + epsi * epsi;
+        den = den * del_neg;
+        T tmp = num / den;
+        if (val(tmp) < 1.0) phi[r] = tmp;
+        else phi[r] = T(1.0);
+      }
+    }
+  }
+
+  // split_fluxes.F90 and quadrant_fluxes.F90: the eight kinetic split fluxes
+  enum FluxKind { GXP, GXN, GYP, GYN, GXI, GXII, GXIII, GXIV };
+
+  // split_fluxes.F90:8-190, quadrant_fluxes.F90:12-248
+  static void flux(FluxKind kind, T G[4], T nx_, T ny_, T u1, T u2, T rho, T pr) {
+    T tx = ny_, ty = -nx_;
+    T ut = u1 * tx + u2 * ty;
+    T un = u1 * nx_ + u2 * ny_;
+    T beta = 0.5 * rho / pr;
+    if (kind == GXP || kind == GXN) {
+      T S1 = ut * sqrt(beta);
+      T B1 = 0.5 * exp(-S1 * S1) / sqrt(PI * beta);
+      T A1 = (kind == GXP) ? 0.5 * (1.0 + erf(S1)) : 0.5 * (1.0 - erf(S1));
+      T pr_by_rho = pr / rho;
+      T u_sqr = ut * ut + un * un;
+      if (kind == GXP) {
+        G[0] = rho * (ut * A1 + B1);
+        T temp1 = pr_by_rho + ut * ut;
+        T temp2 = temp1 * A1 + ut * B1;
+        G[1] = rho * temp2;
+        temp1 = ut * un * A1 + un * B1;
+        G[2] = rho * temp1;
+        temp1 = (7.0 * pr_by_rho) + u_sqr;
+        temp2 = 0.5 * ut * temp1 * A1;
+        temp1 = (6.0 * pr_by_rho) + u_sqr;
+        G[3] = rho * (temp2 + 0.5 * temp1 * B1);
+      } else {
+        G[0] = rho * (ut * A1 - B1);
+        T temp1 = pr_by_rho + ut * ut;
+        T temp2 = temp1 * A1 - ut * B1;
+        G[1] = rho * temp2;
+        temp1 = ut * un * A1 - un * B1;
+        G[2] = rho * temp1;
+        temp1 = (7.0 * pr_by_rho) + u_sqr;
+        temp2 = 0.5 * ut * temp1 * A1;
+        temp1 = (6.0 * pr_by_rho) + u_sqr;
+        G[3] = rho * (temp2 - 0.5 * temp1 * B1);
+      }
+      return;
+    }
+    if (kind == GYP || kind == GYN) {
+      T S2 = un * sqrt(beta);
+      T B2 = 0.5 * exp(-S2 * S2) / sqrt(PI * beta);
+      T A2 = (kind == GYP) ? 0.5 * (1.0 + erf(S2)) : 0.5 * (1.0 - erf(S2));
+      T pr_by_rho = pr / rho;
+      T u_sqr = ut * ut + un * un;
+      if (kind == GYP) {
+        G[0] = rho * (un * A2 + B2);
+        T temp1 = pr_by_rho + un * un;
+        T temp2 = temp1 * A2 + un * B2;
+        G[2] = rho * temp2;
+        temp1 = ut * un * A2 + ut * B2;
+        G[1] = rho * temp1;
+        temp1 = (7.0 * pr_by_rho) + u_sqr;
+        temp2 = 0.5 * un * temp1 * A2;
+        temp1 = (6.0 * pr_by_rho) + u_sqr;
+        G[3] = rho * (temp2 + 0.5 * temp1 * B2);
+      } else {
+        G[0] = rho * (un * A2 - B2);
+        T temp1 = pr_by_rho + un * un;
+        T temp2 = temp1 * A2 - un * B2;
+        G[2] = rho * temp2;
+        temp1 = ut * un * A2 - ut * B2;
+        G[1] = rho * temp1;
+        temp1 = (7.0 * pr_by_rho) + u_sqr;
+        temp2 = 0.5 * un * temp1 * A2;
+        temp1 = (6.0 * pr_by_rho) + u_sqr;
+        G[3] = rho * (temp2 - 0.5 * temp1 * B2);
+      }
+      return;
+    }
+    // quadrant fluxes: A1 sign, A2 sign and the +-B combinations per quadrant
+    T S1 = ut * sqrt(beta);
+    T S2 = un * sqrt(beta);
+    T B1 = 0.5 * exp(-S1 * S1) / sqrt(PI * beta);
+    T B2 = 0.5 * exp(-S2 * S2) / sqrt(PI * beta);
+    bool a1pos = (kind == GXII || kind == GXIII);
+    bool a2pos = (kind == GXIII || kind == GXIV);
+    T A1 = a1pos ? 0.5 * (1.0 + erf(S1)) : 0.5 * (1.0 - erf(S1));
+    T A2 = a2pos ? 0.5 * (1.0 + erf(S2)) : 0.5 * (1.0 - erf(S2));
+    T pr_by_rho = pr / rho;
+    T u_sqr = ut * ut + un * un;
+    // a1pos: (ut*A1 + B1) and (temp2 + temp3); otherwise (ut*A1 - B1), (temp2 - temp3)
+    // a2pos: (un*A2 + B2) and "+ temp4"; otherwise (un*A2 - B2), "- temp4"
+    T t1b = a1pos ? ut * A1 + B1 : ut * A1 - B1;
+    G[0] = rho * A2 * t1b;
+    T temp1 = pr_by_rho + ut * ut;
+    T temp2 = a1pos ? temp1 * A1 + ut * B1 : temp1 * A1 - ut * B1;
+    G[1] = rho * A2 * temp2;
+    temp1 = a1pos ? ut * A1 + B1 : ut * A1 - B1;
+    temp2 = a2pos ? un * A2 + B2 : un * A2 - B2;
+    G[2] = rho * temp1 * temp2;
+    temp1 = (7.0 * pr_by_rho) + u_sqr;
+    temp2 = 0.5 * ut * temp1 * A1;
+    temp1 = (6.0 * pr_by_rho) + u_sqr;
+    T temp3 = 0.5 * B1 * temp1;
+    temp1 = a1pos ? ut * A1 + B1 : ut * A1 - B1;
+    T temp4 = 0.5 * rho * un * B2 * temp1;
+    T inner = a1pos ? temp2 + temp3 : temp2 - temp3;
+    G[3] = a2pos ? rho * A2 * inner + temp4 : rho * A2 * inner - temp4;
+  }
+
+  // The twelve dG routines (wall_fluxes.F90:20-292, outer_fluxes.F90:18-300,
+  // interior_fluxes.F90:19-404) share one body: a limited mid-point reconstruction
+  // of both ends, the kinetic flux at both ends, and a 2x2 least-squares derivative
+  // of G in the point's (t, n) frame.  `solve_y` selects the n-derivative row
+  // (interior_fluxes.F90:305, wall_fluxes.F90:289) instead of the t-derivative (:107).
+  void dG(T G[4], int i, const int32_t* list, int count, FluxKind kind, bool solve_y) const {
+    T sxx = T(0.0), syy = T(0.0), sxy = T(0.0), sxf[4], syf[4];
+    for (int r = 0; r < 4; ++r) { sxf[r] = T(0.0); syf[r] = T(0.0); }
+    T x_i = x[i], y_i = y[i], nx_ = nx[i], ny_ = ny[i];
+    T tx = ny_, ty = -nx_;
+    for (int j = 0; j < count; ++j) {
+      int k = list[j];
+      T delx = x[k] - x_i, dely = y[k] - y_i;
+      T dels = delx * tx + dely * ty;
+      T deln = delx * nx_ + dely * ny_;
+      T dist = sqrt(dels * dels + deln * deln);
+      T weights = pow(dist, p.power);
+      T dels_w = dels * weights, deln_w = deln * weights;
+      sxx = sxx + dels * dels_w;
+      syy = syy + deln * deln_w;
+      sxy = sxy + dels * deln_w;
+      T qt_i[4], qt_k[4], phi_i[4], phi_k[4];
+      for (int r = 0; r < 4; ++r) {
+        qt_i[r] = q[4 * i + r] - 0.5 * (delx * dq[8 * i + 2 * r] + dely * dq[8 * i + 2 * r + 1]);
+        qt_k[r] = q[4 * k + r] - 0.5 * (delx * dq[8 * k + 2 * r] + dely * dq[8 * k + 2 * r + 1]);
+      }
+      venkat_limiter(qt_i, phi_i, i);
+      venkat_limiter(qt_k, phi_k, k);
+      for (int r = 0; r < 4; ++r) {
+        qt_i[r] = q[4 * i + r] - 0.5 * phi_i[r] * (delx * dq[8 * i + 2 * r] + dely * dq[8 * i + 2 * r + 1]);
+        qt_k[r] = q[4 * k + r] - 0.5 * phi_k[r] * (delx * dq[8 * k + 2 * r] + dely * dq[8 * k + 2 * r + 1]);
+      }
+      T u1, u2, rho, pr, G_i[4], G_k[4];
+      qtilde_to_primitive(qt_i, u1, u2, rho, pr);
+      flux(kind, G_i, nx_, ny_, u1, u2, rho, pr);
+      qtilde_to_primitive(qt_k, u1, u2, rho, pr);
+      flux(kind, G_k, nx_, ny_, u1, u2, rho, pr);
+      for (int r = 0; r < 4; ++r) {
+        sxf[r] = sxf[r] + (G_k[r] - G_i[r]) * dels_w;
+        syf[r] = syf[r] + (G_k[r] - G_i[r]) * deln_w;
+      }
+    }
+    T det = sxx * syy - sxy * sxy;
+    T one_by_det = 1.0 / det;
+    for (int r = 0; r < 4; ++r) {
+      if (!solve_y) G[r] = (sxf[r] * syy - syf[r] * sxy) * one_by_det;
+      else G[r] = (syf[r] * sxx - sxf[r] * sxy) * one_by_det;
+    }
+  }
+
+  // flux_residual.F90:14-73 cal_flux_residual
+  void cal_flux_residual() {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      const size_t o = (size_t)i * W;
+      T Gxp[4], Gxn[4], Gyp[4], Gyn[4];
+      if (flag1[i] == 0) {  // wall: wall_fluxes.F90:20-292, flux_residual.F90:23-38
+        dG(Gxp, i, &xpos_c[o], xpos_n[i], GXII, false);
+        dG(Gxn, i, &xneg_c[o], xneg_n[i], GXI, false);
+        dG(Gyn, i, &yneg_c[o], yneg_n[i], GYN, true);
+        for (int r = 0; r < 4; ++r) {
+          T fr = Gxp[r] + Gxn[r] + Gyn[r];
+          flux_res[4 * i + r] = 2.0 * delta[i] * fr;
+        }
+      } else if (flag1[i] == 2) {  // outer: outer_fluxes.F90:18-300, flux_residual.F90:40-53
+        dG(Gxp, i, &xpos_c[o], xpos_n[i], GXIII, false);
+        dG(Gxn, i, &xneg_c[o], xneg_n[i], GXIV, false);
+        dG(Gyp, i, &ypos_c[o], ypos_n[i], GYP, true);
+        for (int r = 0; r < 4; ++r) flux_res[4 * i + r] = delta[i] * (Gxp[r] + Gxn[r] + Gyp[r]);
+      } else if (flag1[i] == 1) {  // interior: interior_fluxes.F90:19-404, :56-70
+        dG(Gxp, i, &xpos_c[o], xpos_n[i], GXP, false);
+        dG(Gxn, i, &xneg_c[o], xneg_n[i], GXN, false);
+        dG(Gyp, i, &ypos_c[o], ypos_n[i], GYP, true);
+        dG(Gyn, i, &yneg_c[o], yneg_n[i], GYN, true);
+        for (int r = 0; r < 4; ++r)
+          flux_res[4 * i + r] = delta[i] * (Gxp[r] + Gxn[r] + Gyp[r] + Gyn[r]);
+      }
+    }
+  }
+
+  // state_update.F90:144-163 primitive_to_conserved (rotated frame)
+  static void primitive_to_conserved(const T* pr_, T nx_, T ny_, T U[4]) {
+    T rho = pr_[0];
+    U[0] = rho;
+    T temp1 = rho * pr_[1];
+    T temp2 = rho * pr_[2];
+    U[3] = 2.5 * pr_[3] + 0.5 * (temp1 * temp1 + temp2 * temp2) / rho;
+    U[1] = temp1 * ny_ - temp2 * nx_;
+    U[2] = temp1 * nx_ + temp2 * ny_;
+  }
+
+  // state_update.F90:239-294 conserved_vector_Ubar (kinetic far-field state)
+  void conserved_vector_Ubar(const T* pr_, T Ubar[4], T nx_, T ny_) const {
+    double u1_inf = q_inf[1], u2_inf = q_inf[2];
+    T tx = ny_, ty = -nx_;
+    T u1_inf_rot = u1_inf * tx + u2_inf * ty;
+    T u2_inf_rot = u1_inf * nx_ + u2_inf * ny_;
+    T temp1 = (u1_inf_rot * u1_inf_rot + u2_inf_rot * u2_inf_rot);
+    T e_inf = PR_INF / (RHO_INF * (GAMMA - 1.0)) + 0.5 * (temp1);
+    double beta_inf = (0.5 * RHO_INF) / PR_INF;
+    T S2 = u2_inf_rot * std::sqrt(beta_inf);
+    T B2_inf = exp(-S2 * S2) / (2.0 * std::sqrt(PI * beta_inf));
+    T A2n_inf = 0.5 * (1.0 - erf(S2));
+    T rho = pr_[0], u1 = pr_[1], u2 = pr_[2], pr = pr_[3];
+    T u1_rot = u1 * tx + u2 * ty;
+    T u2_rot = u1 * nx_ + u2 * ny_;
+    temp1 = (u1_rot * u1_rot + u2_rot * u2_rot);
+    T e = pr / (rho * (GAMMA - 1.0)) + 0.5 * (temp1);
+    T beta = (rho) / (2.0 * pr);
+    S2 = u2_rot * sqrt(beta);
+    T B2 = exp(-S2 * S2) / (2.0 * sqrt(PI * beta));
+    T A2p = 0.5 * (1.0 + erf(S2));
+    Ubar[0] = (RHO_INF * A2n_inf) + (rho * A2p);
+    Ubar[1] = (RHO_INF * u1_inf_rot * A2n_inf) + (rho * u1_rot * A2p);
+    temp1 = RHO_INF * (u2_inf_rot * A2n_inf - B2_inf);
+    T temp2 = rho * (u2_rot * A2p + B2);
+    Ubar[2] = temp1 + temp2;
+    temp1 = (RHO_INF * A2n_inf * e_inf - 0.5 * RHO_INF * u2_inf_rot * B2_inf);
+    temp2 = (rho * A2p * e + 0.5 * rho * u2_rot * B2);
+    Ubar[3] = temp1 + temp2;
+  }
+
+  // state_update.F90:10-141 state_update(rk); rk is 1-based as in the reference
+  void state_update(int rk) {
+    const double obt = 1.0 / 3.0, tbt = 2.0 / 3.0;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n; ++k) {
+      T nx_ = nx[k], ny_ = ny[k], U[4], U_old[4];
+      if (flag1[k] == 2) {
+        conserved_vector_Ubar(&prim[4 * k], U, nx_, ny_);
+        conserved_vector_Ubar(&prim_old[4 * k], U_old, nx_, ny_);
+      } else {
+        primitive_to_conserved(&prim[4 * k], nx_, ny_, U);
+        primitive_to_conserved(&prim_old[4 * k], nx_, ny_, U_old);
+      }
+      T tmp = U[0];
+      for (int r = 0; r < 4; ++r) {
+        if (rk != 3) U[r] = U[r] - (0.5 * p.euler * flux_res[4 * k + r]);
+        else U[r] = tbt * U_old[r] + obt * (U[r] - 0.5 * flux_res[4 * k + r]);
+      }
+      if (flag1[k] == 0) U[2] = T(0.0);
+      T U2_rot = U[1], U3_rot = U[2];
+      U[1] = U2_rot * ny_ + U3_rot * nx_;
+      U[2] = U3_rot * ny_ - U2_rot * nx_;
+      res_sqr_pt[k] = (flag1[k] == 2) ? T(0.0) : (U[0] - tmp) * (U[0] - tmp);
+      prim[4 * k + 0] = U[0];
+      tmp = 1.0 / U[0];
+      prim[4 * k + 1] = U[1] * tmp;
+      prim[4 * k + 2] = U[2] * tmp;
+      prim[4 * k + 3] = 0.4 * (U[3] - (0.5 * tmp) * (U[1] * U[1] + U[2] * U[2]));
+    }
+    // state_update.F90:22-23,58,132: reset per call, summed wall points first, then
+    // interior points, each in point order
+    sum_res_sqr = T(0.0);
+    for (int k : wall_idx) sum_res_sqr = sum_res_sqr + res_sqr_pt[k];
+    for (int k : interior_idx) sum_res_sqr = sum_res_sqr + res_sqr_pt[k];
+  }
+
+  // fpi_solver.F90:13-83 fpi_solver(t) without the objective and file output
+  void fpi_solver(int t) {
+    for (size_t j = 0; j < prim.size(); ++j) prim_old[j] = prim[j];
+    func_delta();
+    for (int rk = 1; rk <= p.rks; ++rk) {
+      eval_q_variables();
+      eval_q_derivatives();
+      for (int it = 0; it < p.inner_iterations; ++it) {
+        eval_q_inner_loop();
+        eval_update_innerloop();
+      }
+      cal_flux_residual();
+      state_update(rk);
+    }
+    res_new = sqrt(sum_res_sqr) / double(n);
+    if (t <= 2 && p.restart == 0) {
+      res_old = res_new;
+      residue = T(0.0);
+    } else {
+      residue = log10(res_new / res_old);
+    }
+  }
+
+  // compute_force_coeffs.F90:11-102 compute_cl_cd_cm (cp per shape point, in order)
+  void compute_cl_cd_cm() {
+    std::vector<T> H(p.shapes, T(0.0)), V(p.shapes, T(0.0)), pm(p.shapes, T(0.0));
+    double tmp = 0.5 * RHO_INF * p.mach * p.mach;
+    for (size_t j = 0; j < shape_idx.size(); ++j) {
+      int m = shape_idx[j], r = right[m], l = left[m];
+      T lx = x[l], ly = y[l], mx = x[m], my = y[m], rx = x[r], ry = y[r];
+      T ds1 = (mx - lx) * (mx - lx) + (my - ly) * (my - ly);
+      ds1 = sqrt(ds1);
+      T ds2 = (rx - mx) * (rx - mx) + (ry - my) * (ry - my);
+      ds2 = sqrt(ds2);
+      T ds = 0.5 * (ds1 + ds2);
+      T nx_ = nx[m], ny_ = ny[m];
+      T c = prim[4 * m + 3] - PR_INF;
+      c = -c / tmp;
+      cp[j] = c;
+      int s = flag2[m] - 1;
+      H[s] = H[s] + c * nx_ * ds;
+      V[s] = V[s] + c * ny_ * ds;
+      pm[s] = pm[s] + (-c * ny_ * ds * (mx - 0.25) + c * nx_ * ds * (my));
+    }
+    for (int s = 0; s < p.shapes; ++s) {
+      Cl[s] = V[s] * std::cos(theta) - H[s] * std::sin(theta);
+      Cd[s] = H[s] * std::cos(theta) + V[s] * std::sin(theta);
+      Cm[s] = pm[s];
+    }
+  }
+};
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------------
+// extern "C" surface used by the Python test harness (oracle/oracle.py)
+// ---------------------------------------------------------------------------------
+using SolverD = orc::Solver<double>;
+
+extern "C" {
+
+struct orc_params {
+  double cfl, vl_const, power, mach, aoa_deg, euler;
+  int32_t rks, inner_iterations, shapes, interior_points_normal_flag, format, restart;
+  double restart_res_old;
+};
+
+void* orc_create(int32_t n, int32_t w, const double* x, const double* y,
+                 const int32_t* left, const int32_t* right, const int32_t* flag1,
+                 const int32_t* flag2, const double* min_dist, const int32_t* nbhs,
+                 const int32_t* conn, const double* nx_file, const double* ny_file,
+                 const orc_params* prm) {
+  auto* s = new SolverD();
+  s->n = n;
+  s->W = w;
+  std::memcpy(&s->p, prm, sizeof(orc::Params));
+  s->x.assign(x, x + n);
+  s->y.assign(y, y + n);
+  s->nx.assign(n, 0.0);
+  s->ny.assign(n, 0.0);
+  if (nx_file) s->nx.assign(nx_file, nx_file + n);
+  if (ny_file) s->ny.assign(ny_file, ny_file + n);
+  s->min_dist.assign(min_dist, min_dist + n);
+  s->left.assign(left, left + n);
+  s->right.assign(right, right + n);
+  s->flag1.assign(flag1, flag1 + n);
+  s->flag2.assign(flag2, flag2 + n);
+  s->nbhs.assign(nbhs, nbhs + n);
+  s->conn.assign(conn, conn + (size_t)n * w);
+  s->build_index_lists();
+  s->allocate();
+  s->setup_case_parameters();
+  s->res_old = prm->restart_res_old;
+  return s;
+}
+
+void orc_destroy(void* h) { delete static_cast<SolverD*>(h); }
+
+void orc_initial_conditions(void* h) { static_cast<SolverD*>(h)->initial_conditions(); }
+void orc_compute_normals(void* h) { static_cast<SolverD*>(h)->compute_normals(); }
+void orc_generate_connectivity(void* h) { static_cast<SolverD*>(h)->generate_connectivity(); }
+void orc_func_delta(void* h) { static_cast<SolverD*>(h)->func_delta(); }
+void orc_eval_q_variables(void* h) { static_cast<SolverD*>(h)->eval_q_variables(); }
+void orc_eval_q_derivatives(void* h) { static_cast<SolverD*>(h)->eval_q_derivatives(); }
+void orc_eval_q_inner_loop(void* h) { static_cast<SolverD*>(h)->eval_q_inner_loop(); }
+void orc_eval_update_innerloop(void* h) { static_cast<SolverD*>(h)->eval_update_innerloop(); }
+void orc_cal_flux_residual(void* h) { static_cast<SolverD*>(h)->cal_flux_residual(); }
+void orc_state_update(void* h, int32_t rk) { static_cast<SolverD*>(h)->state_update(rk); }
+void orc_fpi_solver(void* h, int32_t t) { static_cast<SolverD*>(h)->fpi_solver(t); }
+void orc_compute_cl_cd_cm(void* h) { static_cast<SolverD*>(h)->compute_cl_cd_cm(); }
+
+// q_lskum.F90:59-73: run `iters` iterations from iteration `first_it`, storing the
+// residue of each; stops after the first NaN residue.  Returns iterations done.
+int32_t orc_run(void* h, int32_t first_it, int32_t iters, double* residue_hist,
+                double* res_new_hist) {
+  auto* s = static_cast<SolverD*>(h);
+  int32_t done = 0;
+  for (int32_t t = first_it; t < first_it + iters; ++t) {
+    s->fpi_solver(t);
+    if (residue_hist) residue_hist[done] = s->residue;
+    if (res_new_hist) res_new_hist[done] = s->res_new;
+    ++done;
+    if (std::isnan(s->residue)) break;
+  }
+  return done;
+}
+
+// Scalar probe (for a point-wise debugger): the kinetic flux `kind` of one state
+// in the frame (nx, ny); and the qtilde -> primitive map.
+void orc_flux(int32_t kind, double nx, double ny, double u1, double u2, double rho,
+              double pr, double* G) {
+  SolverD::flux(static_cast<SolverD::FluxKind>(kind), G, nx, ny, u1, u2, rho, pr);
+}
+void orc_qtilde_to_primitive(const double* qt, double* out4) {
+  SolverD::qtilde_to_primitive(qt, out4[1], out4[2], out4[0], out4[3]);
+}
+
+double* orc_field(void* h, const char* name, int64_t* len) {
+  auto* s = static_cast<SolverD*>(h);
+  std::string f(name);
+  std::vector<double>* v = nullptr;
+  if (f == "prim") v = &s->prim;
+  else if (f == "prim_old") v = &s->prim_old;
+  else if (f == "q") v = &s->q;
+  else if (f == "dq") v = &s->dq;
+  else if (f == "qm") v = &s->qm;
+  else if (f == "temp") v = &s->temp;
+  else if (f == "flux_res") v = &s->flux_res;
+  else if (f == "delta") v = &s->delta;
+  else if (f == "x") v = &s->x;
+  else if (f == "y") v = &s->y;
+  else if (f == "nx") v = &s->nx;
+  else if (f == "ny") v = &s->ny;
+  else if (f == "Cl") v = &s->Cl;
+  else if (f == "Cd") v = &s->Cd;
+  else if (f == "Cm") v = &s->Cm;
+  else if (f == "cp") v = &s->cp;
+  else if (f == "res_sqr_pt") v = &s->res_sqr_pt;
+  if (!v) { *len = 0; return nullptr; }
+  *len = (int64_t)v->size();
+  return v->data();
+}
+
+int32_t* orc_ifield(void* h, const char* name, int64_t* len) {
+  auto* s = static_cast<SolverD*>(h);
+  std::string f(name);
+  std::vector<int32_t>* v = nullptr;
+  if (f == "xpos_nbhs") v = &s->xpos_n;
+  else if (f == "xneg_nbhs") v = &s->xneg_n;
+  else if (f == "ypos_nbhs") v = &s->ypos_n;
+  else if (f == "yneg_nbhs") v = &s->yneg_n;
+  else if (f == "xpos_conn") v = &s->xpos_c;
+  else if (f == "xneg_conn") v = &s->xneg_c;
+  else if (f == "ypos_conn") v = &s->ypos_c;
+  else if (f == "yneg_conn") v = &s->yneg_c;
+  else if (f == "shape_idx") v = &s->shape_idx;
+  if (!v) { *len = 0; return nullptr; }
+  *len = (int64_t)v->size();
+  return v->data();
+}
+
+double orc_scalar(void* h, const char* name) {
+  auto* s = static_cast<SolverD*>(h);
+  std::string f(name);
+  if (f == "sum_res_sqr") return s->sum_res_sqr;
+  if (f == "res_old") return s->res_old;
+  if (f == "res_new") return s->res_new;
+  if (f == "residue") return s->residue;
+  return NAN;
+}
+
+}  // extern "C"
