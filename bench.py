@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the LSKUM fixed-point iteration on one MI355X.
+
+Workload (one "step" = one fpi_solver iteration over the whole cloud): NACA0012
+O-cloud 640 x 250 = 160,000 points, 24-neighbour cap, second order with the
+Venkatakrishnan limiter (K = 5), 3 inner q-derivative iterations, M 0.85, AoA 1 deg,
+CFL 0.2, SSPRK(4,3) — synthetic cloud from mfcfd_amd.cloud, freestream start.
+
+Both precision paths run the same W + K iterations from freestream; the timed region
+is the K iterations (inputs resident in HBM, HIP graphs replayed back to back).
+``value`` is the mixed path's point-iterations/s when its Cl and Cd stay within 1e-4
+of the FP64 path's, otherwise the FP64 path's.  The roofline figures are for the
+dominant kernel (the fused flux + state update), timed with HIP events on the
+solver's stream; the CPU baseline times the FP64 oracle (a C++ restatement of the
+reference, OpenMP) on a few iterations of the same cloud.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(n_wall=640, n_radial=250, mach=0.85, aoa=1.0, cfl=0.2, vl_const=5.0,
+                inner_iterations=3, seed=0)
+
+
+def flux_update_bytes_per_point(state_bytes: int, mean_nbhs: float) -> float:
+    """Algorithmic HBM bytes of one fused flux+update launch, per point.
+
+    Reads of the point's own records (q 4, dq 8, qm 8, prim 4, prim_old 4 words of
+    state; eps 1 state word; xy, nrm 4 doubles; half-stencil solve coefficients 12
+    doubles; delta 1 double; nbhs 4 B; conn 4 B per slot) and writes (prim, q: 8
+    state words).  Neighbour gathers are served from cache and not counted.
+    """
+    w = state_bytes
+    reads = (4 + 8 + 8 + 4 + 4 + 1) * w + (4 + 12 + 1) * 8 + 4 + 4 * mean_nbhs
+    writes = 8 * w
+    return reads + writes
+
+
+def iteration_bytes_per_point(state_bytes: int, mean_nbhs: float, inner: int, stages: int = 4) -> float:
+    """Algorithmic bytes per point-iteration, B = S*[(1+I)(2g+20w+4N) + (5g+29w+5N)]
+    + (2g+9w+4N) with g = 8 (geometry) and w the state word size."""
+    g, w, N = 8.0, float(state_bytes), mean_nbhs
+    return stages * ((1 + inner) * (2 * g + 20 * w + 4 * N) + (5 * g + 29 * w + 5 * N)) + (2 * g + 9 * w + 4 * N)
+
+
+def run_gpu(cloud, precision: str, warmup: int, steps: int):
+    from mfcfd_amd.solver import LskumSolver
+    p = {k: WORKLOAD[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
+    s = LskumSolver(cloud, precision=precision, **p)
+    print(f"[bench] {precision}: warmup {warmup}", file=sys.stderr, flush=True)
+    s.q_lskum(warmup, first_it=1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run_async(steps, first_it=warmup + 1)
+    s.sync()
+    dt = time.perf_counter() - t0
+    res, rn, cd = s.q_lskum(1, first_it=warmup + steps + 1)
+    cl, cdv, cm, _ = s.compute_cl_cd_cm()
+    prof = s.profile_kernels(3)
+    lay = s.layout()
+    out = dict(seconds=dt, ms_per_step=1e3 * dt / steps,
+               point_iters_per_s=cloud.n * steps / dt, cl=float(cl[0]), cd=float(cdv[0]),
+               residue=float(res[-1]) if len(res) else None, kernels=prof, layout=lay)
+    s.close()
+    print(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/iter "
+          f"{out['point_iters_per_s']:.3e} pt-it/s", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_baseline(cloud, iters: int):
+    from oracle.oracle import Oracle, num_threads
+    o = Oracle(cloud, cfl=WORKLOAD["cfl"], mach=WORKLOAD["mach"], aoa_deg=WORKLOAD["aoa"],
+               vl_const=WORKLOAD["vl_const"], inner_iterations=WORKLOAD["inner_iterations"]).setup()
+    print(f"[bench] cpu baseline: {iters} oracle iterations", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    o.run(iters)
+    dt = time.perf_counter() - t0
+    return dict(value=cloud.n * iters / dt, unit="point-iters/s", cores=num_threads(),
+                kind="port",
+                sample=f"{iters} FP64 iterations of the same {cloud.n}-point cloud by the C++ "
+                       f"oracle (restatement of src_mpi_primal), OpenMP over points, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the LSKUM path here is single-GPU "
+              "(multi-GPU halo exchange is design-only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from mfcfd_amd.cloud import naca_ogrid
+    t0 = time.perf_counter()
+    cloud = naca_ogrid(WORKLOAD["n_wall"], WORKLOAD["n_radial"], seed=WORKLOAD["seed"])
+    print(f"[bench] cloud {cloud.n} points in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    mean_nbhs = float(cloud.nbhs.mean())
+
+    fp64 = run_gpu(cloud, "fp64", args.warmup, args.steps)
+    mixed = run_gpu(cloud, "mixed", args.warmup, args.steps)
+    dcl = abs(mixed["cl"] - fp64["cl"])
+    dcd = abs(mixed["cd"] - fp64["cd"])
+    use_mixed = dcl <= 1e-4 and dcd <= 1e-4
+    best = mixed if use_mixed else fp64
+    sb = 4 if use_mixed else 8
+
+    fu_ms = best["kernels"]["flux_update"][0]
+    per_launch = flux_update_bytes_per_point(sb, mean_nbhs) * cloud.n
+    achieved = per_launch / (fu_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_flux_update.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        traffic = t.get("fp32" if use_mixed else "fp64")
+
+    it_bytes = iteration_bytes_per_point(sb, mean_nbhs, WORKLOAD["inner_iterations"])
+    line = {
+        "metric": "point-iters/s + HBM GB/s, FP32-mixed vs FP64 at dCl<=1e-4; adjoint/primal cost",
+        "value": best["point_iters_per_s"],
+        "unit": "point-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if use_mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": "NACA0012 O-cloud 640x250 (160,000 points), 2nd order, "
+                               "Venkatakrishnan K=5, 3 inner iterations, M 0.85, AoA 1, CFL 0.2",
+                   "points": cloud.n, "mean_nbhs": round(mean_nbhs, 3),
+                   "precision": "mixed" if use_mixed else "fp64", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_flux_update", "kernel_ms": fu_ms,
+                     "algorithmic_bytes_per_launch": per_launch},
+        "iteration_hbm_gbs": it_bytes * best["point_iters_per_s"] / 1e9,
+        "fp64_point_iters_per_s": fp64["point_iters_per_s"],
+        "mixed_point_iters_per_s": mixed["point_iters_per_s"],
+        "mixed_over_fp64": mixed["point_iters_per_s"] / fp64["point_iters_per_s"],
+        "mixed_vs_fp64": {"dCl": dcl, "dCd": dcd, "cl_fp64": fp64["cl"], "cd_fp64": fp64["cd"]},
+        "kernel_ms": {"fp64": {k: v[0] for k, v in fp64["kernels"].items()},
+                      "mixed": {k: v[0] for k, v in mixed["kernels"].items()}},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cloud, args.cpu_iters)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/* mfcfd.h -- C ABI of the MI355X (gfx950) LSKUM fixed-point iteration.
+ *
+ * The reference (Nischay-Pro/mfcfd, src_mpi_primal) has no FFI: its hot path is a set
+ * of argument-less Fortran module procedures that mutate module state
+ * (data_structure.F90:13-49).  This ABI replaces that module state with an opaque
+ * context and exports one entry point per reference procedure, plus a fused,
+ * graph-captured driver for the whole q_lskum loop.  A Fortran host binds it through
+ * ISO_C_BINDING (interfaces in mfcfd_amd/fortran/mfcfd_api.f90).
+ *
+ * Conventions
+ *  - Every function returns MF_OK (0) or a negative error code; mf_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - Host arrays are owned by the caller and use the reference's layouts and its
+ *    original point ordering: prim (4,N), dq/qm (2,4,N) column-major, 1-based
+ *    neighbour indices in conn(N,W).
+ *  - The context owns all device memory and one HIP stream; a context is not
+ *    thread-safe.  Only one GPU is used (the current HIP device at mf_create).
+ */
+#ifndef MFCFD_H
+#define MFCFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_OK 0
+#define MF_ERR_HIP (-1)      /* a HIP runtime call failed                        */
+#define MF_ERR_INVALID (-2)  /* invalid argument or cloud                         */
+#define MF_ERR_STATE (-3)    /* call not valid in the context's current state     */
+
+#define MF_FP64 0   /* every step in FP64, as the reference                       */
+#define MF_MIXED 1  /* FP32 state, flux, limiter, update; FP64 LSQ, norms, forces */
+
+typedef struct mf_ctx mf_ctx;
+
+/* One point cloud, as read by point_preprocessor.F90:177-546 (single rank). */
+typedef struct mf_geometry {
+  int32_t n_points;        /* max_points = local_points                              */
+  int32_t max_nbhs;        /* W, leading dimension of conn (reference: 20)           */
+  const double *x, *y;     /* [N] coordinates                                        */
+  const double *nx, *ny;   /* [N] file normals, read only when format == 2; may be NULL */
+  const double *min_dist;  /* [N]                                                     */
+  const int32_t *left;     /* [N] 1-based left neighbour of boundary points (else 0)  */
+  const int32_t *right;    /* [N] 1-based right neighbour                             */
+  const int32_t *flag_1;   /* [N] 0 wall, 1 interior, 2 outer                         */
+  const int32_t *flag_2;   /* [N] shape id (1..shapes) of wall points, else 0         */
+  const int32_t *nbhs;     /* [N] neighbour counts                                    */
+  const int32_t *conn;     /* conn(N,W) column-major, 1-based: conn[(k-1)*N + (i-1)] */
+} mf_geometry;
+
+/* Run options (readcase.F90:10-125 names and defaults). */
+typedef struct mf_params {
+  double cfl;              /* -cfl                                                   */
+  double vl_const;         /* -vl_const (Venkatakrishnan K), default 150             */
+  double power;            /* -power, LSQ weights dist**power, default 0             */
+  double mach;             /* Mach (parameter.F90:5)                                 */
+  double aoa;              /* angle of attack, degrees (parameter.F90:6)             */
+  double euler;            /* 1 for ssprk43, 2 for first (readcase.F90:50-60)        */
+  double res_old;          /* residual reference when restart == 1                   */
+  int32_t rks;             /* RK stages: 4 (ssprk43) or 1 (first)                    */
+  int32_t inner_iterations;/* -inner_iterations                                      */
+  int32_t shapes;          /* -shapes                                                */
+  int32_t interior_points_normal_flag; /* 0: (0,1), 1: (1,0)                         */
+  int32_t format;          /* 1 legacy, 2 quadtree (file normals for interior points) */
+  int32_t restart;         /* 0: res_old taken at iterations 1-2; 1: given           */
+  int32_t precision;       /* MF_FP64 or MF_MIXED                                    */
+  int32_t reorder;         /* 1: Hilbert-curve point order on the device (default)   */
+  int32_t block_size;      /* threads per block; 0 = built-in choice                 */
+  int32_t reserved;        /* must be 0                                              */
+} mf_params;
+
+/* Fills `p` with readcase.F90 defaults (cfl 0, vl_const 150, power 0, ssprk43, ...). */
+void mf_default_params(mf_params *p);
+
+/* Builds the context: normals (point_normals.F90:8-105), stencil split
+ * (generate_connectivity.F90:8-256), device layout, freestream initial state
+ * (initial_conditions.F90:9-24).  Replaces readcase/read_hdf5input_point_data/
+ * allocate_soln/initial_conditions/compute_normals/generate_connectivity as the
+ * module-state setup done in meshfree_solver.F90:42-70 and q_lskum.F90:33-34. */
+int mf_create(const mf_geometry *geom, const mf_params *params, mf_ctx **ctx);
+int mf_destroy(mf_ctx *ctx);
+const char *mf_last_error(void);
+
+/* point%prim in/out, (4,N) column-major, original order (initial_conditions.F90:36-70
+ * restart read; post_processing.F90:11-39 writer). */
+int mf_set_prim(mf_ctx *ctx, const double *prim);
+int mf_get_prim(mf_ctx *ctx, double *prim);
+
+/* Fields of the module state, original order, reference layouts. */
+#define MF_FIELD_Q 1         /* q(4,N)                                */
+#define MF_FIELD_DQ 2        /* dq(2,4,N)                             */
+#define MF_FIELD_QM 3        /* qm(2,4,N)                             */
+#define MF_FIELD_FLUX_RES 4  /* flux_res(4,N) (last cal_flux_residual) */
+#define MF_FIELD_DELTA 5     /* delta(N)                              */
+#define MF_FIELD_PRIM_OLD 6  /* prim_old(4,N)                         */
+#define MF_FIELD_NX 7        /* nx(N)                                 */
+#define MF_FIELD_NY 8        /* ny(N)                                 */
+int mf_get_field(mf_ctx *ctx, int32_t field, double *out);
+/* Accepts MF_FIELD_Q, MF_FIELD_DQ, MF_FIELD_QM, MF_FIELD_DELTA, MF_FIELD_PRIM_OLD. */
+int mf_set_field(mf_ctx *ctx, int32_t field, const double *in);
+
+/* One entry point per reference procedure, acting on the context's state. */
+int mf_func_delta(mf_ctx *ctx);            /* state_update.F90:190-236   */
+int mf_eval_q_variables(mf_ctx *ctx);      /* q_variables.F90:8-37       */
+int mf_eval_q_derivatives(mf_ctx *ctx);    /* q_variables.F90:41-112     */
+/* eval_q_inner_loop followed by eval_update_innerloop (q_variables.F90:115-227) */
+int mf_eval_q_inner_loop(mf_ctx *ctx);
+int mf_cal_flux_residual(mf_ctx *ctx);     /* flux_residual.F90:14-73    */
+/* state_update.F90:10-141; rk is 1-based; returns sum_res_sqr of this call */
+int mf_state_update(mf_ctx *ctx, int32_t rk, double *sum_res_sqr);
+/* compute_force_coeffs.F90:11-102 on the current prim: per-shape Cl, Cd, Cm
+ * (arrays of `shapes`) and cp of every shape point in original point order. */
+int mf_compute_cl_cd_cm(mf_ctx *ctx, double *cl, double *cd, double *cm, double *cp);
+int32_t mf_shape_points(mf_ctx *ctx);
+
+/* fpi_solver.F90:13-83 repeated `iters` times from iteration `first_it`, i.e. the
+ * q_lskum.F90:59-73 loop, fused and replayed as a HIP graph.  Per iteration it stores
+ * residue (fpi_solver.F90:63-70) and res_new, and Cd of shape 1 (the reference's
+ * vector_cost_func).  Stops after the first NaN residue, as q_lskum.F90:71 does;
+ * *iters_done reports how many ran.  Any history pointer may be NULL. */
+int mf_run(mf_ctx *ctx, int32_t first_it, int32_t iters, double *residue_hist,
+           double *res_new_hist, double *cd_hist, int32_t *iters_done);
+/* Same iterations, queued without any host synchronisation or history copy-out:
+ * for timing.  mf_sync waits for the context's stream. */
+int mf_run_async(mf_ctx *ctx, int32_t first_it, int32_t iters);
+int mf_sync(mf_ctx *ctx);
+
+/* Kernel timing: runs `iters` fused iterations launched one kernel at a time with HIP
+ * events around every launch on the context's stream, and returns the mean duration
+ * in ms of each kernel kind: [0] lsq, [1] inner, [2] flux+update, [3] finalize,
+ * [4] forces.  Count of launches of each kind per iteration in `launches`. */
+#define MF_NUM_KERNEL_KINDS 5
+int mf_profile_kernels(mf_ctx *ctx, int32_t iters, double *mean_ms, int32_t *launches);
+
+/* Device layout facts for bandwidth accounting: bytes of each device array class. */
+typedef struct mf_layout_info {
+  int64_t n_points, n_wall, n_outer, n_interior, n_shape;
+  int64_t max_nbhs, total_nbhs;  /* sum of nbhs                                   */
+  int64_t state_bytes;           /* bytes per scalar of the state arrays (8 or 4) */
+} mf_layout_info;
+int mf_layout(mf_ctx *ctx, mf_layout_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFCFD_H */

@@ -236,47 +236,100 @@ def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
 # 2 ypos, 3 yneg
 _REQUIRED_BITS = {0: (0, 1, 3), 1: (0, 1, 2, 3), 2: (0, 1, 2)}
 _MIN_HALF = 3
+# 1 - (correlation of delta_s, delta_n)^2 of a half-stencil's 2x2 normal matrix
+_MIN_COND = 0.05
+
+
+def _half_quality(ds: np.ndarray, dn: np.ndarray, member: np.ndarray) -> float:
+    if member.sum() < _MIN_HALF:
+        return 0.0
+    a, b = ds[member], dn[member]
+    sxx, syy, sxy = (a * a).sum(), (b * b).sum(), (a * b).sum()
+    if sxx <= 0 or syy <= 0:
+        return 0.0
+    return float((sxx * syy - sxy * sxy) / (sxx * syy))
+
+
+def _rotated(cloud: PointCloud, i: int, idx: np.ndarray):
+    delx = cloud.x[idx] - cloud.x[i]
+    dely = cloud.y[idx] - cloud.y[i]
+    tx, ty = cloud.ny[i], -cloud.nx[i]
+    return delx * tx + dely * ty, delx * cloud.nx[i] + dely * cloud.ny[i]
 
 
 def _repair_stencils(cloud: PointCloud, tree, max_nbhs: int) -> int:
-    """Re-pick neighbours of points whose half-stencils hold fewer than 3 points.
+    """Re-pick neighbours of points whose half-stencils are too small or degenerate.
 
-    The nearest-first pick can leave a half-plane (e.g. behind a trailing-edge fan)
-    nearly empty, which makes that side's 2x2 least-squares system singular.  For such
-    points the stencil is re-chosen from the 4*max_nbhs nearest: nearest first, but
-    reserving room so every half-plane the point's class uses gets at least 3 points.
+    The nearest-first pick can leave a half-plane (e.g. behind a trailing-edge fan, or
+    outward of a strongly stretched ring) with fewer than 3 points or with nearly
+    collinear points, which makes that side's 2x2 least-squares system (nearly)
+    singular.  For such points the stencil is re-chosen from a widening set of nearest
+    candidates: each needed half-plane first gets its nearest members until it holds
+    3 points and 1 - corr(delta_s, delta_n)^2 >= 0.05, then the rest is filled nearest
+    first.
     """
-    mask = split_stencils(cloud, cloud.nx, cloud.ny)
     n_bad = 0
-    for cls, bits in _REQUIRED_BITS.items():
-        sel = cloud.flag_1 == cls
-        counts = np.stack([((mask & (1 << b)) != 0).sum(axis=1) for b in bits], axis=1)
-        bad = np.where(sel & (counts.min(axis=1) < _MIN_HALF))[0]
-        for i in bad:
-            _, cand = tree.query([cloud.x[i], cloud.y[i]], k=4 * max_nbhs + 1)
-            cand = [int(c) for c in cand if c != i and c < cloud.n]
-            m = _mask_row(cloud, i, np.array(cand))
+    # vectorised screen of every point's half-stencils
+    N, W = cloud.conn.shape
+    valid = cloud.conn >= 0
+    kk = np.where(valid, cloud.conn, 0)
+    delx = cloud.x[kk] - cloud.x[:, None]
+    dely = cloud.y[kk] - cloud.y[:, None]
+    dsa = delx * cloud.ny[:, None] - dely * cloud.nx[:, None]
+    dna = delx * cloud.nx[:, None] + dely * cloud.ny[:, None]
+    mask = split_stencils(cloud, cloud.nx, cloud.ny)
+    bad = np.zeros(N, dtype=bool)
+    for b in range(4):
+        need = np.isin(cloud.flag_1, [c for c, bb in _REQUIRED_BITS.items() if b in bb])
+        mem = ((mask >> b) & 1).astype(bool) & valid
+        cnt = mem.sum(axis=1)
+        sxx = np.where(mem, dsa * dsa, 0).sum(axis=1)
+        syy = np.where(mem, dna * dna, 0).sum(axis=1)
+        sxy = np.where(mem, dsa * dna, 0).sum(axis=1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            qual = np.where((sxx > 0) & (syy > 0), (sxx * syy - sxy * sxy) / (sxx * syy), 0.0)
+        bad |= need & ((cnt < _MIN_HALF) | (qual < _MIN_COND))
+    for i in np.where(bad)[0]:
+        bits = _REQUIRED_BITS[int(cloud.flag_1[i])]
+        kq = 4 * max_nbhs + 1
+        while True:
+            _, cand = tree.query([cloud.x[i], cloud.y[i]], k=min(kq, cloud.n))
+            cand = np.array([int(c) for c in np.atleast_1d(cand) if c != i and c < cloud.n])
+            m = _mask_row(cloud, i, cand)
+            ds, dn = _rotated(cloud, i, cand)
             chosen: list[int] = []
-            have = {b: 0 for b in bits}
-            for b in bits:  # reserve the nearest _MIN_HALF of each needed half-plane
-                for j, c in enumerate(cand):
-                    if have[b] >= _MIN_HALF:
+            # scarcest half first; each half may claim at most an equal share
+            share = max(_MIN_HALF, max_nbhs // len(bits))
+            order = sorted(bits, key=lambda b: int(((m >> b) & 1).sum()))
+            for b in order:
+                mem = [j for j in range(len(cand)) if (m[j] >> b) & 1]
+                taken = [j for j in mem if cand[j] in chosen]
+                for j in mem:
+                    sel = np.zeros(len(cand), dtype=bool)
+                    sel[taken] = True
+                    if len(taken) >= _MIN_HALF and (
+                            _half_quality(ds, dn, sel) >= _MIN_COND or len(taken) >= share):
                         break
-                    if (m[j] >> b) & 1 and c not in chosen:
-                        chosen.append(c)
-                        for bb in bits:
-                            have[bb] += (m[j] >> bb) & 1
-            for c in cand:  # fill the rest nearest first
-                if len(chosen) >= max_nbhs:
-                    break
-                if c not in chosen:
-                    chosen.append(c)
-            chosen = chosen[:max_nbhs]
-            chosen.sort(key=lambda c: (cloud.x[c] - cloud.x[i]) ** 2 + (cloud.y[c] - cloud.y[i]) ** 2)
-            cloud.conn[i, :] = -1
-            cloud.conn[i, :len(chosen)] = chosen
-            cloud.nbhs[i] = len(chosen)
-            n_bad += 1
+                    if j not in taken:
+                        taken.append(j)
+                        if cand[j] not in chosen:
+                            chosen.append(int(cand[j]))
+            filled = all(sum(1 for c in chosen if (m[list(cand).index(c)] >> b) & 1) >= _MIN_HALF
+                         for b in bits)
+            if (filled and len(chosen) <= max_nbhs) or kq >= cloud.n:
+                break
+            kq *= 4
+        for c in cand:  # fill the rest nearest first
+            if len(chosen) >= max_nbhs:
+                break
+            if int(c) not in chosen:
+                chosen.append(int(c))
+        chosen = chosen[:max_nbhs]
+        chosen.sort(key=lambda c: (cloud.x[c] - cloud.x[i]) ** 2 + (cloud.y[c] - cloud.y[i]) ** 2)
+        cloud.conn[i, :] = -1
+        cloud.conn[i, :len(chosen)] = chosen
+        cloud.nbhs[i] = len(chosen)
+        n_bad += 1
     return n_bad
 
 

@@ -1,0 +1,1265 @@
+// MI355X (gfx950) LSKUM fixed-point iteration: kernels and the C ABI of include/mfcfd.h.
+//
+// Host side: builds the device layout from the reference's point arrays (normals,
+// stencil split, Hilbert renumbering, point-fixed least-squares geometry), owns the
+// device state of one cloud, and replays one fpi_solver iteration
+// (fpi_solver.F90:13-83) as a captured HIP graph.  Device side: lskum_device.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mfcfd.h"
+#include "lskum_device.h"
+
+using namespace mf;
+
+// =================================================================================
+// Kernels
+// =================================================================================
+
+// q_variables.F90:8-37 over all points, plus each point's wave speed for func_delta
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __restrict__ prim,
+                                                       V4<S>* __restrict__ q,
+                                                       double* __restrict__ cspeed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const V4<S> p = prim[i];
+  q[i] = q_of_prim(p);
+  cspeed[i] = wave_speed(p);
+}
+
+// q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
+// (state_update.F90:190-236), which gathers over the same stencil.
+template <class S, bool POWER0, bool DELTA>
+__global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__ q,
+                                               V4<S>* __restrict__ dq, V4<S>* __restrict__ qm,
+                                               const double* __restrict__ cspeed,
+                                               double* __restrict__ delta, double cfl,
+                                               double power, const Ctl* __restrict__ ctl) {
+  if (ctl && ctl->halt) return;
+  const int i = point_id();
+  if (i >= g.n) return;
+  const D2 pi = g.xy[i];
+  const V4<S> qi = q[i];
+  V4<S> qmax = qi, qmin = qi;
+  double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
+  double min_delt = 1.0;
+  const int nb = g.nbhs[i];
+  const uint32_t* cp = g.conn + i;
+  for (int s = 0; s < nb; ++s) {
+    const int k = (int)(cp[(size_t)s * g.n] >> 4);
+    const V4<S> qk = q[k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (qk[r] > qmax[r]) qmax[r] = qk[r];
+      if (qk[r] < qmin[r]) qmin[r] = qk[r];
+    }
+    const D2 pk = g.xy[k];
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    double w = 1.0;
+    if (!POWER0 || DELTA) {
+      const double dist = sqrt(delx * delx + dely * dely);
+      if (!POWER0) w = pow(dist, power);
+      if (DELTA) {
+        double dt = dist / cspeed[k];
+        dt = cfl * dt;
+        if (min_delt > dt) min_delt = dt;
+      }
+    }
+    const double wx = w * delx, wy = w * dely;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double d = (double)qk[r] - (double)qi[r];
+      sxq[r] = sxq[r] + wx * d;
+      syq[r] = syq[r] + wy * d;
+    }
+  }
+  const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
+  V4<S> ox, oy;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    ox[r] = (S)((sxq[r] * gg.y - syq[r] * gg.z) * gg.w);
+    oy[r] = (S)((syq[r] * gg.x - sxq[r] * gg.z) * gg.w);
+  }
+  dq[2 * i] = ox;
+  dq[2 * i + 1] = oy;
+  qm[2 * i] = qmax;
+  qm[2 * i + 1] = qmin;
+  if (DELTA) delta[i] = min_delt;
+}
+
+// q_variables.F90:115-227 eval_q_inner_loop + eval_update_innerloop (Jacobi: reads
+// dq_in, writes dq_out)
+template <class S, bool POWER0>
+__global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict__ q,
+                                                 const V4<S>* __restrict__ dq_in,
+                                                 V4<S>* __restrict__ dq_out, double power,
+                                                 const Ctl* __restrict__ ctl) {
+  if (ctl && ctl->halt) return;
+  const int i = point_id();
+  if (i >= g.n) return;
+  const D2 pi = g.xy[i];
+  const V4<S> qi = q[i], dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
+  double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nb = g.nbhs[i];
+  const uint32_t* cp = g.conn + i;
+  for (int s = 0; s < nb; ++s) {
+    const int k = (int)(cp[(size_t)s * g.n] >> 4);
+    const D2 pk = g.xy[k];
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    double w = 1.0;
+    if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+    const V4<S> qk = q[k], dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
+    const double wx = w * delx, wy = w * dely;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
+      const double temp2 = (double)qk[r] - 0.5 * (delx * (double)dxk[r] + dely * (double)dyk[r]);
+      sxq[r] = sxq[r] + (wx * (temp2 - temp1));
+      syq[r] = syq[r] + (wy * (temp2 - temp1));
+    }
+  }
+  const V4d gg = g.lsq_geo[i];
+  V4<S> ox, oy;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    ox[r] = (S)(gg.w * (sxq[r] * gg.y - syq[r] * gg.z));
+    oy[r] = (S)(gg.w * (syq[r] * gg.x - sxq[r] * gg.z));
+  }
+  dq_out[2 * i] = ox;
+  dq_out[2 * i + 1] = oy;
+}
+
+template <class S, bool POWER0>
+__device__ inline void flux_dispatch(const Geom& g, const State<S>& st, int i, double delta,
+                                     double power, double fr[4]) {
+  if (i < g.nw) flux_point<WALL, S, POWER0>(g, st, i, delta, power, fr);
+  else if (i < g.nw + g.no) flux_point<OUTER, S, POWER0>(g, st, i, delta, power, fr);
+  else flux_point<INTERIOR, S, POWER0>(g, st, i, delta, power, fr);
+}
+
+// flux_residual.F90:14-73 cal_flux_residual (unfused: stores flux_res)
+template <class S, bool POWER0>
+__global__ __launch_bounds__(BLOCK, 2) void k_flux(Geom g, State<S> st,
+                                                const double* __restrict__ delta,
+                                                V4d* __restrict__ flux_res, double power) {
+  const int i = point_id();
+  if (i >= g.n) return;
+  double fr[4];
+  flux_dispatch<S, POWER0>(g, st, i, delta[i], power, fr);
+  flux_res[i] = V4d{fr[0], fr[1], fr[2], fr[3]};
+}
+
+template <class S>
+__device__ inline V4<S> update_dispatch(const Geom& g, int i, const V4<S>& p, const V4<S>& po,
+                                        const double fr[4], int rk, double euler,
+                                        const Freestream& fs, double* rs) {
+  const D2 nn = g.nrm[i];
+  if (i < g.nw) return update_point<WALL, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
+  if (i < g.nw + g.no) return update_point<OUTER, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
+  return update_point<INTERIOR, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
+}
+
+// state_update.F90:10-141 state_update(rk) (unfused: in place, partial residual sums)
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_update(Geom g, V4<S>* __restrict__ prim,
+                                                  const V4<S>* __restrict__ prim_old,
+                                                  const V4d* __restrict__ flux_res,
+                                                  double* __restrict__ partial, int rk,
+                                                  double euler, Freestream fs) {
+  const int i = point_id();
+  double rs = 0.0;
+  if (i < g.n) {
+    const V4d f = flux_res[i];
+    const double fr[4] = {f.x, f.y, f.z, f.w};
+    prim[i] = update_dispatch<S>(g, i, prim[i], prim_old[i], fr, rk, euler, fs, &rs);
+  }
+  const double t = block_sum(rs);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Fused cal_flux_residual + state_update(rk) + eval_q_variables of the new state.
+// Reads the stage's (q, dq, qm, prim), writes the next (prim, q) buffers; stage 1 also
+// saves prim_old (fpi_solver.F90:20-22), the last stage the residual partial sums and
+// the wave speeds for the next iteration's func_delta.
+template <class S, bool POWER0>
+__global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
+    Geom g, State<S> st, const V4<S>* __restrict__ prim, V4<S>* __restrict__ prim_old,
+    V4<S>* __restrict__ prim_out, V4<S>* __restrict__ q_out, double* __restrict__ cspeed,
+    const double* __restrict__ delta, double* __restrict__ partial, const Ctl* __restrict__ ctl,
+    int rk, int last, double euler, double power, Freestream fs) {
+  if (ctl->halt) return;
+  const int i = point_id();
+  double rs = 0.0;
+  if (i < g.n) {
+    double fr[4];
+    flux_dispatch<S, POWER0>(g, st, i, delta[i], power, fr);
+    const V4<S> p = prim[i];
+    if (rk == 1) prim_old[i] = p;
+    const V4<S> po = (rk == 3) ? prim_old[i] : p;
+    const V4<S> pn = update_dispatch<S>(g, i, p, po, fr, rk, euler, fs, &rs);
+    prim_out[i] = pn;
+    q_out[i] = q_of_prim(pn);
+    if (last) cspeed[i] = wave_speed(pn);
+  }
+  if (last) {
+    const double t = block_sum(rs);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+// state_update.F90:190-236 func_delta (unfused)
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_func_delta(Geom g, const V4<S>* __restrict__ prim,
+                                                      double* __restrict__ delta, double cfl) {
+  const int i = point_id();
+  if (i >= g.n) return;
+  const D2 pi = g.xy[i];
+  double min_delt = 1.0;
+  const int nb = g.nbhs[i];
+  for (int s = 0; s < nb; ++s) {
+    const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
+    const D2 pk = g.xy[k];
+    double dist = (pk.x - pi.x) * (pk.x - pi.x) + (pk.y - pi.y) * (pk.y - pi.y);
+    dist = sqrt(dist);
+    double dt = dist / wave_speed(prim[k]);
+    dt = cfl * dt;
+    if (min_delt > dt) min_delt = dt;
+  }
+  delta[i] = min_delt;
+}
+
+// Sum of the residual partials and the fpi_solver.F90:57-70 residue logic; one block.
+__global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ partial, int np,
+                                                    int n, int restart, int first_it,
+                                                    Ctl* __restrict__ ctl,
+                                                    double* __restrict__ res_hist,
+                                                    double* __restrict__ resnew_hist) {
+  if (ctl->halt) return;
+  double v = 0.0;
+  for (int j = threadIdx.x; j < np; j += blockDim.x) v += partial[j];
+  const double tot = block_sum(v);
+  if (threadIdx.x == 0) {
+    const int it = ctl->it;
+    const int t = first_it + it;
+    const double res_new = sqrt(tot) / (double)n;
+    double residue;
+    if (t <= 2 && restart == 0) {
+      ctl->res_old = res_new;
+      residue = 0.0;
+    } else {
+      residue = log10(res_new / ctl->res_old);
+    }
+    res_hist[it] = residue;
+    resnew_hist[it] = res_new;
+    if (isnan(residue)) ctl->halt = 1;
+    ctl->it = it + 1;
+  }
+}
+
+__global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old) {
+  ctl->it = 0;
+  ctl->halt = 0;
+  if (set_res_old) ctl->res_old = res_old;
+}
+
+// compute_force_coeffs.F90:11-102 on shape points (one block).  sgeo per shape point:
+// (nx, ny, ds, mx - 0.25, my); Cl/Cd/Cm per shape into clcdcm[3*s + {0,1,2}].
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_forces(const V4<S>* __restrict__ prim,
+                                                  const int32_t* __restrict__ sh_m,
+                                                  const int32_t* __restrict__ sh_id,
+                                                  const double* __restrict__ sgeo, int nsh,
+                                                  int shapes, double cp_scale, double ct,
+                                                  double st, double* __restrict__ cp,
+                                                  double* __restrict__ clcdcm,
+                                                  double* __restrict__ cd_hist,
+                                                  const Ctl* __restrict__ ctl) {
+  if (ctl && ctl->halt) return;
+  const double pr_inf = 1.0 / 1.4;
+  for (int s = 0; s < shapes; ++s) {
+    double H = 0.0, V = 0.0, M = 0.0;
+    for (int j = threadIdx.x; j < nsh; j += blockDim.x) {
+      const double c0 = (double)prim[sh_m[j]].w - pr_inf;
+      const double c = -c0 / cp_scale;
+      if (s == 0) cp[j] = c;
+      if (sh_id[j] != s) continue;
+      const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
+      H += c * nx * ds;
+      V += c * ny * ds;
+      M += (-c * ny * ds * sgeo[5 * j + 3] + c * nx * ds * sgeo[5 * j + 4]);
+    }
+    H = block_sum(H);
+    __syncthreads();
+    V = block_sum(V);
+    __syncthreads();
+    M = block_sum(M);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double cl = V * ct - H * st, cd = H * ct + V * st;
+      clcdcm[3 * s] = cl;
+      clcdcm[3 * s + 1] = cd;
+      clcdcm[3 * s + 2] = M;
+      if (s == 0 && ctl && cd_hist) cd_hist[ctl->it] = cd;
+    }
+  }
+}
+
+// Sum of partials into one scalar (unfused state_update's sum_res_sqr)
+__global__ __launch_bounds__(BLOCK) void k_sum(const double* __restrict__ partial, int np,
+                                               double* __restrict__ out) {
+  double v = 0.0;
+  for (int j = threadIdx.x; j < np; j += blockDim.x) v += partial[j];
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// =================================================================================
+// Host side
+// =================================================================================
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(MF_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                              \
+  } while (0)
+
+static const double kPi = 4.0 * std::atan(1.0);  // parameter.F90:11
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct mf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mf_params p{};
+  int n = 0, W = 0, nw = 0, no = 0, ni = 0, nshape = 0;
+  int64_t total_nbhs = 0;
+  size_t ssz = 8;  // bytes per state scalar
+  bool power0 = true;
+  std::vector<int32_t> perm;   // internal -> original
+  std::vector<int32_t> iperm;  // original -> internal
+  std::vector<double> nx_h, ny_h;  // original order
+  Geom g{};
+  Freestream fs{};
+  double theta = 0, cp_scale = 1;
+  // device arrays
+  std::vector<void*> allocs;
+  void* eps = nullptr;
+  void* prim[2] = {nullptr, nullptr};
+  void* q[2] = {nullptr, nullptr};
+  void* dq[2] = {nullptr, nullptr};
+  void* qm = nullptr;
+  void* prim_old = nullptr;
+  int cur = 0, dq_cur = 0;
+  V4d* flux_res = nullptr;
+  double *delta = nullptr, *cspeed = nullptr, *partial = nullptr, *scalar = nullptr;
+  int npartial = 0;
+  Ctl* ctl = nullptr;
+  double *res_hist = nullptr, *resnew_hist = nullptr, *cd_hist = nullptr;
+  int hist_cap = 0;
+  int32_t *sh_m = nullptr, *sh_id = nullptr;
+  double *sgeo = nullptr, *cp = nullptr, *clcdcm = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_first_it = 0;
+};
+
+template <class T>
+static int dev_alloc(mf_ctx* c, T** ptr, size_t count) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256)));
+  c->allocs.push_back(p);
+  *ptr = static_cast<T*>(p);
+  return MF_OK;
+}
+
+template <class T>
+static int upload(mf_ctx* c, T** ptr, const std::vector<T>& h) {
+  int rc = dev_alloc(c, ptr, h.size());
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(*ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return MF_OK;
+}
+
+// Hilbert index of (x, y) in [0, 2^order)^2
+static uint64_t hilbert_d(uint32_t order, uint64_t x, uint64_t y) {
+  uint64_t d = 0;
+  const uint64_t n = 1ull << order;
+  for (uint64_t s = n >> 1; s > 0; s >>= 1) {
+    const uint64_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+    d += s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+static inline unsigned split_mask(int flag1, double dels, double deln) {
+  // generate_connectivity.F90:39-256
+  unsigned m = 0;
+  if (dels <= 0.0) m |= XPOS;
+  if (dels >= 0.0) m |= XNEG;
+  if (flag1 == 1) {
+    if (deln <= 0.0) m |= YPOS;
+    if (deln >= 0.0) m |= YNEG;
+  } else if (flag1 == 0) {
+    m |= YNEG;
+  } else {
+    m |= YPOS;
+  }
+  return m;
+}
+
+template <class S>
+static int launch_q_variables(mf_ctx* c) {
+  const int nb = (c->n + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL((k_q_variables<S>), dim3(nb), dim3(BLOCK), 0, c->stream, c->n,
+                     (const V4<S>*)c->prim[c->cur], (V4<S>*)c->q[c->cur], c->cspeed);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+static int q_variables(mf_ctx* c) {
+  return c->ssz == 8 ? launch_q_variables<double>(c) : launch_q_variables<float>(c);
+}
+
+static int upload_prim_original(mf_ctx* c, const double* prim) {
+  const int n = c->n;
+  if (c->ssz == 8) {
+    std::vector<V4d> h(n);
+    for (int j = 0; j < n; ++j) {
+      const int o = c->perm[j];
+      h[j] = V4d{prim[4 * o], prim[4 * o + 1], prim[4 * o + 2], prim[4 * o + 3]};
+    }
+    HIPCHK(hipMemcpy(c->prim[c->cur], h.data(), n * sizeof(V4d), hipMemcpyHostToDevice));
+  } else {
+    std::vector<V4<float>> h(n);
+    for (int j = 0; j < n; ++j) {
+      const int o = c->perm[j];
+      h[j] = V4<float>{(float)prim[4 * o], (float)prim[4 * o + 1], (float)prim[4 * o + 2],
+                       (float)prim[4 * o + 3]};
+    }
+    HIPCHK(hipMemcpy(c->prim[c->cur], h.data(), n * sizeof(V4<float>), hipMemcpyHostToDevice));
+  }
+  return MF_OK;
+}
+
+// Downloads `count` V4<S> records per point from `src` and scatters them to the
+// reference layout (point-major, `count*4` doubles per point, original order).
+static int download_v4(mf_ctx* c, const void* src, int count, std::vector<double>& out) {
+  const int n = c->n;
+  out.assign((size_t)n * count * 4, 0.0);
+  if (c->ssz == 8) {
+    std::vector<V4d> h((size_t)n * count);
+    HIPCHK(hipMemcpy(h.data(), src, h.size() * sizeof(V4d), hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < count; ++a)
+        for (int r = 0; r < 4; ++r) out[((size_t)c->perm[j] * count + a) * 4 + r] = h[(size_t)count * j + a][r];
+  } else {
+    std::vector<V4<float>> h((size_t)n * count);
+    HIPCHK(hipMemcpy(h.data(), src, h.size() * sizeof(V4<float>), hipMemcpyDeviceToHost));
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < count; ++a)
+        for (int r = 0; r < 4; ++r) out[((size_t)c->perm[j] * count + a) * 4 + r] = h[(size_t)count * j + a][r];
+  }
+  return MF_OK;
+}
+
+static int upload_v4(mf_ctx* c, void* dst, int count, const double* in) {
+  const int n = c->n;
+  if (c->ssz == 8) {
+    std::vector<V4d> h((size_t)n * count);
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < count; ++a)
+        for (int r = 0; r < 4; ++r) h[(size_t)count * j + a][r] = in[((size_t)c->perm[j] * count + a) * 4 + r];
+    HIPCHK(hipMemcpy(dst, h.data(), h.size() * sizeof(V4d), hipMemcpyHostToDevice));
+  } else {
+    std::vector<V4<float>> h((size_t)n * count);
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < count; ++a)
+        for (int r = 0; r < 4; ++r) h[(size_t)count * j + a][r] = (float)in[((size_t)c->perm[j] * count + a) * 4 + r];
+    HIPCHK(hipMemcpy(dst, h.data(), h.size() * sizeof(V4<float>), hipMemcpyHostToDevice));
+  }
+  return MF_OK;
+}
+
+extern "C" {
+
+const char* mf_last_error(void) { return g_err.c_str(); }
+
+void mf_default_params(mf_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->cfl = 0.0;
+  p->vl_const = 150.0;
+  p->power = 0.0;
+  p->mach = 0.63;
+  p->aoa = 2.0;
+  p->euler = 1.0;
+  p->rks = 4;
+  p->inner_iterations = 0;
+  p->shapes = 1;
+  p->interior_points_normal_flag = 0;
+  p->format = 1;
+  p->restart = 0;
+  p->precision = MF_FP64;
+  p->reorder = 1;
+}
+
+int mf_destroy(mf_ctx* c) {
+  if (!c) return MF_OK;
+  for (auto& gexec : c->graph)
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+  for (void* a : c->allocs) (void)hipFree(a);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MF_OK;
+}
+
+static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c);
+
+int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
+  if (!geo || !prm || !out) return set_err(MF_ERR_INVALID, "mf_create: null argument");
+  *out = nullptr;
+  mf_ctx* c = new mf_ctx();
+  int rc = create_impl(geo, prm, c);
+  if (rc != MF_OK) {
+    std::string keep = g_err;
+    mf_destroy(c);
+    g_err = keep;
+    return rc;
+  }
+  *out = c;
+  return MF_OK;
+}
+
+}  // extern "C"
+
+static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) {
+  const int n = geo->n_points, W = geo->max_nbhs;
+  if (n <= 0 || W <= 0) return set_err(MF_ERR_INVALID, "n_points and max_nbhs must be > 0");
+  if ((int64_t)n >= (1ll << 27)) return set_err(MF_ERR_INVALID, "n_points must be < 2^27");
+  if (!geo->x || !geo->y || !geo->min_dist || !geo->left || !geo->right || !geo->flag_1 ||
+      !geo->flag_2 || !geo->nbhs || !geo->conn)
+    return set_err(MF_ERR_INVALID, "mf_geometry has a null array");
+  if (prm->precision != MF_FP64 && prm->precision != MF_MIXED)
+    return set_err(MF_ERR_INVALID, "precision must be MF_FP64 or MF_MIXED");
+  if (prm->rks != 1 && prm->rks != 4) return set_err(MF_ERR_INVALID, "rks must be 1 or 4");
+  if (prm->inner_iterations < 0) return set_err(MF_ERR_INVALID, "inner_iterations < 0");
+  if (prm->shapes < 1) return set_err(MF_ERR_INVALID, "shapes must be >= 1");
+  if (prm->format == 2 && (!geo->nx || !geo->ny))
+    return set_err(MF_ERR_INVALID, "format 2 needs file normals");
+  if (prm->block_size != 0 && prm->block_size != BLOCK)
+    return set_err(MF_ERR_INVALID, "block_size must be 0 or %d", BLOCK);
+  c->p = *prm;
+  c->n = n;
+  c->W = W;
+  c->ssz = prm->precision == MF_FP64 ? 8 : 4;
+  c->power0 = (prm->power == 0.0);
+
+  // ---- validate the cloud (point_preprocessor.F90:357-385 reads the same fields)
+  const int32_t* conn = geo->conn;
+  auto nbr = [&](int i, int k) { return conn[(size_t)k * n + i] - 1; };  // 0-based
+  for (int i = 0; i < n; ++i) {
+    const int f = geo->flag_1[i];
+    if (f < 0 || f > 2) return set_err(MF_ERR_INVALID, "point %d: flag_1 %d not in {0,1,2}", i + 1, f);
+    if (geo->flag_2[i] > prm->shapes || geo->flag_2[i] < 0)
+      return set_err(MF_ERR_INVALID, "point %d: flag_2 %d outside 0..shapes (shapes value wrong)", i + 1, geo->flag_2[i]);
+    if (geo->flag_2[i] > 0 && f != 0)
+      return set_err(MF_ERR_INVALID, "point %d: flag_2 > 0 on a non-wall point", i + 1);
+    if (geo->nbhs[i] < 0 || geo->nbhs[i] > W)
+      return set_err(MF_ERR_INVALID, "point %d: nbhs %d outside 0..%d", i + 1, geo->nbhs[i], W);
+    for (int k = 0; k < geo->nbhs[i]; ++k) {
+      const int j = nbr(i, k);
+      if (j < 0 || j >= n) return set_err(MF_ERR_INVALID, "point %d: conn(%d) = %d out of range", i + 1, k + 1, j + 1);
+    }
+    if (f != 1) {
+      const int l = geo->left[i] - 1, r = geo->right[i] - 1;
+      if (l < 0 || l >= n || r < 0 || r >= n)
+        return set_err(MF_ERR_INVALID, "boundary point %d: left/right out of range", i + 1);
+    }
+  }
+
+  // ---- normals: point_normals.F90:8-105
+  std::vector<double> nx(n, 0.0), ny(n, 0.0);
+  for (int m = 0; m < n; ++m) {
+    const int f = geo->flag_1[m];
+    if (f == 0 || f == 2) {
+      const int l = geo->left[m] - 1, r = geo->right[m] - 1;
+      const double lx = geo->x[l], ly = geo->y[l], mx = geo->x[m], my = geo->y[m];
+      const double rx = geo->x[r], ry = geo->y[r];
+      const double nx1 = my - ly, nx2 = ry - my, ny1 = mx - lx, ny2 = rx - mx;
+      double vx = 0.5 * (nx1 + nx2), vy = 0.5 * (ny1 + ny2);
+      const double det = std::sqrt(vx * vx + vy * vy);
+      nx[m] = -vx / det;
+      ny[m] = vy / det;
+    } else if (prm->format == 2) {
+      nx[m] = geo->nx[m];
+      ny[m] = geo->ny[m];
+    } else if (prm->interior_points_normal_flag == 0) {
+      nx[m] = 0.0;
+      ny[m] = 1.0;
+    } else {
+      nx[m] = 1.0;
+      ny[m] = 0.0;
+    }
+  }
+  c->nx_h = nx;
+  c->ny_h = ny;
+
+  // ---- renumbering: class-major (wall, outer, interior), Hilbert order within a class
+  std::vector<uint64_t> key(n, 0);
+  if (prm->reorder) {
+    double x0 = geo->x[0], x1 = geo->x[0], y0 = geo->y[0], y1 = geo->y[0];
+    for (int i = 0; i < n; ++i) {
+      x0 = std::min(x0, geo->x[i]); x1 = std::max(x1, geo->x[i]);
+      y0 = std::min(y0, geo->y[i]); y1 = std::max(y1, geo->y[i]);
+    }
+    const double span = std::max(x1 - x0, y1 - y0) * (1.0 + 1e-9) + 1e-300;
+    const uint32_t order = 21;
+    const double scale = (double)((1u << order) - 1) / span;
+    for (int i = 0; i < n; ++i)
+      key[i] = hilbert_d(order, (uint64_t)((geo->x[i] - x0) * scale), (uint64_t)((geo->y[i] - y0) * scale));
+  }
+  auto cls = [&](int i) { return geo->flag_1[i] == 0 ? 0 : (geo->flag_1[i] == 2 ? 1 : 2); };
+  c->perm.resize(n);
+  std::iota(c->perm.begin(), c->perm.end(), 0);
+  std::stable_sort(c->perm.begin(), c->perm.end(), [&](int a, int b) {
+    const int ca = cls(a), cb = cls(b);
+    if (ca != cb) return ca < cb;
+    return key[a] < key[b];
+  });
+  c->iperm.resize(n);
+  for (int j = 0; j < n; ++j) c->iperm[c->perm[j]] = j;
+  for (int i = 0; i < n; ++i) {
+    if (geo->flag_1[i] == 0) ++c->nw;
+    else if (geo->flag_1[i] == 2) ++c->no;
+    else ++c->ni;
+  }
+
+  // ---- point-fixed geometry, accumulated in the reference's order (original points)
+  std::vector<D2> xy(n), nrm(n);
+  std::vector<int32_t> nbh_d(n);
+  std::vector<uint32_t> conn_d((size_t)W * n, 0u);
+  std::vector<V4d> lsq(n), c1(n), sxyv(n), obd(n);
+  std::vector<double> eps_d(n);
+  for (int j = 0; j < n; ++j) {
+    const int i = c->perm[j];
+    xy[j] = D2{geo->x[i], geo->y[i]};
+    nrm[j] = D2{nx[i], ny[i]};
+    nbh_d[j] = geo->nbhs[i];
+    c->total_nbhs += geo->nbhs[i];
+    // limiters.F90:88-89, (VL_CONST*min_dist)**3 as (e*e)*e
+    double e = prm->vl_const * geo->min_dist[i];
+    eps_d[j] = e * e * e;
+    const double xi = geo->x[i], yi = geo->y[i];
+    const double tx = ny[i], ty = -nx[i];
+    // q_variables.F90:84-96 full-stencil sums
+    double sxx = 0, syy = 0, sxy = 0;
+    // wall/outer/interior_fluxes: per half-stencil sums in the rotated frame
+    double hxx[4] = {0, 0, 0, 0}, hyy[4] = {0, 0, 0, 0}, hxy[4] = {0, 0, 0, 0};
+    for (int k = 0; k < geo->nbhs[i]; ++k) {
+      const int o = nbr(i, k);
+      const double delx = geo->x[o] - xi, dely = geo->y[o] - yi;
+      const double dist = std::sqrt(delx * delx + dely * dely);
+      const double w = std::pow(dist, prm->power);
+      sxx = sxx + delx * delx * w;
+      syy = syy + dely * dely * w;
+      sxy = sxy + delx * dely * w;
+      const double dels = delx * tx + dely * ty;
+      const double deln = delx * nx[i] + dely * ny[i];
+      const unsigned m = split_mask(geo->flag_1[i], dels, deln);
+      conn_d[(size_t)k * n + j] = ((uint32_t)c->iperm[o] << 4) | m;
+      const double distr = std::sqrt(dels * dels + deln * deln);
+      const double wr = std::pow(distr, prm->power);
+      const double dsw = dels * wr, dnw = deln * wr;
+      for (int d = 0; d < 4; ++d) {
+        if (!(m & (1u << d))) continue;
+        hxx[d] = hxx[d] + dels * dsw;
+        hyy[d] = hyy[d] + deln * dnw;
+        hxy[d] = hxy[d] + dels * dnw;
+      }
+    }
+    const double det = sxx * syy - sxy * sxy;
+    lsq[j] = V4d{sxx, syy, sxy, 1.0 / det};
+    double cc[4], ob[4];
+    for (int d = 0; d < 4; ++d) {
+      const double dt = hxx[d] * hyy[d] - hxy[d] * hxy[d];
+      ob[d] = 1.0 / dt;
+      cc[d] = (d < 2) ? hyy[d] : hxx[d];
+    }
+    c1[j] = V4d{cc[0], cc[1], cc[2], cc[3]};
+    sxyv[j] = V4d{hxy[0], hxy[1], hxy[2], hxy[3]};
+    obd[j] = V4d{ob[0], ob[1], ob[2], ob[3]};
+  }
+
+  // ---- forces: shape points in original order (compute_force_coeffs.F90:40-78)
+  std::vector<int32_t> sh_m, sh_id;
+  std::vector<double> sgeo;
+  for (int m = 0; m < n; ++m) {
+    if (geo->flag_2[m] <= 0) continue;
+    const int l = geo->left[m] - 1, r = geo->right[m] - 1;
+    const double lx = geo->x[l], ly = geo->y[l], mx = geo->x[m], my = geo->y[m];
+    const double rx = geo->x[r], ry = geo->y[r];
+    double ds1 = (mx - lx) * (mx - lx) + (my - ly) * (my - ly);
+    ds1 = std::sqrt(ds1);
+    double ds2 = (rx - mx) * (rx - mx) + (ry - my) * (ry - my);
+    ds2 = std::sqrt(ds2);
+    const double ds = 0.5 * (ds1 + ds2);
+    sh_m.push_back(c->iperm[m]);
+    sh_id.push_back(geo->flag_2[m] - 1);
+    sgeo.insert(sgeo.end(), {nx[m], ny[m], ds, mx - 0.25, my});
+  }
+  c->nshape = (int)sh_m.size();
+
+  // ---- parameters: parameter.F90:16-34, compute_force_coeffs.F90:34
+  c->theta = prm->aoa * kPi / 180.0;
+  c->fs.u1_inf = prm->mach * std::cos(c->theta);
+  c->fs.u2_inf = prm->mach * std::sin(c->theta);
+  c->cp_scale = 0.5 * 1.0 * prm->mach * prm->mach;
+
+  // ---- device
+  HIPCHK(hipGetDevice(&c->device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int rc;
+  D2 *d_xy, *d_nrm;
+  int32_t* d_nbhs;
+  uint32_t* d_conn;
+  V4d *d_lsq, *d_c1, *d_sxy, *d_obd;
+  if ((rc = upload(c, &d_xy, xy)) || (rc = upload(c, &d_nrm, nrm)) ||
+      (rc = upload(c, &d_nbhs, nbh_d)) || (rc = upload(c, &d_conn, conn_d)) ||
+      (rc = upload(c, &d_lsq, lsq)) || (rc = upload(c, &d_c1, c1)) ||
+      (rc = upload(c, &d_sxy, sxyv)) || (rc = upload(c, &d_obd, obd)))
+    return rc;
+  c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no};
+  if (c->ssz == 8) {
+    double* e;
+    if ((rc = upload(c, &e, eps_d))) return rc;
+    c->eps = e;
+  } else {
+    std::vector<float> ef(eps_d.begin(), eps_d.end());
+    float* e;
+    if ((rc = upload(c, &e, ef))) return rc;
+    c->eps = e;
+  }
+  const size_t v4b = 4 * c->ssz;
+  for (int b = 0; b < 2; ++b) {
+    char *pp, *qq, *dd;
+    if ((rc = dev_alloc(c, &pp, (size_t)n * v4b)) || (rc = dev_alloc(c, &qq, (size_t)n * v4b)) ||
+        (rc = dev_alloc(c, &dd, (size_t)2 * n * v4b)))
+      return rc;
+    c->prim[b] = pp;
+    c->q[b] = qq;
+    c->dq[b] = dd;
+  }
+  char *qmb, *pob;
+  if ((rc = dev_alloc(c, &qmb, (size_t)2 * n * v4b)) || (rc = dev_alloc(c, &pob, (size_t)n * v4b))) return rc;
+  c->qm = qmb;
+  c->prim_old = pob;
+  c->npartial = (n + BLOCK - 1) / BLOCK;
+  if ((rc = dev_alloc(c, &c->flux_res, n)) || (rc = dev_alloc(c, &c->delta, n)) ||
+      (rc = dev_alloc(c, &c->cspeed, n)) || (rc = dev_alloc(c, &c->partial, c->npartial)) ||
+      (rc = dev_alloc(c, &c->scalar, 4)) || (rc = dev_alloc(c, &c->ctl, 1)))
+    return rc;
+  HIPCHK(hipMemset(c->flux_res, 0, (size_t)n * sizeof(V4d)));
+  HIPCHK(hipMemset(c->delta, 0, (size_t)n * sizeof(double)));
+  HIPCHK(hipMemset(c->prim_old, 0, (size_t)n * v4b));
+  HIPCHK(hipMemset(c->dq[0], 0, (size_t)2 * n * v4b));
+  HIPCHK(hipMemset(c->qm, 0, (size_t)2 * n * v4b));
+  Ctl ctl0{0, 0, prm->restart ? prm->res_old : 0.0};
+  HIPCHK(hipMemcpy(c->ctl, &ctl0, sizeof(Ctl), hipMemcpyHostToDevice));
+  if (c->nshape > 0) {
+    if ((rc = upload(c, &c->sh_m, sh_m)) || (rc = upload(c, &c->sh_id, sh_id)) ||
+        (rc = upload(c, &c->sgeo, sgeo)) || (rc = dev_alloc(c, &c->cp, c->nshape)))
+      return rc;
+  }
+  if ((rc = dev_alloc(c, &c->clcdcm, 3 * prm->shapes))) return rc;
+
+  // ---- initial_conditions.F90:9-24: freestream
+  std::vector<double> prim0((size_t)4 * n);
+  for (int i = 0; i < n; ++i) {
+    prim0[4 * i] = 1.0;
+    prim0[4 * i + 1] = c->fs.u1_inf;
+    prim0[4 * i + 2] = c->fs.u2_inf;
+    prim0[4 * i + 3] = 1.0 / 1.4;
+  }
+  if ((rc = upload_prim_original(c, prim0.data()))) return rc;
+  if ((rc = q_variables(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MF_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// kernel launchers
+// ---------------------------------------------------------------------------------
+
+static inline int nblocks(const mf_ctx* c) { return (c->n + BLOCK - 1) / BLOCK; }
+
+template <class S>
+static State<S> state_of(mf_ctx* c, int qbuf, int dqbuf) {
+  return State<S>{(const V4<S>*)c->q[qbuf], (const V4<S>*)c->dq[dqbuf], (const V4<S>*)c->qm,
+                  (const S*)c->eps};
+}
+
+template <class S>
+static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl) {
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  auto q = (const V4<S>*)c->q[qbuf];
+  auto dq = (V4<S>*)c->dq[0];
+  auto qm = (V4<S>*)c->qm;
+  const double cfl = c->p.cfl, pw = c->p.power;
+  if (c->power0) {
+    if (with_delta)
+      hipLaunchKernelGGL((k_lsq<S, true, true>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
+    else
+      hipLaunchKernelGGL((k_lsq<S, true, false>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
+  } else {
+    if (with_delta)
+      hipLaunchKernelGGL((k_lsq<S, false, true>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
+    else
+      hipLaunchKernelGGL((k_lsq<S, false, false>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
+  }
+}
+
+template <class S>
+static void launch_inner(mf_ctx* c, int qbuf, int din, const Ctl* ctl) {
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  auto q = (const V4<S>*)c->q[qbuf];
+  auto a = (const V4<S>*)c->dq[din];
+  auto b = (V4<S>*)c->dq[din ^ 1];
+  if (c->power0)
+    hipLaunchKernelGGL((k_inner<S, true>), gr, bl, 0, c->stream, c->g, q, a, b, c->p.power, ctl);
+  else
+    hipLaunchKernelGGL((k_inner<S, false>), gr, bl, 0, c->stream, c->g, q, a, b, c->p.power, ctl);
+}
+
+template <class S>
+static void launch_flux_update(mf_ctx* c, int buf, int dqbuf, int rk, bool last) {
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  State<S> st = state_of<S>(c, buf, dqbuf);
+  auto prim = (const V4<S>*)c->prim[buf];
+  auto pold = (V4<S>*)c->prim_old;
+  auto pout = (V4<S>*)c->prim[buf ^ 1];
+  auto qout = (V4<S>*)c->q[buf ^ 1];
+  if (c->power0)
+    hipLaunchKernelGGL((k_flux_update<S, true>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
+                       c->cspeed, c->delta, c->partial, c->ctl, rk, last ? 1 : 0, c->p.euler, c->p.power, c->fs);
+  else
+    hipLaunchKernelGGL((k_flux_update<S, false>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
+                       c->cspeed, c->delta, c->partial, c->ctl, rk, last ? 1 : 0, c->p.euler, c->p.power, c->fs);
+}
+
+template <class S>
+static void launch_forces(mf_ctx* c, int buf, const Ctl* ctl, double* cd_hist) {
+  hipLaunchKernelGGL((k_forces<S>), dim3(1), dim3(BLOCK), 0, c->stream, (const V4<S>*)c->prim[buf],
+                     c->sh_m, c->sh_id, c->sgeo, c->nshape, c->p.shapes, c->cp_scale,
+                     std::cos(c->theta), std::sin(c->theta), c->cp, c->clcdcm, cd_hist, ctl);
+}
+
+static void launch_finalize(mf_ctx* c) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, c->npartial,
+                     c->n, c->p.restart, c->graph_first_it, c->ctl, c->res_hist, c->resnew_hist);
+}
+
+// Kernel-kind ids used by the profiler
+enum { KK_LSQ = 0, KK_INNER = 1, KK_FLUX = 2, KK_FIN = 3, KK_FORCES = 4 };
+
+struct Timer {
+  hipStream_t s;
+  std::vector<hipEvent_t>* ev;  // pairs
+  std::vector<int>* kind;
+  void begin(int) {
+    if (!ev) return;
+    hipEvent_t a;
+    (void)hipEventCreate(&a);
+    (void)hipEventRecord(a, s);
+    ev->push_back(a);
+  }
+  void end(int k) {
+    if (!ev) return;
+    hipEvent_t b;
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(b, s);
+    ev->push_back(b);
+    kind->push_back(k);
+  }
+};
+
+// One fpi_solver iteration (fpi_solver.F90:20-70) starting from buffer `phase`;
+// returns the phase after it.  Used for graph capture and for profiled eager runs.
+template <class S>
+static int record_iteration(mf_ctx* c, int phase, Timer* tm) {
+  int buf = phase;
+  const int I = c->p.inner_iterations;
+  for (int rk = 1; rk <= c->p.rks; ++rk) {
+    if (tm) tm->begin(KK_LSQ);
+    launch_lsq<S>(c, buf, rk == 1, c->ctl);
+    if (tm) tm->end(KK_LSQ);
+    int d = 0;
+    for (int it = 0; it < I; ++it) {
+      if (tm) tm->begin(KK_INNER);
+      launch_inner<S>(c, buf, d, c->ctl);
+      if (tm) tm->end(KK_INNER);
+      d ^= 1;
+    }
+    if (tm) tm->begin(KK_FLUX);
+    launch_flux_update<S>(c, buf, d, rk, rk == c->p.rks);
+    if (tm) tm->end(KK_FLUX);
+    buf ^= 1;
+  }
+  if (c->nshape > 0) {
+    if (tm) tm->begin(KK_FORCES);
+    launch_forces<S>(c, buf, c->ctl, c->cd_hist);
+    if (tm) tm->end(KK_FORCES);
+  }
+  if (tm) tm->begin(KK_FIN);
+  launch_finalize(c);
+  if (tm) tm->end(KK_FIN);
+  return buf;
+}
+
+static int record(mf_ctx* c, int phase, Timer* tm) {
+  return c->ssz == 8 ? record_iteration<double>(c, phase, tm) : record_iteration<float>(c, phase, tm);
+}
+
+static int ensure_history(mf_ctx* c, int iters) {
+  if (iters <= c->hist_cap && c->res_hist) return MF_OK;
+  // histories are captured into the graphs: reallocate and drop the graphs
+  for (auto& gexec : c->graph)
+    if (gexec) {
+      HIPCHK(hipGraphExecDestroy(gexec));
+      gexec = nullptr;
+    }
+  int cap = std::max(iters, 1024);
+  int rc;
+  if ((rc = dev_alloc(c, &c->res_hist, cap)) || (rc = dev_alloc(c, &c->resnew_hist, cap)) ||
+      (rc = dev_alloc(c, &c->cd_hist, cap)))
+    return rc;
+  c->hist_cap = cap;
+  return MF_OK;
+}
+
+static int build_graph(mf_ctx* c, int phase) {
+  if (c->graph[phase]) return MF_OK;
+  hipGraph_t gr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  record(c, phase, nullptr);
+  HIPCHK(hipStreamEndCapture(c->stream, &gr));
+  hipError_t e = hipGraphInstantiate(&c->graph[phase], gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) return set_err(MF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  return MF_OK;
+}
+
+static int run_queue(mf_ctx* c, int32_t first_it, int32_t iters) {
+  int rc;
+  if ((rc = ensure_history(c, iters))) return rc;
+  // refresh q and wave speeds from prim: the reference routines may have changed prim
+  if ((rc = q_variables(c))) return rc;
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old);
+  HIPCHK(hipGetLastError());
+  // first_it is a kernel argument of k_finalize: a new value needs new graphs
+  if (c->graph_first_it != first_it) {
+    for (auto& gexec : c->graph)
+      if (gexec) {
+        HIPCHK(hipGraphExecDestroy(gexec));
+        gexec = nullptr;
+      }
+    c->graph_first_it = first_it;
+  }
+  int phase = c->cur;
+  for (int it = 0; it < iters; ++it) {
+    if ((rc = build_graph(c, phase))) return rc;
+    HIPCHK(hipGraphLaunch(c->graph[phase], c->stream));
+    phase ^= (c->p.rks & 1);
+  }
+  c->dq_cur = c->p.inner_iterations & 1;
+  return MF_OK;
+}
+
+extern "C" {
+
+int mf_run_async(mf_ctx* c, int32_t first_it, int32_t iters) {
+  if (!c || iters < 0) return set_err(MF_ERR_INVALID, "mf_run_async: bad argument");
+  const int start = c->cur;
+  int rc = run_queue(c, first_it, iters);
+  if (rc) return rc;
+  c->cur = start ^ ((iters * c->p.rks) & 1);
+  return MF_OK;
+}
+
+int mf_sync(mf_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MF_OK;
+}
+
+int mf_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue_hist,
+           double* res_new_hist, double* cd_hist, int32_t* iters_done) {
+  if (!c || iters < 0) return set_err(MF_ERR_INVALID, "mf_run: bad argument");
+  const int start = c->cur;
+  int rc = run_queue(c, first_it, iters);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  Ctl ctl;
+  HIPCHK(hipMemcpy(&ctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  const int done = ctl.it;
+  c->cur = start ^ ((done * c->p.rks) & 1);
+  if (residue_hist && done) HIPCHK(hipMemcpy(residue_hist, c->res_hist, done * sizeof(double), hipMemcpyDeviceToHost));
+  if (res_new_hist && done) HIPCHK(hipMemcpy(res_new_hist, c->resnew_hist, done * sizeof(double), hipMemcpyDeviceToHost));
+  if (cd_hist && done && c->nshape > 0) HIPCHK(hipMemcpy(cd_hist, c->cd_hist, done * sizeof(double), hipMemcpyDeviceToHost));
+  if (iters_done) *iters_done = done;
+  return MF_OK;
+}
+
+int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launches) {
+  if (!c || iters <= 0 || !mean_ms) return set_err(MF_ERR_INVALID, "mf_profile_kernels: bad argument");
+  int rc;
+  if ((rc = ensure_history(c, iters))) return rc;
+  if ((rc = q_variables(c))) return rc;
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old);
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;
+  Timer tm{c->stream, &ev, &kind};
+  int phase = c->cur;
+  for (int it = 0; it < iters; ++it) phase = record(c, phase, &tm);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double sum[MF_NUM_KERNEL_KINDS] = {0};
+  int cnt[MF_NUM_KERNEL_KINDS] = {0};
+  for (size_t j = 0; j < kind.size(); ++j) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev[2 * j], ev[2 * j + 1]));
+    sum[kind[j]] += ms;
+    cnt[kind[j]] += 1;
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < MF_NUM_KERNEL_KINDS; ++k) {
+    mean_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    if (launches) launches[k] = cnt[k] / iters;
+  }
+  Ctl ctl;
+  HIPCHK(hipMemcpy(&ctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  c->cur = c->cur ^ ((ctl.it * c->p.rks) & 1);
+  c->dq_cur = c->p.inner_iterations & 1;
+  return MF_OK;
+}
+
+int mf_set_prim(mf_ctx* c, const double* prim) {
+  if (!c || !prim) return set_err(MF_ERR_INVALID, "mf_set_prim: null argument");
+  int rc = upload_prim_original(c, prim);
+  if (rc) return rc;
+  return q_variables(c);
+}
+
+int mf_get_prim(mf_ctx* c, double* prim) {
+  if (!c || !prim) return set_err(MF_ERR_INVALID, "mf_get_prim: null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> out;
+  int rc = download_v4(c, c->prim[c->cur], 1, out);
+  if (rc) return rc;
+  std::memcpy(prim, out.data(), out.size() * sizeof(double));
+  return MF_OK;
+}
+
+int mf_get_field(mf_ctx* c, int32_t field, double* out) {
+  if (!c || !out) return set_err(MF_ERR_INVALID, "mf_get_field: null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int n = c->n;
+  std::vector<double> v;
+  int rc = MF_OK;
+  switch (field) {
+    case MF_FIELD_Q: rc = download_v4(c, c->q[c->cur], 1, v); break;
+    case MF_FIELD_PRIM_OLD: rc = download_v4(c, c->prim_old, 1, v); break;
+    case MF_FIELD_DQ:
+    case MF_FIELD_QM: {
+      std::vector<double> t;
+      rc = download_v4(c, field == MF_FIELD_DQ ? c->dq[c->dq_cur] : c->qm, 2, t);
+      if (rc) return rc;
+      // (point, a, r) -> reference (a fastest, then r): out[i*8 + r*2 + a]
+      v.assign((size_t)8 * n, 0.0);
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a)
+          for (int r = 0; r < 4; ++r) v[(size_t)8 * i + 2 * r + a] = t[(size_t)8 * i + 4 * a + r];
+      break;
+    }
+    case MF_FIELD_FLUX_RES: {
+      std::vector<V4d> h(n);
+      HIPCHK(hipMemcpy(h.data(), c->flux_res, n * sizeof(V4d), hipMemcpyDeviceToHost));
+      v.assign((size_t)4 * n, 0.0);
+      for (int j = 0; j < n; ++j)
+        for (int r = 0; r < 4; ++r) v[(size_t)4 * c->perm[j] + r] = h[j][r];
+      break;
+    }
+    case MF_FIELD_DELTA: {
+      std::vector<double> h(n);
+      HIPCHK(hipMemcpy(h.data(), c->delta, n * sizeof(double), hipMemcpyDeviceToHost));
+      v.assign(n, 0.0);
+      for (int j = 0; j < n; ++j) v[c->perm[j]] = h[j];
+      break;
+    }
+    case MF_FIELD_NX: v = c->nx_h; break;
+    case MF_FIELD_NY: v = c->ny_h; break;
+    default: return set_err(MF_ERR_INVALID, "mf_get_field: unknown field %d", field);
+  }
+  if (rc) return rc;
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return MF_OK;
+}
+
+int mf_set_field(mf_ctx* c, int32_t field, const double* in) {
+  if (!c || !in) return set_err(MF_ERR_INVALID, "mf_set_field: null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int n = c->n;
+  switch (field) {
+    case MF_FIELD_Q: return upload_v4(c, c->q[c->cur], 1, in);
+    case MF_FIELD_PRIM_OLD: return upload_v4(c, c->prim_old, 1, in);
+    case MF_FIELD_DQ:
+    case MF_FIELD_QM: {
+      std::vector<double> t((size_t)8 * n);
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a)
+          for (int r = 0; r < 4; ++r) t[(size_t)8 * i + 4 * a + r] = in[(size_t)8 * i + 2 * r + a];
+      return upload_v4(c, field == MF_FIELD_DQ ? c->dq[c->dq_cur] : c->qm, 2, t.data());
+    }
+    case MF_FIELD_DELTA: {
+      std::vector<double> h(n);
+      for (int j = 0; j < n; ++j) h[j] = in[c->perm[j]];
+      HIPCHK(hipMemcpy(c->delta, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      return MF_OK;
+    }
+    default: return set_err(MF_ERR_INVALID, "mf_set_field: field %d is not settable", field);
+  }
+}
+
+int mf_eval_q_variables(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  return q_variables(c);
+}
+
+int mf_func_delta(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  if (c->ssz == 8)
+    hipLaunchKernelGGL(k_func_delta<double>, gr, bl, 0, c->stream, c->g, (const V4d*)c->prim[c->cur], c->delta, c->p.cfl);
+  else
+    hipLaunchKernelGGL(k_func_delta<float>, gr, bl, 0, c->stream, c->g, (const V4<float>*)c->prim[c->cur], c->delta, c->p.cfl);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+int mf_eval_q_derivatives(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (c->ssz == 8) launch_lsq<double>(c, c->cur, false, nullptr);
+  else launch_lsq<float>(c, c->cur, false, nullptr);
+  HIPCHK(hipGetLastError());
+  c->dq_cur = 0;
+  return MF_OK;
+}
+
+int mf_eval_q_inner_loop(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (c->ssz == 8) launch_inner<double>(c, c->cur, c->dq_cur, nullptr);
+  else launch_inner<float>(c, c->cur, c->dq_cur, nullptr);
+  HIPCHK(hipGetLastError());
+  c->dq_cur ^= 1;
+  return MF_OK;
+}
+
+int mf_cal_flux_residual(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  if (c->ssz == 8) {
+    State<double> st = state_of<double>(c, c->cur, c->dq_cur);
+    if (c->power0) hipLaunchKernelGGL((k_flux<double, true>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
+    else hipLaunchKernelGGL((k_flux<double, false>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
+  } else {
+    State<float> st = state_of<float>(c, c->cur, c->dq_cur);
+    if (c->power0) hipLaunchKernelGGL((k_flux<float, true>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
+    else hipLaunchKernelGGL((k_flux<float, false>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
+  }
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+int mf_state_update(mf_ctx* c, int32_t rk, double* sum_res_sqr) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (rk < 1 || rk > 4) return set_err(MF_ERR_INVALID, "rk must be 1..4");
+  const dim3 gr(nblocks(c)), bl(BLOCK);
+  if (c->ssz == 8)
+    hipLaunchKernelGGL(k_update<double>, gr, bl, 0, c->stream, c->g, (V4d*)c->prim[c->cur],
+                       (const V4d*)c->prim_old, c->flux_res, c->partial, rk, c->p.euler, c->fs);
+  else
+    hipLaunchKernelGGL(k_update<float>, gr, bl, 0, c->stream, c->g, (V4<float>*)c->prim[c->cur],
+                       (const V4<float>*)c->prim_old, c->flux_res, c->partial, rk, c->p.euler, c->fs);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, c->npartial, c->scalar);
+  HIPCHK(hipGetLastError());
+  if (sum_res_sqr) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(sum_res_sqr, c->scalar, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return MF_OK;
+}
+
+int32_t mf_shape_points(mf_ctx* c) { return c ? c->nshape : 0; }
+
+int mf_compute_cl_cd_cm(mf_ctx* c, double* cl, double* cd, double* cm, double* cp) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (c->nshape == 0) return set_err(MF_ERR_STATE, "cloud has no shape points");
+  if (c->ssz == 8) launch_forces<double>(c, c->cur, nullptr, nullptr);
+  else launch_forces<float>(c, c->cur, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> h(3 * c->p.shapes);
+  HIPCHK(hipMemcpy(h.data(), c->clcdcm, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->p.shapes; ++s) {
+    if (cl) cl[s] = h[3 * s];
+    if (cd) cd[s] = h[3 * s + 1];
+    if (cm) cm[s] = h[3 * s + 2];
+  }
+  if (cp) HIPCHK(hipMemcpy(cp, c->cp, c->nshape * sizeof(double), hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+int mf_layout(mf_ctx* c, mf_layout_info* info) {
+  if (!c || !info) return set_err(MF_ERR_INVALID, "null argument");
+  info->n_points = c->n;
+  info->n_wall = c->nw;
+  info->n_outer = c->no;
+  info->n_interior = c->ni;
+  info->n_shape = c->nshape;
+  info->max_nbhs = c->W;
+  info->total_nbhs = c->total_nbhs;
+  info->state_bytes = (int64_t)c->ssz;
+  return MF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+"""Host-side mirror of the reference's hot-path interface, over the C ABI.
+
+The reference drives its iteration through argument-less module procedures
+(src_mpi_primal: ``fpi_solver``, ``func_delta``, ``eval_q_variables``,
+``eval_q_derivatives``, ``eval_q_inner_loop``, ``cal_flux_residual``,
+``state_update``, ``compute_cl_cd_cm``) over module state.  :class:`LskumSolver`
+exposes the same procedure names on an MI355X context that owns that state; arrays
+in and out use the reference's layouts and original point order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .cloud import PointCloud
+
+_FIELDS = {"q": (L.MF_FIELD_Q, (4,)), "dq": (L.MF_FIELD_DQ, (4, 2)), "qm": (L.MF_FIELD_QM, (4, 2)),
+           "flux_res": (L.MF_FIELD_FLUX_RES, (4,)), "delta": (L.MF_FIELD_DELTA, ()),
+           "prim_old": (L.MF_FIELD_PRIM_OLD, (4,)), "nx": (L.MF_FIELD_NX, ()),
+           "ny": (L.MF_FIELD_NY, ())}
+
+# readcase.F90:10-125 option names -> mf_params fields
+_TSCHEME = {"ssprk43": (4, 1.0), "first": (1, 2.0)}
+
+
+def make_params(**kw) -> L.MfParams:
+    """mf_params from reference option names (cfl, mach, aoa, vl_const, power,
+    inner_iterations, tscheme, shapes, interior_points_normal_flag, format, restart,
+    res_old) plus precision ('fp64'|'mixed') and reorder."""
+    p = L.MfParams()
+    L.lib().mf_default_params(C.byref(p))
+    kw = dict(kw)
+    if "tscheme" in kw:
+        p.rks, p.euler = _TSCHEME[kw.pop("tscheme")]
+    if "aoa_deg" in kw:
+        kw["aoa"] = kw.pop("aoa_deg")
+    prec = kw.pop("precision", "fp64")
+    p.precision = {"fp64": L.MF_FP64, "mixed": L.MF_MIXED}[prec] if isinstance(prec, str) else int(prec)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown option {k}")
+        setattr(p, k, type(getattr(p, k))(v))
+    return p
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class LskumSolver:
+    """One point cloud on the GPU, with the reference's hot-path procedures."""
+
+    def __init__(self, cloud: PointCloud, **params):
+        self.cloud = cloud
+        self.params = make_params(**params)
+        n, w = cloud.n, cloud.max_nbhs
+        # reference conventions: 1-based indices, conn(N, W) column-major
+        conn_f = np.where(cloud.conn >= 0, cloud.conn + 1, 0).astype(np.int32)
+        self._keep = dict(
+            x=np.ascontiguousarray(cloud.x, dtype=np.float64),
+            y=np.ascontiguousarray(cloud.y, dtype=np.float64),
+            nx=np.ascontiguousarray(cloud.nx, dtype=np.float64),
+            ny=np.ascontiguousarray(cloud.ny, dtype=np.float64),
+            min_dist=np.ascontiguousarray(cloud.min_dist, dtype=np.float64),
+            left=(cloud.left + 1).astype(np.int32), right=(cloud.right + 1).astype(np.int32),
+            flag_1=cloud.flag_1.astype(np.int32), flag_2=cloud.flag_2.astype(np.int32),
+            nbhs=cloud.nbhs.astype(np.int32), conn=np.asfortranarray(conn_f).ravel(order="F"))
+        g = L.MfGeometry(n_points=n, max_nbhs=w,
+                         **{k: v.ctypes.data for k, v in self._keep.items()})
+        h = C.c_void_p()
+        L.check(L.lib().mf_create(C.byref(g), C.byref(self.params), C.byref(h)), "mf_create")
+        self.h = h
+        self.n = n
+        self.shapes = self.params.shapes
+        self.n_shape = L.lib().mf_shape_points(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().mf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state
+    @property
+    def prim(self) -> np.ndarray:
+        out = np.empty((self.n, 4))
+        L.check(L.lib().mf_get_prim(self.h, _dp(out)), "mf_get_prim")
+        return out
+
+    @prim.setter
+    def prim(self, value: np.ndarray) -> None:
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(self.n, 4)
+        L.check(L.lib().mf_set_prim(self.h, _dp(v)), "mf_set_prim")
+
+    def field(self, name: str) -> np.ndarray:
+        fid, shp = _FIELDS[name]
+        out = np.empty((self.n,) + shp)
+        L.check(L.lib().mf_get_field(self.h, fid, _dp(out)), f"mf_get_field({name})")
+        return out
+
+    def set_field(self, name: str, value: np.ndarray) -> None:
+        fid, shp = _FIELDS[name]
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape((self.n,) + shp)
+        L.check(L.lib().mf_set_field(self.h, fid, _dp(v)), f"mf_set_field({name})")
+
+    # ---- reference procedures
+    def func_delta(self):
+        L.check(L.lib().mf_func_delta(self.h), "mf_func_delta")
+
+    def eval_q_variables(self):
+        L.check(L.lib().mf_eval_q_variables(self.h), "mf_eval_q_variables")
+
+    def eval_q_derivatives(self):
+        L.check(L.lib().mf_eval_q_derivatives(self.h), "mf_eval_q_derivatives")
+
+    def eval_q_inner_loop(self):
+        """eval_q_inner_loop followed by eval_update_innerloop."""
+        L.check(L.lib().mf_eval_q_inner_loop(self.h), "mf_eval_q_inner_loop")
+
+    def cal_flux_residual(self):
+        L.check(L.lib().mf_cal_flux_residual(self.h), "mf_cal_flux_residual")
+
+    def state_update(self, rk: int) -> float:
+        s = C.c_double()
+        L.check(L.lib().mf_state_update(self.h, int(rk), C.byref(s)), "mf_state_update")
+        return s.value
+
+    def compute_cl_cd_cm(self):
+        """(Cl, Cd, Cm) per shape and cp per shape point (compute_force_coeffs.F90)."""
+        cl, cd, cm = (np.empty(self.shapes) for _ in range(3))
+        cp = np.empty(max(self.n_shape, 1))
+        L.check(L.lib().mf_compute_cl_cd_cm(self.h, _dp(cl), _dp(cd), _dp(cm), _dp(cp)),
+                "mf_compute_cl_cd_cm")
+        return cl, cd, cm, cp[: self.n_shape]
+
+    def q_lskum(self, max_iters: int, first_it: int = 1):
+        """q_lskum.F90:59-73: returns (residue, res_new, cd) per iteration run."""
+        res, rn, cd = (np.empty(max(max_iters, 1)) for _ in range(3))
+        done = C.c_int32()
+        L.check(L.lib().mf_run(self.h, int(first_it), int(max_iters), _dp(res), _dp(rn), _dp(cd),
+                               C.byref(done)), "mf_run")
+        d = done.value
+        return res[:d], rn[:d], cd[:d]
+
+    def fpi_solver(self, t: int) -> float:
+        """One iteration (fpi_solver.F90:13-83); returns its residue."""
+        res, _, _ = self.q_lskum(1, first_it=t)
+        return float(res[0]) if len(res) else float("nan")
+
+    def run_async(self, iters: int, first_it: int = 1):
+        L.check(L.lib().mf_run_async(self.h, int(first_it), int(iters)), "mf_run_async")
+
+    def sync(self):
+        L.check(L.lib().mf_sync(self.h), "mf_sync")
+
+    def profile_kernels(self, iters: int) -> dict:
+        ms = np.zeros(L.MF_NUM_KERNEL_KINDS)
+        cnt = np.zeros(L.MF_NUM_KERNEL_KINDS, dtype=np.int32)
+        L.check(L.lib().mf_profile_kernels(self.h, int(iters), _dp(ms),
+                                           cnt.ctypes.data_as(C.POINTER(C.c_int32))),
+                "mf_profile_kernels")
+        return {k: (float(ms[j]), int(cnt[j])) for j, k in enumerate(L.KERNEL_KINDS)}
+
+    def layout(self) -> dict:
+        li = L.MfLayout()
+        L.check(L.lib().mf_layout(self.h, C.byref(li)), "mf_layout")
+        return {f: getattr(li, f) for f, _ in L.MfLayout._fields_}

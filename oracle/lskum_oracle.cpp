@@ -28,6 +28,10 @@
 
 #include "dual.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace orc {
 
 // parameter.F90:8-11
@@ -663,6 +667,15 @@ void* orc_create(int32_t n, int32_t w, const double* x, const double* y,
 }
 
 void orc_destroy(void* h) { delete static_cast<SolverD*>(h); }
+
+// OpenMP threads the point loops use
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 void orc_initial_conditions(void* h) { static_cast<SolverD*>(h)->initial_conditions(); }
 void orc_compute_normals(void* h) { static_cast<SolverD*>(h)->compute_normals(); }

@@ -71,6 +71,7 @@ def _load():
         lib.orc_scalar.restype = C.c_double
         lib.orc_flux.argtypes = [C.c_int32] + [C.c_double] * 6 + [dp]
         lib.orc_qtilde_to_primitive.argtypes = [dp, dp]
+        lib.orc_num_threads.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -178,6 +179,11 @@ class Oracle:
         return _load().orc_scalar(self.h, name.encode())
 
 
+def num_threads() -> int:
+    """OpenMP threads the oracle's point loops use."""
+    return int(_load().orc_num_threads())
+
+
 def flux(kind: int, nx, ny, u1, u2, rho, pr) -> np.ndarray:
     """One kinetic split flux (0 Gxp 1 Gxn 2 Gyp 3 Gyn 4 GxI 5 GxII 6 GxIII 7 GxIV)."""
     G = np.zeros(4)
@@ -211,6 +217,8 @@ class RefRoutines:
         lib.ref_set_split.argtypes = [C.c_int32] + [C.c_void_p] * 8
         lib.ref_field.argtypes = [C.c_int32, C.c_int32, C.c_void_p]
         lib.ref_qtilde_to_primitive.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ref_flux.argtypes = [C.c_int32] + [C.c_double] * 6 + [C.c_void_p]
+        lib.ref_venkat.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
         self.n, self.w = cloud.n, cloud.max_nbhs
         self._keep = [np.ascontiguousarray(a) for a in (
             cloud.x.astype(np.float64), cloud.y.astype(np.float64),
@@ -251,3 +259,15 @@ class RefRoutines:
         out = np.zeros(4)
         self.lib.ref_qtilde_to_primitive(qt.ctypes.data, out.ctypes.data)
         return out
+
+    def flux(self, kind: int, nx, ny, u1, u2, rho, pr) -> np.ndarray:
+        G = np.zeros(4)
+        self.lib.ref_flux(kind, nx, ny, u1, u2, rho, pr, G.ctypes.data)
+        return G
+
+    def venkat(self, k0: int, qt) -> np.ndarray:
+        """venkat_limiter for 0-based point k0 on the module's q, qm, min_dist."""
+        qt = np.ascontiguousarray(qt, dtype=np.float64)
+        phi = np.zeros(4)
+        self.lib.ref_venkat(k0 + 1, qt.ctypes.data, phi.ctypes.data)
+        return phi

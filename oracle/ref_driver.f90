@@ -17,6 +17,9 @@ module ref_driver
     use q_variables_mod
     use flux_residual_mod
     use point_normals_mod
+    use split_fluxes_mod
+    use quadrant_fluxes_mod
+    use limiters_mod
     implicit none
 
 contains
@@ -232,6 +235,40 @@ contains
         qq = qt
         call qtilde_to_primitive(qq, u1, u2, rho, pr)
         out(1) = rho; out(2) = u1; out(3) = u2; out(4) = pr
+    end subroutine
+
+    ! One kinetic split flux: kind 0 Gxp, 1 Gxn, 2 Gyp, 3 Gyn, 4 GxI, 5 GxII,
+    ! 6 GxIII, 7 GxIV (split_fluxes.F90, quadrant_fluxes.F90)
+    subroutine ref_flux(kind, cnx, cny, cu1, cu2, crho, cpr, g) bind(C, name="ref_flux")
+        integer(c_int), value :: kind
+        real(c_double), value :: cnx, cny, cu1, cu2, crho, cpr
+        real(c_double), intent(out) :: g(4)
+        real(8) :: gg(4), nx_, ny_, u1, u2, rho, pr
+        nx_ = cnx; ny_ = cny; u1 = cu1; u2 = cu2; rho = crho; pr = cpr
+        select case (kind)
+        case (0); call flux_Gxp(gg, nx_, ny_, u1, u2, rho, pr)
+        case (1); call flux_Gxn(gg, nx_, ny_, u1, u2, rho, pr)
+        case (2); call flux_Gyp(gg, nx_, ny_, u1, u2, rho, pr)
+        case (3); call flux_Gyn(gg, nx_, ny_, u1, u2, rho, pr)
+        case (4); call flux_quad_GxI(gg, nx_, ny_, u1, u2, rho, pr)
+        case (5); call flux_quad_GxII(gg, nx_, ny_, u1, u2, rho, pr)
+        case (6); call flux_quad_GxIII(gg, nx_, ny_, u1, u2, rho, pr)
+        case (7); call flux_quad_GxIV(gg, nx_, ny_, u1, u2, rho, pr)
+        end select
+        g = gg
+    end subroutine
+
+    ! venkat_limiter of point k (1-based) on the module's current q, qm, min_dist
+    subroutine ref_venkat(k, qt, phi) bind(C, name="ref_venkat")
+        integer(c_int), value :: k
+        real(c_double), intent(in) :: qt(4)
+        real(c_double), intent(out) :: phi(4)
+        real(8) :: qq(4), pp(4)
+        integer :: kk
+        qq = qt
+        kk = k
+        call venkat_limiter(qq, pp, kk)
+        phi = pp
     end subroutine
 
 end module ref_driver

@@ -1,0 +1,222 @@
+"""GPU parity: the MI355X path against the FP64 oracle on the same clouds and states.
+
+Every test goes through the C ABI (libmfcfd_hip.so).  The oracle (oracle/) is the
+checker: it restates src_mpi_primal in C++ and is itself pinned against the
+reference's own compiled routines and closed forms (tests/test_oracle.py).
+
+Tolerances (relative to the field's max magnitude):
+  * single routines in FP64: 1e-12 (q, dq, qm, delta), 1e-10 (flux_res: an LSQ
+    derivative of differences of nearly equal fluxes, where libm-vs-device
+    transcendentals at the ulp level are amplified by cancellation);
+  * FP64 trajectories: per-point prim within 1e-10 after the iterations run, residue
+    history within 1e-8 absolute (it is a log10), Cd within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = dict(cfl=0.2, mach=0.63, aoa=2.0, vl_const=5.0, inner_iterations=2)
+
+
+def _oracle(cloud, **kw):
+    from oracle.oracle import Oracle
+    p = dict(PARAMS)
+    p.update(kw)
+    p["aoa_deg"] = p.pop("aoa")
+    return Oracle(cloud, **p).setup()
+
+
+def _gpu(cloud, **kw):
+    from mfcfd_amd.solver import LskumSolver
+    p = dict(PARAMS)
+    p.update(kw)
+    return LskumSolver(cloud, **p)
+
+
+@pytest.fixture(scope="module")
+def developed(small_cloud):
+    """Oracle state after 40 FP64 iterations from freestream (limiter active)."""
+    o = _oracle(small_cloud)
+    print(f"[developed] oracle 40 iterations on {small_cloud.n} points", flush=True)
+    o.run(40)
+    return o
+
+
+def test_normals_match(small_cloud):
+    o = _oracle(small_cloud)
+    with _gpu(small_cloud) as g:
+        assert rel_err(g.field("nx"), o.field("nx")) == 0.0
+        assert rel_err(g.field("ny"), o.field("ny")) == 0.0
+
+
+def test_q_variables(small_cloud, developed):
+    o = developed
+    with _gpu(small_cloud) as g:
+        g.prim = o.field("prim")
+        g.eval_q_variables()
+        o.eval_q_variables()
+        assert rel_err(g.field("q"), o.field("q")) < 1e-14
+
+
+def test_q_derivatives_and_inner_loop(small_cloud, developed):
+    o = developed
+    with _gpu(small_cloud) as g:
+        g.prim = o.field("prim")
+        o.eval_q_variables()
+        g.eval_q_derivatives()
+        o.eval_q_derivatives()
+        assert rel_err(g.field("qm"), o.field("qm")) == 0.0
+        assert rel_err(g.field("dq"), o.field("dq")) < 1e-12
+        for _ in range(3):
+            g.eval_q_inner_loop()
+            o.eval_q_inner_loop()
+            o.eval_update_innerloop()
+            assert rel_err(g.field("dq"), o.field("dq")) < 1e-12
+
+
+def test_func_delta(small_cloud, developed):
+    o = developed
+    with _gpu(small_cloud) as g:
+        g.prim = o.field("prim")
+        g.func_delta()
+        o.func_delta()
+        assert rel_err(g.field("delta"), o.field("delta")) < 1e-14
+
+
+def test_flux_residual_same_inputs(small_cloud, developed):
+    """cal_flux_residual on identical q, dq, qm, delta (oracle's, copied in)."""
+    o = developed
+    o.eval_q_variables()
+    o.eval_q_derivatives()
+    o.eval_q_inner_loop()
+    o.eval_update_innerloop()
+    o.func_delta()
+    o.cal_flux_residual()
+    with _gpu(small_cloud) as g:
+        g.prim = o.field("prim")
+        g.set_field("q", o.field("q"))
+        g.set_field("dq", o.field("dq"))
+        g.set_field("qm", o.field("qm"))
+        g.set_field("delta", o.field("delta"))
+        g.cal_flux_residual()
+        e = rel_err(g.field("flux_res"), o.field("flux_res"))
+        print(f"flux_res rel err {e:.3e}")
+        assert e < 1e-10
+
+
+@pytest.mark.parametrize("rk", [1, 2, 3, 4])
+def test_state_update(small_cloud, developed, rk):
+    """One full stage through the reference routines on both sides."""
+    o = developed
+    prim0 = o.field("prim").copy()
+    with _gpu(small_cloud) as g:
+        g.prim = prim0
+        g.set_field("prim_old", prim0 * (1.0 + 1e-3))
+        o.field("prim_old")[:] = prim0 * (1.0 + 1e-3)
+        for s in (g, o):
+            s.func_delta()
+            s.eval_q_variables()
+            s.eval_q_derivatives()
+        g.cal_flux_residual()
+        o.cal_flux_residual()
+        sg = g.state_update(rk)
+        o.state_update(rk)
+        so = o.scalar("sum_res_sqr")
+        assert rel_err(g.prim, o.field("prim")) < 1e-12
+        assert abs(sg - so) <= 1e-10 * abs(so)
+        o.field("prim")[:] = prim0
+
+
+@pytest.mark.parametrize("inner,tscheme", [(0, "ssprk43"), (2, "ssprk43"), (1, "first")])
+def test_iterations_fp64(small_cloud, inner, tscheme):
+    """q_lskum loop: 30 fused, graph-replayed iterations from freestream."""
+    o = _oracle(small_cloud, inner_iterations=inner,
+                rks=4 if tscheme == "ssprk43" else 1, euler=1.0 if tscheme == "ssprk43" else 2.0)
+    res_o, rn_o = o.run(30)
+    o.compute_cl_cd_cm()
+    with _gpu(small_cloud, inner_iterations=inner, tscheme=tscheme) as g:
+        res_g, rn_g, cd_g = g.q_lskum(30)
+        assert len(res_g) == 30
+        ep = rel_err(g.prim, o.field("prim"))
+        print(f"prim rel err after 30 its: {ep:.3e}")
+        assert ep < 1e-10
+        assert np.abs(res_g - res_o).max() < 1e-8
+        assert rel_err(rn_g, rn_o) < 1e-10
+        cl, cd, cm, cp = g.compute_cl_cd_cm()
+        assert abs(cd[0] - o.field("Cd")[0]) <= 1e-10 * max(abs(o.field("Cd")[0]), 1e-12)
+        assert abs(cl[0] - o.field("Cl")[0]) <= 1e-10 * max(abs(o.field("Cl")[0]), 1e-12)
+        assert rel_err(cp, o.field("cp")) < 1e-10
+        assert abs(cd_g[-1] - cd[0]) <= 1e-14 * max(abs(cd[0]), 1e-300)
+
+
+def test_split_runs_continue(small_cloud):
+    """Two mf_run calls (10 + 10 iterations) equal one of 20, residue included."""
+    with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
+        r1, _, _ = a.q_lskum(10, first_it=1)
+        r2, _, _ = a.q_lskum(10, first_it=11)
+        rb, _, _ = b.q_lskum(20, first_it=1)
+        assert np.array_equal(np.concatenate([r1, r2]), rb)
+        assert np.array_equal(a.prim, b.prim)
+
+
+def test_reorder_invariance(small_cloud):
+    """Hilbert renumbering changes only the memory order, not the arithmetic."""
+    with _gpu(small_cloud, reorder=1) as a, _gpu(small_cloud, reorder=0) as b:
+        a.q_lskum(15)
+        b.q_lskum(15)
+        assert rel_err(a.prim, b.prim) < 1e-13
+
+
+def test_nan_residue_stops_loop(small_cloud):
+    """q_lskum.F90:71 exits after the first NaN residue; so does mf_run."""
+    o = _oracle(small_cloud)
+    prim = o.field("prim")
+    prim[100, 3] = -0.5  # negative pressure: log of a negative number
+    res_o, _ = o.run(10)
+    with _gpu(small_cloud) as g:
+        p = g.prim
+        p[100, 3] = -0.5
+        g.prim = p
+        res_g, _, _ = g.q_lskum(10)
+    assert len(res_g) == len(res_o)
+    assert np.isnan(res_g[-1]) and np.isnan(res_o[-1])
+
+
+def test_c1_first_order_fp64(c1_cloud):
+    """The 9,600-point first-order FP64 case (no inner iterations, M 0.63, AoA 2),
+    100 iterations: GPU and oracle trajectories agree."""
+    o = _oracle(c1_cloud, inner_iterations=0, vl_const=150.0)
+    res_o, _ = o.run(100)
+    o.compute_cl_cd_cm()
+    with _gpu(c1_cloud, inner_iterations=0, vl_const=150.0) as g:
+        res_g, _, cd_g = g.q_lskum(100)
+        ep = rel_err(g.prim, o.field("prim"))
+        print(f"C1 prim rel err after 100 its: {ep:.3e}; cd {cd_g[-1]:.12e} vs {o.field('Cd')[0]:.12e}")
+        assert ep < 1e-10
+        assert np.abs(res_g - res_o).max() < 1e-8
+        assert abs(cd_g[-1] - o.field("Cd")[0]) <= 1e-10 * abs(o.field("Cd")[0])
+
+
+def test_mixed_precision_close(small_cloud):
+    """Mixed path (FP32 state/flux/limiter/update, FP64 LSQ and sums) stays close to
+    FP64 after 60 iterations: |dCl|, |dCd| <= 1e-4 and prim within 1e-4."""
+    with _gpu(small_cloud) as a, _gpu(small_cloud, precision="mixed") as b:
+        a.q_lskum(60)
+        b.q_lskum(60)
+        cla, cda, _, _ = a.compute_cl_cd_cm()
+        clb, cdb, _, _ = b.compute_cl_cd_cm()
+        print(f"mixed: dCl {abs(cla[0]-clb[0]):.2e} dCd {abs(cda[0]-cdb[0]):.2e} "
+              f"prim {rel_err(b.prim, a.prim):.2e}")
+        assert abs(cla[0] - clb[0]) <= 1e-4
+        assert abs(cda[0] - cdb[0]) <= 1e-4
+        assert rel_err(b.prim, a.prim) < 1e-4
+
+
+def test_profile_kernels_runs(small_cloud):
+    with _gpu(small_cloud) as g:
+        t = g.profile_kernels(3)
+        assert t["flux_update"][0] > 0 and t["flux_update"][1] == 4
+        assert t["lsq"][1] == 4 and t["inner"][1] == 8
