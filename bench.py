@@ -94,18 +94,44 @@ def cpu_baseline(cloud, iters: int):
                        f"{dt:.1f} s")
 
 
+COPY_PROBE_BYTES = 256 << 20
+
+
+def probe():
+    import ctypes as C
+    from mfcfd_amd import _lib
+    from mfcfd_amd.cloud import naca_ogrid
+    from mfcfd_amd.solver import LskumSolver
+    ms = C.c_double()
+    _lib.check(_lib.lib().mf_copy_probe(COPY_PROBE_BYTES, 1, C.byref(ms)), "mf_copy_probe")
+    cloud = naca_ogrid(WORKLOAD["n_wall"], WORKLOAD["n_radial"], seed=WORKLOAD["seed"])
+    p = {k: WORKLOAD[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
+    for prec in ("fp64", "mixed"):
+        with LskumSolver(cloud, precision=prec, **p) as s:
+            s.q_lskum(2)
+    print(json.dumps({"probe": "ok", "copy_bytes": COPY_PROBE_BYTES, "copy_ms": ms.value,
+                      "points": cloud.n, "mean_nbhs": float(cloud.nbhs.mean())}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
+                         "2 mixed iterations of the workload (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the LSKUM path here is single-GPU "
               "(multi-GPU halo exchange is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.probe:
+        probe()
+        return
 
     from mfcfd_amd.cloud import naca_ogrid
     t0 = time.perf_counter()

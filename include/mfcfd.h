@@ -68,7 +68,11 @@ typedef struct mf_params {
   int32_t precision;       /* MF_FP64 or MF_MIXED                                    */
   int32_t reorder;         /* 1: Hilbert-curve point order on the device (default)   */
   int32_t block_size;      /* threads per block; 0 = built-in choice                 */
-  int32_t reserved;        /* must be 0                                              */
+  int32_t first_order;     /* 1: -solution_accuracy first: q-tilde = q at both ends of
+                              every edge (no reconstruction, no limiter); 0: second order
+                              (default).  readcase.F90:107-115 parses the option but the
+                              reference's primal kernels never read it; this is the
+                              first-order scheme that option names. */
 } mf_params;
 
 /* Fills `p` with readcase.F90 defaults (cfl 0, vl_const 150, power 0, ssprk43, ...). */
@@ -141,6 +145,11 @@ typedef struct mf_layout_info {
   int64_t state_bytes;           /* bytes per scalar of the state arrays (8 or 4) */
 } mf_layout_info;
 int mf_layout(mf_ctx *ctx, mf_layout_info *info);
+
+/* Measurement aid with no counterpart in the reference: a plain device copy of
+ * n_bytes (each byte read once and written once), `reps` times, timed with HIP
+ * events; the known-size kernel against which HBM traffic counters are calibrated. */
+int mf_copy_probe(int64_t n_bytes, int32_t reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

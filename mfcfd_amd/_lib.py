@@ -41,7 +41,7 @@ class MfParams(C.Structure):
                 ("res_old", C.c_double), ("rks", C.c_int32), ("inner_iterations", C.c_int32),
                 ("shapes", C.c_int32), ("interior_points_normal_flag", C.c_int32),
                 ("format", C.c_int32), ("restart", C.c_int32), ("precision", C.c_int32),
-                ("reorder", C.c_int32), ("block_size", C.c_int32), ("reserved", C.c_int32)]
+                ("reorder", C.c_int32), ("block_size", C.c_int32), ("first_order", C.c_int32)]
 
 
 class MfLayout(C.Structure):
@@ -74,6 +74,7 @@ SIGNATURES = [
     ("mf_sync", C.c_int, [_VP]),
     ("mf_profile_kernels", C.c_int, [_VP, C.c_int32, _DP, _IP]),
     ("mf_layout", C.c_int, [_VP, C.POINTER(MfLayout)]),
+    ("mf_copy_probe", C.c_int, [C.c_int64, C.c_int32, _DP]),
 ]
 
 _lib = None

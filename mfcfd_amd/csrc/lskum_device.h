@@ -76,6 +76,7 @@ struct Geom {
   const V4d* dg_sxy;
   const V4d* dg_obd;
   int32_t n, nw, no;  // points; wall count; outer count (interior = rest)
+  int32_t first_order;  // 1: q-tilde = q at both ends (no reconstruction)
 };
 
 struct Ctl {
@@ -343,8 +344,15 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
 
     const S dx = (S)delx, dy = (S)dely;
     S qti[4], qtk[4];
-    reconstruct(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
-    {
+    if (g.first_order) {
+      const V4<S> qk = st.q[k];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        qti[r] = qi[r];
+        qtk[r] = qk[r];
+      }
+    } else {
+      reconstruct(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
       const V4<S> qk = st.q[k];
       const V4<S> dxk = st.dq[2 * k], dyk = st.dq[2 * k + 1];
       const V4<S> qmaxk = st.qm[2 * k], qmink = st.qm[2 * k + 1];

@@ -322,6 +322,14 @@ __global__ __launch_bounds__(BLOCK) void k_sum(const double* __restrict__ partia
   if (threadIdx.x == 0) *out = t;
 }
 
+// Plain vectorised copy: the known-traffic kernel for counter calibration
+__global__ __launch_bounds__(BLOCK) void k_copy(const D2* __restrict__ a, D2* __restrict__ b,
+                                                int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x)
+    b[j] = a[j];
+}
+
 // =================================================================================
 // Host side
 // =================================================================================
@@ -577,6 +585,8 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
   if (prm->shapes < 1) return set_err(MF_ERR_INVALID, "shapes must be >= 1");
   if (prm->format == 2 && (!geo->nx || !geo->ny))
     return set_err(MF_ERR_INVALID, "format 2 needs file normals");
+  if (prm->first_order != 0 && prm->first_order != 1)
+    return set_err(MF_ERR_INVALID, "first_order must be 0 or 1");
   if (prm->block_size != 0 && prm->block_size != BLOCK)
     return set_err(MF_ERR_INVALID, "block_size must be 0 or %d", BLOCK);
   c->p = *prm;
@@ -759,7 +769,7 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
       (rc = upload(c, &d_lsq, lsq)) || (rc = upload(c, &d_c1, c1)) ||
       (rc = upload(c, &d_sxy, sxyv)) || (rc = upload(c, &d_obd, obd)))
     return rc;
-  c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no};
+  c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no, prm->first_order ? 1 : 0};
   if (c->ssz == 8) {
     double* e;
     if ((rc = upload(c, &e, eps_d))) return rc;
@@ -1246,6 +1256,32 @@ int mf_compute_cl_cd_cm(mf_ctx* c, double* cl, double* cd, double* cm, double* c
     if (cm) cm[s] = h[3 * s + 2];
   }
   if (cp) HIPCHK(hipMemcpy(cp, c->cp, c->nshape * sizeof(double), hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+int mf_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {
+  if (n_bytes < 16 || reps < 1) return set_err(MF_ERR_INVALID, "mf_copy_probe: bad size");
+  const int64_t n = n_bytes / 16;
+  D2 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n * 16));
+  HIPCHK(hipMalloc(&b, n * 16));
+  HIPCHK(hipMemset(a, 0, n * 16));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int grid = 4096;
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLOCK), 0, 0, a, b, n);  // warm
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BLOCK), 0, 0, a, b, n);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
   return MF_OK;
 }
 

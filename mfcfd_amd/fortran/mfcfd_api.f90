@@ -1,0 +1,137 @@
+! ISO_C_BINDING interfaces of include/mfcfd.h for a Fortran host.
+!
+! The reference drives its hot path through argument-less module procedures over
+! module state (src_mpi_primal data_structure.F90:13-49); a host that keeps the
+! reference's I/O (readcase, point reader, writers) calls these functions instead.
+module mfcfd_api
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: MF_OK = 0, MF_ERR_HIP = -1, MF_ERR_INVALID = -2, MF_ERR_STATE = -3
+    integer(c_int32_t), parameter :: MF_FP64 = 0, MF_MIXED = 1
+    integer(c_int32_t), parameter :: MF_FIELD_Q = 1, MF_FIELD_DQ = 2, MF_FIELD_QM = 3, &
+        MF_FIELD_FLUX_RES = 4, MF_FIELD_DELTA = 5, MF_FIELD_PRIM_OLD = 6, MF_FIELD_NX = 7, &
+        MF_FIELD_NY = 8
+
+    type, bind(C) :: mf_geometry
+        integer(c_int32_t) :: n_points, max_nbhs
+        type(c_ptr) :: x, y, nx, ny, min_dist, left, right, flag_1, flag_2, nbhs, conn
+    end type
+
+    type, bind(C) :: mf_params
+        real(c_double) :: cfl, vl_const, power, mach, aoa, euler, res_old
+        integer(c_int32_t) :: rks, inner_iterations, shapes, interior_points_normal_flag
+        integer(c_int32_t) :: format, restart, precision, reorder, block_size, first_order
+    end type
+
+    interface
+        subroutine mf_default_params(p) bind(C, name="mf_default_params")
+            import
+            type(mf_params), intent(out) :: p
+        end subroutine
+        integer(c_int) function mf_create(g, p, ctx) bind(C, name="mf_create")
+            import
+            type(mf_geometry), intent(in) :: g
+            type(mf_params), intent(in) :: p
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function mf_destroy(ctx) bind(C, name="mf_destroy")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function mf_last_error() bind(C, name="mf_last_error")
+            import
+        end function
+        integer(c_int) function mf_set_prim(ctx, prim) bind(C, name="mf_set_prim")
+            import
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: prim(4, *)
+        end function
+        integer(c_int) function mf_get_prim(ctx, prim) bind(C, name="mf_get_prim")
+            import
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: prim(4, *)
+        end function
+        integer(c_int) function mf_get_field(ctx, field, out) bind(C, name="mf_get_field")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: field
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function mf_set_field(ctx, field, in) bind(C, name="mf_set_field")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: field
+            real(c_double), intent(in) :: in(*)
+        end function
+        integer(c_int) function mf_func_delta(ctx) bind(C, name="mf_func_delta")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_eval_q_variables(ctx) bind(C, name="mf_eval_q_variables")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_eval_q_derivatives(ctx) bind(C, name="mf_eval_q_derivatives")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_eval_q_inner_loop(ctx) bind(C, name="mf_eval_q_inner_loop")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_cal_flux_residual(ctx) bind(C, name="mf_cal_flux_residual")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_state_update(ctx, rk, sum_res_sqr) bind(C, name="mf_state_update")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: rk
+            real(c_double), intent(out) :: sum_res_sqr
+        end function
+        integer(c_int) function mf_compute_cl_cd_cm(ctx, cl, cd, cm, cp) bind(C, name="mf_compute_cl_cd_cm")
+            import
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: cl(*), cd(*), cm(*), cp(*)
+        end function
+        integer(c_int32_t) function mf_shape_points(ctx) bind(C, name="mf_shape_points")
+            import
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_run(ctx, first_it, iters, residue, res_new, cd, done) bind(C, name="mf_run")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: first_it, iters
+            real(c_double), intent(out) :: residue(*), res_new(*), cd(*)
+            integer(c_int32_t), intent(out) :: done
+        end function
+        integer(c_int) function mf_run_async(ctx, first_it, iters) bind(C, name="mf_run_async")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: first_it, iters
+        end function
+        integer(c_int) function mf_sync(ctx) bind(C, name="mf_sync")
+            import
+            type(c_ptr), value :: ctx
+        end function
+    end interface
+
+contains
+
+    ! mf_last_error() as a Fortran string
+    function mf_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        character(kind=c_char), pointer :: cs(:)
+        integer :: n
+        call c_f_pointer(mf_last_error(), cs, [1024])
+        n = 0
+        do while (n < 1024)
+            if (cs(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: msg)
+        msg = transfer(cs(1:n), msg)
+    end function
+
+end module mfcfd_api

@@ -28,7 +28,7 @@ _TSCHEME = {"ssprk43": (4, 1.0), "first": (1, 2.0)}
 def make_params(**kw) -> L.MfParams:
     """mf_params from reference option names (cfl, mach, aoa, vl_const, power,
     inner_iterations, tscheme, shapes, interior_points_normal_flag, format, restart,
-    res_old) plus precision ('fp64'|'mixed') and reorder."""
+    res_old, solution_accuracy) plus precision ('fp64'|'mixed') and reorder."""
     p = L.MfParams()
     L.lib().mf_default_params(C.byref(p))
     kw = dict(kw)
@@ -36,6 +36,9 @@ def make_params(**kw) -> L.MfParams:
         p.rks, p.euler = _TSCHEME[kw.pop("tscheme")]
     if "aoa_deg" in kw:
         kw["aoa"] = kw.pop("aoa_deg")
+    acc = kw.pop("solution_accuracy", None)
+    if acc is not None:
+        kw["first_order"] = {"second": 0, "first": 1}[acc]
     prec = kw.pop("precision", "fp64")
     p.precision = {"fp64": L.MF_FP64, "mixed": L.MF_MIXED}[prec] if isinstance(prec, str) else int(prec)
     for k, v in kw.items():

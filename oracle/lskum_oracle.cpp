@@ -54,6 +54,7 @@ struct Params {
   double cfl, vl_const, power, mach, aoa_deg, euler;
   int32_t rks, inner_iterations, shapes, interior_points_normal_flag, format, restart;
   double restart_res_old;
+  int32_t first_order;  // -solution_accuracy first: q-tilde = q (see dG)
 };
 
 template <class T>
@@ -434,12 +435,20 @@ struct Solver {
       syy = syy + deln * deln_w;
       sxy = sxy + dels * deln_w;
       T qt_i[4], qt_k[4], phi_i[4], phi_k[4];
-      for (int r = 0; r < 4; ++r) {
+      // -solution_accuracy first (readcase.F90:107-115) is parsed but never read by
+      // the reference's primal kernels; first order here means no reconstruction,
+      // q-tilde = q at both ends, so dG is the plain LSQ of G(q_k) - G(q_i).
+      if (p.first_order) {
+        for (int r = 0; r < 4; ++r) { phi_i[r] = T(0.0); phi_k[r] = T(0.0); }
+      }
+      for (int r = 0; r < 4 && !p.first_order; ++r) {
         qt_i[r] = q[4 * i + r] - 0.5 * (delx * dq[8 * i + 2 * r] + dely * dq[8 * i + 2 * r + 1]);
         qt_k[r] = q[4 * k + r] - 0.5 * (delx * dq[8 * k + 2 * r] + dely * dq[8 * k + 2 * r + 1]);
       }
-      venkat_limiter(qt_i, phi_i, i);
-      venkat_limiter(qt_k, phi_k, k);
+      if (!p.first_order) {
+        venkat_limiter(qt_i, phi_i, i);
+        venkat_limiter(qt_k, phi_k, k);
+      }
       for (int r = 0; r < 4; ++r) {
         qt_i[r] = q[4 * i + r] - 0.5 * phi_i[r] * (delx * dq[8 * i + 2 * r] + dely * dq[8 * i + 2 * r + 1]);
         qt_k[r] = q[4 * k + r] - 0.5 * phi_k[r] * (delx * dq[8 * k + 2 * r] + dely * dq[8 * k + 2 * r + 1]);
@@ -635,6 +644,7 @@ struct orc_params {
   double cfl, vl_const, power, mach, aoa_deg, euler;
   int32_t rks, inner_iterations, shapes, interior_points_normal_flag, format, restart;
   double restart_res_old;
+  int32_t first_order;
 };
 
 void* orc_create(int32_t n, int32_t w, const double* x, const double* y,

@@ -28,7 +28,8 @@ class OrcParams(C.Structure):
                 ("mach", C.c_double), ("aoa_deg", C.c_double), ("euler", C.c_double),
                 ("rks", C.c_int32), ("inner_iterations", C.c_int32), ("shapes", C.c_int32),
                 ("interior_points_normal_flag", C.c_int32), ("format", C.c_int32),
-                ("restart", C.c_int32), ("restart_res_old", C.c_double)]
+                ("restart", C.c_int32), ("restart_res_old", C.c_double),
+                ("first_order", C.c_int32)]
 
 
 def build(force: bool = False) -> None:
@@ -88,7 +89,7 @@ def default_params(**kw) -> dict:
     """readcase.F90:10-125 defaults (ssprk43, venkat 150, power 0, one shape)."""
     p = dict(cfl=0.0, vl_const=150.0, power=0.0, mach=0.63, aoa_deg=2.0, euler=1.0, rks=4,
              inner_iterations=0, shapes=1, interior_points_normal_flag=0, format=1,
-             restart=0, restart_res_old=0.0)
+             restart=0, restart_res_old=0.0, first_order=0)
     p.update(kw)
     return p
 

@@ -312,3 +312,32 @@ def test_nan_exit(cloud):
     o.field("prim")[5, 3] = -1.0
     res, _ = o.run(20)
     assert len(res) == 3 and np.isnan(res[-1])
+
+
+def test_first_order_is_lsq_of_point_fluxes(cloud):
+    """first order: q-tilde = q, so each half's derivative is the plain LSQ of
+    G(q_k) - G(q_i) with the point's (t, n) frame fluxes; checked for one interior
+    point against a numpy solve of the 2x2 normal equations."""
+    o = Oracle(cloud, cfl=0.2, mach=0.63, aoa_deg=2.0, first_order=1).setup()
+    o.run(5)
+    o.func_delta()
+    o.eval_q_variables()
+    o.eval_q_derivatives()
+    o.cal_flux_residual()
+    i = int(np.where(cloud.flag_1 == 1)[0][37])
+    prim = o.field("prim")
+    nx, ny = o.field("nx")[i], o.field("ny")[i]
+    total = np.zeros(4)
+    for half, kind, ydir in (("xpos", 0, False), ("xneg", 1, False), ("ypos", 2, True), ("yneg", 3, True)):
+        lst = o.ifield(half + "_conn")[i, :o.ifield(half + "_nbhs")[i]]
+        def G(j):
+            r, u1, u2, p = prim[j]
+            return flux(kind, nx, ny, u1, u2, r, p)
+        dx = cloud.x[lst] - cloud.x[i]
+        dy = cloud.y[lst] - cloud.y[i]
+        ds, dn = dx * ny - dy * nx, dx * nx + dy * ny
+        A = np.array([[ds @ ds, ds @ dn], [ds @ dn, dn @ dn]])
+        dG = np.array([G(j) - G(i) for j in lst])
+        sol = np.linalg.solve(A, np.vstack([ds @ dG, dn @ dG]))
+        total += sol[1] if ydir else sol[0]
+    assert np.allclose(o.field("flux_res")[i], o.field("delta")[i] * total, rtol=1e-9, atol=1e-14)
