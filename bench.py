@@ -94,6 +94,39 @@ def cpu_baseline(cloud, iters: int):
                        f"{dt:.1f} s")
 
 
+ADJ_WORKLOAD = dict(n_wall=320, n_radial=125, mach=0.8, aoa=1.25, cfl=0.2, vl_const=5.0,
+                    inner_iterations=3, seed=0, iterations=2000, snaps=32)
+
+
+def run_adjoint(iters: int, snaps: int):
+    """Cd adjoint of the NACA0012 320x125 cloud (40,000 points), second order, M 0.8,
+    AoA 1.25: `iters` primal iterations with `snaps` binomial checkpoints, then the
+    reverse sweep.  Cost is reported against a plain primal run of the same length."""
+    from mfcfd_amd.cloud import naca_ogrid
+    from mfcfd_amd.solver import LskumSolver
+    w = ADJ_WORKLOAD
+    cloud = naca_ogrid(w["n_wall"], w["n_radial"], seed=w["seed"])
+    p = {k: w[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
+    with LskumSolver(cloud, **p) as s:
+        s.q_lskum(2)
+    with LskumSolver(cloud, **p) as s:
+        s.sync()
+        t0 = time.perf_counter()
+        s.run_async(iters)
+        s.sync()
+        primal_s = time.perf_counter() - t0
+    with LskumSolver(cloud, **p) as s:
+        print(f"[bench] adjoint: {iters} iterations, {snaps} snapshots", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        J, dx, dy, st = s.adjoint(iters, snaps=snaps)
+        adj_s = time.perf_counter() - t0
+    return {"workload": f"NACA0012 320x125 ({cloud.n} points), 2nd order, 3 inner, M 0.8, AoA 1.25, "
+                        f"J = Cd, {iters} iterations, {snaps} snapshots",
+            "J": J, "primal_s": primal_s, "adjoint_total_s": adj_s,
+            "adjoint_over_primal": adj_s / primal_s, "stats": st,
+            "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+
+
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -120,6 +153,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-iters", type=int, default=200,
+                    help="iterations of the adjoint cost measurement (0: skip)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only the adjoint measurement, at its full 2000 iterations")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
                          "2 mixed iterations of the workload (for rocprofv3 --pmc)")
@@ -131,6 +168,9 @@ def main():
 
     if args.probe:
         probe()
+        return
+    if args.adjoint:
+        print(json.dumps(run_adjoint(ADJ_WORKLOAD["iterations"], ADJ_WORKLOAD["snaps"])))
         return
 
     from mfcfd_amd.cloud import naca_ogrid
@@ -188,6 +228,8 @@ def main():
         "kernel_ms": {"fp64": {k: v[0] for k, v in fp64["kernels"].items()},
                       "mixed": {k: v[0] for k, v in mixed["kernels"].items()}},
     }
+    if args.adjoint_iters > 0:
+        line["adjoint"] = run_adjoint(args.adjoint_iters, ADJ_WORKLOAD["snaps"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cloud, args.cpu_iters)
     print(json.dumps(line))

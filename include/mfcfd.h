@@ -146,6 +146,28 @@ typedef struct mf_layout_info {
 } mf_layout_info;
 int mf_layout(mf_ctx *ctx, mf_layout_info *info);
 
+/* Adjoint of the iteration (replaces the Tapenade q_lskum_chkpts_b driver of
+ * src_mpi_adjoint, sensitivity_files/q_lskum_chkpts_b.F90:23-300).
+ * J = sum over shapes of Cd after `iters` iterations from the context's current state
+ * (the reference seeds vector_cost_funcb = 1 per shape).  mf_adjoint_forward runs the
+ * primal with a binomial checkpoint schedule of `snaps` prim snapshots in HBM and
+ * returns J; mf_adjoint_backward runs the reverse sweep and returns dJ/dx, dJ/dy per
+ * point in original order (the reference's pointb%x, pointb%y, written to
+ * sensitivity/sensitivity.dat by src_mpi_adjoint/post_processing.F90:41-66).  The
+ * context is left holding the final primal state.  FP64 contexts with power = 0. */
+typedef struct mf_adjoint_stats {
+  int64_t forward_steps;  /* iterations advanced: the primal sweep plus recomputation */
+  int64_t taped_steps;    /* iterations re-run with their stage intermediates taped  */
+  int32_t snapshots;      /* prim snapshots held at the peak                         */
+  int32_t reserved;
+  double primal_ms;       /* the primal sweep (mf_adjoint_forward)                   */
+  double adjoint_ms;      /* the reverse sweep (mf_adjoint_backward)                 */
+} mf_adjoint_stats;
+int mf_adjoint_forward(mf_ctx *ctx, int32_t first_it, int32_t iters, int32_t snaps, double *J);
+int mf_adjoint_backward(mf_ctx *ctx, double *dJdx, double *dJdy, mf_adjoint_stats *stats);
+int mf_adjoint(mf_ctx *ctx, int32_t first_it, int32_t iters, int32_t snaps, double *J,
+               double *dJdx, double *dJdy, mf_adjoint_stats *stats);
+
 /* Measurement aid with no counterpart in the reference: a plain device copy of
  * n_bytes (each byte read once and written once), `reps` times, timed with HIP
  * events; the known-size kernel against which HBM traffic counters are calibrated. */

@@ -50,6 +50,12 @@ class MfLayout(C.Structure):
                 ("total_nbhs", C.c_int64), ("state_bytes", C.c_int64)]
 
 
+class MfAdjointStats(C.Structure):
+    _fields_ = [("forward_steps", C.c_int64), ("taped_steps", C.c_int64),
+                ("snapshots", C.c_int32), ("reserved", C.c_int32),
+                ("primal_ms", C.c_double), ("adjoint_ms", C.c_double)]
+
+
 # (name, restype, argtypes) of every symbol include/mfcfd.h declares
 _VP, _DP, _IP = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
 SIGNATURES = [
@@ -75,6 +81,10 @@ SIGNATURES = [
     ("mf_profile_kernels", C.c_int, [_VP, C.c_int32, _DP, _IP]),
     ("mf_layout", C.c_int, [_VP, C.POINTER(MfLayout)]),
     ("mf_copy_probe", C.c_int, [C.c_int64, C.c_int32, _DP]),
+    ("mf_adjoint_forward", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    ("mf_adjoint_backward", C.c_int, [_VP, _DP, _DP, C.POINTER(MfAdjointStats)]),
+    ("mf_adjoint", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _DP, _DP, _DP,
+                             C.POINTER(MfAdjointStats)]),
 ]
 
 _lib = None

@@ -1,0 +1,479 @@
+// Reverse (adjoint) mode of the LSKUM iteration, FP64, for gfx950.
+//
+// Hand-written reverse of every forward routine in lskum_device.h, in the order the
+// forward evaluates them: each *_rev function takes the adjoints of a routine's
+// outputs and adds the adjoints of its inputs.  Forward intermediates are recomputed
+// from the routine's inputs (no tape inside a point).  Non-differentiable choices
+// follow the primal's branch exactly, as the reference's Tapenade adjoint does
+// (src_mpi_adjoint/sensitivity_files/limiters_b.F90:43-164: phi = 1 and phi capped
+// at 1 carry no derivative; q_variables_b.F90:188-223: qm's max/min pass the adjoint
+// to the selected point; state_update_b.F90:465-471: func_delta's min to its
+// minimising neighbour).  d erf(s)/ds = 2/sqrt(pi) exp(-s^2), as the reference
+// patched it (split_fluxes_b.F90:82-83).
+#pragma once
+#include "lskum_device.h"
+
+namespace mf {
+
+constexpr double TWO_BY_SQRT_PI = 1.1283791670955126;
+
+// ---------------------------------------------------------------------------------
+// q-tilde -> primitive -> flux preliminaries, with every intermediate kept
+struct KinF {
+  double qt[4];
+  double beta, tmp, u1, u2, T1, T2, rho, p, ut, un, b, sb, spb, S1, S2, ex1, ex2, B1, B2, e1,
+      e2, prr, usq;
+};
+
+struct KinB {
+  double rho, ut, un, prr, usq, B1, B2, e1, e2;
+};
+
+__host__ __device__ inline void kinb_zero(KinB& b) {
+  b.rho = b.ut = b.un = b.prr = b.usq = b.B1 = b.B2 = b.e1 = b.e2 = 0.0;
+}
+
+// Same operations, in the same order, as kin_of_qtilde<double>
+__host__ __device__ inline void kin_fwd(const double qt[4], double nx, double ny, KinF& k) {
+  for (int r = 0; r < 4; ++r) k.qt[r] = qt[r];
+  k.beta = -qt[3] * 0.5;
+  k.tmp = 0.5 / k.beta;
+  k.u1 = qt[1] * k.tmp;
+  k.u2 = qt[2] * k.tmp;
+  k.T1 = qt[0] + k.beta * (k.u1 * k.u1 + k.u2 * k.u2);
+  k.T2 = k.T1 - (m_log(k.beta) / Consts<double>::GAMMA_M1);
+  k.rho = m_exp(k.T2);
+  k.p = k.rho * k.tmp;
+  const double tx = ny, ty = -nx;
+  k.ut = k.u1 * tx + k.u2 * ty;
+  k.un = k.u1 * nx + k.u2 * ny;
+  k.b = 0.5 * k.rho / k.p;
+  k.sb = m_sqrt(k.b);
+  k.spb = m_sqrt(Consts<double>::PI * k.b);
+  k.S1 = k.ut * k.sb;
+  k.S2 = k.un * k.sb;
+  k.ex1 = m_exp(-k.S1 * k.S1);
+  k.ex2 = m_exp(-k.S2 * k.S2);
+  k.B1 = 0.5 * k.ex1 / k.spb;
+  k.B2 = 0.5 * k.ex2 / k.spb;
+  k.e1 = m_erf(k.S1);
+  k.e2 = m_erf(k.S2);
+  k.prr = k.p / k.rho;
+  k.usq = k.ut * k.ut + k.un * k.un;
+}
+
+__host__ __device__ inline Kin<double> kin_of(const KinF& f) {
+  Kin<double> k;
+  k.rho = f.rho;
+  k.ut = f.ut;
+  k.un = f.un;
+  k.pr_by_rho = f.prr;
+  k.u_sqr = f.usq;
+  k.B1 = f.B1;
+  k.B2 = f.B2;
+  k.e1 = f.e1;
+  k.e2 = f.e2;
+  return k;
+}
+
+// reverse of flux_gx (split_fluxes.F90:8-96)
+__host__ __device__ inline void flux_gx_rev(const KinF& k, bool plus, const double Gb[4], KinB& kb) {
+  const double s = plus ? 1.0 : -1.0;
+  const double A1 = 0.5 * (1.0 + s * k.e1);
+  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
+  const double F0 = k.ut * A1 + s * k.B1;
+  const double F1 = (k.prr + k.ut * k.ut) * A1 + s * k.ut * k.B1;
+  const double F2 = k.ut * k.un * A1 + s * k.un * k.B1;
+  const double F3 = 0.5 * k.ut * a7 * A1 + s * 0.5 * a6 * k.B1;
+  kb.rho += Gb[0] * F0 + Gb[1] * F1 + Gb[2] * F2 + Gb[3] * F3;
+  const double g0 = k.rho * Gb[0], g1 = k.rho * Gb[1], g2 = k.rho * Gb[2], g3 = k.rho * Gb[3];
+  const double A1b = g0 * k.ut + g1 * (k.prr + k.ut * k.ut) + g2 * k.ut * k.un + g3 * 0.5 * k.ut * a7;
+  kb.B1 += s * (g0 + g1 * k.ut + g2 * k.un + g3 * 0.5 * a6);
+  kb.ut += g0 * A1 + g1 * (2.0 * k.ut * A1 + s * k.B1) + g2 * k.un * A1 + g3 * 0.5 * a7 * A1;
+  kb.un += g2 * (k.ut * A1 + s * k.B1);
+  kb.prr += g1 * A1 + g3 * (3.5 * k.ut * A1 + 3.0 * s * k.B1);
+  kb.usq += g3 * 0.5 * (k.ut * A1 + s * k.B1);
+  kb.e1 += 0.5 * s * A1b;
+}
+
+// reverse of flux_gy (split_fluxes.F90:99-190); outputs 1 and 2 swap roles
+__host__ __device__ inline void flux_gy_rev(const KinF& k, bool plus, const double Gb[4], KinB& kb) {
+  const double s = plus ? 1.0 : -1.0;
+  const double A2 = 0.5 * (1.0 + s * k.e2);
+  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
+  const double F0 = k.un * A2 + s * k.B2;
+  const double F1 = k.ut * k.un * A2 + s * k.ut * k.B2;
+  const double F2 = (k.prr + k.un * k.un) * A2 + s * k.un * k.B2;
+  const double F3 = 0.5 * k.un * a7 * A2 + s * 0.5 * a6 * k.B2;
+  kb.rho += Gb[0] * F0 + Gb[1] * F1 + Gb[2] * F2 + Gb[3] * F3;
+  const double h0 = k.rho * Gb[0], hT = k.rho * Gb[1], hN = k.rho * Gb[2], h3 = k.rho * Gb[3];
+  const double A2b = h0 * k.un + hN * (k.prr + k.un * k.un) + hT * k.ut * k.un + h3 * 0.5 * k.un * a7;
+  kb.B2 += s * (h0 + hN * k.un + hT * k.ut + h3 * 0.5 * a6);
+  kb.un += h0 * A2 + hN * (2.0 * k.un * A2 + s * k.B2) + hT * k.ut * A2 + h3 * 0.5 * a7 * A2;
+  kb.ut += hT * (k.un * A2 + s * k.B2);
+  kb.prr += hN * A2 + h3 * (3.5 * k.un * A2 + 3.0 * s * k.B2);
+  kb.usq += h3 * 0.5 * (k.un * A2 + s * k.B2);
+  kb.e2 += 0.5 * s * A2b;
+}
+
+// reverse of flux_quad (quadrant_fluxes.F90:12-248)
+__host__ __device__ inline void flux_quad_rev(const KinF& k, bool a1p, bool a2p, const double Gb[4],
+                                              KinB& kb) {
+  const double s1 = a1p ? 1.0 : -1.0, s2 = a2p ? 1.0 : -1.0;
+  const double A1 = 0.5 * (1.0 + s1 * k.e1), A2 = 0.5 * (1.0 + s2 * k.e2);
+  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
+  const double t1b = k.ut * A1 + s1 * k.B1;
+  const double P = (k.prr + k.ut * k.ut) * A1 + s1 * k.ut * k.B1;
+  const double Q = k.un * A2 + s2 * k.B2;
+  const double temp2 = 0.5 * k.ut * a7 * A1, temp3 = 0.5 * k.B1 * a6;
+  const double R = temp2 + s1 * temp3;
+  double rhob = Gb[0] * A2 * t1b + Gb[1] * A2 * P + Gb[2] * t1b * Q + Gb[3] * A2 * R;
+  double A2b = Gb[0] * k.rho * t1b + Gb[1] * k.rho * P + Gb[3] * k.rho * R;
+  double t1bb = Gb[0] * k.rho * A2 + Gb[2] * k.rho * Q;
+  const double Pb = Gb[1] * k.rho * A2;
+  const double Qb = Gb[2] * k.rho * t1b;
+  const double Rb = Gb[3] * k.rho * A2;
+  // temp4 = 0.5 rho un B2 t1b enters G3 with sign s2
+  const double t4b = s2 * Gb[3];
+  rhob += t4b * 0.5 * k.un * k.B2 * t1b;
+  double unb = t4b * 0.5 * k.rho * k.B2 * t1b;
+  double B2b = t4b * 0.5 * k.rho * k.un * t1b;
+  t1bb += t4b * 0.5 * k.rho * k.un * k.B2;
+  double utb = 0.0, prrb = 0.0, usqb = 0.0, A1b = 0.0, B1b = 0.0;
+  // R = temp2 + s1 temp3
+  utb += Rb * 0.5 * a7 * A1;
+  prrb += Rb * 3.5 * k.ut * A1;
+  usqb += Rb * 0.5 * k.ut * A1;
+  A1b += Rb * 0.5 * k.ut * a7;
+  const double t3b = s1 * Rb;
+  B1b += t3b * 0.5 * a6;
+  prrb += t3b * 3.0 * k.B1;
+  usqb += t3b * 0.5 * k.B1;
+  // Q = un A2 + s2 B2
+  unb += Qb * A2;
+  A2b += Qb * k.un;
+  B2b += Qb * s2;
+  // P = (prr + ut^2) A1 + s1 ut B1
+  prrb += Pb * A1;
+  utb += Pb * (2.0 * k.ut * A1 + s1 * k.B1);
+  A1b += Pb * (k.prr + k.ut * k.ut);
+  B1b += Pb * s1 * k.ut;
+  // t1b = ut A1 + s1 B1
+  utb += t1bb * A1;
+  A1b += t1bb * k.ut;
+  B1b += t1bb * s1;
+  kb.rho += rhob;
+  kb.ut += utb;
+  kb.un += unb;
+  kb.prr += prrb;
+  kb.usq += usqb;
+  kb.B1 += B1b;
+  kb.B2 += B2b;
+  kb.e1 += 0.5 * s1 * A1b;
+  kb.e2 += 0.5 * s2 * A2b;
+}
+
+// reverse of kin_fwd: adds d/d(qt) into qtb and d/d(nx, ny) into nxb, nyb
+__host__ __device__ inline void kin_rev(const KinF& k, const KinB& kb, double nx, double ny,
+                                        double qtb[4], double& nxb, double& nyb) {
+  double utb = kb.ut, unb = kb.un, rhob = kb.rho;
+  // usq = ut^2 + un^2
+  utb += 2.0 * k.ut * kb.usq;
+  unb += 2.0 * k.un * kb.usq;
+  // prr = p / rho
+  double pb = kb.prr / k.rho;
+  rhob -= kb.prr * k.prr / k.rho;
+  // e = erf(S), B = 0.5 exp(-S^2) / spb
+  double S1b = kb.e1 * TWO_BY_SQRT_PI * k.ex1 - 2.0 * k.S1 * k.B1 * kb.B1;
+  double S2b = kb.e2 * TWO_BY_SQRT_PI * k.ex2 - 2.0 * k.S2 * k.B2 * kb.B2;
+  const double spbb = -(kb.B1 * k.B1 + kb.B2 * k.B2) / k.spb;
+  // S1 = ut sb, S2 = un sb
+  utb += S1b * k.sb;
+  unb += S2b * k.sb;
+  const double sbb = S1b * k.ut + S2b * k.un;
+  // sb = sqrt(b), spb = sqrt(pi b)
+  const double bb = sbb / (2.0 * k.sb) + spbb * Consts<double>::PI / (2.0 * k.spb);
+  // b = 0.5 rho / p
+  rhob += bb * 0.5 / k.p;
+  pb -= bb * k.b / k.p;
+  // ut = u1 ny - u2 nx, un = u1 nx + u2 ny
+  double u1b = utb * ny + unb * nx;
+  double u2b = -utb * nx + unb * ny;
+  nxb += -utb * k.u2 + unb * k.u1;
+  nyb += utb * k.u1 + unb * k.u2;
+  // p = rho tmp
+  rhob += pb * k.tmp;
+  double tmpb = pb * k.rho;
+  // rho = exp(T2), T2 = T1 - log(beta)/(gamma-1)
+  const double T2b = rhob * k.rho;
+  const double T1b = T2b;
+  double betab = -T2b / (Consts<double>::GAMMA_M1 * k.beta);
+  // T1 = qt1 + beta (u1^2 + u2^2)
+  qtb[0] += T1b;
+  betab += T1b * (k.u1 * k.u1 + k.u2 * k.u2);
+  u1b += T1b * 2.0 * k.beta * k.u1;
+  u2b += T1b * 2.0 * k.beta * k.u2;
+  // u1 = qt2 tmp, u2 = qt3 tmp
+  qtb[1] += u1b * k.tmp;
+  qtb[2] += u2b * k.tmp;
+  tmpb += u1b * k.qt[1] + u2b * k.qt[2];
+  // tmp = 0.5 / beta, beta = -0.5 qt4
+  betab -= tmpb * k.tmp / k.beta;
+  qtb[3] += -0.5 * betab;
+}
+
+// reverse of venkat (limiters.F90:64-110), one component
+__host__ __device__ inline void venkat_rev(double qt0, double q, double qmax, double qmin,
+                                           double epsi, double phib, double& qt0b, double& qb,
+                                           double& qmaxb, double& qminb) {
+  const double dn = qt0 - q;
+  if (fabs(dn) <= Consts<double>::THRESH) return;
+  const bool use_max = dn > 0.0;
+  const double dp = (use_max ? qmax : qmin) - q;
+  const double e2 = epsi * epsi;
+  const double num = (dp * dp + e2) * dn + 2.0 * dn * dn * dp;
+  const double D = dp * dp + 2.0 * dn * dn + dn * dp + e2;
+  const double den = D * dn;
+  const double t = num / den;
+  if (!(t < 1.0)) return;
+  const double numb = phib / den;
+  const double denb = -phib * t / den;
+  const double dnb = numb * ((dp * dp + e2) + 4.0 * dn * dp) + denb * (D + dn * (4.0 * dn + dp));
+  const double dpb = numb * (2.0 * dp * dn + 2.0 * dn * dn) + denb * (dn * (2.0 * dp + dn));
+  qt0b += dnb;
+  qb -= dnb;
+  if (use_max) qmaxb += dpb;
+  else qminb += dpb;
+  qb -= dpb;
+}
+
+// Adjoints of one end's point record
+struct EndB {
+  double q[4], dx[4], dy[4], qmax[4], qmin[4];
+};
+
+__host__ __device__ inline void endb_zero(EndB& e) {
+  for (int r = 0; r < 4; ++r) e.q[r] = e.dx[r] = e.dy[r] = e.qmax[r] = e.qmin[r] = 0.0;
+}
+
+// reverse of reconstruct (wall_fluxes.F90:79-86)
+__host__ __device__ inline void reconstruct_rev(const V4d& q, const V4d& dx, const V4d& dy,
+                                                const V4d& qmax, const V4d& qmin, double eps,
+                                                double delx, double dely, const double qtb[4],
+                                                EndB& eb, double& delxb, double& delyb) {
+  for (int r = 0; r < 4; ++r) {
+    const double d = delx * dx[r] + dely * dy[r];
+    const double qt0 = q[r] - 0.5 * d;
+    const double phi = venkat(qt0, q[r], qmax[r], qmin[r], eps);
+    eb.q[r] += qtb[r];
+    const double phib = -0.5 * d * qtb[r];
+    double db = -0.5 * phi * qtb[r];
+    double qt0b = 0.0;
+    venkat_rev(qt0, q[r], qmax[r], qmin[r], eps, phib, qt0b, eb.q[r], eb.qmax[r], eb.qmin[r]);
+    eb.q[r] += qt0b;
+    db += -0.5 * qt0b;
+    eb.dx[r] += db * delx;
+    eb.dy[r] += db * dely;
+    delxb += db * dx[r];
+    delyb += db * dy[r];
+  }
+}
+
+// One end's split fluxes for an edge (as end_fluxes<CLS>) from its KinF
+template <int CLS>
+__host__ __device__ inline void end_fluxes_f(const KinF& f, bool xp, bool yp, double Gx[4], double Gy[4]) {
+  end_fluxes<CLS, double>(kin_of(f), xp, yp, Gx, Gy);
+}
+
+template <int CLS>
+__host__ __device__ inline void end_fluxes_rev(const KinF& f, bool xp, bool yp, const double Gxb[4],
+                                               const double Gyb[4], KinB& kb) {
+  if (CLS == INTERIOR) {
+    flux_gx_rev(f, xp, Gxb, kb);
+    flux_gy_rev(f, yp, Gyb, kb);
+  } else {
+    const bool a2p = (CLS == OUTER);
+    flux_quad_rev(f, xp, a2p, Gxb, kb);
+    flux_gy_rev(f, a2p, Gyb, kb);
+  }
+}
+
+// Adjoints of the per-point half-stencil coefficients (c1, sxy, obd of each half)
+struct HalfCoefB {
+  double c1[4], sxy[4], obd[4];
+};
+
+// reverse of coef_x / coef_y for half `d` (0 xpos, 1 xneg, 2 ypos, 3 yneg)
+__host__ __device__ inline void coef_rev(const HalfCoef& h, int d, bool ydir, double dsw, double dnw,
+                                         double cb, HalfCoefB& hb, double& dswb, double& dnwb) {
+  double c1 = 0, sxy = 0, obd = 0;
+  for (int j = 0; j < 4; ++j)
+    if (j == d) {
+      c1 = h.c1[j];
+      sxy = h.sxy[j];
+      obd = h.obd[j];
+    }
+  const double a = ydir ? dnw : dsw, bq = ydir ? dsw : dnw;
+  // c = (a c1 - bq sxy) obd
+  const double ab = cb * c1 * obd, bqb = -cb * sxy * obd;
+  for (int j = 0; j < 4; ++j)
+    if (j == d) {
+      hb.c1[j] += cb * a * obd;
+      hb.sxy[j] += -cb * bq * obd;
+      hb.obd[j] += cb * (a * c1 - bq * sxy);
+    }
+  if (ydir) {
+    dnwb += ab;
+    dswb += bqb;
+  } else {
+    dswb += ab;
+    dnwb += bqb;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// state update
+
+// reverse of prim_to_cons<double> (state_update.F90:144-163)
+__host__ __device__ inline void prim_to_cons_rev(const V4d& p, double nx, double ny, const double Ub[4],
+                                                 double pb[4], double& nxb, double& nyb) {
+  const double rho = p.x, t1 = rho * p.y, t2 = rho * p.z;
+  // U4 = 2.5 p + 0.5 (t1^2 + t2^2) / rho; U2 = t1 ny - t2 nx; U3 = t1 nx + t2 ny
+  const double t1b = Ub[1] * ny + Ub[2] * nx + Ub[3] * t1 / rho;
+  const double t2b = -Ub[1] * nx + Ub[2] * ny + Ub[3] * t2 / rho;
+  nxb += -Ub[1] * t2 + Ub[2] * t1;
+  nyb += Ub[1] * t1 + Ub[2] * t2;
+  pb[0] += Ub[0] - Ub[3] * 0.5 * (t1 * t1 + t2 * t2) / (rho * rho) + t1b * p.y + t2b * p.z;
+  pb[1] += t1b * rho;
+  pb[2] += t2b * rho;
+  pb[3] += 2.5 * Ub[3];
+}
+
+// reverse of cons_ubar<double> (state_update.F90:239-294)
+__host__ __device__ inline void cons_ubar_rev(const V4d& p, double nx, double ny, const Freestream& fs,
+                                              const double Ubb[4], double pb[4], double& nxb,
+                                              double& nyb) {
+  const double rho_inf = 1.0, pr_inf = 1.0 / 1.4, gm1 = 1.4 - 1.0, pi = Consts<double>::PI;
+  const double u1i = fs.u1_inf, u2i = fs.u2_inf;
+  const double tx = ny, ty = -nx;
+  const double u1ir = u1i * tx + u2i * ty, u2ir = u1i * nx + u2i * ny;
+  const double e_inf = pr_inf / (rho_inf * gm1) + 0.5 * (u1ir * u1ir + u2ir * u2ir);
+  const double beta_i = (0.5 * rho_inf) / pr_inf;
+  const double S2i = u2ir * sqrt(beta_i);
+  const double exi = exp(-S2i * S2i);
+  const double B2i = exi / (2.0 * sqrt(pi * beta_i));
+  const double A2ni = 0.5 * (1.0 - erf(S2i));
+  const double rho = p.x, u1 = p.y, u2 = p.z, pr = p.w;
+  const double u1r = u1 * tx + u2 * ty, u2r = u1 * nx + u2 * ny;
+  const double e = pr / (rho * gm1) + 0.5 * (u1r * u1r + u2r * u2r);
+  const double beta = rho / (2.0 * pr);
+  const double S2 = u2r * sqrt(beta);
+  const double ex = exp(-S2 * S2);
+  const double B2 = ex / (2.0 * sqrt(pi * beta));
+  const double A2p = 0.5 * (1.0 + erf(S2));
+  // freestream half
+  const double A2nib = rho_inf * (Ubb[0] + Ubb[1] * u1ir + Ubb[2] * u2ir + Ubb[3] * e_inf);
+  double u1irb = Ubb[1] * rho_inf * A2ni;
+  double u2irb = Ubb[2] * rho_inf * A2ni - Ubb[3] * 0.5 * rho_inf * B2i;
+  const double B2ib = -Ubb[2] * rho_inf - Ubb[3] * 0.5 * rho_inf * u2ir;
+  const double e_infb = Ubb[3] * rho_inf * A2ni;
+  u1irb += e_infb * u1ir;
+  u2irb += e_infb * u2ir;
+  const double S2ib = -A2nib * 0.5 * TWO_BY_SQRT_PI * exi + B2ib * B2i * (-2.0 * S2i);
+  u2irb += S2ib * sqrt(beta_i);
+  nyb += u1irb * u1i + u2irb * u2i;
+  nxb += -u1irb * u2i + u2irb * u1i;
+  // interior half
+  double rhob = Ubb[0] * A2p + Ubb[1] * u1r * A2p + Ubb[2] * (u2r * A2p + B2) + Ubb[3] * (A2p * e + 0.5 * u2r * B2);
+  const double A2pb = Ubb[0] * rho + Ubb[1] * rho * u1r + Ubb[2] * rho * u2r + Ubb[3] * rho * e;
+  double u1rb = Ubb[1] * rho * A2p;
+  double u2rb = Ubb[2] * rho * A2p + Ubb[3] * 0.5 * rho * B2;
+  const double B2b = Ubb[2] * rho + Ubb[3] * 0.5 * rho * u2r;
+  const double eb = Ubb[3] * rho * A2p;
+  double prb = eb / (rho * gm1);
+  rhob -= eb * pr / (rho * rho * gm1);
+  u1rb += eb * u1r;
+  u2rb += eb * u2r;
+  double S2b = A2pb * 0.5 * TWO_BY_SQRT_PI * ex + B2b * B2 * (-2.0 * S2);
+  double betab = -B2b * B2 / (2.0 * beta);
+  u2rb += S2b * sqrt(beta);
+  betab += S2b * u2r / (2.0 * sqrt(beta));
+  rhob += betab / (2.0 * pr);
+  prb -= betab * beta / pr;
+  pb[0] += rhob;
+  pb[1] += u1rb * tx + u2rb * nx;
+  pb[2] += u1rb * ty + u2rb * ny;
+  pb[3] += prb;
+  nxb += -u1rb * u2 + u2rb * u1;
+  nyb += u1rb * u1 + u2rb * u2;
+}
+
+// reverse of update_point<CLS, double> (state_update.F90:10-141): from the adjoint of
+// the new prim `pn` (pnb), adds adjoints of the old prim, prim_old (stage 3),
+// flux_res and the normal.  The post-update conserved state is rebuilt from pn.
+template <int CLS>
+__host__ __device__ inline void update_rev(const V4d& p, const V4d& pold, const V4d& pn, double nx,
+                                           double ny, int rk, double euler, const Freestream& fs,
+                                           const V4d& pnb, double pb[4], double poldb[4],
+                                           double frb[4], double& nxb, double& nyb) {
+  const double obt = 1.0 / 3.0, tbt = 2.0 / 3.0, half_euler = 0.5 * euler;
+  const double W1 = pn.x, W2 = pn.y * W1, W3 = pn.z * W1;
+  const double Un1 = W2 * ny - W3 * nx;
+  const double Un2 = (CLS == WALL) ? 0.0 : W2 * nx + W3 * ny;
+  // prim = (W1, W2 t, W3 t, 0.4 (W4 - 0.5 t (W2^2 + W3^2))), t = 1/W1
+  const double t = 1.0 / W1;
+  const double W4b = 0.4 * pnb.w;
+  const double W2b = pnb.y * t - 0.4 * pnb.w * t * W2;
+  const double W3b = pnb.z * t - 0.4 * pnb.w * t * W3;
+  const double tb = pnb.y * W2 + pnb.z * W3 - 0.2 * pnb.w * (W2 * W2 + W3 * W3);
+  const double W1b = pnb.x - tb * t * t;
+  // W2 = Un1 ny + Un2 nx, W3 = Un2 ny - Un1 nx
+  double Unb[4] = {W1b, W2b * ny - W3b * nx, W2b * nx + W3b * ny, W4b};
+  nyb += W2b * Un1 + W3b * Un2;
+  nxb += W2b * Un2 - W3b * Un1;
+  if (CLS == WALL) Unb[2] = 0.0;  // no-penetration: U(3) = 0 after the update
+  double Ub[4], Uob[4] = {0, 0, 0, 0};
+  for (int r = 0; r < 4; ++r) {
+    if (rk != 3) {
+      Ub[r] = Unb[r];
+      frb[r] += -half_euler * Unb[r];
+    } else {
+      Uob[r] = tbt * Unb[r];
+      Ub[r] = obt * Unb[r];
+      frb[r] += -0.5 * obt * Unb[r];
+    }
+  }
+  if (CLS == OUTER) {
+    cons_ubar_rev(p, nx, ny, fs, Ub, pb, nxb, nyb);
+    if (rk == 3) cons_ubar_rev(pold, nx, ny, fs, Uob, poldb, nxb, nyb);
+  } else {
+    prim_to_cons_rev(p, nx, ny, Ub, pb, nxb, nyb);
+    if (rk == 3) prim_to_cons_rev(pold, nx, ny, Uob, poldb, nxb, nyb);
+  }
+}
+
+// reverse of q_of_prim<double> (q_variables.F90:8-37)
+__host__ __device__ inline void q_of_prim_rev(const V4d& p, const double qb[4], double pb[4]) {
+  const double rho = p.x, u1 = p.y, u2 = p.z, pr = p.w;
+  const double beta = 0.5 * rho / pr;
+  const double betab = qb[0] * (2.5 / beta - (u1 * u1 + u2 * u2)) + 2.0 * u1 * qb[1] + 2.0 * u2 * qb[2] - 2.0 * qb[3];
+  pb[0] += qb[0] / rho + betab * 0.5 / pr;
+  pb[3] += -betab * beta / pr;
+  pb[1] += -2.0 * beta * u1 * qb[0] + 2.0 * beta * qb[1];
+  pb[2] += -2.0 * beta * u2 * qb[0] + 2.0 * beta * qb[2];
+}
+
+// reverse of wave_speed (state_update.F90:223-225): mod_u + 3 sqrt(p/rho)
+__host__ __device__ inline void wave_speed_rev(const V4d& p, double cb, double pb[4]) {
+  const double mod_u = sqrt(p.y * p.y + p.z * p.z);
+  if (mod_u > 0.0) {
+    pb[1] += cb * p.y / mod_u;
+    pb[2] += cb * p.z / mod_u;
+  }
+  const double z = p.w / p.x;
+  const double zb = cb * 3.0 / (2.0 * sqrt(z));
+  pb[3] += zb / p.x;
+  pb[0] -= zb * z / p.x;
+}
+
+}  // namespace mf

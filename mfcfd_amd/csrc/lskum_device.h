@@ -42,16 +42,16 @@ struct alignas(16) D2 {
 };
 
 // Scalar math per precision (ocml); FP64 follows libm semantics to within an ulp.
-__device__ inline double m_exp(double v) { return exp(v); }
-__device__ inline float m_exp(float v) { return expf(v); }
-__device__ inline double m_log(double v) { return log(v); }
-__device__ inline float m_log(float v) { return logf(v); }
-__device__ inline double m_erf(double v) { return erf(v); }
-__device__ inline float m_erf(float v) { return erff(v); }
-__device__ inline double m_sqrt(double v) { return sqrt(v); }
-__device__ inline float m_sqrt(float v) { return sqrtf(v); }
-__device__ inline double m_abs(double v) { return fabs(v); }
-__device__ inline float m_abs(float v) { return fabsf(v); }
+__host__ __device__ inline double m_exp(double v) { return exp(v); }
+__host__ __device__ inline float m_exp(float v) { return expf(v); }
+__host__ __device__ inline double m_log(double v) { return log(v); }
+__host__ __device__ inline float m_log(float v) { return logf(v); }
+__host__ __device__ inline double m_erf(double v) { return erf(v); }
+__host__ __device__ inline float m_erf(float v) { return erff(v); }
+__host__ __device__ inline double m_sqrt(double v) { return sqrt(v); }
+__host__ __device__ inline float m_sqrt(float v) { return sqrtf(v); }
+__host__ __device__ inline double m_abs(double v) { return fabs(v); }
+__host__ __device__ inline float m_abs(float v) { return fabsf(v); }
 
 template <class S> struct Consts;
 template <> struct Consts<double> {
@@ -100,7 +100,7 @@ __device__ inline S to_s(double v) { return (S)v; }
 // ---------------------------------------------------------------------------------
 // q_variables.F90:8-37 (one point)
 template <class S>
-__device__ inline V4<S> q_of_prim(const V4<S>& p) {
+__host__ __device__ inline V4<S> q_of_prim(const V4<S>& p) {
   S rho = p.x, u1 = p.y, u2 = p.z, pr = p.w;
   S beta = (S)0.5 * rho / pr;
   V4<S> q;
@@ -115,7 +115,7 @@ __device__ inline V4<S> q_of_prim(const V4<S>& p) {
 // state_update.F90:223-225: mod_u + 3 sqrt(p/rho) of a neighbour, the denominator of
 // its local time step (stored per point so func_delta gathers one value)
 template <class S>
-__device__ inline double wave_speed(const V4<S>& p) {
+__host__ __device__ inline double wave_speed(const V4<S>& p) {
   double rho = p.x, u1 = p.y, u2 = p.z, pr = p.w;
   double mod_u = sqrt(u1 * u1 + u2 * u2);
   return mod_u + 3.0 * sqrt(pr / rho);
@@ -123,7 +123,7 @@ __device__ inline double wave_speed(const V4<S>& p) {
 
 // limiters.F90:64-110 venkat_limiter, one component
 template <class S>
-__device__ inline S venkat(S qt, S qk, S qmax, S qmin, S epsi) {
+__host__ __device__ inline S venkat(S qt, S qk, S qmax, S qmin, S epsi) {
   S del_neg = qt - qk;
   if (m_abs(del_neg) <= Consts<S>::THRESH) return (S)1.0;
   S del_pos = (del_neg > (S)0) ? qmax - qk : qmin - qk;
@@ -146,7 +146,7 @@ struct Kin {
 
 // q_variables.F90:230-256 qtilde_to_primitive, then the flux preliminaries
 template <class S>
-__device__ inline Kin<S> kin_of_qtilde(const S qt[4], S nx, S ny) {
+__host__ __device__ inline Kin<S> kin_of_qtilde(const S qt[4], S nx, S ny) {
   S beta = -qt[3] * (S)0.5;
   S tmp = (S)0.5 / beta;
   S u1 = qt[1] * tmp;
@@ -177,11 +177,11 @@ __device__ inline Kin<S> kin_of_qtilde(const S qt[4], S nx, S ny) {
 // x +- y with the sign chosen per lane: bitwise equal to the reference's separate
 // "+" and "-" routines (x + (-y) == x - y in IEEE arithmetic)
 template <class S>
-__device__ inline S pm(S x, S y, bool plus) { return plus ? x + y : x - y; }
+__host__ __device__ inline S pm(S x, S y, bool plus) { return plus ? x + y : x - y; }
 
 // split_fluxes.F90:8-96 flux_Gxp (plus) / flux_Gxn (minus)
 template <class S>
-__device__ inline void flux_gx(const Kin<S>& k, bool plus, S G[4]) {
+__host__ __device__ inline void flux_gx(const Kin<S>& k, bool plus, S G[4]) {
   S A1 = (S)0.5 * pm((S)1.0, k.e1, plus);
   G[0] = k.rho * pm(k.ut * A1, k.B1, plus);
   S temp1 = k.pr_by_rho + k.ut * k.ut;
@@ -197,7 +197,7 @@ __device__ inline void flux_gx(const Kin<S>& k, bool plus, S G[4]) {
 
 // split_fluxes.F90:99-190 flux_Gyp (plus) / flux_Gyn (minus)
 template <class S>
-__device__ inline void flux_gy(const Kin<S>& k, bool plus, S G[4]) {
+__host__ __device__ inline void flux_gy(const Kin<S>& k, bool plus, S G[4]) {
   S A2 = (S)0.5 * pm((S)1.0, k.e2, plus);
   G[0] = k.rho * pm(k.un * A2, k.B2, plus);
   S temp1 = k.pr_by_rho + k.un * k.un;
@@ -214,7 +214,7 @@ __device__ inline void flux_gy(const Kin<S>& k, bool plus, S G[4]) {
 // quadrant_fluxes.F90:12-248: GxI (a1-, a2-), GxII (a1+, a2-), GxIII (a1+, a2+),
 // GxIV (a1-, a2+)
 template <class S>
-__device__ inline void flux_quad(const Kin<S>& k, bool a1p, bool a2p, S G[4]) {
+__host__ __device__ inline void flux_quad(const Kin<S>& k, bool a1p, bool a2p, S G[4]) {
   S A1 = (S)0.5 * pm((S)1.0, k.e1, a1p);
   S A2 = (S)0.5 * pm((S)1.0, k.e2, a2p);
   S t1b = pm(k.ut * A1, k.B1, a1p);
@@ -243,7 +243,7 @@ struct State {
 // Limited mid-point reconstruction of one end (wall_fluxes.F90:79-86): qtilde =
 // q - 0.5*phi*(delx*dq_x + dely*dq_y), phi from the end's own qm and epsilon.
 template <class S>
-__device__ inline void reconstruct(const V4<S>& q, const V4<S>& dx, const V4<S>& dy,
+__host__ __device__ inline void reconstruct(const V4<S>& q, const V4<S>& dx, const V4<S>& dy,
                                    const V4<S>& qmax, const V4<S>& qmin, S eps, S delx,
                                    S dely, S qt[4]) {
 #pragma unroll
@@ -257,7 +257,7 @@ __device__ inline void reconstruct(const V4<S>& q, const V4<S>& dx, const V4<S>&
 // Split fluxes of one end for one edge: the x flux of the neighbour's x half and
 // the y flux of its y half.  `xp`/`yp` pick the + or - half.
 template <int CLS, class S>
-__device__ inline void end_fluxes(const Kin<S>& k, bool xp, bool yp, S Gx[4], S Gy[4]) {
+__host__ __device__ inline void end_fluxes(const Kin<S>& k, bool xp, bool yp, S Gx[4], S Gy[4]) {
   if (CLS == INTERIOR) {
     flux_gx(k, xp, Gx);
     flux_gy(k, yp, Gy);
@@ -282,13 +282,13 @@ struct HalfCoef {
 
 // `first` selects half 0 (xpos) / 2 (ypos), else half 1 (xneg) / 3 (yneg); selects,
 // not a runtime array index, so the coefficients stay in registers
-__device__ inline double coef_x(const HalfCoef& h, bool first, double dsw, double dnw) {
+__host__ __device__ inline double coef_x(const HalfCoef& h, bool first, double dsw, double dnw) {
   const double c1 = first ? h.c1[0] : h.c1[1];
   const double sxy = first ? h.sxy[0] : h.sxy[1];
   const double obd = first ? h.obd[0] : h.obd[1];
   return (dsw * c1 - dnw * sxy) * obd;
 }
-__device__ inline double coef_y(const HalfCoef& h, bool first, double dsw, double dnw) {
+__host__ __device__ inline double coef_y(const HalfCoef& h, bool first, double dsw, double dnw) {
   const double c1 = first ? h.c1[2] : h.c1[3];
   const double sxy = first ? h.sxy[2] : h.sxy[3];
   const double obd = first ? h.obd[2] : h.obd[3];
@@ -409,7 +409,7 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
 
 // state_update.F90:144-163 primitive_to_conserved (rotated frame)
 template <class S>
-__device__ inline void prim_to_cons(const V4<S>& p, S nx, S ny, S U[4]) {
+__host__ __device__ inline void prim_to_cons(const V4<S>& p, S nx, S ny, S U[4]) {
   S rho = p.x;
   U[0] = rho;
   S temp1 = rho * p.y;
@@ -425,7 +425,7 @@ struct Freestream {
 
 // state_update.F90:239-294 conserved_vector_Ubar
 template <class S>
-__device__ inline void cons_ubar(const V4<S>& p, S nx, S ny, const Freestream& fs, S Ub[4]) {
+__host__ __device__ inline void cons_ubar(const V4<S>& p, S nx, S ny, const Freestream& fs, S Ub[4]) {
   const S rho_inf = (S)1.0, pr_inf = (S)(1.0 / 1.4), gm1 = (S)(1.4 - 1.0);
   const S pi = Consts<S>::PI;
   S u1_inf = (S)fs.u1_inf, u2_inf = (S)fs.u2_inf;
@@ -460,7 +460,7 @@ __device__ inline void cons_ubar(const V4<S>& p, S nx, S ny, const Freestream& f
 // state_update.F90:10-141 for one point of class CLS.  Returns the updated prim and
 // (U1_new - U1_old)^2 (0 for outer points, which the residual skips).
 template <int CLS, class S>
-__device__ inline V4<S> update_point(const V4<S>& p, const V4<S>& pold, const double fr[4],
+__host__ __device__ inline V4<S> update_point(const V4<S>& p, const V4<S>& pold, const double fr[4],
                                      double nxd, double nyd, int rk, double euler,
                                      const Freestream& fs, double* res_sqr) {
   const S nx = (S)nxd, ny = (S)nyd;

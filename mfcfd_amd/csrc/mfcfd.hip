@@ -394,6 +394,11 @@ struct mf_ctx {
   double *sgeo = nullptr, *cp = nullptr, *clcdcm = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_first_it = 0;
+  // kept for the adjoint: internal ids of left/right (-1 for interior points), the
+  // half-stencil sums, the shape points
+  std::vector<int32_t> left_i, right_i, h_sh_m;
+  std::vector<V4d> h_hxx, h_hyy, h_hxy;
+  struct AdjState* adj = nullptr;
 };
 
 template <class T>
@@ -542,8 +547,12 @@ void mf_default_params(mf_params* p) {
   p->reorder = 1;
 }
 
+struct AdjState;
+static void adj_release(mf_ctx* c);
+
 int mf_destroy(mf_ctx* c) {
   if (!c) return MF_OK;
+  adj_release(c);
   for (auto& gexec : c->graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
   for (void* a : c->allocs) (void)hipFree(a);
@@ -680,6 +689,9 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
   std::vector<int32_t> nbh_d(n);
   std::vector<uint32_t> conn_d((size_t)W * n, 0u);
   std::vector<V4d> lsq(n), c1(n), sxyv(n), obd(n);
+  c->h_hxx.assign(n, V4d{0, 0, 0, 0});
+  c->h_hyy.assign(n, V4d{0, 0, 0, 0});
+  c->h_hxy.assign(n, V4d{0, 0, 0, 0});
   std::vector<double> eps_d(n);
   for (int j = 0; j < n; ++j) {
     const int i = c->perm[j];
@@ -727,6 +739,9 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
       cc[d] = (d < 2) ? hyy[d] : hxx[d];
     }
     c1[j] = V4d{cc[0], cc[1], cc[2], cc[3]};
+    c->h_hxx[j] = V4d{hxx[0], hxx[1], hxx[2], hxx[3]};
+    c->h_hyy[j] = V4d{hyy[0], hyy[1], hyy[2], hyy[3]};
+    c->h_hxy[j] = V4d{hxy[0], hxy[1], hxy[2], hxy[3]};
     sxyv[j] = V4d{hxy[0], hxy[1], hxy[2], hxy[3]};
     obd[j] = V4d{ob[0], ob[1], ob[2], ob[3]};
   }
@@ -749,6 +764,16 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
     sgeo.insert(sgeo.end(), {nx[m], ny[m], ds, mx - 0.25, my});
   }
   c->nshape = (int)sh_m.size();
+  c->h_sh_m = sh_m;
+  c->left_i.assign(n, -1);
+  c->right_i.assign(n, -1);
+  for (int j = 0; j < n; ++j) {
+    const int i = c->perm[j];
+    if (geo->flag_1[i] != 1) {
+      c->left_i[j] = c->iperm[geo->left[i] - 1];
+      c->right_i[j] = c->iperm[geo->right[i] - 1];
+    }
+  }
 
   // ---- parameters: parameter.F90:16-34, compute_force_coeffs.F90:34
   c->theta = prm->aoa * kPi / 180.0;
@@ -1299,3 +1324,13 @@ int mf_layout(mf_ctx* c, mf_layout_info* info) {
 }
 
 }  // extern "C"
+
+#include "mfcfd_adjoint.hip"
+
+static void adj_release(mf_ctx* c) {
+  if (!c->adj) return;
+  if (c->adj->e0) (void)hipEventDestroy(c->adj->e0);
+  if (c->adj->e1) (void)hipEventDestroy(c->adj->e1);
+  delete c->adj;
+  c->adj = nullptr;
+}

@@ -178,3 +178,28 @@ class LskumSolver:
         li = L.MfLayout()
         L.check(L.lib().mf_layout(self.h, C.byref(li)), "mf_layout")
         return {f: getattr(li, f) for f, _ in L.MfLayout._fields_}
+
+    # ---- adjoint (src_mpi_adjoint q_lskum_chkpts_b)
+    def adjoint(self, iters: int, snaps: int = 32, first_it: int = 1):
+        """J = sum of Cd after `iters` iterations from the current state, and dJ/dx,
+        dJ/dy per point (the reference's pointb%x, pointb%y).  Returns
+        (J, dJdx, dJdy, stats)."""
+        J = C.c_double()
+        dx, dy = np.empty(self.n), np.empty(self.n)
+        st = L.MfAdjointStats()
+        L.check(L.lib().mf_adjoint(self.h, int(first_it), int(iters), int(snaps), C.byref(J),
+                                   _dp(dx), _dp(dy), C.byref(st)), "mf_adjoint")
+        return J.value, dx, dy, {f: getattr(st, f) for f, _ in L.MfAdjointStats._fields_}
+
+    def adjoint_forward(self, iters: int, snaps: int = 32, first_it: int = 1) -> float:
+        J = C.c_double()
+        L.check(L.lib().mf_adjoint_forward(self.h, int(first_it), int(iters), int(snaps),
+                                           C.byref(J)), "mf_adjoint_forward")
+        return J.value
+
+    def adjoint_backward(self):
+        dx, dy = np.empty(self.n), np.empty(self.n)
+        st = L.MfAdjointStats()
+        L.check(L.lib().mf_adjoint_backward(self.h, _dp(dx), _dp(dy), C.byref(st)),
+                "mf_adjoint_backward")
+        return dx, dy, {f: getattr(st, f) for f, _ in L.MfAdjointStats._fields_}

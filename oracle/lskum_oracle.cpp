@@ -780,4 +780,74 @@ double orc_scalar(void* h, const char* name) {
   return NAN;
 }
 
+
+// ---------------------------------------------------------------------------------
+// Tangent (forward mode) of the same iteration in dual numbers: seeds dx, dy on the
+// coordinates and returns J = sum over shapes of Cd and its directional derivative.
+// The stencil split is computed from the values (fixed), as in the reference.
+using SolverT = orc::Solver<orc::Dual>;
+
+void* orc_dual_create(int32_t n, int32_t w, const double* x, const double* y,
+                      const int32_t* left, const int32_t* right, const int32_t* flag1,
+                      const int32_t* flag2, const double* min_dist, const int32_t* nbhs,
+                      const int32_t* conn, const double* nx_file, const double* ny_file,
+                      const orc_params* prm) {
+  auto* s = new SolverT();
+  s->n = n;
+  s->W = w;
+  std::memcpy(&s->p, prm, sizeof(orc::Params));
+  s->x.assign(x, x + n);
+  s->y.assign(y, y + n);
+  s->nx.assign(n, orc::Dual(0.0));
+  s->ny.assign(n, orc::Dual(0.0));
+  if (nx_file) s->nx.assign(nx_file, nx_file + n);
+  if (ny_file) s->ny.assign(ny_file, ny_file + n);
+  s->min_dist.assign(min_dist, min_dist + n);
+  s->left.assign(left, left + n);
+  s->right.assign(right, right + n);
+  s->flag1.assign(flag1, flag1 + n);
+  s->flag2.assign(flag2, flag2 + n);
+  s->nbhs.assign(nbhs, nbhs + n);
+  s->conn.assign(conn, conn + (size_t)n * w);
+  s->build_index_lists();
+  s->allocate();
+  s->setup_case_parameters();
+  s->res_old = orc::Dual(prm->restart_res_old);
+  return s;
+}
+
+void orc_dual_seed(void* h, const double* dx, const double* dy) {
+  auto* s = static_cast<SolverT*>(h);
+  for (int i = 0; i < s->n; ++i) {
+    s->x[i].d = dx[i];
+    s->y[i].d = dy[i];
+  }
+}
+
+void orc_dual_setup(void* h) {
+  auto* s = static_cast<SolverT*>(h);
+  s->initial_conditions();
+  s->compute_normals();
+  s->generate_connectivity();
+}
+
+void orc_dual_run(void* h, int32_t first_it, int32_t iters) {
+  auto* s = static_cast<SolverT*>(h);
+  for (int32_t t = first_it; t < first_it + iters; ++t) s->fpi_solver(t);
+}
+
+void orc_dual_cd(void* h, double* J, double* dJ) {
+  auto* s = static_cast<SolverT*>(h);
+  s->compute_cl_cd_cm();
+  double v = 0.0, d = 0.0;
+  for (auto& c : s->Cd) {
+    v += c.v;
+    d += c.d;
+  }
+  *J = v;
+  *dJ = d;
+}
+
+void orc_dual_destroy(void* h) { delete static_cast<SolverT*>(h); }
+
 }  // extern "C"

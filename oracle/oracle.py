@@ -73,6 +73,13 @@ def _load():
         lib.orc_flux.argtypes = [C.c_int32] + [C.c_double] * 6 + [dp]
         lib.orc_qtilde_to_primitive.argtypes = [dp, dp]
         lib.orc_num_threads.restype = C.c_int32
+        lib.orc_dual_create.restype = vp
+        lib.orc_dual_create.argtypes = lib.orc_create.argtypes
+        lib.orc_dual_seed.argtypes = [vp, dp, dp]
+        lib.orc_dual_setup.argtypes = [vp]
+        lib.orc_dual_run.argtypes = [vp, C.c_int32, C.c_int32]
+        lib.orc_dual_cd.argtypes = [vp, dp, dp]
+        lib.orc_dual_destroy.argtypes = [vp]
         _lib = lib
     return _lib
 
@@ -178,6 +185,35 @@ class Oracle:
 
     def scalar(self, name: str) -> float:
         return _load().orc_scalar(self.h, name.encode())
+
+
+def tangent_cd(cloud, vx, vy, iters: int, first_it: int = 1, **params):
+    """Forward-mode derivative of J = sum of Cd after `iters` iterations from
+    freestream, in the direction (vx, vy) of the point coordinates.  Returns (J, dJ)."""
+    lib = _load()
+    prm = OrcParams(**default_params(**params))
+    n, w = cloud.n, cloud.max_nbhs
+    keep = [np.ascontiguousarray(a) for a in (
+        cloud.x.astype(np.float64), cloud.y.astype(np.float64),
+        cloud.left.astype(np.int32), cloud.right.astype(np.int32),
+        cloud.flag_1.astype(np.int32), cloud.flag_2.astype(np.int32),
+        cloud.min_dist.astype(np.float64), cloud.nbhs.astype(np.int32),
+        np.where(cloud.conn >= 0, cloud.conn, 0).astype(np.int32),
+        cloud.nx.astype(np.float64), cloud.ny.astype(np.float64))]
+    x, y, l, r, f1, f2, md, nb, cn, nx, ny = keep
+    h = lib.orc_dual_create(n, w, _dp(x), _dp(y), _ip(l), _ip(r), _ip(f1), _ip(f2), _dp(md),
+                            _ip(nb), _ip(cn), _dp(nx), _dp(ny), C.byref(prm))
+    try:
+        vx = np.ascontiguousarray(vx, dtype=np.float64)
+        vy = np.ascontiguousarray(vy, dtype=np.float64)
+        lib.orc_dual_seed(h, _dp(vx), _dp(vy))
+        lib.orc_dual_setup(h)
+        lib.orc_dual_run(h, first_it, iters)
+        J, dJ = C.c_double(), C.c_double()
+        lib.orc_dual_cd(h, C.byref(J), C.byref(dJ))
+        return J.value, dJ.value
+    finally:
+        lib.orc_dual_destroy(h)
 
 
 def num_threads() -> int:
