@@ -10,8 +10,14 @@ so this module generates one of the same layout:
 * geometric radial spacing starting at ``first_spacing`` chords;
 * seeded uniform jitter of +-``jitter`` x local spacing (radial and tangential
   components) on interior points only;
-* neighbours: every point within ``radius_factor`` x h_i, nearest first, capped at
-  ``max_nbhs``, where h_i = max(radial, tangential) local spacing of point i;
+* neighbours: the points within ``radius_factor`` x h_i, where h_i = max(radial,
+  tangential) local spacing of point i, that see point i through the fluid, capped at
+  ``max_nbhs``: picked round-robin over 8 angular sectors round the point, nearest
+  first within each (a plain nearest-first cap leaves the half-stencil towards the
+  next ring nearly empty where the cloud is strongly stretched), and stored nearest
+  first.  A segment that crosses the airfoil, or joins two wall points on opposite
+  sides of the thin trailing-edge wedge, is no stencil edge (a stencil reaching across
+  the body couples the upper and lower surface flows);
 * flags: wall ``flag_1=0, flag_2=1``; interior ``flag_1=1, flag_2=0``; outer ``flag_1=2``;
 * ``left``/``right`` run clockwise round both the airfoil and the far-field circle, so
   the normal rot90(right-left) of point_normals.F90:36-48 points into the fluid at the
@@ -23,6 +29,7 @@ Arrays are returned in ring-major order (index = ring * n_wall + i), which is al
 """
 from __future__ import annotations
 
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -95,10 +102,172 @@ def _wall_normals(ax: np.ndarray, ay: np.ndarray) -> tuple[np.ndarray, np.ndarra
     return -dy / nrm, dx / nrm
 
 
+class _Visibility:
+    """Which point pairs see each other through the fluid round a convex wall polygon.
+
+    A pair is hidden when its segment enters the airfoil, or when both are wall points
+    whose shorter path along the wall is more than 1.5 x their distance (a chord across
+    the body rather than along one surface).  The symmetric NACA 4-digit section is
+    convex, so its wall polygon splits at the leading and trailing edges into an upper
+    chain y_u(x) (concave) and a lower chain y_l(x) (convex); along a segment p + s (q - p)
+    the depth min(y_u(x) - y, y - y_l(x)) is concave in s, and the segment enters the
+    body exactly when its maximum (golden-section search) is positive.
+    """
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, wall: np.ndarray):
+        self.x, self.y = x, y
+        ax, ay = x[wall], y[wall]
+        elen = np.hypot(np.roll(ax, -1) - ax, np.roll(ay, -1) - ay)
+        self.arc = np.concatenate([[0.0], np.cumsum(elen)])
+        self.perim = float(self.arc[-1])
+        self.pos = np.full(x.shape[0], -1, dtype=np.int64)
+        self.pos[wall] = np.arange(wall.shape[0])
+        nw = ax.shape[0]
+        jl, jt = int(np.argmin(ax)), int(np.argmax(ax))
+        c1 = np.arange(jt, jt + ((jl - jt) % nw) + 1) % nw
+        c2 = np.arange(jl, jl + ((jt - jl) % nw) + 1) % nw
+        chains = []
+        for c in (c1, c2):
+            o = np.argsort(ax[c], kind="stable")
+            chains.append((ax[c][o], ay[c][o]))
+        if chains[0][1].mean() < chains[1][1].mean():
+            chains.reverse()
+        (self.ux, self.uy), (self.lx, self.ly) = chains
+        self.xmin, self.xmax = float(ax.min()), float(ax.max())
+        self.ymin, self.ymax = float(ay.min()), float(ay.max())
+        # lower bound on each point's distance to the wall polygon: nearest wall vertex
+        # minus half the longest edge
+        from scipy.spatial import cKDTree
+        dv, _ = cKDTree(np.column_stack([ax, ay])).query(np.column_stack([x, y]), workers=-1)
+        self.dlow = dv - 0.5 * float(elen.max())
+
+    def _depth(self, xs, ys):
+        return np.minimum(np.interp(xs, self.ux, self.uy) - ys, ys - np.interp(xs, self.lx, self.ly))
+
+    def visible(self, i: np.ndarray, k: np.ndarray) -> np.ndarray:
+        """Boolean mask over the pairs (i[j], k[j])."""
+        i = np.asarray(i, dtype=np.int64).ravel()
+        k = np.asarray(k, dtype=np.int64).ravel()
+        out = np.ones(i.shape[0], dtype=bool)
+        px, py, qx, qy = self.x[i], self.y[i], self.x[k], self.y[k]
+        pi_, pk_ = self.pos[i], self.pos[k]
+        ww = (pi_ >= 0) & (pk_ >= 0)
+        if ww.any():  # wall-wall chords across the body
+            L = np.hypot(qx[ww] - px[ww], qy[ww] - py[ww])
+            fwd = np.mod(self.arc[pk_[ww]] - self.arc[pi_[ww]], self.perim)
+            path = np.minimum(fwd, self.perim - fwd)
+            out[np.where(ww)[0][path > 1.5 * L]] = False
+        # other segments that can reach the body: every point of a segment lies within
+        # half its length of an end, so one whose ends are both farther than that from
+        # the wall stays outside; the rest also need a bounding box meeting the body's
+        half = 0.5 * np.hypot(qx - px, qy - py)
+        cand = np.where(~ww & (np.minimum(self.dlow[i], self.dlow[k]) <= half)
+                        & (np.maximum(px, qx) >= self.xmin) & (np.minimum(px, qx) <= self.xmax)
+                        & (np.maximum(py, qy) >= self.ymin) & (np.minimum(py, qy) <= self.ymax))[0]
+        if cand.size == 0:
+            return out
+        px, py, qx, qy = px[cand], py[cand], qx[cand], qy[cand]
+        dx, dy = qx - px, qy - py
+        # the part of the segment with xmin <= x <= xmax
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sa = np.where(dx != 0, (self.xmin - px) / dx, -np.inf)
+            sb = np.where(dx != 0, (self.xmax - px) / dx, np.inf)
+        s0 = np.clip(np.minimum(sa, sb), 0.0, 1.0)
+        s1 = np.clip(np.maximum(sa, sb), 0.0, 1.0)
+        vert_out = (dx == 0) & ((px < self.xmin) | (px > self.xmax))
+        g = 0.5 * (np.sqrt(5.0) - 1.0)
+        lo, hi = s0, s1
+        m1, m2 = hi - g * (hi - lo), lo + g * (hi - lo)
+        f1 = self._depth(px + m1 * dx, py + m1 * dy)
+        f2 = self._depth(px + m2 * dx, py + m2 * dy)
+        best = np.maximum(self._depth(px + s0 * dx, py + s0 * dy), self._depth(px + s1 * dx, py + s1 * dy))
+        for _ in range(60):
+            left = f1 > f2
+            best = np.maximum(best, np.maximum(f1, f2))
+            hi = np.where(left, m2, hi)
+            lo = np.where(left, lo, m1)
+            n1 = np.where(left, hi - g * (hi - lo), m2)
+            n2 = np.where(left, m1, lo + g * (hi - lo))
+            fn = self._depth(px + np.where(left, n1, n2) * dx, py + np.where(left, n1, n2) * dy)
+            f1, f2 = np.where(left, fn, f2), np.where(left, f1, fn)
+            m1, m2 = n1, n2
+        best = np.maximum(best, np.maximum(f1, f2))
+        hit = (best > 1e-12) & (s1 > s0) & ~vert_out
+        out[cand[hit]] = False
+        return out
+
+
+_SECTORS = 8
+
+
+def _select_stencils(x, y, nx, ny, hh, tree, vis, max_nbhs: int, radius_factor: float,
+                     k_cand: int = 128, chunk: int = 65536, verbose: bool = False):
+    """Neighbours of every point: the visible points within radius_factor x h_i, at most
+    max_nbhs, picked round-robin over 8 angular sectors of the point's (t, n) frame
+    (nearest of each sector first, then the second nearest, ...).  A plain nearest-first
+    cap in a strongly stretched region fills the stencil from the point's own ring and
+    leaves the half-stencil towards the next ring with a few nearly collinear points;
+    balancing by direction keeps every half-plane populated.  The chosen neighbours
+    are stored nearest first."""
+    N = x.shape[0]
+    K = min(k_cand + 1, N)
+    conn = np.full((N, max_nbhs), -1, dtype=np.int32)
+    nbhs = np.zeros(N, dtype=np.int32)
+    for a in range(0, N, chunk):
+        b = min(N, a + chunk)
+        idx = np.arange(a, b)
+        dd, ii = tree.query(np.column_stack([x[a:b], y[a:b]]), k=K, workers=-1)
+        ok = (ii != idx[:, None]) & (dd <= radius_factor * hh[a:b, None]) & np.isfinite(dd)
+        ii = np.where(ok, ii, 0)
+        dx = x[ii] - x[idx, None]
+        dy = y[ii] - y[idx, None]
+        ds = dx * ny[idx, None] - dy * nx[idx, None]
+        dn = dx * nx[idx, None] + dy * ny[idx, None]
+        sec = np.floor((np.arctan2(dn, ds) + np.pi) * (_SECTORS / (2.0 * np.pi))).astype(np.int64)
+        sec = np.minimum(sec, _SECTORS - 1)
+        col = np.arange(K)[None, :]
+        checked = np.zeros(ii.shape, dtype=bool)
+        # pick, then test the visibility of what was picked; a hidden pick is dropped
+        # and the point picks again (a few rounds: only points near the body change)
+        while True:
+            # rank of each candidate within its sector (candidates come nearest first)
+            s_ok = np.where(ok, sec, _SECTORS)
+            order = np.argsort(s_ok * K + col, axis=1, kind="stable")
+            ss = np.take_along_axis(s_ok, order, axis=1)
+            pos = np.broadcast_to(col, ss.shape)
+            start = np.where(np.concatenate([np.ones((ss.shape[0], 1), dtype=bool), ss[:, 1:] != ss[:, :-1]],
+                                            axis=1), pos, 0)
+            start = np.maximum.accumulate(start, axis=1)
+            rank = np.empty_like(order)
+            np.put_along_axis(rank, order, pos - start, axis=1)
+            key = np.where(ok, rank * K + col, np.iinfo(np.int64).max)
+            pick = np.argsort(key, axis=1, kind="stable")[:, :max_nbhs]
+            pick_ok = np.take_along_axis(ok, pick, axis=1)
+            new = pick_ok & ~np.take_along_axis(checked, pick, axis=1)
+            if not new.any():
+                break
+            rows = np.nonzero(new)[0]
+            cols = pick[new]
+            checked[rows, cols] = True
+            hidden = ~vis.visible(idx[rows], ii[rows, cols])
+            if not hidden.any():
+                break
+            ok[rows[hidden], cols[hidden]] = False
+        pick = np.sort(np.where(pick_ok, pick, K), axis=1)  # nearest first, invalid last
+        valid = pick < K
+        chosen = np.take_along_axis(ii, np.minimum(pick, K - 1), axis=1)
+        conn[a:b] = np.where(valid, chosen, -1)
+        nbhs[a:b] = valid.sum(axis=1)
+        if verbose:
+            print(f"[cloud] stencils {b}/{N}", file=sys.stderr, flush=True)
+    return conn, nbhs
+
+
 def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
                first_spacing: float = 2e-3, far_radius: float = 15.0,
                jitter: float = 0.1, seed: int = 0, max_nbhs: int = 24,
-               radius_factor: float = 2.5, smooth_passes: int | None = None) -> PointCloud:
+               radius_factor: float = 2.5, smooth_passes: int | None = None,
+               spacing: str = "max", verbose: bool = False) -> PointCloud:
     """Generate an O-type cloud round a NACA 00tt airfoil (see module docstring)."""
     from scipy.spatial import cKDTree
 
@@ -154,7 +323,7 @@ def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
     h_rad[-1] = dist(Px[-1], Py[-1], Px[-2], Py[-2])
     h_rad[1:-1] = 0.5 * (dist(Px[1:-1], Py[1:-1], Px[2:], Py[2:])
                          + dist(Px[1:-1], Py[1:-1], Px[:-2], Py[:-2]))
-    h = np.maximum(h_tan, h_rad)
+    h = {"max": np.maximum, "mean": lambda a, b: 0.5 * (a + b), "min": np.minimum}[spacing](h_tan, h_rad)
 
     if jitter > 0.0 and R > 2:
         rng = np.random.default_rng(seed)
@@ -200,22 +369,12 @@ def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
         nx[sl], ny[sl] = -vx / det, vy / det
 
     tree = cKDTree(np.column_stack([x, y]))
-    dd, ii = tree.query(np.column_stack([x, y]), k=max_nbhs + 1, workers=-1)
-    conn = np.full((N, max_nbhs), -1, dtype=np.int32)
-    nbhs = np.zeros(N, dtype=np.int32)
-    self_col = ii == np.arange(N)[:, None]
-    ok = (~self_col) & (dd <= radius_factor * hh[:, None]) & np.isfinite(dd)
-    # keep nearest-first order, drop self, cap at max_nbhs
-    order = np.argsort(~ok, axis=1, kind="stable")
-    ii_sorted = np.take_along_axis(ii, order, axis=1)
-    ok_sorted = np.take_along_axis(ok, order, axis=1)
-    nbhs[:] = ok_sorted.sum(axis=1).clip(max=max_nbhs)
-    keep = ok_sorted[:, :max_nbhs]
-    conn[:, :] = np.where(keep, ii_sorted[:, :max_nbhs], -1)
+    vis = _Visibility(x, y, np.arange(W))
+    conn, nbhs = _select_stencils(x, y, nx, ny, hh, tree, vis, max_nbhs, radius_factor, verbose=verbose)
 
     cloud = PointCloud(x=x, y=y, nx=nx, ny=ny, min_dist=np.zeros(N), left=left,
                        right=right, flag_1=flag_1, flag_2=flag_2, nbhs=nbhs, conn=conn)
-    repaired = _repair_stencils(cloud, tree, max_nbhs)
+    repaired = _repair_stencils(cloud, tree, max_nbhs, vis)
     valid = cloud.conn >= 0
     kk = np.where(valid, cloud.conn, 0)
     dist_k = np.where(valid, np.hypot(x[kk] - x[:, None], y[kk] - y[:, None]), np.inf)
@@ -226,7 +385,7 @@ def naca_ogrid(n_wall: int, n_radial: int, thickness: float = 0.12,
     meta = dict(generator="naca_ogrid", n_wall=W, n_radial=R, thickness=thickness,
                 first_spacing=first_spacing, far_radius=far_radius, jitter=jitter,
                 seed=seed, max_nbhs=max_nbhs, radius_factor=radius_factor,
-                radial_ratio=r, h_rule="max(radial, tangential) local spacing",
+                radial_ratio=r, h_rule=f"{spacing}(radial, tangential) local spacing",
                 repaired_points=int(repaired))
     cloud.meta = meta
     return cloud
@@ -257,7 +416,7 @@ def _rotated(cloud: PointCloud, i: int, idx: np.ndarray):
     return delx * tx + dely * ty, delx * cloud.nx[i] + dely * cloud.ny[i]
 
 
-def _repair_stencils(cloud: PointCloud, tree, max_nbhs: int) -> int:
+def _repair_stencils(cloud: PointCloud, tree, max_nbhs: int, vis: _Visibility | None = None) -> int:
     """Re-pick neighbours of points whose half-stencils are too small or degenerate.
 
     The nearest-first pick can leave a half-plane (e.g. behind a trailing-edge fan, or
@@ -295,6 +454,8 @@ def _repair_stencils(cloud: PointCloud, tree, max_nbhs: int) -> int:
         while True:
             _, cand = tree.query([cloud.x[i], cloud.y[i]], k=min(kq, cloud.n))
             cand = np.array([int(c) for c in np.atleast_1d(cand) if c != i and c < cloud.n])
+            if vis is not None:
+                cand = cand[vis.visible(np.full(cand.shape[0], i), cand)]
             m = _mask_row(cloud, i, cand)
             ds, dn = _rotated(cloud, i, cand)
             chosen: list[int] = []

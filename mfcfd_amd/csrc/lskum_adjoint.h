@@ -256,11 +256,22 @@ __host__ __device__ inline void endb_zero(EndB& e) {
   for (int r = 0; r < 4; ++r) e.q[r] = e.dx[r] = e.dy[r] = e.qmax[r] = e.qmin[r] = 0.0;
 }
 
-// reverse of reconstruct (wall_fluxes.F90:79-86)
-__host__ __device__ inline void reconstruct_rev(const V4d& q, const V4d& dx, const V4d& dy,
-                                                const V4d& qmax, const V4d& qmin, double eps,
-                                                double delx, double dely, const double qtb[4],
-                                                EndB& eb, double& delxb, double& delyb) {
+// Adjoints of one end's point record with the reconstruction's increment kept as one
+// number per component: d = delx dq_x + dely dq_y, so the dq adjoints are db*delx and
+// db*dely (recomputed by whoever holds delx, dely)
+struct EndD {
+  double q[4], d[4], qmax[4], qmin[4];
+};
+
+__host__ __device__ inline void endd_zero(EndD& e) {
+  for (int r = 0; r < 4; ++r) e.q[r] = e.d[r] = e.qmax[r] = e.qmin[r] = 0.0;
+}
+
+// reverse of reconstruct, keeping d's adjoint (see EndD); adds delx, dely adjoints
+__host__ __device__ inline void reconstruct_rev_d(const V4d& q, const V4d& dx, const V4d& dy,
+                                                  const V4d& qmax, const V4d& qmin, double eps,
+                                                  double delx, double dely, const double qtb[4],
+                                                  EndD& eb, double& delxb, double& delyb) {
   for (int r = 0; r < 4; ++r) {
     const double d = delx * dx[r] + dely * dy[r];
     const double qt0 = q[r] - 0.5 * d;
@@ -272,10 +283,26 @@ __host__ __device__ inline void reconstruct_rev(const V4d& q, const V4d& dx, con
     venkat_rev(qt0, q[r], qmax[r], qmin[r], eps, phib, qt0b, eb.q[r], eb.qmax[r], eb.qmin[r]);
     eb.q[r] += qt0b;
     db += -0.5 * qt0b;
-    eb.dx[r] += db * delx;
-    eb.dy[r] += db * dely;
+    eb.d[r] += db;
     delxb += db * dx[r];
     delyb += db * dy[r];
+  }
+}
+
+// reverse of reconstruct (wall_fluxes.F90:79-86)
+__host__ __device__ inline void reconstruct_rev(const V4d& q, const V4d& dx, const V4d& dy,
+                                                const V4d& qmax, const V4d& qmin, double eps,
+                                                double delx, double dely, const double qtb[4],
+                                                EndB& eb, double& delxb, double& delyb) {
+  EndD t;
+  endd_zero(t);
+  reconstruct_rev_d(q, dx, dy, qmax, qmin, eps, delx, dely, qtb, t, delxb, delyb);
+  for (int r = 0; r < 4; ++r) {
+    eb.q[r] += t.q[r];
+    eb.dx[r] += t.d[r] * delx;
+    eb.dy[r] += t.d[r] * dely;
+    eb.qmax[r] += t.qmax[r];
+    eb.qmin[r] += t.qmin[r];
   }
 }
 

@@ -8,8 +8,11 @@
 //
 // Per reversed iteration the stage intermediates are recomputed from that iteration's
 // prim (one taped forward iteration, the forward kernels writing into tape buffers),
-// then reversed stage by stage.  Contributions to a neighbour's adjoints are added with
-// hardware FP64 atomics; a point's own adjoints are summed in registers and added once.
+// then reversed stage by stage.  A point's own adjoints are summed in registers and
+// written once; contributions to a neighbour's adjoints are written per edge into a
+// slot-major edge buffer and gathered by the neighbour over the transposed
+// connectivity, so the stencil reverses use no atomics (the argmin of func_delta, the
+// force integral and the normals, a few contributions per point, still do).
 // Iterates are checkpointed with a binomial (revolve-style) schedule of prim snapshots
 // in HBM (q_lskum_chkpts_b.F90:116-279 drives Griewank's REVOLV the same way).
 #include "lskum_adjoint.h"
@@ -28,13 +31,21 @@ __device__ inline void aadd2(D2* p, double x, double y) {
   aadd(&p->y, y);
 }
 
+// Per-edge adjoint contributions to the neighbour's record, slot-major like conn:
+// field f of the edge in slot s of point i lives at eb[f * W * N + s * N + i].  The
+// flux reverse writes 18 fields (q(4), d(4), qmax(4), qmin(4), delx, dely; d being the
+// reconstruction increment delx dq_x + dely dq_y), the LSQ reverses 6 (q(4), delx, dely)
+constexpr int EF_FLUX = 18;
+
 struct AdjOut {
-  V4d* qb;      // adjoint of the stage's q
+  V4d* qb;      // adjoint of the stage's q (own contributions)
   V4d* dqb;     // adjoint of the dq the kernel reads [2N]
   V4d* qmb;     // adjoint of qm [2N]
   double* deltab;
   D2* xyb;      // adjoint of the coordinates
   D2* nrmb;     // adjoint of the normals
+  double* eb;   // edge buffer
+  int64_t ebs;  // field stride W * N
 };
 
 // Half-stencil geometry sums (sum ds^2, dn^2, ds dn per half), kept for the adjoint
@@ -44,15 +55,36 @@ struct HalfSums {
   const V4d* hxy;
 };
 
+// Transposed connectivity: for point k, the edges (i, slot s) whose neighbour is k,
+// as edge ids s*N + i and their owners i
+struct Trans {
+  const int32_t* ptr;  // [N+1]
+  const int32_t* eid;  // [E]
+  const int32_t* own;  // [E]
+};
+
+// q-tilde of one end of an edge (the point's own q when first order)
+__device__ inline void end_qtilde(const State<double>& st, int p, double delx, double dely, int first_order,
+                                  double qt[4]) {
+  const V4d q = st.q[p];
+  if (first_order) {
+    for (int r = 0; r < 4; ++r) qt[r] = q[r];
+  } else {
+    reconstruct(q, st.dq[2 * p], st.dq[2 * p + 1], st.qm[2 * p], st.qm[2 * p + 1], st.eps[p], delx, dely, qt);
+  }
+}
+
+// Reverse of one point's flux_res (wall/outer/interior_dGx_pos..neg with the edge
+// coefficients).  Per edge, two passes over the ends, each recomputing the end's
+// q-tilde and kinetic intermediates instead of keeping both ends live: the first
+// gives the adjoints of the edge's coefficients (and delta's), the second reverses
+// the end's fluxes into its record.
 template <int CLS>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<double>& st, int i,
                                double delta, const V4d& frbv, const AdjOut& o) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
-  const V4d qi = st.q[i], dxi = st.dq[2 * i], dyi = st.dq[2 * i + 1];
-  const V4d qmaxi = st.qm[2 * i], qmini = st.qm[2 * i + 1];
-  const double epsi = st.eps[i];
   HalfCoef hc;
   {
     const V4d a = g.dg_c1[i], b = g.dg_sxy[i], c = g.dg_obd[i];
@@ -65,11 +97,12 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
   const double scale = (CLS == WALL) ? 2.0 * delta : delta;
   double accb[4];
   for (int r = 0; r < 4; ++r) accb[r] = scale * frbv[r];
-  double acc[4] = {0, 0, 0, 0};
+  double dsum = 0.0;  // frb . (flux_res / scale): delta's adjoint
   HalfCoefB hcb;
   for (int d = 0; d < 4; ++d) hcb.c1[d] = hcb.sxy[d] = hcb.obd[d] = 0.0;
-  EndB own;
-  endb_zero(own);
+  EndD own;
+  endd_zero(own);
+  double odx[4] = {0, 0, 0, 0}, ody[4] = {0, 0, 0, 0};  // own dq adjoints
   double xib = 0.0, yib = 0.0, nxb = 0.0, nyb = 0.0;
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
@@ -79,105 +112,118 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     const unsigned m = pk & 15u;
     const D2 pkk = g.xy[k];
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
-    const double dels = delx * tx + dely * ty;
-    const double deln = delx * nx + dely * ny;
-    const double dsw = dels, dnw = deln;
+    const double dsw = delx * tx + dely * ty;
+    const double dnw = delx * nx + dely * ny;
     const bool xp = (m & XPOS) != 0u;
     const bool tie_x = xp && (m & XNEG) != 0u;
     const bool yp = (CLS == INTERIOR) ? (m & YPOS) != 0u : (CLS == OUTER);
     const bool tie_y = (CLS == INTERIOR) && yp && (m & YNEG) != 0u;
+    const bool tie = tie_x || tie_y;
     const double cx = coef_x(hc, xp, dsw, dnw);
     const double cy = coef_y(hc, yp, dsw, dnw);
-    const V4d qk = st.q[k], dxk = st.dq[2 * k], dyk = st.dq[2 * k + 1];
-    const V4d qmaxk = st.qm[2 * k], qmink = st.qm[2 * k + 1];
-    const double epsk = st.eps[k];
-    double qti[4], qtk[4];
-    if (g.first_order) {
+    const double cx2 = tie_x ? coef_x(hc, false, dsw, dnw) : 0.0;
+    const double cy2 = tie_y ? coef_y(hc, false, dsw, dnw) : 0.0;
+    // pass 1: adjoints of the coefficients, sum over ends of sign * accb . G
+    double cxb = 0.0, cyb = 0.0, cx2b = 0.0, cy2b = 0.0;
+#pragma unroll 1
+    for (int e = 0; e < 2; ++e) {
+      double qt[4];
+      end_qtilde(st, e ? k : i, delx, dely, g.first_order, qt);
+      KinF f;
+      kin_fwd(qt, nx, ny, f);
+      double Gx[4], Gy[4];
+      end_fluxes_f<CLS>(f, xp, yp, Gx, Gy);
+      const double sg = e ? 1.0 : -1.0;
+      double ax = 0.0, ay = 0.0, fd = 0.0;
       for (int r = 0; r < 4; ++r) {
-        qti[r] = qi[r];
-        qtk[r] = qk[r];
+        ax += accb[r] * Gx[r];
+        ay += accb[r] * Gy[r];
+        fd += frbv[r] * (Gx[r] * cx + Gy[r] * cy);
       }
-    } else {
-      reconstruct(qi, dxi, dyi, qmaxi, qmini, epsi, delx, dely, qti);
-      reconstruct(qk, dxk, dyk, qmaxk, qmink, epsk, delx, dely, qtk);
+      cxb += sg * ax;
+      cyb += sg * ay;
+      if (tie) {
+        double Hx[4], Hy[4];
+        end_fluxes_f<CLS>(f, false, false, Hx, Hy);
+        double hx = 0.0, hy = 0.0;
+        for (int r = 0; r < 4; ++r) {
+          hx += accb[r] * Hx[r];
+          hy += accb[r] * Hy[r];
+          fd += frbv[r] * (Hx[r] * cx2 + Hy[r] * cy2);
+        }
+        cx2b += sg * hx;
+        cy2b += sg * hy;
+      }
+      dsum += sg * fd;
     }
-    KinF fi, fk;
-    kin_fwd(qti, nx, ny, fi);
-    kin_fwd(qtk, nx, ny, fk);
-    double Gxi[4], Gyi[4], Gxk[4], Gyk[4];
-    end_fluxes_f<CLS>(fi, xp, yp, Gxi, Gyi);
-    end_fluxes_f<CLS>(fk, xp, yp, Gxk, Gyk);
-    double cxb = 0.0, cyb = 0.0, Gxb[4], Gyb[4], Gxbn[4], Gybn[4];
-    for (int r = 0; r < 4; ++r) {
-      const double dgx = Gxk[r] - Gxi[r], dgy = Gyk[r] - Gyi[r];
-      acc[r] += dgx * cx + dgy * cy;
-      cxb += accb[r] * dgx;
-      cyb += accb[r] * dgy;
-      Gxb[r] = accb[r] * cx;
-      Gyb[r] = accb[r] * cy;
-      Gxbn[r] = -Gxb[r];
-      Gybn[r] = -Gyb[r];
-    }
-    KinB kbi, kbk;
-    kinb_zero(kbi);
-    kinb_zero(kbk);
-    end_fluxes_rev<CLS>(fk, xp, yp, Gxb, Gyb, kbk);
-    end_fluxes_rev<CLS>(fi, xp, yp, Gxbn, Gybn, kbi);
     double dswb = 0.0, dnwb = 0.0;
     coef_rev(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
     coef_rev(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
-    if (tie_x || tie_y) {
-      // the neighbour is also in the xneg (yneg) half, with the opposite split
-      double Hxi[4], Hyi[4], Hxk[4], Hyk[4];
-      end_fluxes_f<CLS>(fi, false, false, Hxi, Hyi);
-      end_fluxes_f<CLS>(fk, false, false, Hxk, Hyk);
-      const double cx2 = coef_x(hc, false, dsw, dnw), cy2 = coef_y(hc, false, dsw, dnw);
-      double cx2b = 0.0, cy2b = 0.0, Hxb[4], Hyb[4], Hxbn[4], Hybn[4];
-      for (int r = 0; r < 4; ++r) {
-        const double dx2 = tie_x ? Hxk[r] - Hxi[r] : 0.0, dy2 = tie_y ? Hyk[r] - Hyi[r] : 0.0;
-        acc[r] += dx2 * cx2 + dy2 * cy2;
-        cx2b += accb[r] * dx2;
-        cy2b += accb[r] * dy2;
-        Hxb[r] = tie_x ? accb[r] * cx2 : 0.0;
-        Hyb[r] = tie_y ? accb[r] * cy2 : 0.0;
-        Hxbn[r] = -Hxb[r];
-        Hybn[r] = -Hyb[r];
-      }
-      end_fluxes_rev<CLS>(fk, false, false, Hxb, Hyb, kbk);
-      end_fluxes_rev<CLS>(fi, false, false, Hxbn, Hybn, kbi);
-      if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
-      if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
-    }
-    double qtib[4] = {0, 0, 0, 0}, qtkb[4] = {0, 0, 0, 0};
-    kin_rev(fi, kbi, nx, ny, qtib, nxb, nyb);
-    kin_rev(fk, kbk, nx, ny, qtkb, nxb, nyb);
+    if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
+    if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
     double delxb = 0.0, delyb = 0.0;
-    EndB nbr;
-    endb_zero(nbr);
-    if (g.first_order) {
+    double* ep = o.eb + (size_t)s * g.n + i;
+    // pass 2: reverse of each end's fluxes, kinetic intermediates and reconstruction
+#pragma unroll 1
+    for (int e = 1; e >= 0; --e) {
+      const int p = e ? k : i;
+      double qt[4];
+      end_qtilde(st, p, delx, dely, g.first_order, qt);
+      KinF f;
+      kin_fwd(qt, nx, ny, f);
+      const double sg = e ? 1.0 : -1.0;
+      double Gxb[4], Gyb[4];
       for (int r = 0; r < 4; ++r) {
-        own.q[r] += qtib[r];
-        nbr.q[r] += qtkb[r];
+        Gxb[r] = sg * accb[r] * cx;
+        Gyb[r] = sg * accb[r] * cy;
       }
-    } else {
-      reconstruct_rev(qi, dxi, dyi, qmaxi, qmini, epsi, delx, dely, qtib, own, delxb, delyb);
-      reconstruct_rev(qk, dxk, dyk, qmaxk, qmink, epsk, delx, dely, qtkb, nbr, delxb, delyb);
+      KinB kb;
+      kinb_zero(kb);
+      end_fluxes_rev<CLS>(f, xp, yp, Gxb, Gyb, kb);
+      if (tie) {
+        for (int r = 0; r < 4; ++r) {
+          Gxb[r] = sg * accb[r] * cx2;
+          Gyb[r] = sg * accb[r] * cy2;
+        }
+        end_fluxes_rev<CLS>(f, false, false, Gxb, Gyb, kb);
+      }
+      double qtb[4] = {0, 0, 0, 0};
+      kin_rev(f, kb, nx, ny, qtb, nxb, nyb);
+      EndD d;
+      endd_zero(d);
+      if (g.first_order) {
+        for (int r = 0; r < 4; ++r) d.q[r] = qtb[r];
+      } else {
+        reconstruct_rev_d(st.q[p], st.dq[2 * p], st.dq[2 * p + 1], st.qm[2 * p], st.qm[2 * p + 1], st.eps[p],
+                          delx, dely, qtb, d, delxb, delyb);
+      }
+      if (e) {
+        for (int r = 0; r < 4; ++r) {
+          ep[(size_t)r * o.ebs] = d.q[r];
+          ep[(size_t)(4 + r) * o.ebs] = d.d[r];
+          ep[(size_t)(8 + r) * o.ebs] = d.qmax[r];
+          ep[(size_t)(12 + r) * o.ebs] = d.qmin[r];
+        }
+      } else {
+        for (int r = 0; r < 4; ++r) {
+          own.q[r] += d.q[r];
+          own.qmax[r] += d.qmax[r];
+          own.qmin[r] += d.qmin[r];
+          // d = delx dq_x + dely dq_y of this edge: its adjoint splits on delx, dely
+          odx[r] += d.d[r] * delx;
+          ody[r] += d.d[r] * dely;
+        }
+      }
     }
-    // dels = delx ny - dely nx, deln = delx nx + dely ny
+    // dsw = delx ny - dely nx, dnw = delx nx + dely ny
     delxb += dswb * ny + dnwb * nx;
     delyb += -dswb * nx + dnwb * ny;
     nxb += -dswb * dely + dnwb * delx;
     nyb += dswb * delx + dnwb * dely;
     xib -= delxb;
     yib -= delyb;
-    aadd2(&o.xyb[k], delxb, delyb);
-    aadd4(&o.qb[k], nbr.q);
-    if (!g.first_order) {
-      aadd4(&o.dqb[2 * k], nbr.dx);
-      aadd4(&o.dqb[2 * k + 1], nbr.dy);
-      aadd4(&o.qmb[2 * k], nbr.qmax);
-      aadd4(&o.qmb[2 * k + 1], nbr.qmin);
-    }
+    ep[(size_t)16 * o.ebs] = delxb;
+    ep[(size_t)17 * o.ebs] = delyb;
   }
   // half-stencil geometry: c1 = sum dn^2 (x halves) or sum ds^2 (y halves), sxy = sum
   // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2)
@@ -217,21 +263,24 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     nyb += delsb * delx + delnb * dely;
     xib -= delxb;
     yib -= delyb;
-    aadd2(&o.xyb[k], delxb, delyb);
+    double* ep = o.eb + (size_t)s * g.n + i;
+    ep[(size_t)16 * o.ebs] += delxb;
+    ep[(size_t)17 * o.ebs] += delyb;
   }
-  aadd2(&o.xyb[i], xib, yib);
-  aadd4(&o.qb[i], own.q);
-  if (!g.first_order) {
-    aadd4(&o.dqb[2 * i], own.dx);
-    aadd4(&o.dqb[2 * i + 1], own.dy);
-    aadd4(&o.qmb[2 * i], own.qmax);
-    aadd4(&o.qmb[2 * i + 1], own.qmin);
+  // own contributions, written (q, dq, qm) or added (xy) by this thread alone; the
+  // edges' contributions to their neighbours are gathered by k_gather_flux
+  {
+    const D2 t = o.xyb[i];
+    o.xyb[i] = D2{t.x + xib, t.y + yib};
+    o.qb[i] = V4d{own.q[0], own.q[1], own.q[2], own.q[3]};
+    o.dqb[2 * i] = V4d{odx[0], odx[1], odx[2], odx[3]};
+    o.dqb[2 * i + 1] = V4d{ody[0], ody[1], ody[2], ody[3]};
+    o.qmb[2 * i] = V4d{own.qmax[0], own.qmax[1], own.qmax[2], own.qmax[3]};
+    o.qmb[2 * i + 1] = V4d{own.qmin[0], own.qmin[1], own.qmin[2], own.qmin[3]};
   }
   o.nrmb[i].x += nxb;
   o.nrmb[i].y += nyb;
-  double db = 0.0;
-  for (int r = 0; r < 4; ++r) db += frbv[r] * acc[r];
-  o.deltab[i] += (CLS == WALL) ? 2.0 * db : db;  // delta enters every stage
+  o.deltab[i] += (CLS == WALL) ? 2.0 * dsum : dsum;  // delta enters every stage
 }
 
 __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<double> st,
@@ -275,13 +324,17 @@ __global__ __launch_bounds__(BLOCK) void k_update_rev(Geom g, const V4d* __restr
   nrmb[i].y += nyb;
 }
 
-// reverse of k_inner (one inner iteration): dq_out adjoint -> dq_in, q, xy adjoints
+// reverse of k_inner (one inner iteration): dq_out adjoint -> dq_in, q, xy adjoints.
+// Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
+// adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
+// neighbour's dq adjoint being -0.5 (delx, dely) times its q adjoint (k_gather_lsq).
 __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restrict__ q,
                                                      const V4d* __restrict__ dq_in,
                                                      const V4d* __restrict__ dq_out,
                                                      const V4d* __restrict__ dqb_out,
                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ qb,
-                                                     D2* __restrict__ xyb) {
+                                                     D2* __restrict__ xyb, double* __restrict__ eb,
+                                                     int64_t ebs) {
   const int i = point_id();
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
@@ -315,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
     const V4d qk = q[k], dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
-    double qkb[4], dxkb[4], dykb[4];
+    double* ep = eb + (size_t)s * g.n + i;
     for (int r = 0; r < 4; ++r) {
       const double t1 = qi[r] - 0.5 * (dx * dxi[r] + dy * dyi[r]);
       const double t2 = qk[r] - 0.5 * (dx * dxk[r] + dy * dyk[r]);
@@ -323,32 +376,34 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
       const double Db = sqb[r] * dx + syqb[r] * dy;
       dxb += sqb[r] * D - 0.5 * Db * (dxk[r] - dxi[r]);
       dyb += syqb[r] * D - 0.5 * Db * (dyk[r] - dyi[r]);
-      qkb[r] = Db;
-      dxkb[r] = -0.5 * dx * Db;
-      dykb[r] = -0.5 * dy * Db;
+      ep[(size_t)r * ebs] = Db;
       qib[r] -= Db;
       dxib[r] += 0.5 * dx * Db;
       dyib[r] += 0.5 * dy * Db;
     }
-    aadd4(&qb[k], qkb);
-    aadd4(&dqb_in[2 * k], dxkb);
-    aadd4(&dqb_in[2 * k + 1], dykb);
-    aadd2(&xyb[k], dxb, dyb);
+    ep[(size_t)4 * ebs] = dxb;
+    ep[(size_t)5 * ebs] = dyb;
     xib -= dxb;
     yib -= dyb;
   }
-  aadd4(&qb[i], qib);
-  aadd4(&dqb_in[2 * i], dxib);
-  aadd4(&dqb_in[2 * i + 1], dyib);
-  aadd2(&xyb[i], xib, yib);
+  const V4d a = qb[i];
+  qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
+  dqb_in[2 * i] = V4d{dxib[0], dxib[1], dxib[2], dxib[3]};
+  dqb_in[2 * i + 1] = V4d{dyib[0], dyib[1], dyib[2], dyib[3]};
+  const D2 t = xyb[i];
+  xyb[i] = D2{t.x + xib, t.y + yib};
 }
 
-// reverse of k_lsq (eval_q_derivatives): dq and qm adjoints -> q, xy adjoints
+// reverse of k_lsq (eval_q_derivatives): dq and qm adjoints -> q, xy adjoints.  Own
+// contributions added to qb, xyb; per edge, the neighbour's q adjoint (with the qm
+// adjoint of the components it maximises or minimises) and the edge's (delx, dely)
+// adjoint go to the edge buffer (fields 0-5, k_gather_lsq)
 __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict__ q,
                                                    const V4d* __restrict__ dq,
                                                    const V4d* __restrict__ dqb,
                                                    const V4d* __restrict__ qmb, V4d* __restrict__ qb,
-                                                   D2* __restrict__ xyb) {
+                                                   D2* __restrict__ xyb, double* __restrict__ eb,
+                                                   int64_t ebs) {
   const int i = point_id();
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
@@ -375,9 +430,9 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   const V4d qi = q[i];
   const V4d mxb = qmb[2 * i], mnb = qmb[2 * i + 1];
   // qm (q_variables.F90:68-82): the max/min over {i} + stencil, strict > and <, the
-  // adjoint going to the selected point
+  // adjoint going to the selected point (slot -1: the point itself)
   V4d qmax = qi, qmin = qi;
-  int amax[4] = {i, i, i, i}, amin[4] = {i, i, i, i};
+  int smax[4] = {-1, -1, -1, -1}, smin[4] = {-1, -1, -1, -1};
   double qib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
   const int nb = g.nbhs[i];
@@ -387,36 +442,126 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
     const V4d qk = q[k];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
-    double qkb[4];
+    double* ep = eb + (size_t)s * g.n + i;
     for (int r = 0; r < 4; ++r) {
       if (qk[r] > qmax[r]) {
         qmax[r] = qk[r];
-        amax[r] = k;
+        smax[r] = s;
       }
       if (qk[r] < qmin[r]) {
         qmin[r] = qk[r];
-        amin[r] = k;
+        smin[r] = s;
       }
       const double D = qk[r] - qi[r];
       const double Db = sqb[r] * dx + syqb[r] * dy;
       dxb += sqb[r] * D;
       dyb += syqb[r] * D;
-      qkb[r] = Db;
+      ep[(size_t)r * ebs] = Db;
       qib[r] -= Db;
     }
-    aadd4(&qb[k], qkb);
-    aadd2(&xyb[k], dxb, dyb);
+    ep[(size_t)4 * ebs] = dxb;
+    ep[(size_t)5 * ebs] = dyb;
     xib -= dxb;
     yib -= dyb;
   }
   for (int r = 0; r < 4; ++r) {
-    if (amax[r] == i) qib[r] += mxb[r];
-    else aadd(&qb[amax[r]].x + r, mxb[r]);
-    if (amin[r] == i) qib[r] += mnb[r];
-    else aadd(&qb[amin[r]].x + r, mnb[r]);
+    if (smax[r] < 0) qib[r] += mxb[r];
+    else eb[(size_t)r * ebs + (size_t)smax[r] * g.n + i] += mxb[r];
+    if (smin[r] < 0) qib[r] += mnb[r];
+    else eb[(size_t)r * ebs + (size_t)smin[r] * g.n + i] += mnb[r];
   }
-  aadd4(&qb[i], qib);
-  aadd2(&xyb[i], xib, yib);
+  const V4d a = qb[i];
+  qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
+  const D2 t = xyb[i];
+  xyb[i] = D2{t.x + xib, t.y + yib};
+}
+
+// Gathers of the edge buffer into the neighbour's adjoints, over the transposed
+// connectivity (each point reads the edges that name it; no atomics)
+__global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const double* __restrict__ eb,
+                                                       int64_t ebs, V4d* __restrict__ qb,
+                                                       V4d* __restrict__ dqb, V4d* __restrict__ qmb,
+                                                       D2* __restrict__ xyb) {
+  const int k = point_id();
+  if (k >= g.n) return;
+  const D2 pk = g.xy[k];
+  double aq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
+  double amx[4] = {0, 0, 0, 0}, amn[4] = {0, 0, 0, 0};
+  double xb = 0.0, yb = 0.0;
+  const int j1 = tr.ptr[k + 1];
+  for (int j = tr.ptr[k]; j < j1; ++j) {
+    const size_t e = (size_t)tr.eid[j];
+    const double* ep = eb + e;
+    for (int r = 0; r < 4; ++r) aq[r] += ep[(size_t)r * ebs];
+    xb += ep[(size_t)16 * ebs];
+    yb += ep[(size_t)17 * ebs];
+    if (!g.first_order) {
+      const D2 pi = g.xy[tr.own[j]];
+      const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+      for (int r = 0; r < 4; ++r) {
+        const double d = ep[(size_t)(4 + r) * ebs];
+        ax[r] += d * delx;
+        ay[r] += d * dely;
+        amx[r] += ep[(size_t)(8 + r) * ebs];
+        amn[r] += ep[(size_t)(12 + r) * ebs];
+      }
+    }
+  }
+  V4d t = qb[k];
+  qb[k] = V4d{t.x + aq[0], t.y + aq[1], t.z + aq[2], t.w + aq[3]};
+  if (!g.first_order) {
+    t = dqb[2 * k];
+    dqb[2 * k] = V4d{t.x + ax[0], t.y + ax[1], t.z + ax[2], t.w + ax[3]};
+    t = dqb[2 * k + 1];
+    dqb[2 * k + 1] = V4d{t.x + ay[0], t.y + ay[1], t.z + ay[2], t.w + ay[3]};
+    t = qmb[2 * k];
+    qmb[2 * k] = V4d{t.x + amx[0], t.y + amx[1], t.z + amx[2], t.w + amx[3]};
+    t = qmb[2 * k + 1];
+    qmb[2 * k + 1] = V4d{t.x + amn[0], t.y + amn[1], t.z + amn[2], t.w + amn[3]};
+  }
+  const D2 p = xyb[k];
+  xyb[k] = D2{p.x + xb, p.y + yb};
+}
+
+// with_dq: the inner-loop payload (dq adjoint = -0.5 (delx, dely) q adjoint)
+template <bool WITH_DQ>
+__global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const double* __restrict__ eb,
+                                                      int64_t ebs, V4d* __restrict__ qb,
+                                                      V4d* __restrict__ dqb, D2* __restrict__ xyb) {
+  const int k = point_id();
+  if (k >= g.n) return;
+  const D2 pk = g.xy[k];
+  double aq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
+  double xb = 0.0, yb = 0.0;
+  const int j1 = tr.ptr[k + 1];
+  for (int j = tr.ptr[k]; j < j1; ++j) {
+    const double* ep = eb + (size_t)tr.eid[j];
+    double v[4];
+    for (int r = 0; r < 4; ++r) {
+      v[r] = ep[(size_t)r * ebs];
+      aq[r] += v[r];
+    }
+    xb += ep[(size_t)4 * ebs];
+    yb += ep[(size_t)5 * ebs];
+    if (WITH_DQ) {
+      const D2 pi = g.xy[tr.own[j]];
+      const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+      for (int r = 0; r < 4; ++r) {
+        ax[r] += -0.5 * delx * v[r];
+        ay[r] += -0.5 * dely * v[r];
+      }
+    }
+  }
+  V4d t = qb[k];
+  qb[k] = V4d{t.x + aq[0], t.y + aq[1], t.z + aq[2], t.w + aq[3]};
+  if (WITH_DQ) {
+    t = dqb[2 * k];
+    dqb[2 * k] = V4d{t.x + ax[0], t.y + ax[1], t.z + ax[2], t.w + ax[3]};
+    t = dqb[2 * k + 1];
+    dqb[2 * k + 1] = V4d{t.x + ay[0], t.y + ay[1], t.z + ay[2], t.w + ay[3]};
+  }
+  const D2 p = xyb[k];
+  xyb[k] = D2{p.x + xb, p.y + yb};
 }
 
 // reverse of eval_q_variables: pb += dq/dprim^T qb (own point)
@@ -566,6 +711,10 @@ struct AdjState {
   int32_t *sh_l = nullptr, *sh_r = nullptr;
   int32_t *bd_m = nullptr, *bd_l = nullptr, *bd_r = nullptr;
   int nbd = 0;
+  // edge buffer (EF_FLUX fields of W*N doubles) and the transposed connectivity
+  double* eb = nullptr;
+  int64_t ebs = 0;
+  int32_t *t_ptr = nullptr, *t_eid = nullptr, *t_own = nullptr;
   // checkpoints
   std::vector<V4d*> snap;       // prim snapshots
   std::vector<int> snap_t;      // iteration index held by each slot (-1 free)
@@ -617,30 +766,35 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
   int rc;
   if ((rc = adj_zero(c, A.poldb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.deltab, n))) return rc;
   HalfSums hs{A.hxx, A.hyy, A.hxy};
+  const Trans tr{A.t_ptr, A.t_eid, A.t_own};
   for (int s = A.S - 1; s >= 0; --s) {
     // update: Pb (adjoint of prim_{s+1}) -> Pstage (prim_s), poldb, frb, nrmb
     hipLaunchKernelGGL(k_update_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tprim[s], (const V4d*)A.tpold,
                        (const V4d*)A.tprim[s + 1], (const V4d*)A.Pb, A.Pstage, A.poldb, A.frb, A.nrmb, s + 1,
                        c->p.euler, c->fs);
-    if ((rc = adj_zero(c, A.qb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.qmb, 8 * (int64_t)n)) ||
-        (rc = adj_zero(c, A.dqb[A.I & 1], 8 * (int64_t)n)))
-      return rc;
     State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
-    AdjOut o{A.qb, A.dqb[A.I & 1], A.qmb, A.deltab, A.xyb, A.nrmb};
-    // flux: frb -> q, dq, qm, delta, xy, nrm
+    V4d* dqb_last = A.dqb[A.I & 1];
+    AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.ebs};
+    // flux: frb -> q, dq, qm (written), delta, xy, nrm; neighbours' parts gathered
     hipLaunchKernelGGL(k_flux_rev, gr, bl, 0, c->stream, c->g, hs, st, (const double*)A.tdelta,
                        (const V4d*)A.frb, o);
+    hipLaunchKernelGGL(k_gather_flux, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs, A.qb,
+                       dqb_last, A.qmb, A.xyb);
     // inner iterations, last first: dq^(j) adjoint -> dq^(j-1)
     for (int j = A.I; j >= 1; --j) {
       const int cur = j & 1, prv = (j - 1) & 1;
-      if ((rc = adj_zero(c, A.dqb[prv], 8 * (int64_t)n))) return rc;
       hipLaunchKernelGGL(k_inner_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
                          (const V4d*)A.tdq[s * (A.I + 1) + j - 1], (const V4d*)A.tdq[s * (A.I + 1) + j],
-                         (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb);
+                         (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, A.ebs);
+      hipLaunchKernelGGL(k_gather_lsq<true>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs,
+                         A.qb, A.dqb[prv], A.xyb);
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
     hipLaunchKernelGGL(k_lsq_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
-                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb);
+                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb,
+                       A.eb, A.ebs);
+    hipLaunchKernelGGL(k_gather_lsq<false>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs, A.qb,
+                       (V4d*)nullptr, A.xyb);
     // q = q(prim_s): into the stage-input adjoint
     hipLaunchKernelGGL(k_qvar_rev, dim3(nblocks(c)), bl, 0, c->stream, n, (const V4d*)A.tprim[s],
                        (const V4d*)A.qb, A.Pstage);
@@ -719,6 +873,32 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   if ((rc = upload(c, &A.bd_m, bm)) || (rc = upload(c, &A.bd_l, bl)) || (rc = upload(c, &A.bd_r, br)))
     return rc;
   if (c->nshape > 0 && ((rc = upload(c, &A.sh_l, sl)) || (rc = upload(c, &A.sh_r, sr)))) return rc;
+  // transposed connectivity: for each point, the edges (slot-major ids s*N + i) that
+  // name it, in increasing owner order
+  {
+    const int W = c->W;
+    if ((int64_t)W * n > INT32_MAX) return set_err(MF_ERR_INVALID, "edge ids exceed int32");
+    std::vector<uint32_t> hc((size_t)W * n);
+    std::vector<int32_t> hn(n);
+    HIPCHK(hipMemcpy(hc.data(), c->g.conn, hc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hn.data(), c->g.nbhs, hn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ptr(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+      for (int s = 0; s < hn[i]; ++s) ++ptr[(hc[(size_t)s * n + i] >> 4) + 1];
+    for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
+    std::vector<int32_t> eid(ptr[n]), own(ptr[n]), cur(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int s = 0; s < hn[i]; ++s) {
+        const int k = (int)(hc[(size_t)s * n + i] >> 4);
+        eid[cur[k]] = s * n + i;
+        own[cur[k]] = i;
+        ++cur[k];
+      }
+    if ((rc = upload(c, &A.t_ptr, ptr)) || (rc = upload(c, &A.t_eid, eid)) || (rc = upload(c, &A.t_own, own)))
+      return rc;
+    A.ebs = (int64_t)W * n;
+    if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * A.ebs))) return rc;
+  }
   A.snap.assign(snaps, nullptr);
   for (auto& p : A.snap) if ((rc = dev_alloc(c, &p, n))) return rc;
   A.snap_t.assign(snaps, -1);
