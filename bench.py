@@ -61,6 +61,8 @@ def run_gpu(cloud, precision: str, warmup: int, steps: int):
     s = LskumSolver(cloud, precision=precision, **p)
     print(f"[bench] {precision}: warmup {warmup}", file=sys.stderr, flush=True)
     s.q_lskum(warmup, first_it=1)
+    tune = s.autotune()
+    print(f"[bench] {precision}: lanes per point {tune}", file=sys.stderr, flush=True)
     s.sync()
     t0 = time.perf_counter()
     s.run_async(steps, first_it=warmup + 1)
@@ -72,7 +74,8 @@ def run_gpu(cloud, precision: str, warmup: int, steps: int):
     lay = s.layout()
     out = dict(seconds=dt, ms_per_step=1e3 * dt / steps,
                point_iters_per_s=cloud.n * steps / dt, cl=float(cl[0]), cd=float(cdv[0]),
-               residue=float(res[-1]) if len(res) else None, kernels=prof, layout=lay)
+               residue=float(res[-1]) if len(res) else None, kernels=prof, layout=lay,
+               tuning=tune)
     s.close()
     print(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/iter "
           f"{out['point_iters_per_s']:.3e} pt-it/s", file=sys.stderr, flush=True)
@@ -113,21 +116,28 @@ def run_adjoint(iters: int, snaps: int):
     with LskumSolver(cloud, **p) as s:
         s.q_lskum(2)
     with LskumSolver(cloud, **p) as s:
+        tune = s.autotune()
         s.sync()
         t0 = time.perf_counter()
         s.run_async(iters)
         s.sync()
         primal_s = time.perf_counter() - t0
-    with LskumSolver(cloud, **p) as s:
-        print(f"[bench] adjoint: {iters} iterations, {snaps} snapshots", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        J, dx, dy, st = s.adjoint(iters, snaps=snaps)
-        adj_s = time.perf_counter() - t0
-    return {"workload": f"NACA0012 320x125 ({cloud.n} points), 2nd order, 2 inner, M 0.8, AoA 1.25, "
-                        f"J = Cd, {iters} iterations, {snaps} snapshots",
-            "J": J, "primal_s": primal_s, "adjoint_total_s": adj_s,
-            "adjoint_over_primal": adj_s / primal_s, "stats": st,
-            "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+    out = {"workload": f"NACA0012 320x125 ({cloud.n} points), 2nd order, 2 inner, M 0.8, AoA 1.25, "
+                       f"J = Cd, {iters} iterations", "primal_s": primal_s,
+           "tuning": {k: tune[k] for k in ("lpp_lsq", "lpp_inner", "lpp_flux")}}
+    # the configured binomial schedule, then every iterate kept (iters x 32 B per
+    # point of HBM: no recomputation)
+    for key, ns in (("binomial", snaps), ("store_all", iters)):
+        with LskumSolver(cloud, **p) as s:
+            s.tuning = out["tuning"]
+            print(f"[bench] adjoint: {iters} iterations, {ns} snapshots", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            J, dx, dy, st = s.adjoint(iters, snaps=ns)
+            adj_s = time.perf_counter() - t0
+        out[key] = {"snapshots": ns, "J": J, "adjoint_total_s": adj_s,
+                    "adjoint_over_primal": adj_s / primal_s, "stats": st,
+                    "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+    return out
 
 
 COPY_PROBE_BYTES = 256 << 20

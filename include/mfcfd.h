@@ -138,6 +138,22 @@ int mf_sync(mf_ctx *ctx);
 #define MF_NUM_KERNEL_KINDS 5
 int mf_profile_kernels(mf_ctx *ctx, int32_t iters, double *mean_ms, int32_t *launches);
 
+/* Launch shape of the stencil kernels: lanes per point (1, 2, 4 or 8) of the LSQ
+ * derivative kernel (with func_delta), the inner-iteration kernel and the fused flux
+ * residual + state update.  A point's neighbours are split over that many lanes of a
+ * wavefront and the partial sums combined with shuffles; 1 (the default) keeps the
+ * reference's summation order.  Replaces the thread-block shape option of the CUDA
+ * build (src_cuda data_structure.F90:104 blockx/blocky/blockz, q_lskum.cuf:51). */
+typedef struct mf_tuning {
+  int32_t lpp_lsq, lpp_inner, lpp_flux, reserved;
+  double us_lsq, us_inner, us_flux;  /* mf_autotune: mean launch time of the chosen shape */
+} mf_tuning;
+/* Times every shape on the context's current state (`reps` launches each, default 10),
+ * keeps the fastest of each kernel and reports them.  Leaves prim untouched. */
+int mf_autotune(mf_ctx *ctx, int32_t reps, mf_tuning *out);
+int mf_get_tuning(mf_ctx *ctx, mf_tuning *out);
+int mf_set_tuning(mf_ctx *ctx, const mf_tuning *t);
+
 /* Device layout facts for bandwidth accounting: bytes of each device array class. */
 typedef struct mf_layout_info {
   int64_t n_points, n_wall, n_outer, n_interior, n_shape;

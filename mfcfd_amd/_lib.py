@@ -56,6 +56,12 @@ class MfAdjointStats(C.Structure):
                 ("primal_ms", C.c_double), ("adjoint_ms", C.c_double)]
 
 
+class MfTuning(C.Structure):
+    _fields_ = [("lpp_lsq", C.c_int32), ("lpp_inner", C.c_int32), ("lpp_flux", C.c_int32),
+                ("reserved", C.c_int32), ("us_lsq", C.c_double), ("us_inner", C.c_double),
+                ("us_flux", C.c_double)]
+
+
 # (name, restype, argtypes) of every symbol include/mfcfd.h declares
 _VP, _DP, _IP = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
 SIGNATURES = [
@@ -80,6 +86,9 @@ SIGNATURES = [
     ("mf_sync", C.c_int, [_VP]),
     ("mf_profile_kernels", C.c_int, [_VP, C.c_int32, _DP, _IP]),
     ("mf_layout", C.c_int, [_VP, C.POINTER(MfLayout)]),
+    ("mf_autotune", C.c_int, [_VP, C.c_int32, C.POINTER(MfTuning)]),
+    ("mf_get_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
+    ("mf_set_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
     ("mf_copy_probe", C.c_int, [C.c_int64, C.c_int32, _DP]),
     ("mf_adjoint_forward", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _DP]),
     ("mf_adjoint_backward", C.c_int, [_VP, _DP, _DP, C.POINTER(MfAdjointStats)]),

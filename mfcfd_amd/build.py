@@ -19,7 +19,7 @@ ARCH = os.environ.get("MFCFD_OFFLOAD_ARCH", "gfx950")
 # contraction), so its results can be compared with the FP64 oracle to ~1e-13.
 # Packed FP32 instructions are disabled for the whole device build.
 HIPCC_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++20", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -94,6 +94,35 @@ __device__ inline int xcd_block(int b, int nb) {
 
 __device__ inline int point_id() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
 
+// Lanes per point: a point's stencil loop can be split over LPP consecutive lanes of a
+// wave (edge s goes to lane s mod LPP) and the partial sums combined with shuffles.
+// More lanes per point give more, shorter waves, which fills the chip on small clouds
+// and shortens the tail of a launch; LPP = 1 keeps the reference's summation order.
+template <int LPP>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int off = LPP / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+template <int LPP, class T>
+__device__ inline T group_max(T v) {
+#pragma unroll
+  for (int off = LPP / 2; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
+template <int LPP, class T>
+__device__ inline T group_min(T v) {
+#pragma unroll
+  for (int off = LPP / 2; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  return v;
+}
+
 template <class S>
 __device__ inline S to_s(double v) { return (S)v; }
 
@@ -298,10 +327,11 @@ __host__ __device__ inline double coef_y(const HalfCoef& h, bool first, double d
 // Flux residual of point i (flux_residual.F90:14-73) for class CLS: the sum over the
 // point's half-stencils of the least-squares derivatives of the kinetic split fluxes
 // (wall_fluxes.F90:20-292, outer_fluxes.F90:18-300, interior_fluxes.F90:19-404).
-// Returns flux_res(:, i) in fr.
-template <int CLS, class S, bool POWER0>
+// Returns flux_res(:, i) in fr; with LPP > 1, this lane's share (edges lane, lane +
+// LPP, ...), to be summed over the point's lanes.
+template <int CLS, class S, bool POWER0, int LPP = 1>
 __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, double delta,
-                                  double power, double fr[4]) {
+                                  double power, double fr[4], int lane = 0) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -323,7 +353,7 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const uint32_t pk = cp[(size_t)s * g.n];
     const int k = (int)(pk >> 4);
     const unsigned m = pk & 15u;

@@ -38,15 +38,16 @@ __global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __res
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
 // (state_update.F90:190-236), which gathers over the same stencil.
-template <class S, bool POWER0, bool DELTA>
+template <class S, bool POWER0, bool DELTA, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__ q,
                                                V4<S>* __restrict__ dq, V4<S>* __restrict__ qm,
                                                const double* __restrict__ cspeed,
                                                double* __restrict__ delta, double cfl,
                                                double power, const Ctl* __restrict__ ctl) {
   if (ctl && ctl->halt) return;
-  const int i = point_id();
-  if (i >= g.n) return;
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
+  if (i >= g.n) return;  // whole groups: n * LPP threads are a multiple of LPP
   const D2 pi = g.xy[i];
   const V4<S> qi = q[i];
   V4<S> qmax = qi, qmin = qi;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   double min_delt = 1.0;
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const int k = (int)(cp[(size_t)s * g.n] >> 4);
     const V4<S> qk = q[k];
 #pragma unroll
@@ -82,6 +83,17 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
       syq[r] = syq[r] + wy * d;
     }
   }
+  if (LPP > 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sxq[r] = group_sum<LPP>(sxq[r]);
+      syq[r] = group_sum<LPP>(syq[r]);
+      qmax[r] = group_max<LPP>(qmax[r]);
+      qmin[r] = group_min<LPP>(qmin[r]);
+    }
+    if (DELTA) min_delt = group_min<LPP>(min_delt);
+    if (lane != 0) return;
+  }
   const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
   V4<S> ox, oy;
 #pragma unroll
@@ -98,20 +110,21 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
 
 // q_variables.F90:115-227 eval_q_inner_loop + eval_update_innerloop (Jacobi: reads
 // dq_in, writes dq_out)
-template <class S, bool POWER0>
+template <class S, bool POWER0, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict__ q,
                                                  const V4<S>* __restrict__ dq_in,
                                                  V4<S>* __restrict__ dq_out, double power,
                                                  const Ctl* __restrict__ ctl) {
   if (ctl && ctl->halt) return;
-  const int i = point_id();
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
   const V4<S> qi = q[i], dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const int k = (int)(cp[(size_t)s * g.n] >> 4);
     const D2 pk = g.xy[k];
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
@@ -127,6 +140,14 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
       syq[r] = syq[r] + (wy * (temp2 - temp1));
     }
   }
+  if (LPP > 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sxq[r] = group_sum<LPP>(sxq[r]);
+      syq[r] = group_sum<LPP>(syq[r]);
+    }
+    if (lane != 0) return;
+  }
   const V4d gg = g.lsq_geo[i];
   V4<S> ox, oy;
 #pragma unroll
@@ -138,12 +159,16 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   dq_out[2 * i + 1] = oy;
 }
 
-template <class S, bool POWER0>
+template <class S, bool POWER0, int LPP>
 __device__ inline void flux_dispatch(const Geom& g, const State<S>& st, int i, double delta,
-                                     double power, double fr[4]) {
-  if (i < g.nw) flux_point<WALL, S, POWER0>(g, st, i, delta, power, fr);
-  else if (i < g.nw + g.no) flux_point<OUTER, S, POWER0>(g, st, i, delta, power, fr);
-  else flux_point<INTERIOR, S, POWER0>(g, st, i, delta, power, fr);
+                                     double power, double fr[4], int lane) {
+  if (i < g.nw) flux_point<WALL, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
+  else if (i < g.nw + g.no) flux_point<OUTER, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
+  else flux_point<INTERIOR, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
+  if (LPP > 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
+  }
 }
 
 // flux_residual.F90:14-73 cal_flux_residual (unfused: stores flux_res)
@@ -154,7 +179,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux(Geom g, State<S> st,
   const int i = point_id();
   if (i >= g.n) return;
   double fr[4];
-  flux_dispatch<S, POWER0>(g, st, i, delta[i], power, fr);
+  flux_dispatch<S, POWER0, 1>(g, st, i, delta[i], power, fr, 0);
   flux_res[i] = V4d{fr[0], fr[1], fr[2], fr[3]};
 }
 
@@ -190,18 +215,24 @@ __global__ __launch_bounds__(BLOCK) void k_update(Geom g, V4<S>* __restrict__ pr
 // Reads the stage's (q, dq, qm, prim), writes the next (prim, q) buffers; stage 1 also
 // saves prim_old (fpi_solver.F90:20-22), the last stage the residual partial sums and
 // the wave speeds for the next iteration's func_delta.
-template <class S, bool POWER0>
+template <class S, bool POWER0, int LPP>
 __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     Geom g, State<S> st, const V4<S>* __restrict__ prim, V4<S>* __restrict__ prim_old,
     V4<S>* __restrict__ prim_out, V4<S>* __restrict__ q_out, double* __restrict__ cspeed,
     const double* __restrict__ delta, double* __restrict__ partial, const Ctl* __restrict__ ctl,
     int rk, int last, double euler, double power, Freestream fs) {
   if (ctl->halt) return;
-  const int i = point_id();
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
   double rs = 0.0;
-  if (i < g.n) {
-    double fr[4];
-    flux_dispatch<S, POWER0>(g, st, i, delta[i], power, fr);
+  double fr[4] = {0.0, 0.0, 0.0, 0.0};
+  // every lane reaches the group sums (no early exit before them)
+  if (i < g.n) flux_dispatch<S, POWER0, LPP>(g, st, i, delta[i], power, fr, lane);
+  else if (LPP > 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
+  }
+  if (i < g.n && lane == 0) {
     const V4<S> p = prim[i];
     if (rk == 1) prim_old[i] = p;
     const V4<S> po = (rk == 3) ? prim_old[i] : p;
@@ -211,8 +242,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     if (last) cspeed[i] = wave_speed(pn);
   }
   if (last) {
-    const double t = block_sum(rs);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    const double tot = block_sum(rs);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
   }
 }
 
@@ -399,6 +430,9 @@ struct mf_ctx {
   std::vector<int32_t> left_i, right_i, h_sh_m;
   std::vector<V4d> h_hxx, h_hyy, h_hxy;
   struct AdjState* adj = nullptr;
+  // lanes per point of the stencil kernels (1, 2, 4 or 8; mf_autotune picks them)
+  int lpp_lsq = 1, lpp_inner = 1, lpp_flux = 1;
+  Ctl* tune_ctl = nullptr;
 };
 
 template <class T>
@@ -819,7 +853,8 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
   if ((rc = dev_alloc(c, &qmb, (size_t)2 * n * v4b)) || (rc = dev_alloc(c, &pob, (size_t)n * v4b))) return rc;
   c->qm = qmb;
   c->prim_old = pob;
-  c->npartial = (n + BLOCK - 1) / BLOCK;
+  // room for the partial sums of the largest lanes-per-point grid (8 lanes)
+  c->npartial = (int)(((int64_t)n * 8 + BLOCK - 1) / BLOCK);
   if ((rc = dev_alloc(c, &c->flux_res, n)) || (rc = dev_alloc(c, &c->delta, n)) ||
       (rc = dev_alloc(c, &c->cspeed, n)) || (rc = dev_alloc(c, &c->partial, c->npartial)) ||
       (rc = dev_alloc(c, &c->scalar, 4)) || (rc = dev_alloc(c, &c->ctl, 1)))
@@ -864,52 +899,89 @@ static State<S> state_of(mf_ctx* c, int qbuf, int dqbuf) {
                   (const S*)c->eps};
 }
 
+static inline dim3 lane_blocks(const mf_ctx* c, int lpp) {
+  return dim3((unsigned)(((int64_t)c->n * lpp + BLOCK - 1) / BLOCK));
+}
+
+// LPP dispatch: f.template operator()<L>() for the runtime lanes-per-point value
+template <class F>
+static void with_lpp(int lpp, F&& f) {
+  switch (lpp) {
+    case 2: f.template operator()<2>(); break;
+    case 4: f.template operator()<4>(); break;
+    case 8: f.template operator()<8>(); break;
+    default: f.template operator()<1>(); break;
+  }
+}
+
+// eval_q_derivatives (+ func_delta) of q into dq, qm (and delta)
+template <class S>
+static void launch_lsq_on(mf_ctx* c, const V4<S>* q, V4<S>* dq, V4<S>* qm, const double* cspeed,
+                          double* delta, bool with_delta, const Ctl* ctl, int lpp) {
+  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
+  const double cfl = c->p.cfl, pw = c->p.power;
+  const bool p0 = c->power0;
+  with_lpp(lpp, [&]<int L>() {
+    if (p0 && with_delta)
+      hipLaunchKernelGGL((k_lsq<S, true, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+    else if (p0)
+      hipLaunchKernelGGL((k_lsq<S, true, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+    else if (with_delta)
+      hipLaunchKernelGGL((k_lsq<S, false, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+    else
+      hipLaunchKernelGGL((k_lsq<S, false, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+  });
+}
+
 template <class S>
 static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl) {
-  const dim3 gr(nblocks(c)), bl(BLOCK);
-  auto q = (const V4<S>*)c->q[qbuf];
-  auto dq = (V4<S>*)c->dq[0];
-  auto qm = (V4<S>*)c->qm;
-  const double cfl = c->p.cfl, pw = c->p.power;
-  if (c->power0) {
-    if (with_delta)
-      hipLaunchKernelGGL((k_lsq<S, true, true>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
-    else
-      hipLaunchKernelGGL((k_lsq<S, true, false>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
-  } else {
-    if (with_delta)
-      hipLaunchKernelGGL((k_lsq<S, false, true>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
-    else
-      hipLaunchKernelGGL((k_lsq<S, false, false>), gr, bl, 0, c->stream, c->g, q, dq, qm, c->cspeed, c->delta, cfl, pw, ctl);
-  }
+  launch_lsq_on<S>(c, (const V4<S>*)c->q[qbuf], (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, with_delta,
+                   ctl, c->lpp_lsq);
+}
+
+// one inner iteration: dq_in -> dq_out
+template <class S>
+static void launch_inner_on(mf_ctx* c, const V4<S>* q, const V4<S>* a, V4<S>* b, const Ctl* ctl, int lpp) {
+  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
+  const bool p0 = c->power0;
+  const double pw = c->p.power;
+  with_lpp(lpp, [&]<int L>() {
+    if (p0) hipLaunchKernelGGL((k_inner<S, true, L>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl);
+    else hipLaunchKernelGGL((k_inner<S, false, L>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl);
+  });
 }
 
 template <class S>
 static void launch_inner(mf_ctx* c, int qbuf, int din, const Ctl* ctl) {
-  const dim3 gr(nblocks(c)), bl(BLOCK);
-  auto q = (const V4<S>*)c->q[qbuf];
-  auto a = (const V4<S>*)c->dq[din];
-  auto b = (V4<S>*)c->dq[din ^ 1];
-  if (c->power0)
-    hipLaunchKernelGGL((k_inner<S, true>), gr, bl, 0, c->stream, c->g, q, a, b, c->p.power, ctl);
-  else
-    hipLaunchKernelGGL((k_inner<S, false>), gr, bl, 0, c->stream, c->g, q, a, b, c->p.power, ctl);
+  launch_inner_on<S>(c, (const V4<S>*)c->q[qbuf], (const V4<S>*)c->dq[din], (V4<S>*)c->dq[din ^ 1], ctl,
+                     c->lpp_inner);
+}
+
+// fused flux residual + state update (rk) + q_variables of the new state
+template <class S>
+static void launch_flux_update_on(mf_ctx* c, const State<S>& st, const V4<S>* prim, V4<S>* pold, V4<S>* pout,
+                                  V4<S>* qout, double* cspeed, const double* delta, double* partial,
+                                  const Ctl* ctl, int rk, bool last, int lpp) {
+  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
+  const bool p0 = c->power0;
+  const int lst = last ? 1 : 0;
+  const double eu = c->p.euler, pw = c->p.power;
+  const Freestream fs = c->fs;
+  with_lpp(lpp, [&]<int L>() {
+    if (p0)
+      hipLaunchKernelGGL((k_flux_update<S, true, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
+                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs);
+    else
+      hipLaunchKernelGGL((k_flux_update<S, false, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
+                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs);
+  });
 }
 
 template <class S>
 static void launch_flux_update(mf_ctx* c, int buf, int dqbuf, int rk, bool last) {
-  const dim3 gr(nblocks(c)), bl(BLOCK);
-  State<S> st = state_of<S>(c, buf, dqbuf);
-  auto prim = (const V4<S>*)c->prim[buf];
-  auto pold = (V4<S>*)c->prim_old;
-  auto pout = (V4<S>*)c->prim[buf ^ 1];
-  auto qout = (V4<S>*)c->q[buf ^ 1];
-  if (c->power0)
-    hipLaunchKernelGGL((k_flux_update<S, true>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                       c->cspeed, c->delta, c->partial, c->ctl, rk, last ? 1 : 0, c->p.euler, c->p.power, c->fs);
-  else
-    hipLaunchKernelGGL((k_flux_update<S, false>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                       c->cspeed, c->delta, c->partial, c->ctl, rk, last ? 1 : 0, c->p.euler, c->p.power, c->fs);
+  launch_flux_update_on<S>(c, state_of<S>(c, buf, dqbuf), (const V4<S>*)c->prim[buf], (V4<S>*)c->prim_old,
+                           (V4<S>*)c->prim[buf ^ 1], (V4<S>*)c->q[buf ^ 1], c->cspeed, c->delta, c->partial,
+                           c->ctl, rk, last, c->lpp_flux);
 }
 
 template <class S>
@@ -920,8 +992,9 @@ static void launch_forces(mf_ctx* c, int buf, const Ctl* ctl, double* cd_hist) {
 }
 
 static void launch_finalize(mf_ctx* c) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, c->npartial,
-                     c->n, c->p.restart, c->graph_first_it, c->ctl, c->res_hist, c->resnew_hist);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial,
+                     (int)lane_blocks(c, c->lpp_flux).x, c->n, c->p.restart, c->graph_first_it, c->ctl,
+                     c->res_hist, c->resnew_hist);
 }
 
 // Kernel-kind ids used by the profiler
@@ -1107,6 +1180,114 @@ int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launc
   return MF_OK;
 }
 
+}  // extern "C"
+
+static void drop_graphs(mf_ctx* c) {
+  for (auto& gexec : c->graph)
+    if (gexec) {
+      (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+}
+
+static bool valid_lpp(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
+
+// mean time (us) of `reps` launches of fn on the context's stream, after 2 untimed
+template <class F>
+static int time_launches(mf_ctx* c, int reps, F&& fn, double* us) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  fn();
+  fn();
+  HIPCHK(hipEventRecord(a, c->stream));
+  for (int r = 0; r < reps; ++r) fn();
+  HIPCHK(hipEventRecord(b, c->stream));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *us = 1e3 * (double)ms / reps;
+  return MF_OK;
+}
+
+// The timed launches write only scratch between iterations: dq, qm, and the next
+// buffers of prim and q (stage 2: no prim_old, wave speed or residual writes).
+template <class S>
+static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
+  int rc;
+  if ((rc = q_variables(c))) return rc;
+  if (!c->tune_ctl && (rc = dev_alloc(c, &c->tune_ctl, 1))) return rc;
+  const Ctl z{0, 0, 1.0};
+  HIPCHK(hipMemcpy(c->tune_ctl, &z, sizeof(Ctl), hipMemcpyHostToDevice));
+  const int cur = c->cur;
+  auto q = (const V4<S>*)c->q[cur];
+  const int cand[4] = {1, 2, 4, 8};
+  double best[3] = {1e300, 1e300, 1e300};
+  int pick[3] = {1, 1, 1};
+  for (int lpp : cand) {
+    double us;
+    if ((rc = time_launches(c, reps, [&] {
+           launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp);
+         }, &us)))
+      return rc;
+    if (us < best[0]) best[0] = us, pick[0] = lpp;
+  }
+  launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, 1);
+  for (int lpp : cand) {
+    double us;
+    if ((rc = time_launches(c, reps, [&] {
+           launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp);
+         }, &us)))
+      return rc;
+    if (us < best[1]) best[1] = us, pick[1] = lpp;
+  }
+  const State<S> st = state_of<S>(c, cur, 0);
+  for (int lpp : cand) {
+    double us;
+    if ((rc = time_launches(c, reps, [&] {
+           launch_flux_update_on<S>(c, st, (const V4<S>*)c->prim[cur], (V4<S>*)c->prim_old,
+                                    (V4<S>*)c->prim[cur ^ 1], (V4<S>*)c->q[cur ^ 1], c->cspeed, c->delta,
+                                    c->partial, c->tune_ctl, 2, false, lpp);
+         }, &us)))
+      return rc;
+    if (us < best[2]) best[2] = us, pick[2] = lpp;
+  }
+  c->lpp_lsq = pick[0];
+  c->lpp_inner = pick[1];
+  c->lpp_flux = pick[2];
+  drop_graphs(c);
+  if (t) *t = mf_tuning{pick[0], pick[1], pick[2], 0, best[0], best[1], best[2]};
+  return MF_OK;
+}
+
+extern "C" {
+
+int mf_autotune(mf_ctx* c, int32_t reps, mf_tuning* out) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (reps <= 0) reps = 10;
+  return c->ssz == 8 ? autotune_impl<double>(c, reps, out) : autotune_impl<float>(c, reps, out);
+}
+
+int mf_get_tuning(mf_ctx* c, mf_tuning* out) {
+  if (!c || !out) return set_err(MF_ERR_INVALID, "mf_get_tuning: null argument");
+  *out = mf_tuning{c->lpp_lsq, c->lpp_inner, c->lpp_flux, 0, 0.0, 0.0, 0.0};
+  return MF_OK;
+}
+
+int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
+  if (!c || !t) return set_err(MF_ERR_INVALID, "mf_set_tuning: null argument");
+  if (!valid_lpp(t->lpp_lsq) || !valid_lpp(t->lpp_inner) || !valid_lpp(t->lpp_flux))
+    return set_err(MF_ERR_INVALID, "lanes per point must be 1, 2, 4 or 8");
+  c->lpp_lsq = t->lpp_lsq;
+  c->lpp_inner = t->lpp_inner;
+  c->lpp_flux = t->lpp_flux;
+  drop_graphs(c);
+  return MF_OK;
+}
+
 int mf_set_prim(mf_ctx* c, const double* prim) {
   if (!c || !prim) return set_err(MF_ERR_INVALID, "mf_set_prim: null argument");
   int rc = upload_prim_original(c, prim);
@@ -1255,7 +1436,7 @@ int mf_state_update(mf_ctx* c, int32_t rk, double* sum_res_sqr) {
     hipLaunchKernelGGL(k_update<float>, gr, bl, 0, c->stream, c->g, (V4<float>*)c->prim[c->cur],
                        (const V4<float>*)c->prim_old, c->flux_res, c->partial, rk, c->p.euler, c->fs);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, c->npartial, c->scalar);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, nblocks(c), c->scalar);
   HIPCHK(hipGetLastError());
   if (sum_res_sqr) {
     HIPCHK(hipStreamSynchronize(c->stream));

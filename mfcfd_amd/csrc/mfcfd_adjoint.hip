@@ -31,11 +31,15 @@ __device__ inline void aadd2(D2* p, double x, double y) {
   aadd(&p->y, y);
 }
 
-// Per-edge adjoint contributions to the neighbour's record, slot-major like conn:
-// field f of the edge in slot s of point i lives at eb[f * W * N + s * N + i].  The
-// flux reverse writes 18 fields (q(4), d(4), qmax(4), qmin(4), delx, dely; d being the
-// reconstruction increment delx dq_x + dely dq_y), the LSQ reverses 6 (q(4), delx, dely)
+// Per-edge adjoint contributions to the neighbour's record.  Each edge's fields are
+// contiguous, and the edges are stored in the order of the transposed connectivity
+// (grouped by the neighbour that receives them), so a gather reads one contiguous
+// block per point; epos[s * N + i] is the position of the edge in slot s of point i.
+// The flux reverse writes 18 fields (q(4), d(4), qmax(4), qmin(4), delx, dely; d being
+// the reconstruction increment delx dq_x + dely dq_y), the LSQ reverses 6 (q(4), delx,
+// dely).
 constexpr int EF_FLUX = 18;
+constexpr int EF_LSQ = 6;
 
 struct AdjOut {
   V4d* qb;      // adjoint of the stage's q (own contributions)
@@ -44,8 +48,8 @@ struct AdjOut {
   double* deltab;
   D2* xyb;      // adjoint of the coordinates
   D2* nrmb;     // adjoint of the normals
-  double* eb;   // edge buffer
-  int64_t ebs;  // field stride W * N
+  double* eb;            // edge buffer
+  const int32_t* epos;   // edge positions [W * N]
 };
 
 // Half-stencil geometry sums (sum ds^2, dn^2, ds dn per half), kept for the adjoint
@@ -55,11 +59,10 @@ struct HalfSums {
   const V4d* hxy;
 };
 
-// Transposed connectivity: for point k, the edges (i, slot s) whose neighbour is k,
-// as edge ids s*N + i and their owners i
+// Transposed connectivity: the edges whose neighbour is k sit at positions
+// ptr[k] .. ptr[k+1]-1 of the edge buffer; own[j] is the point that owns edge j
 struct Trans {
   const int32_t* ptr;  // [N+1]
-  const int32_t* eid;  // [E]
   const int32_t* own;  // [E]
 };
 
@@ -75,10 +78,10 @@ __device__ inline void end_qtilde(const State<double>& st, int p, double delx, d
 }
 
 // Reverse of one point's flux_res (wall/outer/interior_dGx_pos..neg with the edge
-// coefficients).  Per edge, two passes over the ends, each recomputing the end's
-// q-tilde and kinetic intermediates instead of keeping both ends live: the first
-// gives the adjoints of the edge's coefficients (and delta's), the second reverses
-// the end's fluxes into its record.
+// coefficients).  Per edge, one end at a time (its q-tilde and kinetic intermediates
+// live only while that end is processed): the end's fluxes feed the adjoints of the
+// edge's coefficients and of delta, and the end's flux adjoints are reversed into its
+// record.
 template <int CLS>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<double>& st, int i,
                                double delta, const V4d& frbv, const AdjOut& o) {
@@ -123,69 +126,53 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     const double cy = coef_y(hc, yp, dsw, dnw);
     const double cx2 = tie_x ? coef_x(hc, false, dsw, dnw) : 0.0;
     const double cy2 = tie_y ? coef_y(hc, false, dsw, dnw) : 0.0;
-    // pass 1: adjoints of the coefficients, sum over ends of sign * accb . G
+    // per end (own, then neighbour): its q-tilde and kinetic intermediates once, the
+    // forward fluxes for the coefficients' adjoints, then the reverse of the fluxes,
+    // the intermediates and the reconstruction
     double cxb = 0.0, cyb = 0.0, cx2b = 0.0, cy2b = 0.0;
+    double delxb = 0.0, delyb = 0.0;
+    D2* ep = reinterpret_cast<D2*>(o.eb + (size_t)o.epos[(size_t)s * g.n + i] * EF_FLUX);
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
-      double qt[4];
-      end_qtilde(st, e ? k : i, delx, dely, g.first_order, qt);
-      KinF f;
-      kin_fwd(qt, nx, ny, f);
-      double Gx[4], Gy[4];
-      end_fluxes_f<CLS>(f, xp, yp, Gx, Gy);
-      const double sg = e ? 1.0 : -1.0;
-      double ax = 0.0, ay = 0.0, fd = 0.0;
-      for (int r = 0; r < 4; ++r) {
-        ax += accb[r] * Gx[r];
-        ay += accb[r] * Gy[r];
-        fd += frbv[r] * (Gx[r] * cx + Gy[r] * cy);
-      }
-      cxb += sg * ax;
-      cyb += sg * ay;
-      if (tie) {
-        double Hx[4], Hy[4];
-        end_fluxes_f<CLS>(f, false, false, Hx, Hy);
-        double hx = 0.0, hy = 0.0;
-        for (int r = 0; r < 4; ++r) {
-          hx += accb[r] * Hx[r];
-          hy += accb[r] * Hy[r];
-          fd += frbv[r] * (Hx[r] * cx2 + Hy[r] * cy2);
-        }
-        cx2b += sg * hx;
-        cy2b += sg * hy;
-      }
-      dsum += sg * fd;
-    }
-    double dswb = 0.0, dnwb = 0.0;
-    coef_rev(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
-    coef_rev(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
-    if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
-    if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
-    double delxb = 0.0, delyb = 0.0;
-    double* ep = o.eb + (size_t)s * g.n + i;
-    // pass 2: reverse of each end's fluxes, kinetic intermediates and reconstruction
-#pragma unroll 1
-    for (int e = 1; e >= 0; --e) {
       const int p = e ? k : i;
       double qt[4];
       end_qtilde(st, p, delx, dely, g.first_order, qt);
       KinF f;
       kin_fwd(qt, nx, ny, f);
       const double sg = e ? 1.0 : -1.0;
-      double Gxb[4], Gyb[4];
-      for (int r = 0; r < 4; ++r) {
-        Gxb[r] = sg * accb[r] * cx;
-        Gyb[r] = sg * accb[r] * cy;
-      }
       KinB kb;
       kinb_zero(kb);
-      end_fluxes_rev<CLS>(f, xp, yp, Gxb, Gyb, kb);
-      if (tie) {
+      {
+        double Gx[4], Gy[4];
+        end_fluxes_f<CLS>(f, xp, yp, Gx, Gy);
+        double ax = 0.0, ay = 0.0, fd = 0.0;
         for (int r = 0; r < 4; ++r) {
-          Gxb[r] = sg * accb[r] * cx2;
-          Gyb[r] = sg * accb[r] * cy2;
+          ax += accb[r] * Gx[r];
+          ay += accb[r] * Gy[r];
+          fd += frbv[r] * (Gx[r] * cx + Gy[r] * cy);
+          Gx[r] = sg * accb[r] * cx;
+          Gy[r] = sg * accb[r] * cy;
         }
-        end_fluxes_rev<CLS>(f, false, false, Gxb, Gyb, kb);
+        cxb += sg * ax;
+        cyb += sg * ay;
+        dsum += sg * fd;
+        end_fluxes_rev<CLS>(f, xp, yp, Gx, Gy, kb);
+      }
+      if (tie) {
+        double Hx[4], Hy[4];
+        end_fluxes_f<CLS>(f, false, false, Hx, Hy);
+        double hx = 0.0, hy = 0.0, fd = 0.0;
+        for (int r = 0; r < 4; ++r) {
+          hx += accb[r] * Hx[r];
+          hy += accb[r] * Hy[r];
+          fd += frbv[r] * (Hx[r] * cx2 + Hy[r] * cy2);
+          Hx[r] = sg * accb[r] * cx2;
+          Hy[r] = sg * accb[r] * cy2;
+        }
+        cx2b += sg * hx;
+        cy2b += sg * hy;
+        dsum += sg * fd;
+        end_fluxes_rev<CLS>(f, false, false, Hx, Hy, kb);
       }
       double qtb[4] = {0, 0, 0, 0};
       kin_rev(f, kb, nx, ny, qtb, nxb, nyb);
@@ -198,12 +185,14 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
                           delx, dely, qtb, d, delxb, delyb);
       }
       if (e) {
-        for (int r = 0; r < 4; ++r) {
-          ep[(size_t)r * o.ebs] = d.q[r];
-          ep[(size_t)(4 + r) * o.ebs] = d.d[r];
-          ep[(size_t)(8 + r) * o.ebs] = d.qmax[r];
-          ep[(size_t)(12 + r) * o.ebs] = d.qmin[r];
-        }
+        ep[0] = D2{d.q[0], d.q[1]};
+        ep[1] = D2{d.q[2], d.q[3]};
+        ep[2] = D2{d.d[0], d.d[1]};
+        ep[3] = D2{d.d[2], d.d[3]};
+        ep[4] = D2{d.qmax[0], d.qmax[1]};
+        ep[5] = D2{d.qmax[2], d.qmax[3]};
+        ep[6] = D2{d.qmin[0], d.qmin[1]};
+        ep[7] = D2{d.qmin[2], d.qmin[3]};
       } else {
         for (int r = 0; r < 4; ++r) {
           own.q[r] += d.q[r];
@@ -215,6 +204,11 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
         }
       }
     }
+    double dswb = 0.0, dnwb = 0.0;
+    coef_rev(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
+    coef_rev(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
+    if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
+    if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
     // dsw = delx ny - dely nx, dnw = delx nx + dely ny
     delxb += dswb * ny + dnwb * nx;
     delyb += -dswb * nx + dnwb * ny;
@@ -222,8 +216,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     nyb += dswb * delx + dnwb * dely;
     xib -= delxb;
     yib -= delyb;
-    ep[(size_t)16 * o.ebs] = delxb;
-    ep[(size_t)17 * o.ebs] = delyb;
+    ep[8] = D2{delxb, delyb};
   }
   // half-stencil geometry: c1 = sum dn^2 (x halves) or sum ds^2 (y halves), sxy = sum
   // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2)
@@ -263,9 +256,9 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     nyb += delsb * delx + delnb * dely;
     xib -= delxb;
     yib -= delyb;
-    double* ep = o.eb + (size_t)s * g.n + i;
-    ep[(size_t)16 * o.ebs] += delxb;
-    ep[(size_t)17 * o.ebs] += delyb;
+    D2* ep = reinterpret_cast<D2*>(o.eb + (size_t)o.epos[(size_t)s * g.n + i] * EF_FLUX);
+    const D2 t = ep[8];
+    ep[8] = D2{t.x + delxb, t.y + delyb};
   }
   // own contributions, written (q, dq, qm) or added (xy) by this thread alone; the
   // edges' contributions to their neighbours are gathered by k_gather_flux
@@ -334,7 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
                                                      const V4d* __restrict__ dqb_out,
                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ qb,
                                                      D2* __restrict__ xyb, double* __restrict__ eb,
-                                                     int64_t ebs) {
+                                                     const int32_t* __restrict__ epos) {
   const int i = point_id();
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
@@ -368,7 +361,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
     const V4d qk = q[k], dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
-    double* ep = eb + (size_t)s * g.n + i;
+    double Dbv[4];
     for (int r = 0; r < 4; ++r) {
       const double t1 = qi[r] - 0.5 * (dx * dxi[r] + dy * dyi[r]);
       const double t2 = qk[r] - 0.5 * (dx * dxk[r] + dy * dyk[r]);
@@ -376,13 +369,15 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
       const double Db = sqb[r] * dx + syqb[r] * dy;
       dxb += sqb[r] * D - 0.5 * Db * (dxk[r] - dxi[r]);
       dyb += syqb[r] * D - 0.5 * Db * (dyk[r] - dyi[r]);
-      ep[(size_t)r * ebs] = Db;
+      Dbv[r] = Db;
       qib[r] -= Db;
       dxib[r] += 0.5 * dx * Db;
       dyib[r] += 0.5 * dy * Db;
     }
-    ep[(size_t)4 * ebs] = dxb;
-    ep[(size_t)5 * ebs] = dyb;
+    D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
+    ep[0] = D2{Dbv[0], Dbv[1]};
+    ep[1] = D2{Dbv[2], Dbv[3]};
+    ep[2] = D2{dxb, dyb};
     xib -= dxb;
     yib -= dyb;
   }
@@ -403,7 +398,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
                                                    const V4d* __restrict__ dqb,
                                                    const V4d* __restrict__ qmb, V4d* __restrict__ qb,
                                                    D2* __restrict__ xyb, double* __restrict__ eb,
-                                                   int64_t ebs) {
+                                                   const int32_t* __restrict__ epos) {
   const int i = point_id();
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
@@ -442,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
     const V4d qk = q[k];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
-    double* ep = eb + (size_t)s * g.n + i;
+    double Dbv[4];
     for (int r = 0; r < 4; ++r) {
       if (qk[r] > qmax[r]) {
         qmax[r] = qk[r];
@@ -456,19 +451,21 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
       const double Db = sqb[r] * dx + syqb[r] * dy;
       dxb += sqb[r] * D;
       dyb += syqb[r] * D;
-      ep[(size_t)r * ebs] = Db;
+      Dbv[r] = Db;
       qib[r] -= Db;
     }
-    ep[(size_t)4 * ebs] = dxb;
-    ep[(size_t)5 * ebs] = dyb;
+    D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
+    ep[0] = D2{Dbv[0], Dbv[1]};
+    ep[1] = D2{Dbv[2], Dbv[3]};
+    ep[2] = D2{dxb, dyb};
     xib -= dxb;
     yib -= dyb;
   }
   for (int r = 0; r < 4; ++r) {
     if (smax[r] < 0) qib[r] += mxb[r];
-    else eb[(size_t)r * ebs + (size_t)smax[r] * g.n + i] += mxb[r];
+    else eb[(size_t)epos[(size_t)smax[r] * g.n + i] * EF_LSQ + r] += mxb[r];
     if (smin[r] < 0) qib[r] += mnb[r];
-    else eb[(size_t)r * ebs + (size_t)smin[r] * g.n + i] += mnb[r];
+    else eb[(size_t)epos[(size_t)smin[r] * g.n + i] * EF_LSQ + r] += mnb[r];
   }
   const V4d a = qb[i];
   qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
@@ -479,9 +476,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
 // Gathers of the edge buffer into the neighbour's adjoints, over the transposed
 // connectivity (each point reads the edges that name it; no atomics)
 __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const double* __restrict__ eb,
-                                                       int64_t ebs, V4d* __restrict__ qb,
-                                                       V4d* __restrict__ dqb, V4d* __restrict__ qmb,
-                                                       D2* __restrict__ xyb) {
+                                                       V4d* __restrict__ qb, V4d* __restrict__ dqb,
+                                                       V4d* __restrict__ qmb, D2* __restrict__ xyb) {
   const int k = point_id();
   if (k >= g.n) return;
   const D2 pk = g.xy[k];
@@ -490,21 +486,32 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
   double xb = 0.0, yb = 0.0;
   const int j1 = tr.ptr[k + 1];
   for (int j = tr.ptr[k]; j < j1; ++j) {
-    const size_t e = (size_t)tr.eid[j];
-    const double* ep = eb + e;
-    for (int r = 0; r < 4; ++r) aq[r] += ep[(size_t)r * ebs];
-    xb += ep[(size_t)16 * ebs];
-    yb += ep[(size_t)17 * ebs];
+    const D2* ep = reinterpret_cast<const D2*>(eb + (size_t)j * EF_FLUX);
+    const D2 q01 = ep[0], q23 = ep[1], w = ep[8];
+    aq[0] += q01.x;
+    aq[1] += q01.y;
+    aq[2] += q23.x;
+    aq[3] += q23.y;
+    xb += w.x;
+    yb += w.y;
     if (!g.first_order) {
       const D2 pi = g.xy[tr.own[j]];
       const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+      const D2 d01 = ep[2], d23 = ep[3];
+      const double d[4] = {d01.x, d01.y, d23.x, d23.y};
       for (int r = 0; r < 4; ++r) {
-        const double d = ep[(size_t)(4 + r) * ebs];
-        ax[r] += d * delx;
-        ay[r] += d * dely;
-        amx[r] += ep[(size_t)(8 + r) * ebs];
-        amn[r] += ep[(size_t)(12 + r) * ebs];
+        ax[r] += d[r] * delx;
+        ay[r] += d[r] * dely;
       }
+      const D2 x01 = ep[4], x23 = ep[5], n01 = ep[6], n23 = ep[7];
+      amx[0] += x01.x;
+      amx[1] += x01.y;
+      amx[2] += x23.x;
+      amx[3] += x23.y;
+      amn[0] += n01.x;
+      amn[1] += n01.y;
+      amn[2] += n23.x;
+      amn[3] += n23.y;
     }
   }
   V4d t = qb[k];
@@ -523,11 +530,11 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
   xyb[k] = D2{p.x + xb, p.y + yb};
 }
 
-// with_dq: the inner-loop payload (dq adjoint = -0.5 (delx, dely) q adjoint)
+// WITH_DQ: the inner-loop payload (dq adjoint = -0.5 (delx, dely) q adjoint)
 template <bool WITH_DQ>
 __global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const double* __restrict__ eb,
-                                                      int64_t ebs, V4d* __restrict__ qb,
-                                                      V4d* __restrict__ dqb, D2* __restrict__ xyb) {
+                                                      V4d* __restrict__ qb, V4d* __restrict__ dqb,
+                                                      D2* __restrict__ xyb) {
   const int k = point_id();
   if (k >= g.n) return;
   const D2 pk = g.xy[k];
@@ -535,14 +542,12 @@ __global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const do
   double xb = 0.0, yb = 0.0;
   const int j1 = tr.ptr[k + 1];
   for (int j = tr.ptr[k]; j < j1; ++j) {
-    const double* ep = eb + (size_t)tr.eid[j];
-    double v[4];
-    for (int r = 0; r < 4; ++r) {
-      v[r] = ep[(size_t)r * ebs];
-      aq[r] += v[r];
-    }
-    xb += ep[(size_t)4 * ebs];
-    yb += ep[(size_t)5 * ebs];
+    const D2* ep = reinterpret_cast<const D2*>(eb + (size_t)j * EF_LSQ);
+    const D2 a = ep[0], b = ep[1], w = ep[2];
+    const double v[4] = {a.x, a.y, b.x, b.y};
+    for (int r = 0; r < 4; ++r) aq[r] += v[r];
+    xb += w.x;
+    yb += w.y;
     if (WITH_DQ) {
       const D2 pi = g.xy[tr.own[j]];
       const double delx = pk.x - pi.x, dely = pk.y - pi.y;
@@ -711,10 +716,10 @@ struct AdjState {
   int32_t *sh_l = nullptr, *sh_r = nullptr;
   int32_t *bd_m = nullptr, *bd_l = nullptr, *bd_r = nullptr;
   int nbd = 0;
-  // edge buffer (EF_FLUX fields of W*N doubles) and the transposed connectivity
+  // edge buffer (EF_FLUX doubles per edge, transposed order), edge positions and the
+  // transposed connectivity
   double* eb = nullptr;
-  int64_t ebs = 0;
-  int32_t *t_ptr = nullptr, *t_eid = nullptr, *t_own = nullptr;
+  int32_t *epos = nullptr, *t_ptr = nullptr, *t_own = nullptr;
   // checkpoints
   std::vector<V4d*> snap;       // prim snapshots
   std::vector<int> snap_t;      // iteration index held by each slot (-1 free)
@@ -735,24 +740,19 @@ static int adj_zero(mf_ctx* c, void* p, int64_t n_doubles) {
 // one taped forward iteration from A.tprim[0]
 static int adj_record(mf_ctx* c, AdjState& A) {
   const int n = c->n;
-  const dim3 gr(nblocks(c)), bl(BLOCK);
+  const dim3 bl(BLOCK);
   hipLaunchKernelGGL((k_q_variables<double>), dim3(nblocks(c)), bl, 0, c->stream, n,
                      (const V4d*)A.tprim[0], A.tq[0], A.tcspeed);
   for (int s = 0; s < A.S; ++s) {
     V4d* dq0 = A.tdq[s * (A.I + 1)];
-    if (s == 0)
-      hipLaunchKernelGGL((k_lsq<double, true, true>), gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s], dq0, A.tqm[s],
-                         A.tcspeed, A.tdelta, c->p.cfl, 0.0, nullptr);
-    else
-      hipLaunchKernelGGL((k_lsq<double, true, false>), gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s], dq0, A.tqm[s],
-                         A.tcspeed, A.tdelta, c->p.cfl, 0.0, nullptr);
+    launch_lsq_on<double>(c, (const V4d*)A.tq[s], dq0, A.tqm[s], A.tcspeed, A.tdelta, s == 0, nullptr, c->lpp_lsq);
     for (int j = 1; j <= A.I; ++j)
-      hipLaunchKernelGGL((k_inner<double, true>), gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
-                         (const V4d*)A.tdq[s * (A.I + 1) + j - 1], A.tdq[s * (A.I + 1) + j], 0.0, nullptr);
+      launch_inner_on<double>(c, (const V4d*)A.tq[s], (const V4d*)A.tdq[s * (A.I + 1) + j - 1],
+                              A.tdq[s * (A.I + 1) + j], nullptr, c->lpp_inner);
     State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
-    hipLaunchKernelGGL((k_flux_update<double, true>), gr, bl, 0, c->stream, c->g, st, (const V4d*)A.tprim[s],
-                       A.tpold, A.tprim[s + 1], A.tq[s + 1], A.tcspeed, (const double*)A.tdelta, A.tpartial,
-                       (const Ctl*)A.tctl, s + 1, s + 1 == A.S ? 1 : 0, c->p.euler, 0.0, c->fs);
+    launch_flux_update_on<double>(c, st, (const V4d*)A.tprim[s], A.tpold, A.tprim[s + 1], A.tq[s + 1], A.tcspeed,
+                                  (const double*)A.tdelta, A.tpartial, (const Ctl*)A.tctl, s + 1, s + 1 == A.S,
+                                  c->lpp_flux);
   }
   HIPCHK(hipGetLastError());
   return MF_OK;
@@ -766,7 +766,7 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
   int rc;
   if ((rc = adj_zero(c, A.poldb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.deltab, n))) return rc;
   HalfSums hs{A.hxx, A.hyy, A.hxy};
-  const Trans tr{A.t_ptr, A.t_eid, A.t_own};
+  const Trans tr{A.t_ptr, A.t_own};
   for (int s = A.S - 1; s >= 0; --s) {
     // update: Pb (adjoint of prim_{s+1}) -> Pstage (prim_s), poldb, frb, nrmb
     hipLaunchKernelGGL(k_update_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tprim[s], (const V4d*)A.tpold,
@@ -774,26 +774,26 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
                        c->p.euler, c->fs);
     State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
     V4d* dqb_last = A.dqb[A.I & 1];
-    AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.ebs};
+    AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
     // flux: frb -> q, dq, qm (written), delta, xy, nrm; neighbours' parts gathered
     hipLaunchKernelGGL(k_flux_rev, gr, bl, 0, c->stream, c->g, hs, st, (const double*)A.tdelta,
                        (const V4d*)A.frb, o);
-    hipLaunchKernelGGL(k_gather_flux, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs, A.qb,
-                       dqb_last, A.qmb, A.xyb);
+    hipLaunchKernelGGL(k_gather_flux, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb, dqb_last,
+                       A.qmb, A.xyb);
     // inner iterations, last first: dq^(j) adjoint -> dq^(j-1)
     for (int j = A.I; j >= 1; --j) {
       const int cur = j & 1, prv = (j - 1) & 1;
       hipLaunchKernelGGL(k_inner_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
                          (const V4d*)A.tdq[s * (A.I + 1) + j - 1], (const V4d*)A.tdq[s * (A.I + 1) + j],
-                         (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, A.ebs);
-      hipLaunchKernelGGL(k_gather_lsq<true>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs,
-                         A.qb, A.dqb[prv], A.xyb);
+                         (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+      hipLaunchKernelGGL(k_gather_lsq<true>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb,
+                         A.dqb[prv], A.xyb);
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
     hipLaunchKernelGGL(k_lsq_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
                        (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb,
-                       A.eb, A.ebs);
-    hipLaunchKernelGGL(k_gather_lsq<false>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.ebs, A.qb,
+                       A.eb, (const int32_t*)A.epos);
+    hipLaunchKernelGGL(k_gather_lsq<false>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb,
                        (V4d*)nullptr, A.xyb);
     // q = q(prim_s): into the stage-input adjoint
     hipLaunchKernelGGL(k_qvar_rev, dim3(nblocks(c)), bl, 0, c->stream, n, (const V4d*)A.tprim[s],
@@ -873,8 +873,8 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   if ((rc = upload(c, &A.bd_m, bm)) || (rc = upload(c, &A.bd_l, bl)) || (rc = upload(c, &A.bd_r, br)))
     return rc;
   if (c->nshape > 0 && ((rc = upload(c, &A.sh_l, sl)) || (rc = upload(c, &A.sh_r, sr)))) return rc;
-  // transposed connectivity: for each point, the edges (slot-major ids s*N + i) that
-  // name it, in increasing owner order
+  // transposed connectivity: the edges that name each point, in increasing owner
+  // order, and each edge's position in that order (the edge buffer's layout)
   {
     const int W = c->W;
     if ((int64_t)W * n > INT32_MAX) return set_err(MF_ERR_INVALID, "edge ids exceed int32");
@@ -886,18 +886,17 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
     for (int i = 0; i < n; ++i)
       for (int s = 0; s < hn[i]; ++s) ++ptr[(hc[(size_t)s * n + i] >> 4) + 1];
     for (int k = 0; k < n; ++k) ptr[k + 1] += ptr[k];
-    std::vector<int32_t> eid(ptr[n]), own(ptr[n]), cur(ptr.begin(), ptr.end() - 1);
+    std::vector<int32_t> own(ptr[n]), pos((size_t)W * n, 0), cur(ptr.begin(), ptr.end() - 1);
     for (int i = 0; i < n; ++i)
       for (int s = 0; s < hn[i]; ++s) {
         const int k = (int)(hc[(size_t)s * n + i] >> 4);
-        eid[cur[k]] = s * n + i;
+        pos[(size_t)s * n + i] = cur[k];
         own[cur[k]] = i;
         ++cur[k];
       }
-    if ((rc = upload(c, &A.t_ptr, ptr)) || (rc = upload(c, &A.t_eid, eid)) || (rc = upload(c, &A.t_own, own)))
+    if ((rc = upload(c, &A.t_ptr, ptr)) || (rc = upload(c, &A.t_own, own)) || (rc = upload(c, &A.epos, pos)))
       return rc;
-    A.ebs = (int64_t)W * n;
-    if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * A.ebs))) return rc;
+    if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1)))) return rc;
   }
   A.snap.assign(snaps, nullptr);
   for (auto& p : A.snap) if ((rc = dev_alloc(c, &p, n))) return rc;

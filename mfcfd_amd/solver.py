@@ -174,6 +174,27 @@ class LskumSolver:
                 "mf_profile_kernels")
         return {k: (float(ms[j]), int(cnt[j])) for j, k in enumerate(L.KERNEL_KINDS)}
 
+    # ---- launch shapes (lanes per point of the stencil kernels)
+    def autotune(self, reps: int = 10) -> dict:
+        """Time every lanes-per-point shape on the current state and keep the fastest
+        of each stencil kernel; returns the choice and the measured launch times."""
+        t = L.MfTuning()
+        L.check(L.lib().mf_autotune(self.h, int(reps), C.byref(t)), "mf_autotune")
+        return {f: getattr(t, f) for f, _ in L.MfTuning._fields_ if f != "reserved"}
+
+    @property
+    def tuning(self) -> dict:
+        t = L.MfTuning()
+        L.check(L.lib().mf_get_tuning(self.h, C.byref(t)), "mf_get_tuning")
+        return {"lpp_lsq": t.lpp_lsq, "lpp_inner": t.lpp_inner, "lpp_flux": t.lpp_flux}
+
+    @tuning.setter
+    def tuning(self, value: dict) -> None:
+        cur = self.tuning
+        cur.update(value)
+        t = L.MfTuning(lpp_lsq=cur["lpp_lsq"], lpp_inner=cur["lpp_inner"], lpp_flux=cur["lpp_flux"])
+        L.check(L.lib().mf_set_tuning(self.h, C.byref(t)), "mf_set_tuning")
+
     def layout(self) -> dict:
         li = L.MfLayout()
         L.check(L.lib().mf_layout(self.h, C.byref(li)), "mf_layout")

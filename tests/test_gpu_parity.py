@@ -170,6 +170,43 @@ def test_reorder_invariance(small_cloud):
         assert rel_err(a.prim, b.prim) < 1e-13
 
 
+@pytest.mark.parametrize("lpp", [2, 4, 8])
+def test_lanes_per_point_shapes(small_cloud, lpp):
+    """Splitting each point's stencil over 2/4/8 lanes only reorders sums: the
+    trajectory stays within 1e-12 of the one-lane run and of the oracle's tolerance."""
+    o = _oracle(small_cloud)
+    o.run(20)
+    with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
+        b.tuning = {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": lpp}
+        assert b.tuning == {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": lpp}
+        ra, _, cda = a.q_lskum(20)
+        rb, _, cdb = b.q_lskum(20)
+        assert rel_err(b.prim, a.prim) < 1e-12
+        assert np.abs(ra - rb).max() < 1e-9
+        assert abs(cda[-1] - cdb[-1]) <= 1e-12 * abs(cda[-1])
+        assert rel_err(b.prim, o.field("prim")) < 1e-10
+
+
+def test_autotune_and_invalid_shape(small_cloud):
+    """mf_autotune picks a valid shape per kernel and leaves the state untouched;
+    mf_set_tuning rejects shapes other than 1, 2, 4, 8."""
+    from mfcfd_amd._lib import MfError
+    with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
+        a.q_lskum(5)
+        before = a.prim.copy()
+        t = a.autotune(reps=3)
+        assert all(t[k] in (1, 2, 4, 8) for k in ("lpp_lsq", "lpp_inner", "lpp_flux"))
+        assert all(t[k] > 0 for k in ("us_lsq", "us_inner", "us_flux"))
+        assert np.array_equal(a.prim, before)
+        # the tuned context continues the same trajectory as an untuned one
+        b.q_lskum(5)
+        ra, _, _ = a.q_lskum(5, first_it=6)
+        rb, _, _ = b.q_lskum(5, first_it=6)
+        assert rel_err(a.prim, b.prim) < 1e-12
+        with pytest.raises(MfError):
+            a.tuning = {"lpp_flux": 3}
+
+
 def test_nan_residue_stops_loop(small_cloud):
     """q_lskum.F90:71 exits after the first NaN residue; so does mf_run."""
     o = _oracle(small_cloud)
