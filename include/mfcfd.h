@@ -147,6 +147,9 @@ int mf_profile_kernels(mf_ctx *ctx, int32_t iters, double *mean_ms, int32_t *lau
 typedef struct mf_tuning {
   int32_t lpp_lsq, lpp_inner, lpp_flux, reserved;
   double us_lsq, us_inner, us_flux;  /* mf_autotune: mean launch time of the chosen shape */
+  /* reverse kernels of the adjoint (flux, LSQ, inner iteration, edge gathers): 0 lets
+   * the first mf_adjoint_forward time the shapes and keep the fastest */
+  int32_t lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather;
 } mf_tuning;
 /* Times every shape on the context's current state (`reps` launches each, default 10),
  * keeps the fastest of each kernel and reports them.  Leaves prim untouched. */
@@ -178,6 +181,9 @@ typedef struct mf_adjoint_stats {
   int32_t reserved;
   double primal_ms;       /* the primal sweep (mf_adjoint_forward)                   */
   double adjoint_ms;      /* the reverse sweep (mf_adjoint_backward)                 */
+  /* lanes per point of the reverse kernels (flux, LSQ, inner iteration, edge
+   * gathers), timed and chosen by the first mf_adjoint_forward on the context */
+  int32_t lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather;
 } mf_adjoint_stats;
 int mf_adjoint_forward(mf_ctx *ctx, int32_t first_it, int32_t iters, int32_t snaps, double *J);
 int mf_adjoint_backward(mf_ctx *ctx, double *dJdx, double *dJdy, mf_adjoint_stats *stats);

@@ -53,13 +53,16 @@ class MfLayout(C.Structure):
 class MfAdjointStats(C.Structure):
     _fields_ = [("forward_steps", C.c_int64), ("taped_steps", C.c_int64),
                 ("snapshots", C.c_int32), ("reserved", C.c_int32),
-                ("primal_ms", C.c_double), ("adjoint_ms", C.c_double)]
+                ("primal_ms", C.c_double), ("adjoint_ms", C.c_double),
+                ("lpp_flux_rev", C.c_int32), ("lpp_lsq_rev", C.c_int32),
+                ("lpp_inner_rev", C.c_int32), ("lpp_gather", C.c_int32)]
 
 
 class MfTuning(C.Structure):
     _fields_ = [("lpp_lsq", C.c_int32), ("lpp_inner", C.c_int32), ("lpp_flux", C.c_int32),
                 ("reserved", C.c_int32), ("us_lsq", C.c_double), ("us_inner", C.c_double),
-                ("us_flux", C.c_double)]
+                ("us_flux", C.c_double), ("lpp_flux_rev", C.c_int32), ("lpp_lsq_rev", C.c_int32),
+                ("lpp_inner_rev", C.c_int32), ("lpp_gather", C.c_int32)]
 
 
 # (name, restype, argtypes) of every symbol include/mfcfd.h declares

@@ -432,6 +432,7 @@ struct mf_ctx {
   struct AdjState* adj = nullptr;
   // lanes per point of the stencil kernels (1, 2, 4 or 8; mf_autotune picks them)
   int lpp_lsq = 1, lpp_inner = 1, lpp_flux = 1;
+  int lpp_rev[4] = {0, 0, 0, 0};  // adjoint: flux, lsq, inner, gather (0: timed)
   Ctl* tune_ctl = nullptr;
 };
 
@@ -583,6 +584,7 @@ void mf_default_params(mf_params* p) {
 
 struct AdjState;
 static void adj_release(mf_ctx* c);
+static void adj_forget_tuning(mf_ctx* c);
 
 int mf_destroy(mf_ctx* c) {
   if (!c) return MF_OK;
@@ -1259,7 +1261,9 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   c->lpp_inner = pick[1];
   c->lpp_flux = pick[2];
   drop_graphs(c);
-  if (t) *t = mf_tuning{pick[0], pick[1], pick[2], 0, best[0], best[1], best[2]};
+  if (t)
+    *t = mf_tuning{pick[0], pick[1], pick[2], 0, best[0], best[1], best[2],
+                   c->lpp_rev[0], c->lpp_rev[1], c->lpp_rev[2], c->lpp_rev[3]};
   return MF_OK;
 }
 
@@ -1273,7 +1277,8 @@ int mf_autotune(mf_ctx* c, int32_t reps, mf_tuning* out) {
 
 int mf_get_tuning(mf_ctx* c, mf_tuning* out) {
   if (!c || !out) return set_err(MF_ERR_INVALID, "mf_get_tuning: null argument");
-  *out = mf_tuning{c->lpp_lsq, c->lpp_inner, c->lpp_flux, 0, 0.0, 0.0, 0.0};
+  *out = mf_tuning{c->lpp_lsq, c->lpp_inner, c->lpp_flux, 0, 0.0, 0.0, 0.0,
+                   c->lpp_rev[0], c->lpp_rev[1], c->lpp_rev[2], c->lpp_rev[3]};
   return MF_OK;
 }
 
@@ -1281,9 +1286,15 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   if (!c || !t) return set_err(MF_ERR_INVALID, "mf_set_tuning: null argument");
   if (!valid_lpp(t->lpp_lsq) || !valid_lpp(t->lpp_inner) || !valid_lpp(t->lpp_flux))
     return set_err(MF_ERR_INVALID, "lanes per point must be 1, 2, 4 or 8");
+  const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
+  for (int k = 0; k < 4; ++k)
+    if (rv[k] != 0 && !valid_lpp(rv[k]))
+      return set_err(MF_ERR_INVALID, "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8");
   c->lpp_lsq = t->lpp_lsq;
   c->lpp_inner = t->lpp_inner;
   c->lpp_flux = t->lpp_flux;
+  for (int k = 0; k < 4; ++k) c->lpp_rev[k] = rv[k];
+  adj_forget_tuning(c);
   drop_graphs(c);
   return MF_OK;
 }

@@ -82,9 +82,9 @@ __device__ inline void end_qtilde(const State<double>& st, int p, double delx, d
 // live only while that end is processed): the end's fluxes feed the adjoints of the
 // edge's coefficients and of delta, and the end's flux adjoints are reversed into its
 // record.
-template <int CLS>
+template <int CLS, int LPP>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<double>& st, int i,
-                               double delta, const V4d& frbv, const AdjOut& o) {
+                               double delta, const V4d& frbv, const AdjOut& o, int lane) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -109,7 +109,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
   double xib = 0.0, yib = 0.0, nxb = 0.0, nyb = 0.0;
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const uint32_t pk = cp[(size_t)s * g.n];
     const int k = (int)(pk >> 4);
     const unsigned m = pk & 15u;
@@ -219,7 +219,14 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     ep[8] = D2{delxb, delyb};
   }
   // half-stencil geometry: c1 = sum dn^2 (x halves) or sum ds^2 (y halves), sxy = sum
-  // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2)
+  // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2); every lane needs the point's totals
+  if (LPP > 1) {
+    for (int d = 0; d < 4; ++d) {
+      hcb.c1[d] = group_sum<LPP>(hcb.c1[d]);
+      hcb.sxy[d] = group_sum<LPP>(hcb.sxy[d]);
+      hcb.obd[d] = group_sum<LPP>(hcb.obd[d]);
+    }
+  }
   double hxxb[4], hyyb[4], hxyb[4];
   {
     const V4d hxx = hs.hxx[i], hyy = hs.hyy[i], hxy = hs.hxy[i];
@@ -236,7 +243,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
       }
     }
   }
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const uint32_t pk = cp[(size_t)s * g.n];
     const int k = (int)(pk >> 4);
     const unsigned m = pk & 15u;
@@ -260,8 +267,24 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     const D2 t = ep[8];
     ep[8] = D2{t.x + delxb, t.y + delyb};
   }
-  // own contributions, written (q, dq, qm) or added (xy) by this thread alone; the
-  // edges' contributions to their neighbours are gathered by k_gather_flux
+  // own contributions, summed over the point's lanes and written (q, dq, qm) or
+  // added (xy, normals, delta) by its first lane alone; the edges' contributions to
+  // their neighbours are gathered by k_gather_flux
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) {
+      own.q[r] = group_sum<LPP>(own.q[r]);
+      own.qmax[r] = group_sum<LPP>(own.qmax[r]);
+      own.qmin[r] = group_sum<LPP>(own.qmin[r]);
+      odx[r] = group_sum<LPP>(odx[r]);
+      ody[r] = group_sum<LPP>(ody[r]);
+    }
+    xib = group_sum<LPP>(xib);
+    yib = group_sum<LPP>(yib);
+    nxb = group_sum<LPP>(nxb);
+    nyb = group_sum<LPP>(nyb);
+    dsum = group_sum<LPP>(dsum);
+    if (lane != 0) return;
+  }
   {
     const D2 t = o.xyb[i];
     o.xyb[i] = D2{t.x + xib, t.y + yib};
@@ -276,14 +299,16 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
   o.deltab[i] += (CLS == WALL) ? 2.0 * dsum : dsum;  // delta enters every stage
 }
 
+template <int LPP>
 __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<double> st,
                                                        const double* __restrict__ delta,
                                                        const V4d* __restrict__ frb, AdjOut o) {
-  const int i = point_id();
-  if (i >= g.n) return;
-  if (i < g.nw) flux_point_rev<WALL>(g, hs, st, i, delta[i], frb[i], o);
-  else if (i < g.nw + g.no) flux_point_rev<OUTER>(g, hs, st, i, delta[i], frb[i], o);
-  else flux_point_rev<INTERIOR>(g, hs, st, i, delta[i], frb[i], o);
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
+  if (i >= g.n) return;  // whole lane groups
+  if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
+  else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
+  else flux_point_rev<INTERIOR, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
 }
 
 // reverse of the state update of stage rk: from the adjoint of the stage's new prim
@@ -321,6 +346,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_rev(Geom g, const V4d* __restr
 // Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
 // adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
 // neighbour's dq adjoint being -0.5 (delx, dely) times its q adjoint (k_gather_lsq).
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restrict__ q,
                                                      const V4d* __restrict__ dq_in,
                                                      const V4d* __restrict__ dq_out,
@@ -328,7 +354,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ qb,
                                                      D2* __restrict__ xyb, double* __restrict__ eb,
                                                      const int32_t* __restrict__ epos) {
-  const int i = point_id();
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
   const V4d gg = g.lsq_geo[i];
@@ -355,7 +382,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   double qib[4] = {0, 0, 0, 0}, dxib[4] = {0, 0, 0, 0}, dyib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
   const int nb = g.nbhs[i];
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
     const D2 pk = g.xy[k];
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
@@ -381,25 +408,37 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
     xib -= dxb;
     yib -= dyb;
   }
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) {
+      qib[r] = group_sum<LPP>(qib[r]);
+      dxib[r] = group_sum<LPP>(dxib[r]);
+      dyib[r] = group_sum<LPP>(dyib[r]);
+    }
+    xib = group_sum<LPP>(xib);
+    yib = group_sum<LPP>(yib);
+    if (lane != 0) return;
+  }
   const V4d a = qb[i];
   qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
   dqb_in[2 * i] = V4d{dxib[0], dxib[1], dxib[2], dxib[3]};
   dqb_in[2 * i + 1] = V4d{dyib[0], dyib[1], dyib[2], dyib[3]};
-  const D2 t = xyb[i];
-  xyb[i] = D2{t.x + xib, t.y + yib};
+  const D2 pxy = xyb[i];
+  xyb[i] = D2{pxy.x + xib, pxy.y + yib};
 }
 
 // reverse of k_lsq (eval_q_derivatives): dq and qm adjoints -> q, xy adjoints.  Own
 // contributions added to qb, xyb; per edge, the neighbour's q adjoint (with the qm
 // adjoint of the components it maximises or minimises) and the edge's (delx, dely)
 // adjoint go to the edge buffer (fields 0-5, k_gather_lsq)
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict__ q,
                                                    const V4d* __restrict__ dq,
                                                    const V4d* __restrict__ dqb,
                                                    const V4d* __restrict__ qmb, V4d* __restrict__ qb,
                                                    D2* __restrict__ xyb, double* __restrict__ eb,
                                                    const int32_t* __restrict__ epos) {
-  const int i = point_id();
+  const int t = point_id();
+  const int i = t / LPP, lane = t % LPP;
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
   const V4d gg = g.lsq_geo[i];
@@ -431,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   double qib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
   const int nb = g.nbhs[i];
-  for (int s = 0; s < nb; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
     const D2 pk = g.xy[k];
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
@@ -461,31 +500,66 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
     xib -= dxb;
     yib -= dyb;
   }
+  if (LPP > 1) {
+    // the point's extremes over its lanes: the larger (smaller) value, on ties the
+    // lower slot, the point itself (slot -1) first, as the sequential strict > and <
+    for (int r = 0; r < 4; ++r) {
+      for (int off = LPP / 2; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(qmax[r], off, 64);
+        const int os = __shfl_xor(smax[r], off, 64);
+        if (ov > qmax[r] || (ov == qmax[r] && os < smax[r])) {
+          qmax[r] = ov;
+          smax[r] = os;
+        }
+        const double nv = __shfl_xor(qmin[r], off, 64);
+        const int ns = __shfl_xor(smin[r], off, 64);
+        if (nv < qmin[r] || (nv == qmin[r] && ns < smin[r])) {
+          qmin[r] = nv;
+          smin[r] = ns;
+        }
+      }
+    }
+  }
+  // the selected neighbour's entry is updated by the lane that wrote it
   for (int r = 0; r < 4; ++r) {
-    if (smax[r] < 0) qib[r] += mxb[r];
-    else eb[(size_t)epos[(size_t)smax[r] * g.n + i] * EF_LSQ + r] += mxb[r];
-    if (smin[r] < 0) qib[r] += mnb[r];
-    else eb[(size_t)epos[(size_t)smin[r] * g.n + i] * EF_LSQ + r] += mnb[r];
+    if (smax[r] < 0) {
+      if (lane == 0) qib[r] += mxb[r];
+    } else if (smax[r] % LPP == lane) {
+      eb[(size_t)epos[(size_t)smax[r] * g.n + i] * EF_LSQ + r] += mxb[r];
+    }
+    if (smin[r] < 0) {
+      if (lane == 0) qib[r] += mnb[r];
+    } else if (smin[r] % LPP == lane) {
+      eb[(size_t)epos[(size_t)smin[r] * g.n + i] * EF_LSQ + r] += mnb[r];
+    }
+  }
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) qib[r] = group_sum<LPP>(qib[r]);
+    xib = group_sum<LPP>(xib);
+    yib = group_sum<LPP>(yib);
+    if (lane != 0) return;
   }
   const V4d a = qb[i];
   qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
-  const D2 t = xyb[i];
-  xyb[i] = D2{t.x + xib, t.y + yib};
+  const D2 pxy = xyb[i];
+  xyb[i] = D2{pxy.x + xib, pxy.y + yib};
 }
 
 // Gathers of the edge buffer into the neighbour's adjoints, over the transposed
 // connectivity (each point reads the edges that name it; no atomics)
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const double* __restrict__ eb,
                                                        V4d* __restrict__ qb, V4d* __restrict__ dqb,
                                                        V4d* __restrict__ qmb, D2* __restrict__ xyb) {
-  const int k = point_id();
+  const int t = point_id();
+  const int k = t / LPP, lane = t % LPP;
   if (k >= g.n) return;
   const D2 pk = g.xy[k];
   double aq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
   double amx[4] = {0, 0, 0, 0}, amn[4] = {0, 0, 0, 0};
   double xb = 0.0, yb = 0.0;
   const int j1 = tr.ptr[k + 1];
-  for (int j = tr.ptr[k]; j < j1; ++j) {
+  for (int j = tr.ptr[k] + lane; j < j1; j += LPP) {
     const D2* ep = reinterpret_cast<const D2*>(eb + (size_t)j * EF_FLUX);
     const D2 q01 = ep[0], q23 = ep[1], w = ep[8];
     aq[0] += q01.x;
@@ -514,10 +588,22 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
       amn[3] += n23.y;
     }
   }
-  V4d t = qb[k];
-  qb[k] = V4d{t.x + aq[0], t.y + aq[1], t.z + aq[2], t.w + aq[3]};
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) {
+      aq[r] = group_sum<LPP>(aq[r]);
+      ax[r] = group_sum<LPP>(ax[r]);
+      ay[r] = group_sum<LPP>(ay[r]);
+      amx[r] = group_sum<LPP>(amx[r]);
+      amn[r] = group_sum<LPP>(amn[r]);
+    }
+    xb = group_sum<LPP>(xb);
+    yb = group_sum<LPP>(yb);
+    if (lane != 0) return;
+  }
+  V4d t4 = qb[k];
+  qb[k] = V4d{t4.x + aq[0], t4.y + aq[1], t4.z + aq[2], t4.w + aq[3]};
   if (!g.first_order) {
-    t = dqb[2 * k];
+    V4d t = dqb[2 * k];
     dqb[2 * k] = V4d{t.x + ax[0], t.y + ax[1], t.z + ax[2], t.w + ax[3]};
     t = dqb[2 * k + 1];
     dqb[2 * k + 1] = V4d{t.x + ay[0], t.y + ay[1], t.z + ay[2], t.w + ay[3]};
@@ -531,17 +617,18 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
 }
 
 // WITH_DQ: the inner-loop payload (dq adjoint = -0.5 (delx, dely) q adjoint)
-template <bool WITH_DQ>
+template <bool WITH_DQ, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const double* __restrict__ eb,
                                                       V4d* __restrict__ qb, V4d* __restrict__ dqb,
                                                       D2* __restrict__ xyb) {
-  const int k = point_id();
+  const int t = point_id();
+  const int k = t / LPP, lane = t % LPP;
   if (k >= g.n) return;
   const D2 pk = g.xy[k];
   double aq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
   double xb = 0.0, yb = 0.0;
   const int j1 = tr.ptr[k + 1];
-  for (int j = tr.ptr[k]; j < j1; ++j) {
+  for (int j = tr.ptr[k] + lane; j < j1; j += LPP) {
     const D2* ep = reinterpret_cast<const D2*>(eb + (size_t)j * EF_LSQ);
     const D2 a = ep[0], b = ep[1], w = ep[2];
     const double v[4] = {a.x, a.y, b.x, b.y};
@@ -557,10 +644,22 @@ __global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const do
       }
     }
   }
-  V4d t = qb[k];
-  qb[k] = V4d{t.x + aq[0], t.y + aq[1], t.z + aq[2], t.w + aq[3]};
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) {
+      aq[r] = group_sum<LPP>(aq[r]);
+      if (WITH_DQ) {
+        ax[r] = group_sum<LPP>(ax[r]);
+        ay[r] = group_sum<LPP>(ay[r]);
+      }
+    }
+    xb = group_sum<LPP>(xb);
+    yb = group_sum<LPP>(yb);
+    if (lane != 0) return;
+  }
+  V4d t4 = qb[k];
+  qb[k] = V4d{t4.x + aq[0], t4.y + aq[1], t4.z + aq[2], t4.w + aq[3]};
   if (WITH_DQ) {
-    t = dqb[2 * k];
+    V4d t = dqb[2 * k];
     dqb[2 * k] = V4d{t.x + ax[0], t.y + ax[1], t.z + ax[2], t.w + ax[3]};
     t = dqb[2 * k + 1];
     dqb[2 * k + 1] = V4d{t.x + ay[0], t.y + ay[1], t.z + ay[2], t.w + ay[3]};
@@ -727,6 +826,9 @@ struct AdjState {
   double J = 0.0;
   int64_t fwd_steps = 0, taped = 0;
   int peak_snaps = 0;
+  // lanes per point of the reverse kernels, timed once per context (adj_tune)
+  bool tuned = false;
+  int lpp_frev = 1, lpp_lrev = 1, lpp_irev = 1, lpp_gath = 1;
   float primal_ms = 0.f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -758,6 +860,52 @@ static int adj_record(mf_ctx* c, AdjState& A) {
   return MF_OK;
 }
 
+// launchers of the reverse kernels at a given lanes-per-point shape
+static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lpp) {
+  const HalfSums hs{A.hxx, A.hyy, A.hxy};
+  const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
+  const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
+  with_lpp(lpp, [&]<int L>() {
+    hipLaunchKernelGGL(k_flux_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
+                       (const double*)A.tdelta, (const V4d*)A.frb, o);
+  });
+}
+
+static void launch_gather_flux(mf_ctx* c, AdjState& A, V4d* dqb_last, int lpp) {
+  const Trans tr{A.t_ptr, A.t_own};
+  with_lpp(lpp, [&]<int L>() {
+    hipLaunchKernelGGL(k_gather_flux<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, tr,
+                       (const double*)A.eb, A.qb, dqb_last, A.qmb, A.xyb);
+  });
+}
+
+// inner iteration j of stage s: dq^(j) adjoint (dqb[j & 1]) -> dq^(j-1)
+static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
+  const int cur = j & 1, prv = (j - 1) & 1;
+  with_lpp(lpp, [&]<int L>() {
+    hipLaunchKernelGGL(k_inner_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, (const V4d*)A.tq[s],
+                       (const V4d*)A.tdq[s * (A.I + 1) + j - 1], (const V4d*)A.tdq[s * (A.I + 1) + j],
+                       (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+  });
+}
+
+template <bool WITH_DQ>
+static void launch_gather_lsq(mf_ctx* c, AdjState& A, V4d* dqb, int lpp) {
+  const Trans tr{A.t_ptr, A.t_own};
+  with_lpp(lpp, [&]<int L>() {
+    hipLaunchKernelGGL((k_gather_lsq<WITH_DQ, L>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, tr,
+                       (const double*)A.eb, A.qb, dqb, A.xyb);
+  });
+}
+
+static void launch_lsq_rev(mf_ctx* c, AdjState& A, int s, int lpp) {
+  with_lpp(lpp, [&]<int L>() {
+    hipLaunchKernelGGL(k_lsq_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, (const V4d*)A.tq[s],
+                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb,
+                       A.eb, (const int32_t*)A.epos);
+  });
+}
+
 // reverse of the taped iteration: A.Pb holds the adjoint of its output prim and
 // receives the adjoint of its input prim
 static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
@@ -765,36 +913,23 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
   const dim3 gr(nblocks(c)), bl(BLOCK);
   int rc;
   if ((rc = adj_zero(c, A.poldb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.deltab, n))) return rc;
-  HalfSums hs{A.hxx, A.hyy, A.hxy};
-  const Trans tr{A.t_ptr, A.t_own};
   for (int s = A.S - 1; s >= 0; --s) {
     // update: Pb (adjoint of prim_{s+1}) -> Pstage (prim_s), poldb, frb, nrmb
     hipLaunchKernelGGL(k_update_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tprim[s], (const V4d*)A.tpold,
                        (const V4d*)A.tprim[s + 1], (const V4d*)A.Pb, A.Pstage, A.poldb, A.frb, A.nrmb, s + 1,
                        c->p.euler, c->fs);
-    State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
     V4d* dqb_last = A.dqb[A.I & 1];
-    AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
     // flux: frb -> q, dq, qm (written), delta, xy, nrm; neighbours' parts gathered
-    hipLaunchKernelGGL(k_flux_rev, gr, bl, 0, c->stream, c->g, hs, st, (const double*)A.tdelta,
-                       (const V4d*)A.frb, o);
-    hipLaunchKernelGGL(k_gather_flux, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb, dqb_last,
-                       A.qmb, A.xyb);
+    launch_flux_rev(c, A, s, dqb_last, A.lpp_frev);
+    launch_gather_flux(c, A, dqb_last, A.lpp_gath);
     // inner iterations, last first: dq^(j) adjoint -> dq^(j-1)
     for (int j = A.I; j >= 1; --j) {
-      const int cur = j & 1, prv = (j - 1) & 1;
-      hipLaunchKernelGGL(k_inner_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
-                         (const V4d*)A.tdq[s * (A.I + 1) + j - 1], (const V4d*)A.tdq[s * (A.I + 1) + j],
-                         (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
-      hipLaunchKernelGGL(k_gather_lsq<true>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb,
-                         A.dqb[prv], A.xyb);
+      launch_inner_rev(c, A, s, j, A.lpp_irev);
+      launch_gather_lsq<true>(c, A, A.dqb[(j - 1) & 1], A.lpp_gath);
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
-    hipLaunchKernelGGL(k_lsq_rev, gr, bl, 0, c->stream, c->g, (const V4d*)A.tq[s],
-                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb,
-                       A.eb, (const int32_t*)A.epos);
-    hipLaunchKernelGGL(k_gather_lsq<false>, gr, bl, 0, c->stream, c->g, tr, (const double*)A.eb, A.qb,
-                       (V4d*)nullptr, A.xyb);
+    launch_lsq_rev(c, A, s, A.lpp_lrev);
+    launch_gather_lsq<false>(c, A, (V4d*)nullptr, A.lpp_gath);
     // q = q(prim_s): into the stage-input adjoint
     hipLaunchKernelGGL(k_qvar_rev, dim3(nblocks(c)), bl, 0, c->stream, n, (const V4d*)A.tprim[s],
                        (const V4d*)A.qb, A.Pstage);
@@ -805,6 +940,45 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
                      c->p.cfl, A.Pb, A.xyb);
   hipLaunchKernelGGL(k_axpy, dim3(1024), bl, 0, c->stream, (double*)A.Pb, (const double*)A.poldb, (int64_t)4 * n);
   HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+static int adj_copy(mf_ctx* c, void* dst, const void* src);
+
+static void adj_forget_tuning(mf_ctx* c) {
+  if (c->adj) c->adj->tuned = false;
+}
+
+// Picks the lanes per point of each reverse kernel by timing every shape on a tape of
+// the context's current state.  The timed launches write only buffers that the sweep
+// zeroes or overwrites before use (edge buffer, stage adjoints, xy/normal adjoints).
+static int adj_tune(mf_ctx* c, AdjState& A) {
+  int rc;
+  const int n = c->n;
+  if ((rc = adj_copy(c, A.tprim[0], c->prim[c->cur])) || (rc = adj_record(c, A))) return rc;
+  if ((rc = adj_zero(c, A.frb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.dqb[0], 8 * (int64_t)n)) ||
+      (rc = adj_zero(c, A.dqb[1], 8 * (int64_t)n)) || (rc = adj_zero(c, A.qmb, 8 * (int64_t)n)) ||
+      (rc = adj_zero(c, A.qb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.deltab, n)) ||
+      (rc = adj_zero(c, A.xyb, 2 * (int64_t)n)) || (rc = adj_zero(c, A.nrmb, 2 * (int64_t)n)))
+    return rc;
+  const int reps = 5, cand[4] = {1, 2, 4, 8};
+  double best[4] = {1e300, 1e300, 1e300, 1e300};
+  int pick[4] = {1, 1, 1, 1};
+  for (int lpp : cand) {
+    double us[4] = {0, 0, 0, 0};
+    if ((rc = time_launches(c, reps, [&] { launch_flux_rev(c, A, 0, A.dqb[A.I & 1], lpp); }, &us[0])) ||
+        (rc = time_launches(c, reps, [&] { launch_lsq_rev(c, A, 0, lpp); }, &us[1])) ||
+        (rc = time_launches(c, reps, [&] { launch_gather_flux(c, A, A.dqb[A.I & 1], lpp); }, &us[3])))
+      return rc;
+    if (A.I > 0 && (rc = time_launches(c, reps, [&] { launch_inner_rev(c, A, 0, 1, lpp); }, &us[2]))) return rc;
+    for (int k = 0; k < 4; ++k)
+      if (us[k] < best[k]) best[k] = us[k], pick[k] = lpp;
+  }
+  A.lpp_frev = pick[0];
+  A.lpp_lrev = pick[1];
+  A.lpp_irev = pick[2];
+  A.lpp_gath = pick[3];
+  A.tuned = true;
   return MF_OK;
 }
 
@@ -977,6 +1151,13 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   AdjState& A = *c->adj;
   int rc;
   if ((rc = adj_alloc(c, A, snaps))) return rc;
+  const bool pinned = c->lpp_rev[0] && c->lpp_rev[1] && c->lpp_rev[2] && c->lpp_rev[3];
+  if (!A.tuned && !pinned && (rc = adj_tune(c, A))) return rc;
+  // shapes pinned with mf_set_tuning override the timed ones
+  if (c->lpp_rev[0]) A.lpp_frev = c->lpp_rev[0];
+  if (c->lpp_rev[1]) A.lpp_lrev = c->lpp_rev[1];
+  if (c->lpp_rev[2]) A.lpp_irev = c->lpp_rev[2];
+  if (c->lpp_rev[3]) A.lpp_gath = c->lpp_rev[3];
   A.first_it = first_it;
   A.iters = iters;
   A.fwd_steps = A.taped = 0;
@@ -1071,6 +1252,10 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
     stats->reserved = 0;
     stats->primal_ms = A.primal_ms;
     stats->adjoint_ms = bms;
+    stats->lpp_flux_rev = A.lpp_frev;
+    stats->lpp_lsq_rev = A.lpp_lrev;
+    stats->lpp_inner_rev = A.lpp_irev;
+    stats->lpp_gather = A.lpp_gath;
   }
   return MF_OK;
 }

@@ -182,17 +182,22 @@ class LskumSolver:
         L.check(L.lib().mf_autotune(self.h, int(reps), C.byref(t)), "mf_autotune")
         return {f: getattr(t, f) for f, _ in L.MfTuning._fields_ if f != "reserved"}
 
+    _SHAPES = ("lpp_lsq", "lpp_inner", "lpp_flux", "lpp_flux_rev", "lpp_lsq_rev", "lpp_inner_rev",
+               "lpp_gather")
+
     @property
     def tuning(self) -> dict:
+        """Lanes per point of each stencil kernel (the reverse ones 0 = timed by the
+        first adjoint run)."""
         t = L.MfTuning()
         L.check(L.lib().mf_get_tuning(self.h, C.byref(t)), "mf_get_tuning")
-        return {"lpp_lsq": t.lpp_lsq, "lpp_inner": t.lpp_inner, "lpp_flux": t.lpp_flux}
+        return {k: getattr(t, k) for k in self._SHAPES}
 
     @tuning.setter
     def tuning(self, value: dict) -> None:
         cur = self.tuning
         cur.update(value)
-        t = L.MfTuning(lpp_lsq=cur["lpp_lsq"], lpp_inner=cur["lpp_inner"], lpp_flux=cur["lpp_flux"])
+        t = L.MfTuning(**{k: cur[k] for k in self._SHAPES})
         L.check(L.lib().mf_set_tuning(self.h, C.byref(t)), "mf_set_tuning")
 
     def layout(self) -> dict:

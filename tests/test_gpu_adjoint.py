@@ -115,6 +115,24 @@ def test_adjoint_schedule_invariance(small_cloud):
     assert out[0][3]["forward_steps"] > out[1][3]["forward_steps"] >= out[2][3]["forward_steps"]
 
 
+@pytest.mark.parametrize("lpp", [2, 4, 8])
+def test_adjoint_reverse_shapes(small_cloud, lpp):
+    """The reverse kernels split over 2/4/8 lanes per point (partial sums combined with
+    shuffles; the qm argmax resolved over the lanes) give the one-lane gradient up to
+    summation order, and the same J."""
+    out = []
+    for shape in (1, lpp):
+        with _gpu(small_cloud) as g:
+            g.tuning = {"lpp_flux_rev": shape, "lpp_lsq_rev": shape, "lpp_inner_rev": shape,
+                        "lpp_gather": shape}
+            J, dx, dy, st = g.adjoint(12, snaps=4)
+            assert (st["lpp_flux_rev"], st["lpp_lsq_rev"], st["lpp_inner_rev"], st["lpp_gather"]) == \
+                (shape,) * 4
+            out.append((J, np.concatenate([dx, dy])))
+    assert out[0][0] == out[1][0]
+    assert np.abs(out[1][1] - out[0][1]).max() <= 1e-12 * np.abs(out[0][1]).max()
+
+
 def test_adjoint_leaves_final_state(small_cloud):
     with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
         a.adjoint(10, snaps=3)

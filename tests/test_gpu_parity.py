@@ -178,7 +178,8 @@ def test_lanes_per_point_shapes(small_cloud, lpp):
     o.run(20)
     with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
         b.tuning = {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": lpp}
-        assert b.tuning == {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": lpp}
+        assert {k: b.tuning[k] for k in ("lpp_lsq", "lpp_inner", "lpp_flux")} == \
+            {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": lpp}
         ra, _, cda = a.q_lskum(20)
         rb, _, cdb = b.q_lskum(20)
         assert rel_err(b.prim, a.prim) < 1e-12
