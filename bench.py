@@ -55,9 +55,9 @@ def iteration_bytes_per_point(state_bytes: int, mean_nbhs: float, inner: int, st
     return stages * ((1 + inner) * (2 * g + 20 * w + 4 * N) + (5 * g + 29 * w + 5 * N)) + (2 * g + 9 * w + 4 * N)
 
 
-def run_gpu(cloud, precision: str, warmup: int, steps: int):
+def run_gpu(cloud, precision: str, warmup: int, steps: int, wl: dict = WORKLOAD):
     from mfcfd_amd.solver import LskumSolver
-    p = {k: WORKLOAD[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
+    p = {k: wl[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
     s = LskumSolver(cloud, precision=precision, **p)
     print(f"[bench] {precision}: warmup {warmup}", file=sys.stderr, flush=True)
     s.q_lskum(warmup, first_it=1)
@@ -140,6 +140,41 @@ def run_adjoint(iters: int, snaps: int):
     return out
 
 
+# 2,560,000 points; the configuration fixes no inner count and no limiter constant:
+# 0 inner iterations (readcase.F90's default) and K = 5 as the 160k workload
+LARGE_WORKLOAD = dict(n_wall=2560, n_radial=1000, mach=1.2, aoa=0.0, cfl=0.2, vl_const=5.0,
+                      inner_iterations=0, seed=0, warmup=100, steps=500)
+
+
+def run_large():
+    """NACA0012 2560x1000 (2.56M points), second order, M 1.2 (bow shock), AoA 0:
+    100 untimed + 500 timed iterations on each precision path."""
+    from mfcfd_amd.cloud import naca_ogrid
+    w = LARGE_WORKLOAD
+    t0 = time.perf_counter()
+    cloud = naca_ogrid(w["n_wall"], w["n_radial"], seed=w["seed"], verbose=True)
+    print(f"[bench] large cloud {cloud.n} points in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    nb = float(cloud.nbhs.mean())
+    out = {"workload": f"NACA0012 O-cloud 2560x1000 ({cloud.n} points), 2nd order, K=5, 0 inner, "
+                       f"M 1.2, AoA 0, CFL 0.2, {w['warmup']} + {w['steps']} timed iterations",
+           "mean_nbhs": round(nb, 3)}
+    for prec in ("fp64", "mixed"):
+        r = run_gpu(cloud, prec, w["warmup"], w["steps"], wl=w)
+        sb = 8 if prec == "fp64" else 4
+        fu = r["kernels"]["flux_update"][0]
+        per_launch = flux_update_bytes_per_point(sb, nb) * cloud.n
+        out[prec] = {"point_iters_per_s": r["point_iters_per_s"], "ms_per_step": r["ms_per_step"],
+                     "cl": r["cl"], "cd": r["cd"], "residue": r["residue"], "tuning": r["tuning"],
+                     "flux_update_ms": fu, "flux_update_hbm_gbs": per_launch / (fu * 1e-3) / 1e9,
+                     "iteration_hbm_gbs": iteration_bytes_per_point(sb, nb, w["inner_iterations"])
+                     * r["point_iters_per_s"] / 1e9,
+                     "kernel_ms": {k: v[0] for k, v in r["kernels"].items()}}
+    out["mixed_over_fp64"] = out["mixed"]["point_iters_per_s"] / out["fp64"]["point_iters_per_s"]
+    out["dCl"] = abs(out["mixed"]["cl"] - out["fp64"]["cl"])
+    out["dCd"] = abs(out["mixed"]["cd"] - out["fp64"]["cd"])
+    return out
+
+
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -171,6 +206,9 @@ def main():
                          "or the workload's full 2000 with --adjoint)")
     ap.add_argument("--adjoint", action="store_true",
                     help="only the adjoint measurement")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 2.56M-point measurement reported beside the headline")
+    ap.add_argument("--large", action="store_true", help="only the 2.56M-point measurement")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
                          "2 mixed iterations of the workload (for rocprofv3 --pmc)")
@@ -186,6 +224,9 @@ def main():
     if args.adjoint:
         iters = ADJ_WORKLOAD["iterations"] if args.adjoint_iters is None else args.adjoint_iters
         print(json.dumps(run_adjoint(iters, ADJ_WORKLOAD["snaps"])))
+        return
+    if args.large:
+        print(json.dumps(run_large()))
         return
     if args.adjoint_iters is None:
         args.adjoint_iters = 200
@@ -247,6 +288,8 @@ def main():
     }
     if args.adjoint_iters > 0:
         line["adjoint"] = run_adjoint(args.adjoint_iters, ADJ_WORKLOAD["snaps"])
+    if not args.no_large:
+        line["large"] = run_large()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cloud, args.cpu_iters)
     print(json.dumps(line))

@@ -1,4 +1,6 @@
-"""Builds the HIP library libmfcfd_hip.so in-tree (hipcc, gfx950).
+"""Builds the HIP library libmfcfd_hip.so in-tree (hipcc, gfx950) and the Fortran host
+program fortran/meshfree_host (amdflang, linked to the library and to the HDF5 C
+library it reads point.h5 with).
 
 Run ``python -m mfcfd_amd.build`` (or ``__graft_entry__.build()``).  hipcc
 cross-compiles for gfx950 without a GPU, so this runs anywhere the ROCm toolchain is.
@@ -33,6 +35,44 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the MI355X library cannot be built")
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+HOST = os.path.join(FORTRAN, "meshfree_host")
+HOST_SOURCES = ["mfcfd_api.f90", "host_io.f90", "meshfree_host.f90"]
+
+
+def amdflang() -> str | None:
+    for cand in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/amdflang",
+                 shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def hdf5_libdir() -> str | None:
+    for d in (os.environ.get("MFCFD_HDF5_LIBDIR"), "/opt/conda/lib"):
+        if d and os.path.exists(os.path.join(d, "libhdf5.so")):
+            return d
+    return None
+
+
+def build_host(verbose: bool = False) -> str | None:
+    """Compile the Fortran host program (None when amdflang or libhdf5 is missing)."""
+    fc, h5 = amdflang(), hdf5_libdir()
+    if fc is None or h5 is None:
+        print("meshfree_host not built: amdflang or libhdf5 missing", file=sys.stderr)
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod:
+        cmd = [fc, "-O2", "-J", mod] + [os.path.join(FORTRAN, f) for f in HOST_SOURCES] + [
+            "-o", HOST + ".tmp", f"-L{HERE}", "-lmfcfd_hip", "-Wl,-rpath,$ORIGIN/..",
+            f"-L{h5}", "-lhdf5", f"-Wl,-rpath,{h5}"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+    os.replace(HOST + ".tmp", HOST)
+    return HOST
+
+
 def build(verbose: bool = False) -> str:
     """Compile csrc/mfcfd.hip into mfcfd_amd/libmfcfd_hip.so; returns the path."""
     src = os.path.join(CSRC, "mfcfd.hip")
@@ -42,6 +82,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

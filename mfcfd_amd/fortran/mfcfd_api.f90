@@ -24,6 +24,19 @@ module mfcfd_api
         integer(c_int32_t) :: format, restart, precision, reorder, block_size, first_order
     end type
 
+    type, bind(C) :: mf_tuning
+        integer(c_int32_t) :: lpp_lsq, lpp_inner, lpp_flux, reserved
+        real(c_double) :: us_lsq, us_inner, us_flux
+        integer(c_int32_t) :: lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather
+    end type
+
+    type, bind(C) :: mf_adjoint_stats
+        integer(c_int64_t) :: forward_steps, taped_steps
+        integer(c_int32_t) :: snapshots, reserved
+        real(c_double) :: primal_ms, adjoint_ms
+        integer(c_int32_t) :: lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather
+    end type
+
     interface
         subroutine mf_default_params(p) bind(C, name="mf_default_params")
             import
@@ -114,6 +127,30 @@ module mfcfd_api
         integer(c_int) function mf_sync(ctx) bind(C, name="mf_sync")
             import
             type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mf_autotune(ctx, reps, t) bind(C, name="mf_autotune")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: reps
+            type(mf_tuning), intent(out) :: t
+        end function
+        integer(c_int) function mf_get_tuning(ctx, t) bind(C, name="mf_get_tuning")
+            import
+            type(c_ptr), value :: ctx
+            type(mf_tuning), intent(out) :: t
+        end function
+        integer(c_int) function mf_set_tuning(ctx, t) bind(C, name="mf_set_tuning")
+            import
+            type(c_ptr), value :: ctx
+            type(mf_tuning), intent(in) :: t
+        end function
+        integer(c_int) function mf_adjoint(ctx, first_it, iters, snaps, J, dJdx, dJdy, stats) &
+                bind(C, name="mf_adjoint")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: first_it, iters, snaps
+            real(c_double), intent(out) :: J, dJdx(*), dJdy(*)
+            type(mf_adjoint_stats), intent(out) :: stats
         end function
     end interface
 
