@@ -175,6 +175,81 @@ def run_large():
     return out
 
 
+# 64 cases: 8 NACA 00tt sections (t ~ U[0.08, 0.15]) on clouds of 20k-120k points
+# (wall 100-480, radial 200-250), each flown at 8 conditions M ~ U[0.5, 0.85],
+# AoA ~ U[-2, 4] (seeded); second order, K = 5, 2 inner iterations, CFL 0.2
+BATCH_WORKLOAD = dict(seed=5, n_airfoils=8, n_cases=64, cfl=0.2, vl_const=5.0, inner_iterations=2,
+                      iterations=1000, warmup=20)
+
+
+def batch_cases():
+    from mfcfd_amd.cloud import naca_ogrid
+    w = BATCH_WORKLOAD
+    rng = np.random.default_rng(w["seed"])
+    th = rng.uniform(0.08, 0.15, w["n_airfoils"])
+    nw = rng.integers(100, 481, w["n_airfoils"])
+    nr = rng.integers(200, 251, w["n_airfoils"])
+    mach = rng.uniform(0.5, 0.85, w["n_cases"])
+    aoa = rng.uniform(-2.0, 4.0, w["n_cases"])
+    clouds = []
+    for a in range(w["n_airfoils"]):
+        t0 = time.perf_counter()
+        clouds.append(naca_ogrid(int(nw[a]), int(nr[a]), thickness=float(th[a]), seed=a))
+        print(f"[bench] batch cloud {a}: NACA00{round(100 * th[a]):02d} {nw[a]}x{nr[a]} "
+              f"({clouds[-1].n} points) {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cases = [(clouds[k % w["n_airfoils"]], float(mach[k]), float(aoa[k])) for k in range(w["n_cases"])]
+    return cases, th
+
+
+def run_batch(iters: int, adjoint: bool = True):
+    """All 64 cases in one context: every graph-replayed iteration is one launch per
+    kernel over all cases.  Mixed and FP64 paths from freestream for `iters`
+    iterations; then the FP64 Cd adjoint of the batch over the same iterations."""
+    from mfcfd_amd.solver import LskumBatch
+    w = BATCH_WORKLOAD
+    cases, th = batch_cases()
+    p = {k: w[k] for k in ("cfl", "vl_const", "inner_iterations")}
+    npts = sum(c.n for c, _, _ in cases)
+    out = {"workload": f"{w['n_cases']} cases on {w['n_airfoils']} NACA 00tt clouds (thickness "
+                       f"{th.min():.3f}-{th.max():.3f}, {min(c.n for c, _, _ in cases)}-"
+                       f"{max(c.n for c, _, _ in cases)} points), M 0.5-0.85, AoA -2..4, 2nd order, "
+                       f"K=5, 2 inner, CFL 0.2, {iters} iterations", "points": npts}
+    cd_of = {}
+    for prec in ("mixed", "fp64"):
+        with LskumBatch(cases, precision=prec, **p) as b:
+            b.run(w["warmup"])
+            tune = b.autotune()
+            b.sync()
+            t0 = time.perf_counter()
+            res, _, _ = b.run(iters, first_it=w["warmup"] + 1)
+            dt = time.perf_counter() - t0
+            _, cd, _, _ = b.compute_cl_cd_cm()
+            cd_of[prec] = cd[:, 0]
+            out[prec] = {"point_iters_per_s": npts * iters / dt, "ms_per_step": 1e3 * dt / iters,
+                         "tuning": {k: tune[k] for k in ("lpp_lsq", "lpp_inner", "lpp_flux")},
+                         "final_residue_max": float(np.nanmax(res[-1]))}
+            print(f"[bench] batch {prec}: {out[prec]['ms_per_step']:.2f} ms/iter "
+                  f"{out[prec]['point_iters_per_s']:.3e} pt-it/s", file=sys.stderr, flush=True)
+    dcd = np.abs(cd_of["mixed"] - cd_of["fp64"])
+    out["mixed_over_fp64"] = out["mixed"]["point_iters_per_s"] / out["fp64"]["point_iters_per_s"]
+    out["per_case_dCd_mixed_vs_fp64"] = {"max": float(dcd.max()), "median": float(np.median(dcd))}
+    out["cd_fp64"] = [round(float(v), 8) for v in cd_of["fp64"]]
+    if adjoint:
+        # every iterate kept when it fits comfortably in HBM (32 B per point per iterate)
+        snaps = int(min(iters + w["warmup"], max(32, 150e9 // (npts * 32))))
+        os.environ["MFCFD_PROGRESS"] = "1"
+        with LskumBatch(cases, **p) as b:
+            b.run(0)
+            t0 = time.perf_counter()
+            J, dx, dy, st = b.adjoint(iters + w["warmup"], snaps=snaps)
+            adt = time.perf_counter() - t0
+        fp64_iter_s = out["fp64"]["ms_per_step"] * 1e-3
+        out["adjoint"] = {"iterations": iters + w["warmup"], "snapshots": snaps, "J_sum_cd": J,
+                          "seconds": adt, "over_primal": adt / (fp64_iter_s * (iters + w["warmup"])),
+                          "stats": st, "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+    return out
+
+
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -209,6 +284,8 @@ def main():
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 2.56M-point measurement reported beside the headline")
     ap.add_argument("--large", action="store_true", help="only the 2.56M-point measurement")
+    ap.add_argument("--batch", action="store_true", help="only the 64-case batched measurement")
+    ap.add_argument("--batch-iters", type=int, default=None, help="iterations of the batched run")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
                          "2 mixed iterations of the workload (for rocprofv3 --pmc)")
@@ -227,6 +304,9 @@ def main():
         return
     if args.large:
         print(json.dumps(run_large()))
+        return
+    if args.batch:
+        print(json.dumps(run_batch(args.batch_iters or BATCH_WORKLOAD["iterations"])))
         return
     if args.adjoint_iters is None:
         args.adjoint_iters = 200

@@ -138,6 +138,27 @@ int mf_sync(mf_ctx *ctx);
 #define MF_NUM_KERNEL_KINDS 5
 int mf_profile_kernels(mf_ctx *ctx, int32_t iters, double *mean_ms, int32_t *launches);
 
+/* A batch: several independent flow cases (each its own cloud, of any size, with its
+ * own Mach number and angle of attack; the other options shared) advanced together by
+ * every launch of one context.  Per-point arrays of the context (mf_get_prim,
+ * mf_set_prim, mf_get_field, dJ/dx and dJ/dy) are the cases' points concatenated in
+ * case order; mf_compute_cl_cd_cm returns ncases * shapes entries, case-major, and cp
+ * of every case's shape points in that order; the adjoint's J is the sum of every
+ * case's Cd (each point's gradient is its own case's dCd/dx, the cases being
+ * independent).  No counterpart in the reference, which runs one case per
+ * process. */
+typedef struct mf_case {
+  const mf_geometry *geometry;
+  double mach, aoa;  /* parameter.F90:5-6 of this case */
+} mf_case;
+int mf_create_batch(const mf_case *cases, int32_t ncases, const mf_params *params, mf_ctx **ctx);
+/* `iters` iterations of every case (fpi_solver.F90:13-83 per case, residual norms per
+ * case; a NaN in one case does not stop the others): residue[it * ncases + c],
+ * res_new likewise, cd[(it * ncases + c) * shapes + s].  Any pointer may be NULL. */
+int mf_batch_run(mf_ctx *ctx, int32_t first_it, int32_t iters, double *residue, double *res_new, double *cd);
+/* Number of cases and the offset of each case's first point (first[ncases] = N). */
+int mf_batch_cases(mf_ctx *ctx, int32_t *ncases, int32_t *first);
+
 /* Launch shape of the stencil kernels: lanes per point (1, 2, 4 or 8) of the LSQ
  * derivative kernel (with func_delta), the inner-iteration kernel and the fused flux
  * residual + state update.  A point's neighbours are split over that many lanes of a

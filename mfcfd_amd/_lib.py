@@ -58,6 +58,10 @@ class MfAdjointStats(C.Structure):
                 ("lpp_inner_rev", C.c_int32), ("lpp_gather", C.c_int32)]
 
 
+class MfCase(C.Structure):
+    _fields_ = [("geometry", C.POINTER(MfGeometry)), ("mach", C.c_double), ("aoa", C.c_double)]
+
+
 class MfTuning(C.Structure):
     _fields_ = [("lpp_lsq", C.c_int32), ("lpp_inner", C.c_int32), ("lpp_flux", C.c_int32),
                 ("reserved", C.c_int32), ("us_lsq", C.c_double), ("us_inner", C.c_double),
@@ -89,6 +93,9 @@ SIGNATURES = [
     ("mf_sync", C.c_int, [_VP]),
     ("mf_profile_kernels", C.c_int, [_VP, C.c_int32, _DP, _IP]),
     ("mf_layout", C.c_int, [_VP, C.POINTER(MfLayout)]),
+    ("mf_create_batch", C.c_int, [C.POINTER(MfCase), C.c_int32, C.POINTER(MfParams), C.POINTER(C.c_void_p)]),
+    ("mf_batch_run", C.c_int, [_VP, C.c_int32, C.c_int32, _DP, _DP, _DP]),
+    ("mf_batch_cases", C.c_int, [_VP, _IP, _IP]),
     ("mf_autotune", C.c_int, [_VP, C.c_int32, C.POINTER(MfTuning)]),
     ("mf_get_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
     ("mf_set_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),

@@ -66,6 +66,8 @@ template <> struct Consts<float> {
   static constexpr float GAMMA_M1 = 0.4f;
 };
 
+struct Freestream;
+
 struct Geom {
   const D2* xy;
   const D2* nrm;
@@ -77,6 +79,10 @@ struct Geom {
   const V4d* dg_obd;
   int32_t n, nw, no;  // points; wall count; outer count (interior = rest)
   int32_t first_order;  // 1: q-tilde = q at both ends (no reconstruction)
+  // a batch of independent cases in one cloud: each point's case and the cases'
+  // freestreams (null for a single case: the kernels' Freestream argument)
+  const int32_t* case_id = nullptr;
+  const Freestream* fs_tab = nullptr;
 };
 
 struct Ctl {

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -189,7 +190,11 @@ __device__ inline V4<S> update_dispatch(const Geom& g, int i, const V4<S>& p, co
                                         const Freestream& fs, double* rs) {
   const D2 nn = g.nrm[i];
   if (i < g.nw) return update_point<WALL, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
-  if (i < g.nw + g.no) return update_point<OUTER, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
+  if (i < g.nw + g.no) {
+    // the far-field state of the point's own case
+    const Freestream f = g.fs_tab ? g.fs_tab[g.case_id[i]] : fs;
+    return update_point<OUTER, S>(p, po, fr, nn.x, nn.y, rk, euler, f, rs);
+  }
   return update_point<INTERIOR, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
 }
 
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     Geom g, State<S> st, const V4<S>* __restrict__ prim, V4<S>* __restrict__ prim_old,
     V4<S>* __restrict__ prim_out, V4<S>* __restrict__ q_out, double* __restrict__ cspeed,
     const double* __restrict__ delta, double* __restrict__ partial, const Ctl* __restrict__ ctl,
-    int rk, int last, double euler, double power, Freestream fs) {
+    int rk, int last, double euler, double power, Freestream fs, double* __restrict__ rs_pt) {
   if (ctl->halt) return;
   const int t = point_id();
   const int i = t / LPP, lane = t % LPP;
@@ -239,7 +244,10 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     const V4<S> pn = update_dispatch<S>(g, i, p, po, fr, rk, euler, fs, &rs);
     prim_out[i] = pn;
     q_out[i] = q_of_prim(pn);
-    if (last) cspeed[i] = wave_speed(pn);
+    if (last) {
+      cspeed[i] = wave_speed(pn);
+      if (rs_pt) rs_pt[i] = rs;
+    }
   }
   if (last) {
     const double tot = block_sum(rs);
@@ -302,47 +310,83 @@ __global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old) {
   if (set_res_old) ctl->res_old = res_old;
 }
 
-// compute_force_coeffs.F90:11-102 on shape points (one block).  sgeo per shape point:
-// (nx, ny, ds, mx - 0.25, my); Cl/Cd/Cm per shape into clcdcm[3*s + {0,1,2}].
+// compute_force_coeffs.F90:11-102, one block per shape id (a batch numbers the shapes
+// of case c as c * shapes + s).  Shape points are grouped by shape id (sh_ptr), each
+// with its (nx, ny, ds, mx - 0.25, my) in sgeo and its ordinal among the shape points
+// in original order (sh_pos, the cp output's order); shape_par per shape id holds
+// (0.5 rho_inf M^2, cos theta, sin theta) of its case.  Cl/Cd/Cm into clcdcm[3 s + k].
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_forces(const V4<S>* __restrict__ prim,
                                                   const int32_t* __restrict__ sh_m,
-                                                  const int32_t* __restrict__ sh_id,
-                                                  const double* __restrict__ sgeo, int nsh,
-                                                  int shapes, double cp_scale, double ct,
-                                                  double st, double* __restrict__ cp,
-                                                  double* __restrict__ clcdcm,
+                                                  const int32_t* __restrict__ sh_pos,
+                                                  const double* __restrict__ sgeo,
+                                                  const int32_t* __restrict__ sh_ptr,
+                                                  const double* __restrict__ shape_par, int nsid,
+                                                  double* __restrict__ cp, double* __restrict__ clcdcm,
                                                   double* __restrict__ cd_hist,
                                                   const Ctl* __restrict__ ctl) {
   if (ctl && ctl->halt) return;
   const double pr_inf = 1.0 / 1.4;
-  for (int s = 0; s < shapes; ++s) {
-    double H = 0.0, V = 0.0, M = 0.0;
-    for (int j = threadIdx.x; j < nsh; j += blockDim.x) {
-      const double c0 = (double)prim[sh_m[j]].w - pr_inf;
-      const double c = -c0 / cp_scale;
-      if (s == 0) cp[j] = c;
-      if (sh_id[j] != s) continue;
-      const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
-      H += c * nx * ds;
-      V += c * ny * ds;
-      M += (-c * ny * ds * sgeo[5 * j + 3] + c * nx * ds * sgeo[5 * j + 4]);
-    }
-    H = block_sum(H);
-    __syncthreads();
-    V = block_sum(V);
-    __syncthreads();
-    M = block_sum(M);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const double cl = V * ct - H * st, cd = H * ct + V * st;
-      clcdcm[3 * s] = cl;
-      clcdcm[3 * s + 1] = cd;
-      clcdcm[3 * s + 2] = M;
-      if (s == 0 && ctl && cd_hist) cd_hist[ctl->it] = cd;
-    }
+  const int s = blockIdx.x;
+  const double cp_scale = shape_par[3 * s], ct = shape_par[3 * s + 1], st = shape_par[3 * s + 2];
+  double H = 0.0, V = 0.0, M = 0.0;
+  const int j1 = sh_ptr[s + 1];
+  for (int j = sh_ptr[s] + threadIdx.x; j < j1; j += blockDim.x) {
+    const double c0 = (double)prim[sh_m[j]].w - pr_inf;
+    const double c = -c0 / cp_scale;
+    cp[sh_pos[j]] = c;
+    const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
+    H += c * nx * ds;
+    V += c * ny * ds;
+    M += (-c * ny * ds * sgeo[5 * j + 3] + c * nx * ds * sgeo[5 * j + 4]);
+  }
+  H = block_sum(H);
+  __syncthreads();
+  V = block_sum(V);
+  __syncthreads();
+  M = block_sum(M);
+  if (threadIdx.x == 0) {
+    const double cl = V * ct - H * st, cd = H * ct + V * st;
+    clcdcm[3 * s] = cl;
+    clcdcm[3 * s + 1] = cd;
+    clcdcm[3 * s + 2] = M;
+    if (ctl && cd_hist) cd_hist[(size_t)ctl->it * nsid + s] = cd;
   }
 }
+
+// Residual norms of a batch, one block per case: sqrt(sum over the case's wall and
+// interior points of (dU1)^2) / (its point count), and fpi_solver.F90:63-70's residue
+// with a res_old per case.  rng per case: wall range, interior range, point count.
+// Cases are independent: a NaN in one does not stop the others.
+__global__ __launch_bounds__(BLOCK) void k_finalize_batch(const double* __restrict__ rs_pt,
+                                                          const int32_t* __restrict__ rng, int ncases,
+                                                          int restart, int first_it,
+                                                          const Ctl* __restrict__ ctl,
+                                                          double* __restrict__ res_old,
+                                                          double* __restrict__ res_hist,
+                                                          double* __restrict__ resnew_hist) {
+  const int c = blockIdx.x;
+  const int* r = rng + 5 * c;
+  double v = 0.0;
+  for (int j = r[0] + threadIdx.x; j < r[1]; j += blockDim.x) v += rs_pt[j];
+  for (int j = r[2] + threadIdx.x; j < r[3]; j += blockDim.x) v += rs_pt[j];
+  const double tot = block_sum(v);
+  if (threadIdx.x == 0) {
+    const int it = ctl->it;
+    const double res_new = sqrt(tot) / (double)r[4];
+    double residue;
+    if (first_it + it <= 2 && restart == 0) {
+      res_old[c] = res_new;
+      residue = 0.0;
+    } else {
+      residue = log10(res_new / res_old[c]);
+    }
+    res_hist[(size_t)it * ncases + c] = residue;
+    resnew_hist[(size_t)it * ncases + c] = res_new;
+  }
+}
+
+__global__ void k_tick(Ctl* ctl) { ctl->it += 1; }
 
 // Sum of partials into one scalar (unfused state_update's sum_res_sqr)
 __global__ __launch_bounds__(BLOCK) void k_sum(const double* __restrict__ partial, int np,
@@ -433,6 +477,16 @@ struct mf_ctx {
   // lanes per point of the stencil kernels (1, 2, 4 or 8; mf_autotune picks them)
   int lpp_lsq = 1, lpp_inner = 1, lpp_flux = 1;
   int lpp_rev[4] = {0, 0, 0, 0};  // adjoint: flux, lsq, inner, gather (0: timed)
+  // shape ids (shapes per case x cases) and their per-id force constants
+  int nsid = 0;
+  int32_t *sh_ptr = nullptr, *sh_pos = nullptr;
+  double* shape_par = nullptr;
+  std::vector<double> h_shape_par;
+  // a batch of cases (ncases > 0): per-point case ids, freestreams, residual pieces
+  int ncases = 0;
+  std::vector<int32_t> case_first;  // original-order offset of each case [ncases+1]
+  int32_t* case_rng = nullptr;      // per case: wall range, interior range, point count
+  double *rs_pt = nullptr, *res_old_case = nullptr;
   Ctl* tune_ctl = nullptr;
 };
 
@@ -597,13 +651,20 @@ int mf_destroy(mf_ctx* c) {
   return MF_OK;
 }
 
-static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c);
+// Cases of a batch: each point's case (original order) and the cases' flow conditions
+struct BatchSpec {
+  int ncases = 0;
+  std::vector<int32_t> case_of, first;  // first[c]: original index of case c's point 1
+  std::vector<double> mach, aoa;
+};
+
+static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, const BatchSpec* bs);
 
 int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
   if (!geo || !prm || !out) return set_err(MF_ERR_INVALID, "mf_create: null argument");
   *out = nullptr;
   mf_ctx* c = new mf_ctx();
-  int rc = create_impl(geo, prm, c);
+  int rc = create_impl(geo, prm, c, nullptr);
   if (rc != MF_OK) {
     std::string keep = g_err;
     mf_destroy(c);
@@ -616,7 +677,7 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 
 }  // extern "C"
 
-static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) {
+static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, const BatchSpec* bs) {
   const int n = geo->n_points, W = geo->max_nbhs;
   if (n <= 0 || W <= 0) return set_err(MF_ERR_INVALID, "n_points and max_nbhs must be > 0");
   if ((int64_t)n >= (1ll << 27)) return set_err(MF_ERR_INVALID, "n_points must be < 2^27");
@@ -705,11 +766,16 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
       key[i] = hilbert_d(order, (uint64_t)((geo->x[i] - x0) * scale), (uint64_t)((geo->y[i] - y0) * scale));
   }
   auto cls = [&](int i) { return geo->flag_1[i] == 0 ? 0 : (geo->flag_1[i] == 2 ? 1 : 2); };
+  // a batch: by case within each class, so every case is one contiguous range of each
+  // class (its residual sums and its neighbour gathers stay together)
+  auto case_of = [&](int i) { return bs ? bs->case_of[i] : 0; };
   c->perm.resize(n);
   std::iota(c->perm.begin(), c->perm.end(), 0);
   std::stable_sort(c->perm.begin(), c->perm.end(), [&](int a, int b) {
     const int ca = cls(a), cb = cls(b);
     if (ca != cb) return ca < cb;
+    const int ka = case_of(a), kb = case_of(b);
+    if (ka != kb) return ka < kb;
     return key[a] < key[b];
   });
   c->iperm.resize(n);
@@ -782,22 +848,36 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
     obd[j] = V4d{ob[0], ob[1], ob[2], ob[3]};
   }
 
-  // ---- forces: shape points in original order (compute_force_coeffs.F90:40-78)
-  std::vector<int32_t> sh_m, sh_id;
+  // ---- forces (compute_force_coeffs.F90:40-78): shape points grouped by shape id
+  // (case * shapes + flag_2 - 1), in original order within an id
+  const int ncase_eff = bs ? bs->ncases : 1;
+  c->nsid = prm->shapes * ncase_eff;
+  std::vector<int32_t> sh_m, sh_id, sh_pos, sh_ptr(c->nsid + 1, 0);
   std::vector<double> sgeo;
-  for (int m = 0; m < n; ++m) {
-    if (geo->flag_2[m] <= 0) continue;
-    const int l = geo->left[m] - 1, r = geo->right[m] - 1;
-    const double lx = geo->x[l], ly = geo->y[l], mx = geo->x[m], my = geo->y[m];
-    const double rx = geo->x[r], ry = geo->y[r];
-    double ds1 = (mx - lx) * (mx - lx) + (my - ly) * (my - ly);
-    ds1 = std::sqrt(ds1);
-    double ds2 = (rx - mx) * (rx - mx) + (ry - my) * (ry - my);
-    ds2 = std::sqrt(ds2);
-    const double ds = 0.5 * (ds1 + ds2);
-    sh_m.push_back(c->iperm[m]);
-    sh_id.push_back(geo->flag_2[m] - 1);
-    sgeo.insert(sgeo.end(), {nx[m], ny[m], ds, mx - 0.25, my});
+  {
+    std::vector<int> order;
+    for (int m = 0; m < n; ++m)
+      if (geo->flag_2[m] > 0) order.push_back(m);
+    std::vector<int32_t> ordinal(n, -1);
+    for (size_t j = 0; j < order.size(); ++j) ordinal[order[j]] = (int32_t)j;
+    auto sid = [&](int m) { return case_of(m) * prm->shapes + geo->flag_2[m] - 1; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sid(a) < sid(b); });
+    for (int m : order) {
+      const int l = geo->left[m] - 1, r = geo->right[m] - 1;
+      const double lx = geo->x[l], ly = geo->y[l], mx = geo->x[m], my = geo->y[m];
+      const double rx = geo->x[r], ry = geo->y[r];
+      double ds1 = (mx - lx) * (mx - lx) + (my - ly) * (my - ly);
+      ds1 = std::sqrt(ds1);
+      double ds2 = (rx - mx) * (rx - mx) + (ry - my) * (ry - my);
+      ds2 = std::sqrt(ds2);
+      const double ds = 0.5 * (ds1 + ds2);
+      sh_m.push_back(c->iperm[m]);
+      sh_id.push_back(sid(m));
+      sh_pos.push_back(ordinal[m]);
+      sgeo.insert(sgeo.end(), {nx[m], ny[m], ds, mx - 0.25, my});
+      ++sh_ptr[sid(m) + 1];
+    }
+    for (int k = 0; k < c->nsid; ++k) sh_ptr[k + 1] += sh_ptr[k];
   }
   c->nshape = (int)sh_m.size();
   c->h_sh_m = sh_m;
@@ -811,11 +891,26 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
     }
   }
 
-  // ---- parameters: parameter.F90:16-34, compute_force_coeffs.F90:34
-  c->theta = prm->aoa * kPi / 180.0;
-  c->fs.u1_inf = prm->mach * std::cos(c->theta);
-  c->fs.u2_inf = prm->mach * std::sin(c->theta);
-  c->cp_scale = 0.5 * 1.0 * prm->mach * prm->mach;
+  // ---- parameters: parameter.F90:16-34, compute_force_coeffs.F90:34, per case
+  std::vector<Freestream> fs_case(ncase_eff);
+  std::vector<double> th_case(ncase_eff), cps_case(ncase_eff);
+  for (int k = 0; k < ncase_eff; ++k) {
+    const double mach = bs ? bs->mach[k] : prm->mach, aoa = bs ? bs->aoa[k] : prm->aoa;
+    th_case[k] = aoa * kPi / 180.0;
+    fs_case[k].u1_inf = mach * std::cos(th_case[k]);
+    fs_case[k].u2_inf = mach * std::sin(th_case[k]);
+    cps_case[k] = 0.5 * 1.0 * mach * mach;
+  }
+  c->theta = th_case[0];
+  c->fs = fs_case[0];
+  c->cp_scale = cps_case[0];
+  c->h_shape_par.resize(3 * (size_t)c->nsid);
+  for (int sd = 0; sd < c->nsid; ++sd) {
+    const int k = sd / prm->shapes;
+    c->h_shape_par[3 * sd] = cps_case[k];
+    c->h_shape_par[3 * sd + 1] = std::cos(th_case[k]);
+    c->h_shape_par[3 * sd + 2] = std::sin(th_case[k]);
+  }
 
   // ---- device
   HIPCHK(hipGetDevice(&c->device));
@@ -870,17 +965,50 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c) 
   HIPCHK(hipMemcpy(c->ctl, &ctl0, sizeof(Ctl), hipMemcpyHostToDevice));
   if (c->nshape > 0) {
     if ((rc = upload(c, &c->sh_m, sh_m)) || (rc = upload(c, &c->sh_id, sh_id)) ||
-        (rc = upload(c, &c->sgeo, sgeo)) || (rc = dev_alloc(c, &c->cp, c->nshape)))
+        (rc = upload(c, &c->sh_pos, sh_pos)) || (rc = upload(c, &c->sgeo, sgeo)) ||
+        (rc = dev_alloc(c, &c->cp, c->nshape)))
       return rc;
   }
-  if ((rc = dev_alloc(c, &c->clcdcm, 3 * prm->shapes))) return rc;
+  if ((rc = upload(c, &c->sh_ptr, sh_ptr)) || (rc = upload(c, &c->shape_par, c->h_shape_par)) ||
+      (rc = dev_alloc(c, &c->clcdcm, 3 * (size_t)c->nsid)))
+    return rc;
 
-  // ---- initial_conditions.F90:9-24: freestream
+  // ---- a batch: case of every point, the cases' freestreams, residual ranges
+  if (bs) {
+    c->ncases = bs->ncases;
+    c->case_first = bs->first;
+    std::vector<int32_t> cid(n), rng(5 * (size_t)bs->ncases, 0);
+    for (int j = 0; j < n; ++j) cid[j] = bs->case_of[c->perm[j]];
+    // points are sorted (class, case): each case is one range of the wall and of the
+    // interior classes
+    for (int k = 0; k < bs->ncases; ++k) {
+      rng[5 * k] = rng[5 * k + 1] = 0;
+      rng[5 * k + 2] = rng[5 * k + 3] = c->nw + c->no;
+      rng[5 * k + 4] = bs->first[k + 1] - bs->first[k];
+    }
+    for (int j = c->nw - 1; j >= 0; --j) rng[5 * cid[j]] = j;
+    for (int j = 0; j < c->nw; ++j) rng[5 * cid[j] + 1] = j + 1;
+    for (int j = n - 1; j >= c->nw + c->no; --j) rng[5 * cid[j] + 2] = j;
+    for (int j = c->nw + c->no; j < n; ++j) rng[5 * cid[j] + 3] = j + 1;
+    int32_t* d_cid;
+    Freestream* d_fs;
+    if ((rc = upload(c, &d_cid, cid)) || (rc = upload(c, &d_fs, fs_case)) || (rc = upload(c, &c->case_rng, rng)) ||
+        (rc = dev_alloc(c, &c->rs_pt, n)) || (rc = dev_alloc(c, &c->res_old_case, bs->ncases)))
+      return rc;
+    HIPCHK(hipMemset(c->rs_pt, 0, (size_t)n * sizeof(double)));
+    std::vector<double> ro(bs->ncases, prm->restart ? prm->res_old : 0.0);
+    HIPCHK(hipMemcpy(c->res_old_case, ro.data(), ro.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->g.case_id = d_cid;
+    c->g.fs_tab = d_fs;
+  }
+
+  // ---- initial_conditions.F90:9-24: freestream (of the point's case)
   std::vector<double> prim0((size_t)4 * n);
   for (int i = 0; i < n; ++i) {
+    const Freestream& f = fs_case[case_of(i)];
     prim0[4 * i] = 1.0;
-    prim0[4 * i + 1] = c->fs.u1_inf;
-    prim0[4 * i + 2] = c->fs.u2_inf;
+    prim0[4 * i + 1] = f.u1_inf;
+    prim0[4 * i + 2] = f.u2_inf;
     prim0[4 * i + 3] = 1.0 / 1.4;
   }
   if ((rc = upload_prim_original(c, prim0.data()))) return rc;
@@ -963,7 +1091,7 @@ static void launch_inner(mf_ctx* c, int qbuf, int din, const Ctl* ctl) {
 template <class S>
 static void launch_flux_update_on(mf_ctx* c, const State<S>& st, const V4<S>* prim, V4<S>* pold, V4<S>* pout,
                                   V4<S>* qout, double* cspeed, const double* delta, double* partial,
-                                  const Ctl* ctl, int rk, bool last, int lpp) {
+                                  const Ctl* ctl, int rk, bool last, int lpp, double* rs_pt = nullptr) {
   const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
   const bool p0 = c->power0;
   const int lst = last ? 1 : 0;
@@ -972,10 +1100,10 @@ static void launch_flux_update_on(mf_ctx* c, const State<S>& st, const V4<S>* pr
   with_lpp(lpp, [&]<int L>() {
     if (p0)
       hipLaunchKernelGGL((k_flux_update<S, true, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs);
+                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
     else
       hipLaunchKernelGGL((k_flux_update<S, false, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs);
+                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
   });
 }
 
@@ -983,17 +1111,24 @@ template <class S>
 static void launch_flux_update(mf_ctx* c, int buf, int dqbuf, int rk, bool last) {
   launch_flux_update_on<S>(c, state_of<S>(c, buf, dqbuf), (const V4<S>*)c->prim[buf], (V4<S>*)c->prim_old,
                            (V4<S>*)c->prim[buf ^ 1], (V4<S>*)c->q[buf ^ 1], c->cspeed, c->delta, c->partial,
-                           c->ctl, rk, last, c->lpp_flux);
+                           c->ctl, rk, last, c->lpp_flux, c->ncases > 0 ? c->rs_pt : nullptr);
 }
 
 template <class S>
 static void launch_forces(mf_ctx* c, int buf, const Ctl* ctl, double* cd_hist) {
-  hipLaunchKernelGGL((k_forces<S>), dim3(1), dim3(BLOCK), 0, c->stream, (const V4<S>*)c->prim[buf],
-                     c->sh_m, c->sh_id, c->sgeo, c->nshape, c->p.shapes, c->cp_scale,
-                     std::cos(c->theta), std::sin(c->theta), c->cp, c->clcdcm, cd_hist, ctl);
+  hipLaunchKernelGGL((k_forces<S>), dim3(c->nsid), dim3(BLOCK), 0, c->stream, (const V4<S>*)c->prim[buf],
+                     c->sh_m, c->sh_pos, c->sgeo, c->sh_ptr, c->shape_par, c->nsid, c->cp, c->clcdcm,
+                     cd_hist, ctl);
 }
 
 static void launch_finalize(mf_ctx* c) {
+  if (c->ncases > 0) {
+    hipLaunchKernelGGL(k_finalize_batch, dim3(c->ncases), dim3(BLOCK), 0, c->stream, c->rs_pt, c->case_rng,
+                       c->ncases, c->p.restart, c->graph_first_it, c->ctl, c->res_old_case, c->res_hist,
+                       c->resnew_hist);
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, c->stream, c->ctl);
+    return;
+  }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial,
                      (int)lane_blocks(c, c->lpp_flux).x, c->n, c->p.restart, c->graph_first_it, c->ctl,
                      c->res_hist, c->resnew_hist);
@@ -1069,9 +1204,10 @@ static int ensure_history(mf_ctx* c, int iters) {
       gexec = nullptr;
     }
   int cap = std::max(iters, 1024);
+  const size_t per = (size_t)std::max(c->ncases, 1);
   int rc;
-  if ((rc = dev_alloc(c, &c->res_hist, cap)) || (rc = dev_alloc(c, &c->resnew_hist, cap)) ||
-      (rc = dev_alloc(c, &c->cd_hist, cap)))
+  if ((rc = dev_alloc(c, &c->res_hist, cap * per)) || (rc = dev_alloc(c, &c->resnew_hist, cap * per)) ||
+      (rc = dev_alloc(c, &c->cd_hist, cap * (size_t)std::max(c->nsid, 1))))
     return rc;
   c->hist_cap = cap;
   return MF_OK;
@@ -1134,6 +1270,7 @@ int mf_sync(mf_ctx* c) {
 int mf_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue_hist,
            double* res_new_hist, double* cd_hist, int32_t* iters_done) {
   if (!c || iters < 0) return set_err(MF_ERR_INVALID, "mf_run: bad argument");
+  if (c->ncases > 0) return set_err(MF_ERR_STATE, "a batch context runs with mf_batch_run");
   const int start = c->cur;
   int rc = run_queue(c, first_it, iters);
   if (rc) return rc;
@@ -1144,7 +1281,11 @@ int mf_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue_hist,
   c->cur = start ^ ((done * c->p.rks) & 1);
   if (residue_hist && done) HIPCHK(hipMemcpy(residue_hist, c->res_hist, done * sizeof(double), hipMemcpyDeviceToHost));
   if (res_new_hist && done) HIPCHK(hipMemcpy(res_new_hist, c->resnew_hist, done * sizeof(double), hipMemcpyDeviceToHost));
-  if (cd_hist && done && c->nshape > 0) HIPCHK(hipMemcpy(cd_hist, c->cd_hist, done * sizeof(double), hipMemcpyDeviceToHost));
+  if (cd_hist && done && c->nshape > 0) {
+    std::vector<double> h((size_t)done * c->nsid);
+    HIPCHK(hipMemcpy(h.data(), c->cd_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < done; ++j) cd_hist[j] = h[(size_t)j * c->nsid];
+  }
   if (iters_done) *iters_done = done;
   return MF_OK;
 }
@@ -1296,6 +1437,95 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   for (int k = 0; k < 4; ++k) c->lpp_rev[k] = rv[k];
   adj_forget_tuning(c);
   drop_graphs(c);
+  return MF_OK;
+}
+
+int mf_batch_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue, double* res_new, double* cd) {
+  if (!c || iters < 0) return set_err(MF_ERR_INVALID, "mf_batch_run: bad argument");
+  if (c->ncases == 0) return set_err(MF_ERR_STATE, "mf_batch_run needs a batch context (mf_create_batch)");
+  const int start = c->cur;
+  int rc = run_queue(c, first_it, iters);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cur = start ^ ((iters * c->p.rks) & 1);
+  const size_t nr = (size_t)iters * c->ncases;
+  if (residue && nr) HIPCHK(hipMemcpy(residue, c->res_hist, nr * sizeof(double), hipMemcpyDeviceToHost));
+  if (res_new && nr) HIPCHK(hipMemcpy(res_new, c->resnew_hist, nr * sizeof(double), hipMemcpyDeviceToHost));
+  if (cd && iters && c->nshape > 0)
+    HIPCHK(hipMemcpy(cd, c->cd_hist, (size_t)iters * c->nsid * sizeof(double), hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+int mf_batch_cases(mf_ctx* c, int32_t* ncases, int32_t* first) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
+  if (ncases) *ncases = c->ncases;
+  if (first)
+    for (size_t k = 0; k < c->case_first.size(); ++k) first[k] = c->case_first[k];
+  return MF_OK;
+}
+
+int mf_create_batch(const mf_case* cases, int32_t ncases, const mf_params* prm, mf_ctx** out) {
+  if (!cases || !prm || !out || ncases < 1) return set_err(MF_ERR_INVALID, "mf_create_batch: bad argument");
+  *out = nullptr;
+  // one cloud of the cases' points, in case order, with their links offset
+  BatchSpec bs;
+  bs.ncases = ncases;
+  bs.first.assign(ncases + 1, 0);
+  int W = 0;
+  int64_t total = 0;
+  for (int k = 0; k < ncases; ++k) {
+    const mf_geometry* g = cases[k].geometry;
+    if (!g || g->n_points <= 0 || g->max_nbhs <= 0 || !g->x || !g->y || !g->min_dist || !g->left || !g->right ||
+        !g->flag_1 || !g->flag_2 || !g->nbhs || !g->conn)
+      return set_err(MF_ERR_INVALID, "case %d: incomplete geometry", k + 1);
+    W = std::max(W, (int)g->max_nbhs);
+    total += g->n_points;
+    if (total >= (1ll << 27)) return set_err(MF_ERR_INVALID, "a batch must hold fewer than 2^27 points");
+    bs.first[k + 1] = (int32_t)total;
+    bs.mach.push_back(cases[k].mach);
+    bs.aoa.push_back(cases[k].aoa);
+  }
+  const int n = (int)total;
+  std::vector<double> x(n), y(n), fnx(n, 0.0), fny(n, 1.0), md(n);
+  std::vector<int32_t> left(n), right(n), f1(n), f2(n), nb(n), conn((size_t)W * n, 0);
+  bs.case_of.resize(n);
+  for (int k = 0; k < ncases; ++k) {
+    const mf_geometry* g = cases[k].geometry;
+    const int o = bs.first[k], m = g->n_points;
+    for (int i = 0; i < m; ++i) {
+      const int j = o + i;
+      x[j] = g->x[i];
+      y[j] = g->y[i];
+      if (g->nx && g->ny) {
+        fnx[j] = g->nx[i];
+        fny[j] = g->ny[i];
+      }
+      md[j] = g->min_dist[i];
+      left[j] = g->left[i] > 0 ? g->left[i] + o : 0;
+      right[j] = g->right[i] > 0 ? g->right[i] + o : 0;
+      f1[j] = g->flag_1[i];
+      f2[j] = g->flag_2[i];
+      nb[j] = g->nbhs[i];
+      if (nb[j] < 0 || nb[j] > g->max_nbhs) return set_err(MF_ERR_INVALID, "case %d point %d: bad nbhs", k + 1, i + 1);
+      for (int s = 0; s < nb[j]; ++s) {
+        const int32_t v = g->conn[(size_t)s * m + i];
+        if (v < 1 || v > m) return set_err(MF_ERR_INVALID, "case %d point %d: conn(%d) = %d out of range", k + 1, i + 1, s + 1, v);
+        conn[(size_t)s * n + j] = v + o;
+      }
+      bs.case_of[j] = k;
+    }
+  }
+  const mf_geometry merged{n, W, x.data(), y.data(), fnx.data(), fny.data(), md.data(), left.data(), right.data(),
+                           f1.data(), f2.data(), nb.data(), conn.data()};
+  mf_ctx* c = new mf_ctx();
+  int rc = create_impl(&merged, prm, c, &bs);
+  if (rc != MF_OK) {
+    std::string keep = g_err;
+    mf_destroy(c);
+    g_err = keep;
+    return rc;
+  }
+  *out = c;
   return MF_OK;
 }
 
@@ -1465,9 +1695,9 @@ int mf_compute_cl_cd_cm(mf_ctx* c, double* cl, double* cd, double* cm, double* c
   else launch_forces<float>(c, c->cur, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<double> h(3 * c->p.shapes);
+  std::vector<double> h(3 * (size_t)c->nsid);
   HIPCHK(hipMemcpy(h.data(), c->clcdcm, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int s = 0; s < c->p.shapes; ++s) {
+  for (int s = 0; s < c->nsid; ++s) {
     if (cl) cl[s] = h[3 * s];
     if (cd) cd[s] = h[3 * s + 1];
     if (cm) cm[s] = h[3 * s + 2];

@@ -330,7 +330,10 @@ __global__ __launch_bounds__(BLOCK) void k_update_rev(Geom g, const V4d* __restr
   double nxb = 0.0, nyb = 0.0;
   const V4d p = prim[i], po = prim_old[i], b = pnb[i];
   if (i < g.nw) update_rev<WALL>(p, po, pn, nn.x, nn.y, rk, euler, fs, b, pbv, poldbv, frbv, nxb, nyb);
-  else if (i < g.nw + g.no) update_rev<OUTER>(p, po, pn, nn.x, nn.y, rk, euler, fs, b, pbv, poldbv, frbv, nxb, nyb);
+  else if (i < g.nw + g.no) {
+    const Freestream f = g.fs_tab ? g.fs_tab[g.case_id[i]] : fs;
+    update_rev<OUTER>(p, po, pn, nn.x, nn.y, rk, euler, f, b, pbv, poldbv, frbv, nxb, nyb);
+  }
   else update_rev<INTERIOR>(p, po, pn, nn.x, nn.y, rk, euler, fs, b, pbv, poldbv, frbv, nxb, nyb);
   pb[i] = V4d{pbv[0], pbv[1], pbv[2], pbv[3]};
   if (rk == 3) {
@@ -725,14 +728,17 @@ __global__ __launch_bounds__(BLOCK) void k_delta_rev(Geom g, const V4d* __restri
 // reverse of compute_cl_cd_cm for J = sum over shapes of Cd (compute_force_coeffs_b.F90)
 __global__ __launch_bounds__(BLOCK) void k_forces_rev(const V4d* __restrict__ prim,
                                                       const int32_t* __restrict__ sh_m,
+                                                      const int32_t* __restrict__ sh_id,
                                                       const int32_t* __restrict__ sh_l,
                                                       const int32_t* __restrict__ sh_r,
                                                       const double* __restrict__ sgeo, int nsh,
-                                                      double cp_scale, double ct, double st,
+                                                      const double* __restrict__ shape_par,
                                                       const D2* __restrict__ xy, V4d* __restrict__ pb,
                                                       D2* __restrict__ xyb, D2* __restrict__ nrmb) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nsh) return;
+  const double* sp = shape_par + 3 * sh_id[j];
+  const double cp_scale = sp[0], ct = sp[1], st = sp[2];
   const double pr_inf = 1.0 / 1.4;
   const int m = sh_m[j], l = sh_l[j], r = sh_r[j];
   const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
@@ -826,6 +832,8 @@ struct AdjState {
   double J = 0.0;
   int64_t fwd_steps = 0, taped = 0;
   int peak_snaps = 0;
+  // progress lines on stderr (environment MFCFD_PROGRESS=1): about 20 per sweep
+  int64_t progress_every = 0;
   // lanes per point of the reverse kernels, timed once per context (adj_tune)
   bool tuned = false;
   int lpp_frev = 1, lpp_lrev = 1, lpp_irev = 1, lpp_gath = 1;
@@ -1097,7 +1105,13 @@ static int adj_taped_reverse(mf_ctx* c, AdjState& A) {
   int rc;
   if ((rc = adj_record(c, A))) return rc;
   A.taped += 1;
-  return adj_reverse_iteration(c, A);
+  if ((rc = adj_reverse_iteration(c, A))) return rc;
+  if (A.progress_every > 0 && A.taped % A.progress_every == 0) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::fprintf(stderr, "[mfcfd] adjoint: %lld of %d iterations reversed\n", (long long)A.taped, A.iters);
+    std::fflush(stderr);
+  }
+  return MF_OK;
 }
 
 // Reverse iterations [t0, t1); snapshot `slot0` holds prim(t0); `free_slots` more free
@@ -1190,7 +1204,7 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   HIPCHK(hipEventSynchronize(A.e1));
   HIPCHK(hipEventElapsedTime(&A.primal_ms, A.e0, A.e1));
   A.peak_snaps = (int)spine.size() + 1;
-  std::vector<double> cl(c->p.shapes), cd(c->p.shapes), cm(c->p.shapes);
+  std::vector<double> cl(c->nsid), cd(c->nsid), cm(c->nsid);
   if ((rc = mf_compute_cl_cd_cm(c, cl.data(), cd.data(), cm.data(), nullptr))) return rc;
   A.J = 0.0;
   for (double v : cd) A.J += v;
@@ -1204,6 +1218,11 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
   AdjState& A = *c->adj;
   const int n = c->n;
   int rc;
+  {
+    const char* e = std::getenv("MFCFD_PROGRESS");
+    A.progress_every = (e && e[0] == '1') ? std::max<int64_t>(1, A.iters / 20) : 0;
+  }
+  A.taped = 0;
   hipEvent_t b0, b1;
   HIPCHK(hipEventCreate(&b0));
   HIPCHK(hipEventCreate(&b1));
@@ -1220,8 +1239,8 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
     return rc;
   // seed: J = sum of Cd over shapes on the final state (objective_function_b.F90:22-23)
   hipLaunchKernelGGL(k_forces_rev, dim3((c->nshape + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream,
-                     (const V4d*)c->prim[c->cur], c->sh_m, A.sh_l, A.sh_r, c->sgeo, c->nshape, c->cp_scale,
-                     std::cos(c->theta), std::sin(c->theta), c->g.xy, A.Pb, A.xyb, A.nrmb);
+                     (const V4d*)c->prim[c->cur], c->sh_m, c->sh_id, A.sh_l, A.sh_r, c->sgeo, c->nshape,
+                     (const double*)c->shape_par, c->g.xy, A.Pb, A.xyb, A.nrmb);
   HIPCHK(hipGetLastError());
   if ((rc = adj_rev_range(c, A, 0, A.iters, 0, (int)A.snap.size() - 1))) return rc;
   // normals depend on x, y of the boundary neighbours (point_normals_b.F90)

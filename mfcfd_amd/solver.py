@@ -52,28 +52,36 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _geometry(cloud: PointCloud):
+    """mf_geometry of a cloud in the reference's conventions (1-based indices, conn(N, W)
+    column-major); returns the struct and the arrays it points into."""
+    conn_f = np.where(cloud.conn >= 0, cloud.conn + 1, 0).astype(np.int32)
+    keep = dict(
+        x=np.ascontiguousarray(cloud.x, dtype=np.float64),
+        y=np.ascontiguousarray(cloud.y, dtype=np.float64),
+        nx=np.ascontiguousarray(cloud.nx, dtype=np.float64),
+        ny=np.ascontiguousarray(cloud.ny, dtype=np.float64),
+        min_dist=np.ascontiguousarray(cloud.min_dist, dtype=np.float64),
+        left=(cloud.left + 1).astype(np.int32), right=(cloud.right + 1).astype(np.int32),
+        flag_1=cloud.flag_1.astype(np.int32), flag_2=cloud.flag_2.astype(np.int32),
+        nbhs=cloud.nbhs.astype(np.int32), conn=np.asfortranarray(conn_f).ravel(order="F"))
+    g = L.MfGeometry(n_points=cloud.n, max_nbhs=cloud.max_nbhs,
+                     **{k: v.ctypes.data for k, v in keep.items()})
+    return g, keep
+
+
 class LskumSolver:
     """One point cloud on the GPU, with the reference's hot-path procedures."""
 
     def __init__(self, cloud: PointCloud, **params):
         self.cloud = cloud
         self.params = make_params(**params)
-        n, w = cloud.n, cloud.max_nbhs
-        # reference conventions: 1-based indices, conn(N, W) column-major
-        conn_f = np.where(cloud.conn >= 0, cloud.conn + 1, 0).astype(np.int32)
-        self._keep = dict(
-            x=np.ascontiguousarray(cloud.x, dtype=np.float64),
-            y=np.ascontiguousarray(cloud.y, dtype=np.float64),
-            nx=np.ascontiguousarray(cloud.nx, dtype=np.float64),
-            ny=np.ascontiguousarray(cloud.ny, dtype=np.float64),
-            min_dist=np.ascontiguousarray(cloud.min_dist, dtype=np.float64),
-            left=(cloud.left + 1).astype(np.int32), right=(cloud.right + 1).astype(np.int32),
-            flag_1=cloud.flag_1.astype(np.int32), flag_2=cloud.flag_2.astype(np.int32),
-            nbhs=cloud.nbhs.astype(np.int32), conn=np.asfortranarray(conn_f).ravel(order="F"))
-        g = L.MfGeometry(n_points=n, max_nbhs=w,
-                         **{k: v.ctypes.data for k, v in self._keep.items()})
+        g, self._keep = _geometry(cloud)
         h = C.c_void_p()
         L.check(L.lib().mf_create(C.byref(g), C.byref(self.params), C.byref(h)), "mf_create")
+        self._init_handle(h, cloud.n)
+
+    def _init_handle(self, h, n):
         self.h = h
         self.n = n
         self.shapes = self.params.shapes
@@ -229,3 +237,51 @@ class LskumSolver:
         L.check(L.lib().mf_adjoint_backward(self.h, _dp(dx), _dp(dy), C.byref(st)),
                 "mf_adjoint_backward")
         return dx, dy, {f: getattr(st, f) for f, _ in L.MfAdjointStats._fields_}
+
+
+class LskumBatch(LskumSolver):
+    """Several independent flow cases in one context (mf_create_batch): every launch
+    advances all of them.  ``cases`` is a list of (cloud, mach, aoa); the other options
+    are shared.  Per-point arrays are the cases' points concatenated in case order."""
+
+    def __init__(self, cases, **params):
+        self.params = make_params(**params)
+        self._geoms = [_geometry(c) for c, _, _ in cases]
+        arr = (L.MfCase * len(cases))()
+        for k, ((cloud, mach, aoa), (g, _)) in enumerate(zip(cases, self._geoms)):
+            arr[k].geometry = C.pointer(g)
+            arr[k].mach = float(mach)
+            arr[k].aoa = float(aoa)
+        self._cases_arr = arr
+        h = C.c_void_p()
+        L.check(L.lib().mf_create_batch(arr, len(cases), C.byref(self.params), C.byref(h)), "mf_create_batch")
+        self.ncases = len(cases)
+        self.clouds = [c for c, _, _ in cases]
+        self._init_handle(h, sum(c.n for c in self.clouds))
+        first = np.zeros(self.ncases + 1, dtype=np.int32)
+        nc = C.c_int32()
+        L.check(L.lib().mf_batch_cases(self.h, C.byref(nc), first.ctypes.data_as(C.POINTER(C.c_int32))),
+                "mf_batch_cases")
+        self.first = first
+
+    def run(self, iters: int, first_it: int = 1):
+        """(residue, res_new) of shape (iters, ncases) and cd of shape (iters, ncases, shapes)."""
+        res = np.empty((max(iters, 1), self.ncases))
+        rn = np.empty_like(res)
+        cd = np.empty((max(iters, 1), self.ncases, self.shapes))
+        L.check(L.lib().mf_batch_run(self.h, int(first_it), int(iters), _dp(res), _dp(rn), _dp(cd)),
+                "mf_batch_run")
+        return res[:iters], rn[:iters], cd[:iters]
+
+    def compute_cl_cd_cm(self):
+        """Cl, Cd, Cm of shape (ncases, shapes) and cp of every shape point."""
+        k = self.ncases * self.shapes
+        cl, cd, cm = (np.empty(k) for _ in range(3))
+        cp = np.empty(max(self.n_shape, 1))
+        L.check(L.lib().mf_compute_cl_cd_cm(self.h, _dp(cl), _dp(cd), _dp(cm), _dp(cp)),
+                "mf_compute_cl_cd_cm")
+        shp = (self.ncases, self.shapes)
+        return cl.reshape(shp), cd.reshape(shp), cm.reshape(shp), cp[: self.n_shape]
+
+    def case_slice(self, k: int) -> slice:
+        return slice(int(self.first[k]), int(self.first[k + 1]))
