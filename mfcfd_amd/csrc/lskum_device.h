@@ -209,6 +209,74 @@ __host__ __device__ inline Kin<S> kin_of_qtilde(const S qt[4], S nx, S ny) {
   return k;
 }
 
+// ---------------------------------------------------------------------------------
+// Single-precision math of the mixed path.  That path is held to |dCl|, |dCd| <= 1e-4
+// of the FP64 path, not to ulp parity, so it uses the hardware transcendental
+// instructions (exp2, log2, reciprocal and reciprocal square root, each about 1 ulp)
+// instead of the correctly rounded library sequences, and erf from the exp(-s^2) the
+// fluxes need anyway.  The FP64 path keeps the library functions and the reference's
+// operation order.
+__device__ inline float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ inline float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// erf(s) given ex = exp(-s^2): Abramowitz & Stegun 7.1.26, erf(x) = 1 - t (a1 + t (a2 +
+// t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x), x >= 0, |error| <= 1.5e-7; odd in s
+__device__ inline float ferf_from_exp(float s, float ex) {
+  const float x = fabsf(s);
+  const float t = frcp(1.0f + 0.3275911f * x);
+  const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float e = 1.0f - poly * ex;
+  return s < 0.0f ? -e : e;
+}
+
+// The flux preliminaries (q-tilde -> primitive -> split-flux pieces) of the mixed
+// path: kin_of_qtilde's quantities with the identities b = rho / (2 p) = beta and
+// p / rho = 1 / (2 beta) used instead of recomputing them
+__device__ inline Kin<float> kin_fast(const float qt[4], float nx, float ny) {
+  const float beta = -qt[3] * 0.5f;
+  const float tmp = 0.5f * frcp(beta);
+  const float u1 = qt[1] * tmp, u2 = qt[2] * tmp;
+  const float temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
+  const float rho = fexp(temp1 - flog(beta) * 2.5f);  // 1 / (gamma - 1) = 2.5
+  Kin<float> k;
+  k.ut = u1 * ny - u2 * nx;
+  k.un = u1 * nx + u2 * ny;
+  const float rs = frsq(beta);
+  const float sb = beta * rs;
+  const float S1 = k.ut * sb, S2 = k.un * sb;
+  const float ex1 = fexp(-S1 * S1), ex2 = fexp(-S2 * S2);
+  const float half_inv_spb = 0.282094792f * rs;  // 0.5 / sqrt(pi beta)
+  k.B1 = ex1 * half_inv_spb;
+  k.B2 = ex2 * half_inv_spb;
+  k.e1 = ferf_from_exp(S1, ex1);
+  k.e2 = ferf_from_exp(S2, ex2);
+  k.pr_by_rho = tmp;
+  k.u_sqr = k.ut * k.ut + k.un * k.un;
+  k.rho = rho;
+  return k;
+}
+
+// venkat with a reciprocal in place of the division (mixed path)
+__device__ inline float venkat_fast(float qt, float qk, float qmax, float qmin, float epsi) {
+  const float del_neg = qt - qk;
+  if (fabsf(del_neg) <= Consts<float>::THRESH) return 1.0f;
+  const float del_pos = (del_neg > 0.0f) ? qmax - qk : qmin - qk;
+  const float e2 = epsi * epsi;
+  const float num = (del_pos * del_pos + e2) * del_neg + 2.0f * del_neg * del_neg * del_pos;
+  const float den = (del_pos * del_pos + 2.0f * del_neg * del_neg + del_neg * del_pos + e2) * del_neg;
+  const float t = num * frcp(den);
+  return (t < 1.0f) ? t : 1.0f;
+}
+
+// The stencil kernels' entry points: FP64 keeps the reference's routines, the mixed
+// path takes the fast ones
+__host__ __device__ inline Kin<double> kin_eval(const double qt[4], double nx, double ny) {
+  return kin_of_qtilde(qt, nx, ny);
+}
+__device__ inline Kin<float> kin_eval(const float qt[4], float nx, float ny) { return kin_fast(qt, nx, ny); }
+
 // x +- y with the sign chosen per lane: bitwise equal to the reference's separate
 // "+" and "-" routines (x + (-y) == x - y in IEEE arithmetic)
 template <class S>
@@ -286,6 +354,23 @@ __host__ __device__ inline void reconstruct(const V4<S>& q, const V4<S>& dx, con
     const S d = delx * dx[r] + dely * dy[r];
     const S phi = venkat(q[r] - (S)0.5 * d, q[r], qmax[r], qmin[r], eps);
     qt[r] = q[r] - (S)0.5 * phi * d;
+  }
+}
+
+// reconstruct for the stencil kernels (see kin_eval)
+__host__ __device__ inline void reconstruct_eval(const V4d& q, const V4d& dx, const V4d& dy, const V4d& qmax,
+                                                 const V4d& qmin, double eps, double delx, double dely,
+                                                 double qt[4]) {
+  reconstruct(q, dx, dy, qmax, qmin, eps, delx, dely, qt);
+}
+__device__ inline void reconstruct_eval(const V4<float>& q, const V4<float>& dx, const V4<float>& dy,
+                                        const V4<float>& qmax, const V4<float>& qmin, float eps, float delx,
+                                        float dely, float qt[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float d = delx * dx[r] + dely * dy[r];
+    const float phi = venkat_fast(q[r] - 0.5f * d, q[r], qmax[r], qmin[r], eps);
+    qt[r] = q[r] - 0.5f * phi * d;
   }
 }
 
@@ -388,11 +473,11 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
         qtk[r] = qk[r];
       }
     } else {
-      reconstruct(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
+      reconstruct_eval(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
       const V4<S> qk = st.q[k];
       const V4<S> dxk = st.dq[2 * k], dyk = st.dq[2 * k + 1];
       const V4<S> qmaxk = st.qm[2 * k], qmink = st.qm[2 * k + 1];
-      reconstruct(qk, dxk, dyk, qmaxk, qmink, st.eps[k], dx, dy, qtk);
+      reconstruct_eval(qk, dxk, dyk, qmaxk, qmink, st.eps[k], dx, dy, qtk);
     }
     // the two ends one after the other: one end's transcendental temporaries live
     // at a time
@@ -403,7 +488,7 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
 #pragma unroll
       for (int r = 0; r < 4; ++r) qt[r] = e ? qtk[r] : qti[r];
       S Gx[4], Gy[4];
-      end_fluxes<CLS>(kin_of_qtilde(qt, nxs, nys), xp, yp, Gx, Gy);
+      end_fluxes<CLS>(kin_eval(qt, nxs, nys), xp, yp, Gx, Gy);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (e) {
@@ -427,9 +512,9 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
       const double cx2 = tie_x ? coef_x(hc, false, dsw, dnw) : 0.0;
       const double cy2 = tie_y ? coef_y(hc, false, dsw, dnw) : 0.0;
       S Gx[4], Gy[4];
-      end_fluxes<CLS>(kin_of_qtilde(qti, nxs, nys), false, false, Gx, Gy);
+      end_fluxes<CLS>(kin_eval(qti, nxs, nys), false, false, Gx, Gy);
       S Hx[4], Hy[4];
-      end_fluxes<CLS>(kin_of_qtilde(qtk, nxs, nys), false, false, Hx, Hy);
+      end_fluxes<CLS>(kin_eval(qtk, nxs, nys), false, false, Hx, Hy);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (tie_x) acc[r] = acc[r] + ((double)Hx[r] - (double)Gx[r]) * cx2;
