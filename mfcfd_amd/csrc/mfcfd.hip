@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -39,12 +40,15 @@ __global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __res
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
 // (state_update.F90:190-236), which gathers over the same stencil.
+// qsum (mixed path): the first-pass sums sum w (dx, dy) (q_k - q_i) are also stored, for
+// the inner iterations (q does not change between them).
 template <class S, bool POWER0, bool DELTA, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__ q,
                                                V4<S>* __restrict__ dq, V4<S>* __restrict__ qm,
                                                const double* __restrict__ cspeed,
                                                double* __restrict__ delta, double cfl,
-                                               double power, const Ctl* __restrict__ ctl) {
+                                               double power, const Ctl* __restrict__ ctl,
+                                               V4<float>* __restrict__ qsum) {
   if (ctl && ctl->halt) return;
   const int t = point_id();
   const int i = t / LPP, lane = t % LPP;
@@ -107,25 +111,59 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   qm[2 * i] = qmax;
   qm[2 * i + 1] = qmin;
   if (DELTA) delta[i] = min_delt;
+  if (qsum) {
+    qsum[2 * i] = V4<float>{(float)sxq[0], (float)sxq[1], (float)sxq[2], (float)sxq[3]};
+    qsum[2 * i + 1] = V4<float>{(float)syq[0], (float)syq[1], (float)syq[2], (float)syq[3]};
+  }
 }
 
 // q_variables.F90:115-227 eval_q_inner_loop + eval_update_innerloop (Jacobi: reads
-// dq_in, writes dq_out)
-template <class S, bool POWER0, int LPP>
+// dq_in, writes dq_out).  SUMS (mixed path): the edge sums of w (dx, dy) ((q_k - q_i)
+// - 0.5 [dx (dq_x,k - dq_x,i) + dy (dq_y,k - dq_y,i)]) start from the LSQ kernel's
+// first-pass sums, so only the neighbours' dq and coordinates are gathered; the FP64
+// path keeps the reference's per-edge form.
+template <class S, bool POWER0, int LPP, bool SUMS>
 __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict__ q,
                                                  const V4<S>* __restrict__ dq_in,
                                                  V4<S>* __restrict__ dq_out, double power,
-                                                 const Ctl* __restrict__ ctl) {
+                                                 const Ctl* __restrict__ ctl,
+                                                 const V4<float>* __restrict__ qsum) {
   if (ctl && ctl->halt) return;
   const int t = point_id();
   const int i = t / LPP, lane = t % LPP;
   if (i >= g.n) return;
   const D2 pi = g.xy[i];
-  const V4<S> qi = q[i], dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
+  const V4<S> dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = lane; s < nb; s += LPP) {
+  if (SUMS) {
+    if (lane == 0) {
+      const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sxq[r] = a[r];
+        syq[r] = b[r];
+      }
+    }
+    for (int s = lane; s < nb; s += LPP) {
+      const int k = (int)(cp[(size_t)s * g.n] >> 4);
+      const D2 pk = g.xy[k];
+      const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+      double w = 1.0;
+      if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+      const V4<S> dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
+      const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double d = delx * (double)(dxk[r] - dxi[r]) + dely * (double)(dyk[r] - dyi[r]);
+        sxq[r] -= hx * d;
+        syq[r] -= hy * d;
+      }
+    }
+  }
+  const V4<S> qi = q[i];
+  for (int s = lane; !SUMS && s < nb; s += LPP) {
     const int k = (int)(cp[(size_t)s * g.n] >> 4);
     const D2 pk = g.xy[k];
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
@@ -487,6 +525,7 @@ struct mf_ctx {
   std::vector<int32_t> case_first;  // original-order offset of each case [ncases+1]
   int32_t* case_rng = nullptr;      // per case: wall range, interior range, point count
   double *rs_pt = nullptr, *res_old_case = nullptr;
+  V4<float>* qsum = nullptr;  // mixed path: the LSQ first-pass sums per point [2N]
   Ctl* tune_ctl = nullptr;
 };
 
@@ -948,6 +987,7 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
   }
   char *qmb, *pob;
   if ((rc = dev_alloc(c, &qmb, (size_t)2 * n * v4b)) || (rc = dev_alloc(c, &pob, (size_t)n * v4b))) return rc;
+  if (c->ssz == 4 && (rc = dev_alloc(c, &c->qsum, (size_t)2 * n))) return rc;
   c->qm = qmb;
   c->prim_old = pob;
   // room for the partial sums of the largest lanes-per-point grid (8 lanes)
@@ -1047,44 +1087,47 @@ static void with_lpp(int lpp, F&& f) {
 // eval_q_derivatives (+ func_delta) of q into dq, qm (and delta)
 template <class S>
 static void launch_lsq_on(mf_ctx* c, const V4<S>* q, V4<S>* dq, V4<S>* qm, const double* cspeed,
-                          double* delta, bool with_delta, const Ctl* ctl, int lpp) {
+                          double* delta, bool with_delta, const Ctl* ctl, int lpp, V4<float>* qsum = nullptr) {
   const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
   const double cfl = c->p.cfl, pw = c->p.power;
   const bool p0 = c->power0;
   with_lpp(lpp, [&]<int L>() {
     if (p0 && with_delta)
-      hipLaunchKernelGGL((k_lsq<S, true, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+      hipLaunchKernelGGL((k_lsq<S, true, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else if (p0)
-      hipLaunchKernelGGL((k_lsq<S, true, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+      hipLaunchKernelGGL((k_lsq<S, true, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else if (with_delta)
-      hipLaunchKernelGGL((k_lsq<S, false, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+      hipLaunchKernelGGL((k_lsq<S, false, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else
-      hipLaunchKernelGGL((k_lsq<S, false, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl);
+      hipLaunchKernelGGL((k_lsq<S, false, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
   });
 }
 
 template <class S>
 static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl) {
   launch_lsq_on<S>(c, (const V4<S>*)c->q[qbuf], (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, with_delta,
-                   ctl, c->lpp_lsq);
+                   ctl, c->lpp_lsq, c->qsum);
 }
 
 // one inner iteration: dq_in -> dq_out
 template <class S>
-static void launch_inner_on(mf_ctx* c, const V4<S>* q, const V4<S>* a, V4<S>* b, const Ctl* ctl, int lpp) {
+static void launch_inner_on(mf_ctx* c, const V4<S>* q, const V4<S>* a, V4<S>* b, const Ctl* ctl, int lpp,
+                            const V4<float>* qsum = nullptr) {
   const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
   const bool p0 = c->power0;
   const double pw = c->p.power;
+  // the mixed path gathers only dq (first-pass sums from the LSQ kernel)
+  constexpr bool SUMS = std::is_same<S, float>::value;
   with_lpp(lpp, [&]<int L>() {
-    if (p0) hipLaunchKernelGGL((k_inner<S, true, L>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl);
-    else hipLaunchKernelGGL((k_inner<S, false, L>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl);
+    if (p0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl, qsum);
+    else hipLaunchKernelGGL((k_inner<S, false, L, SUMS>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl, qsum);
   });
 }
 
 template <class S>
 static void launch_inner(mf_ctx* c, int qbuf, int din, const Ctl* ctl) {
   launch_inner_on<S>(c, (const V4<S>*)c->q[qbuf], (const V4<S>*)c->dq[din], (V4<S>*)c->dq[din ^ 1], ctl,
-                     c->lpp_inner);
+                     c->lpp_inner, c->qsum);
 }
 
 // fused flux residual + state update (rk) + q_variables of the new state
@@ -1373,16 +1416,16 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   for (int lpp : cand) {
     double us;
     if ((rc = time_launches(c, reps, [&] {
-           launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp);
+           launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp, c->qsum);
          }, &us)))
       return rc;
     if (us < best[0]) best[0] = us, pick[0] = lpp;
   }
-  launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, 1);
+  launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, 1, c->qsum);
   for (int lpp : cand) {
     double us;
     if ((rc = time_launches(c, reps, [&] {
-           launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp);
+           launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp, c->qsum);
          }, &us)))
       return rc;
     if (us < best[1]) best[1] = us, pick[1] = lpp;
