@@ -251,6 +251,7 @@ def run_batch(iters: int, adjoint: bool = True):
 
 
 COPY_PROBE_BYTES = 256 << 20
+PROBE_SHAPES = {"lpp_lsq": 2, "lpp_inner": 2, "lpp_flux": 4}
 
 
 def probe():
@@ -264,6 +265,8 @@ def probe():
     p = {k: WORKLOAD[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
     for prec in ("fp64", "mixed"):
         with LskumSolver(cloud, precision=prec, **p) as s:
+            # the launch shapes the autotuner picks for this workload
+            s.tuning = PROBE_SHAPES
             s.q_lskum(2)
     print(json.dumps({"probe": "ok", "copy_bytes": COPY_PROBE_BYTES, "copy_ms": ms.value,
                       "points": cloud.n, "mean_nbhs": float(cloud.nbhs.mean())}))
@@ -288,7 +291,8 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=None, help="iterations of the batched run")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
-                         "2 mixed iterations of the workload (for rocprofv3 --pmc)")
+                         "2 mixed iterations of the workload at the autotuner's usual shapes "
+                         "(for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the LSKUM path here is single-GPU "
@@ -329,12 +333,15 @@ def main():
     fu_ms = best["kernels"]["flux_update"][0]
     per_launch = flux_update_bytes_per_point(sb, mean_nbhs) * cloud.n
     achieved = per_launch / (fu_ms * 1e-3) / 1e9
-    traffic = None
+    traffic = valu_busy = l2_hit = None
     tpath = os.path.join(ROOT, "profiles", "traffic_flux_update.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        traffic = t.get("fp32" if use_mixed else "fp64")
+        key = "fp32" if use_mixed else "fp64"
+        traffic = t.get(key)
+        valu_busy = t.get(f"{key}_valu_busy")
+        l2_hit = t.get(f"{key}_l2_hit")
 
     it_bytes = iteration_bytes_per_point(sb, mean_nbhs, WORKLOAD["inner_iterations"])
     line = {
@@ -357,7 +364,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_flux_update", "kernel_ms": fu_ms,
-                     "algorithmic_bytes_per_launch": per_launch},
+                     "algorithmic_bytes_per_launch": per_launch,
+                     # counters of the same kernel (profiles/): the VALU, not HBM, binds it
+                     "valu_busy_pct": valu_busy, "l2_hit": l2_hit},
         "iteration_hbm_gbs": it_bytes * best["point_iters_per_s"] / 1e9,
         "fp64_point_iters_per_s": fp64["point_iters_per_s"],
         "mixed_point_iters_per_s": mixed["point_iters_per_s"],
