@@ -140,6 +140,81 @@ def run_adjoint(iters: int, snaps: int):
     return out
 
 
+def _boundary_normals(cloud):
+    """Normals as the device computes them (point_normals.F90:8-105): wall and outer
+    points from their left/right neighbours, interior points (0, 1)."""
+    nx, ny = np.zeros(cloud.n), np.ones(cloud.n)
+    b = cloud.flag_1 != 1
+    l, r = cloud.left[b], cloud.right[b]
+    x, y = cloud.x, cloud.y
+    vx = 0.5 * ((y[b] - y[l]) + (y[r] - y[b]))
+    vy = 0.5 * ((x[b] - x[l]) + (x[r] - x[b]))
+    det = np.sqrt(vx * vx + vy * vy)
+    nx[b], ny[b] = -vx / det, vy / det
+    return nx, ny
+
+
+def run_adjoint_fd(iters: int, n_points: int = 16, h: float = 1e-6, save: str | None = None):
+    """dCd/dx, dCd/dy of the adjoint workload against central differences (step h
+    chord) at seeded wall points: each difference re-runs the FP64 primal from
+    freestream on the perturbed cloud (normals and half-stencil splits recomputed by
+    mf_create).  A point is skipped when a perturbation flips a half-stencil split
+    (delta_s or delta_n changing sign), checked before any run.  Over a long
+    trajectory a perturbation of 1e-6 can also switch limiter or argmin branches (J is
+    then not differentiable along the step); such points are kept and show up as the
+    tail of the error distribution."""
+    import dataclasses
+    from mfcfd_amd.cloud import naca_ogrid, split_stencils
+    from mfcfd_amd.solver import LskumSolver
+    w = ADJ_WORKLOAD
+    cloud = naca_ogrid(w["n_wall"], w["n_radial"], seed=w["seed"])
+    p = {k: w[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
+    with LskumSolver(cloud, **p) as s:
+        J, dJdx, dJdy, _ = s.adjoint(iters, snaps=iters)
+    if save:
+        np.savez(save, J=J, dJdx=dJdx, dJdy=dJdy, iters=iters)
+    masks0 = split_stencils(cloud, *_boundary_normals(cloud))
+
+    def perturbed(j, comp, shift):
+        c = dataclasses.replace(cloud, x=cloud.x.copy(), y=cloud.y.copy())
+        (c.x if comp == 0 else c.y)[j] += shift
+        return c
+
+    def flips(c):
+        return not np.array_equal(split_stencils(c, *_boundary_normals(c)), masks0)
+
+    def J_of(c):
+        with LskumSolver(c, **p) as s:
+            _, _, cd = s.q_lskum(iters)
+        return float(cd[-1])
+
+    rng = np.random.default_rng(16)
+    wall = np.where(cloud.flag_1 == 0)[0]
+    scale = float(np.abs(np.concatenate([dJdx[wall], dJdy[wall]])).max())
+    rows, flipped = [], 0
+    for j in rng.permutation(wall):
+        cs = {(comp, sg): perturbed(j, comp, sg * h) for comp in (0, 1) for sg in (1, -1)}
+        if any(flips(c) for c in cs.values()):
+            flipped += 1
+            continue
+        ent = {"point": int(j)}
+        for comp, grad in ((0, dJdx), (1, dJdy)):
+            fd = (J_of(cs[(comp, 1)]) - J_of(cs[(comp, -1)])) / (2 * h)
+            ent["xy"[comp]] = {"adjoint": float(grad[j]), "fd": fd,
+                               "rel_err": abs(grad[j] - fd) / max(abs(fd), 1e-3 * scale)}
+        rows.append(ent)
+        print(f"[bench] fd point {j}: x {ent['x']['rel_err']:.2e} y {ent['y']['rel_err']:.2e}",
+              file=sys.stderr, flush=True)
+        if len(rows) == n_points:
+            break
+    errs = np.array([e[c]["rel_err"] for e in rows for c in "xy"])
+    return {"workload": f"adjoint workload, {iters} iterations, central differences h = {h}",
+            "J": J, "points_used": len(rows), "skipped_split_flip": flipped,
+            "rel_err_median": float(np.median(errs)), "rel_err_worst": float(errs.max()),
+            "within_1e-6": int((errs <= 1e-6).sum()), "within_1e-4": int((errs <= 1e-4).sum()),
+            "of": int(errs.size), "gradient_scale": scale, "points": rows}
+
+
 # 2,560,000 points; the configuration fixes no inner count and no limiter constant:
 # 0 inner iterations (readcase.F90's default) and K = 5 as the 160k workload
 LARGE_WORKLOAD = dict(n_wall=2560, n_radial=1000, mach=1.2, aoa=0.0, cfl=0.2, vl_const=5.0,
@@ -288,6 +363,10 @@ def main():
                     help="skip the 2.56M-point measurement reported beside the headline")
     ap.add_argument("--large", action="store_true", help="only the 2.56M-point measurement")
     ap.add_argument("--batch", action="store_true", help="only the 64-case batched measurement")
+    ap.add_argument("--adjoint-fd", action="store_true",
+                    help="only the adjoint's central-difference check at 16 wall points")
+    ap.add_argument("--fd-step", type=float, default=1e-6, help="step of that check (chord)")
+    ap.add_argument("--save-gradient", default=None, help="write the adjoint's gradient (npz)")
     ap.add_argument("--batch-iters", type=int, default=None, help="iterations of the batched run")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration pass: one 256 MiB copy kernel, then 2 FP64 and "
@@ -311,6 +390,10 @@ def main():
         return
     if args.batch:
         print(json.dumps(run_batch(args.batch_iters or BATCH_WORKLOAD["iterations"])))
+        return
+    if args.adjoint_fd:
+        iters = ADJ_WORKLOAD["iterations"] if args.adjoint_iters is None else args.adjoint_iters
+        print(json.dumps(run_adjoint_fd(iters, h=args.fd_step, save=args.save_gradient)))
         return
     if args.adjoint_iters is None:
         args.adjoint_iters = 200
