@@ -98,7 +98,9 @@ __device__ inline int xcd_block(int b, int nb) {
   return x * q + (x < r ? x : r) + local;
 }
 
-__device__ inline int point_id() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
+// Every stencil kernel is launched with BLOCK threads; the constant spares each wave the
+// load of the dispatch packet's workgroup size before its first address is known.
+__device__ inline int point_id() { return xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; }
 
 // Lanes per point: a point's stencil loop can be split over LPP consecutive lanes of a
 // wave (edge s goes to lane s mod LPP) and the partial sums combined with shuffles.
@@ -127,6 +129,33 @@ __device__ inline T group_min(T v) {
     v = (o < v) ? o : v;
   }
   return v;
+}
+
+// A lane's stencil walk (slots lane, lane + LPP, ...) taken U slots at a time: the U
+// connectivity words are loaded together, then the U neighbour records (load(word)),
+// then each record is used in slot order (use(record, word, valid)).  One slot at a
+// time, each edge costs two dependent memory round trips (word, then record); batched,
+// U edges share them.  Slots past nb load the point's own record (self) and come with
+// valid = false: use() is called for every slot, so the compiler cannot sink a record's
+// load into a branch (which would serialise it again), and must leave its sums unchanged
+// (select, not add) when !valid, so a lane's sums keep the one-slot walk's order and
+// values bit for bit.
+template <int U, int LPP, class Load, class Use>
+__device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, int n, int nb,
+                                             int lane, uint32_t self, Load&& load, Use&& use) {
+  for (int s0 = lane; s0 < nb; s0 += U * LPP) {
+    uint32_t e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u * LPP;
+      e[u] = s < nb ? cp[(size_t)s * n] : self;
+    }
+    decltype(load(self)) rec[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) rec[u] = load(e[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) use(rec[u], e[u], s0 + u * LPP < nb);
+  }
 }
 
 template <class S>
@@ -444,8 +473,12 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
+  // the next slot's connectivity word is loaded while this edge is evaluated, so each
+  // edge starts with one memory round trip (its neighbour's records), not two
+  uint32_t pk_next = lane < nb ? cp[(size_t)lane * g.n] : 0u;
   for (int s = lane; s < nb; s += LPP) {
-    const uint32_t pk = cp[(size_t)s * g.n];
+    const uint32_t pk = pk_next;
+    if (s + LPP < nb) pk_next = cp[(size_t)(s + LPP) * g.n];
     const int k = (int)(pk >> 4);
     const unsigned m = pk & 15u;
     const D2 pkk = g.xy[k];

@@ -26,6 +26,14 @@ using namespace mf;
 // Kernels
 // =================================================================================
 
+// Slots per batch of a stencil walk (walk_stencil): a lane's share of a 24-slot stencil
+// in two to four batches, with the batch's neighbour records kept within the register
+// budget of the kernel's occupancy.
+template <class S, int LPP>
+constexpr int walk_batch() {
+  return LPP >= 4 ? 3 : (sizeof(S) == 4 ? 4 : 2);
+}
+
 // q_variables.F90:8-37 over all points, plus each point's wave speed for func_delta
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __restrict__ prim,
@@ -60,34 +68,44 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   double min_delt = 1.0;
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  for (int s = lane; s < nb; s += LPP) {
-    const int k = (int)(cp[(size_t)s * g.n] >> 4);
-    const V4<S> qk = q[k];
+  struct Nbr {
+    V4<S> q;
+    D2 p;
+    double c;
+  };
+  walk_stencil<walk_batch<S, LPP>(), LPP>(
+      cp, g.n, nb, lane, (uint32_t)i << 4,
+      [&](uint32_t e) {
+        const int k = (int)(e >> 4);
+        return Nbr{q[k], g.xy[k], DELTA ? cspeed[k] : 0.0};
+      },
+      [&](const Nbr& nk, uint32_t, bool valid) {
+        // a slot past nb carries the point's own q: neutral for the strict max/min
+        const V4<S> qk = nk.q;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (qk[r] > qmax[r]) qmax[r] = qk[r];
-      if (qk[r] < qmin[r]) qmin[r] = qk[r];
-    }
-    const D2 pk = g.xy[k];
-    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-    double w = 1.0;
-    if (!POWER0 || DELTA) {
-      const double dist = sqrt(delx * delx + dely * dely);
-      if (!POWER0) w = pow(dist, power);
-      if (DELTA) {
-        double dt = dist / cspeed[k];
-        dt = cfl * dt;
-        if (min_delt > dt) min_delt = dt;
-      }
-    }
-    const double wx = w * delx, wy = w * dely;
+        for (int r = 0; r < 4; ++r) {
+          if (qk[r] > qmax[r]) qmax[r] = qk[r];
+          if (qk[r] < qmin[r]) qmin[r] = qk[r];
+        }
+        const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
+        double w = 1.0;
+        if (!POWER0 || DELTA) {
+          const double dist = sqrt(delx * delx + dely * dely);
+          if (!POWER0) w = pow(dist, power);
+          if (DELTA) {
+            double dt = dist / nk.c;
+            dt = cfl * dt;
+            if (valid && min_delt > dt) min_delt = dt;
+          }
+        }
+        const double wx = w * delx, wy = w * dely;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double d = (double)qk[r] - (double)qi[r];
-      sxq[r] = sxq[r] + wx * d;
-      syq[r] = syq[r] + wy * d;
-    }
-  }
+        for (int r = 0; r < 4; ++r) {
+          const double d = (double)qk[r] - (double)qi[r];
+          sxq[r] = valid ? sxq[r] + wx * d : sxq[r];
+          syq[r] = valid ? syq[r] + wy * d : syq[r];
+        }
+      });
   if (LPP > 1) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -137,7 +155,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = g.nbhs[i];
   const uint32_t* cp = g.conn + i;
-  if (SUMS) {
+  if constexpr (SUMS) {
     if (lane == 0) {
       const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
 #pragma unroll
@@ -146,38 +164,54 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
         syq[r] = b[r];
       }
     }
-    for (int s = lane; s < nb; s += LPP) {
-      const int k = (int)(cp[(size_t)s * g.n] >> 4);
-      const D2 pk = g.xy[k];
-      const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-      double w = 1.0;
-      if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-      const V4<S> dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
-      const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
+    struct Nbr {
+      D2 p;
+      V4<S> dx, dy;
+    };
+    walk_stencil<walk_batch<S, LPP>(), LPP>(
+        cp, g.n, nb, lane, (uint32_t)i << 4,
+        [&](uint32_t e) {
+          const int k = (int)(e >> 4);
+          return Nbr{g.xy[k], dq_in[2 * k], dq_in[2 * k + 1]};
+        },
+        [&](const Nbr& nk, uint32_t, bool valid) {
+          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
+          double w = 1.0;
+          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+          const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double d = delx * (double)(dxk[r] - dxi[r]) + dely * (double)(dyk[r] - dyi[r]);
-        sxq[r] -= hx * d;
-        syq[r] -= hy * d;
-      }
-    }
+          for (int r = 0; r < 4; ++r) {
+            const double d = delx * (double)(nk.dx[r] - dxi[r]) + dely * (double)(nk.dy[r] - dyi[r]);
+            sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
+            syq[r] = valid ? syq[r] - hy * d : syq[r];
+          }
+        });
   }
-  const V4<S> qi = q[i];
-  for (int s = lane; !SUMS && s < nb; s += LPP) {
-    const int k = (int)(cp[(size_t)s * g.n] >> 4);
-    const D2 pk = g.xy[k];
-    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-    double w = 1.0;
-    if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-    const V4<S> qk = q[k], dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
-    const double wx = w * delx, wy = w * dely;
+  if constexpr (!SUMS) {
+    const V4<S> qi = q[i];
+    struct Nbr {
+      D2 p;
+      V4<S> q, dx, dy;
+    };
+    walk_stencil<walk_batch<S, LPP>(), LPP>(
+        cp, g.n, nb, lane, (uint32_t)i << 4,
+        [&](uint32_t e) {
+          const int k = (int)(e >> 4);
+          return Nbr{g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]};
+        },
+        [&](const Nbr& nk, uint32_t, bool valid) {
+          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
+          double w = 1.0;
+          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+          const double wx = w * delx, wy = w * dely;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
-      const double temp2 = (double)qk[r] - 0.5 * (delx * (double)dxk[r] + dely * (double)dyk[r]);
-      sxq[r] = sxq[r] + (wx * (temp2 - temp1));
-      syq[r] = syq[r] + (wy * (temp2 - temp1));
-    }
+          for (int r = 0; r < 4; ++r) {
+            const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
+            const double temp2 = (double)nk.q[r] - 0.5 * (delx * (double)nk.dx[r] + dely * (double)nk.dy[r]);
+            sxq[r] = valid ? sxq[r] + (wx * (temp2 - temp1)) : sxq[r];
+            syq[r] = valid ? syq[r] + (wy * (temp2 - temp1)) : syq[r];
+          }
+        });
   }
   if (LPP > 1) {
 #pragma unroll

@@ -54,3 +54,19 @@ def test_seeded_and_deterministic():
     assert not np.array_equal(a.x, c.x)
     # jitter only moves interior points
     assert np.array_equal(a.x[:40], c.x[:40]) and np.array_equal(a.x[-40:], c.x[-40:])
+
+
+def test_c3_tangent_fixture_cloud():
+    """The stored C3 tangent belongs to the cloud the generator makes today."""
+    import json
+    import os
+    import sys
+    from mfcfd_amd.cloud import naca_ogrid
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_c3_tangent as M
+    with open(os.path.join(here, "c3_tangent.json")) as f:
+        ref = json.load(f)
+    c = naca_ogrid(320, 125, seed=0)
+    assert c.n == ref["points"]
+    assert M.cloud_digest(c) == ref["cloud_sha256"]

@@ -138,3 +138,31 @@ def test_adjoint_leaves_final_state(small_cloud):
         a.adjoint(10, snaps=3)
         b.q_lskum(10)
         assert np.array_equal(a.prim, b.prim)
+
+
+def test_adjoint_matches_tangent_c3():
+    """The 40,000-point workload over 2000 iterations against the oracle tangent stored
+    in tests/golden/c3_tangent.json (every iterate kept, so no recomputation)."""
+    import json
+    import os
+    import sys
+    from mfcfd_amd.cloud import naca_ogrid
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_c3_tangent as M
+    with open(os.path.join(here, "c3_tangent.json")) as f:
+        ref = json.load(f)
+    c = naca_ogrid(320, 125, seed=0)
+    assert M.cloud_digest(c) == ref["cloud_sha256"], "cloud generator changed: remake the fixture"
+    p = dict(ref["params"])
+    p["aoa"] = p.pop("aoa_deg")
+    from mfcfd_amd.solver import LskumSolver
+    iters = ref["iterations"]
+    print(f"[adjoint] c3: {c.n} points, {iters} iterations", flush=True)
+    with LskumSolver(c, **p) as g:
+        J, dJdx, dJdy, st = g.adjoint(iters, snaps=iters)
+    vx, vy = M.direction(c.n)
+    adj = float(dJdx @ vx + dJdy @ vy)
+    print(f"J {J:.15e} vs {ref['J']:.15e}; directional {adj:.14e} vs tangent {ref['dJ']:.14e}")
+    assert abs(J - ref["J"]) <= 1e-12 * abs(ref["J"])
+    assert abs(adj - ref["dJ"]) <= 1e-10 * abs(ref["dJ"])
