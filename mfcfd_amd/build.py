@@ -27,6 +27,12 @@ HIPCC_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
+# The library's translation units and their contraction mode: mfcfd.hip (host side,
+# FP64 kernels, adjoint) without contraction; mfcfd_f32.hip (the mixed path's FP32
+# kernels, held to a tolerance against FP64, not to rounding parity) with FMA
+# contraction.
+UNITS = [("mfcfd.hip", "-ffp-contract=off"), ("mfcfd_f32.hip", "-ffp-contract=fast")]
+
 
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
@@ -74,13 +80,24 @@ def build_host(verbose: bool = False) -> str | None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile csrc/mfcfd.hip into mfcfd_amd/libmfcfd_hip.so; returns the path."""
-    src = os.path.join(CSRC, "mfcfd.hip")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, src]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
+    """Compile csrc/mfcfd.hip and csrc/mfcfd_f32.hip and link them into
+    mfcfd_amd/libmfcfd_hip.so; returns the path."""
+    import tempfile
+    common = [f for f in HIPCC_FLAGS if f not in ("-shared", "-ffp-contract=off")]
+    with tempfile.TemporaryDirectory() as tmpd:
+        objs = []
+        for src, contract in UNITS:
+            obj = os.path.join(tmpd, src.replace(".hip", ".o"))
+            cmd = [hipcc()] + common + [contract, "-c", "-o", obj, os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.check_call(cmd)
+            objs.append(obj)
+        tmp = LIB + ".tmp"
+        cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

@@ -18,335 +18,13 @@
 #include <vector>
 
 #include "../../include/mfcfd.h"
-#include "lskum_device.h"
+#include "lskum_kernels.h"
 
 using namespace mf;
 
 // =================================================================================
 // Kernels
 // =================================================================================
-
-// Slots per batch of a stencil walk (walk_stencil): a lane's share of a 24-slot stencil
-// in two to four batches, with the batch's neighbour records kept within the register
-// budget of the kernel's occupancy.
-template <class S, int LPP>
-constexpr int walk_batch() {
-  return LPP >= 4 ? 3 : (sizeof(S) == 4 ? 4 : 2);
-}
-
-// q_variables.F90:8-37 over all points, plus each point's wave speed for func_delta
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __restrict__ prim,
-                                                       V4<S>* __restrict__ q,
-                                                       double* __restrict__ cspeed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const V4<S> p = prim[i];
-  q[i] = q_of_prim(p);
-  cspeed[i] = wave_speed(p);
-}
-
-// q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
-// (state_update.F90:190-236), which gathers over the same stencil.
-// qsum (mixed path): the first-pass sums sum w (dx, dy) (q_k - q_i) are also stored, for
-// the inner iterations (q does not change between them).
-template <class S, bool POWER0, bool DELTA, int LPP>
-__global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__ q,
-                                               V4<S>* __restrict__ dq, V4<S>* __restrict__ qm,
-                                               const double* __restrict__ cspeed,
-                                               double* __restrict__ delta, double cfl,
-                                               double power, const Ctl* __restrict__ ctl,
-                                               V4<float>* __restrict__ qsum) {
-  if (ctl && ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;  // whole groups: n * LPP threads are a multiple of LPP
-  const D2 pi = g.xy[i];
-  const V4<S> qi = q[i];
-  V4<S> qmax = qi, qmin = qi;
-  double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
-  double min_delt = 1.0;
-  const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
-  struct Nbr {
-    V4<S> q;
-    D2 p;
-    double c;
-  };
-  walk_stencil<walk_batch<S, LPP>(), LPP>(
-      cp, g.n, nb, lane, (uint32_t)i << 4,
-      [&](uint32_t e) {
-        const int k = (int)(e >> 4);
-        return Nbr{q[k], g.xy[k], DELTA ? cspeed[k] : 0.0};
-      },
-      [&](const Nbr& nk, uint32_t, bool valid) {
-        // a slot past nb carries the point's own q: neutral for the strict max/min
-        const V4<S> qk = nk.q;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (qk[r] > qmax[r]) qmax[r] = qk[r];
-          if (qk[r] < qmin[r]) qmin[r] = qk[r];
-        }
-        const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-        double w = 1.0;
-        if (!POWER0 || DELTA) {
-          const double dist = sqrt(delx * delx + dely * dely);
-          if (!POWER0) w = pow(dist, power);
-          if (DELTA) {
-            double dt = dist / nk.c;
-            dt = cfl * dt;
-            if (valid && min_delt > dt) min_delt = dt;
-          }
-        }
-        const double wx = w * delx, wy = w * dely;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double d = (double)qk[r] - (double)qi[r];
-          sxq[r] = valid ? sxq[r] + wx * d : sxq[r];
-          syq[r] = valid ? syq[r] + wy * d : syq[r];
-        }
-      });
-  if (LPP > 1) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      sxq[r] = group_sum<LPP>(sxq[r]);
-      syq[r] = group_sum<LPP>(syq[r]);
-      qmax[r] = group_max<LPP>(qmax[r]);
-      qmin[r] = group_min<LPP>(qmin[r]);
-    }
-    if (DELTA) min_delt = group_min<LPP>(min_delt);
-    if (lane != 0) return;
-  }
-  const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
-  V4<S> ox, oy;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    ox[r] = (S)((sxq[r] * gg.y - syq[r] * gg.z) * gg.w);
-    oy[r] = (S)((syq[r] * gg.x - sxq[r] * gg.z) * gg.w);
-  }
-  dq[2 * i] = ox;
-  dq[2 * i + 1] = oy;
-  qm[2 * i] = qmax;
-  qm[2 * i + 1] = qmin;
-  if (DELTA) delta[i] = min_delt;
-  if (qsum) {
-    qsum[2 * i] = V4<float>{(float)sxq[0], (float)sxq[1], (float)sxq[2], (float)sxq[3]};
-    qsum[2 * i + 1] = V4<float>{(float)syq[0], (float)syq[1], (float)syq[2], (float)syq[3]};
-  }
-}
-
-// q_variables.F90:115-227 eval_q_inner_loop + eval_update_innerloop (Jacobi: reads
-// dq_in, writes dq_out).  SUMS (mixed path): the edge sums of w (dx, dy) ((q_k - q_i)
-// - 0.5 [dx (dq_x,k - dq_x,i) + dy (dq_y,k - dq_y,i)]) start from the LSQ kernel's
-// first-pass sums, so only the neighbours' dq and coordinates are gathered; the FP64
-// path keeps the reference's per-edge form.
-template <class S, bool POWER0, int LPP, bool SUMS>
-__global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict__ q,
-                                                 const V4<S>* __restrict__ dq_in,
-                                                 V4<S>* __restrict__ dq_out, double power,
-                                                 const Ctl* __restrict__ ctl,
-                                                 const V4<float>* __restrict__ qsum) {
-  if (ctl && ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;
-  const D2 pi = g.xy[i];
-  const V4<S> dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
-  double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
-  const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
-  if constexpr (SUMS) {
-    if (lane == 0) {
-      const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        sxq[r] = a[r];
-        syq[r] = b[r];
-      }
-    }
-    struct Nbr {
-      D2 p;
-      V4<S> dx, dy;
-    };
-    walk_stencil<walk_batch<S, LPP>(), LPP>(
-        cp, g.n, nb, lane, (uint32_t)i << 4,
-        [&](uint32_t e) {
-          const int k = (int)(e >> 4);
-          return Nbr{g.xy[k], dq_in[2 * k], dq_in[2 * k + 1]};
-        },
-        [&](const Nbr& nk, uint32_t, bool valid) {
-          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-          double w = 1.0;
-          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-          const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double d = delx * (double)(nk.dx[r] - dxi[r]) + dely * (double)(nk.dy[r] - dyi[r]);
-            sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
-            syq[r] = valid ? syq[r] - hy * d : syq[r];
-          }
-        });
-  }
-  if constexpr (!SUMS) {
-    const V4<S> qi = q[i];
-    struct Nbr {
-      D2 p;
-      V4<S> q, dx, dy;
-    };
-    walk_stencil<walk_batch<S, LPP>(), LPP>(
-        cp, g.n, nb, lane, (uint32_t)i << 4,
-        [&](uint32_t e) {
-          const int k = (int)(e >> 4);
-          return Nbr{g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]};
-        },
-        [&](const Nbr& nk, uint32_t, bool valid) {
-          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-          double w = 1.0;
-          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-          const double wx = w * delx, wy = w * dely;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
-            const double temp2 = (double)nk.q[r] - 0.5 * (delx * (double)nk.dx[r] + dely * (double)nk.dy[r]);
-            sxq[r] = valid ? sxq[r] + (wx * (temp2 - temp1)) : sxq[r];
-            syq[r] = valid ? syq[r] + (wy * (temp2 - temp1)) : syq[r];
-          }
-        });
-  }
-  if (LPP > 1) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      sxq[r] = group_sum<LPP>(sxq[r]);
-      syq[r] = group_sum<LPP>(syq[r]);
-    }
-    if (lane != 0) return;
-  }
-  const V4d gg = g.lsq_geo[i];
-  V4<S> ox, oy;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    ox[r] = (S)(gg.w * (sxq[r] * gg.y - syq[r] * gg.z));
-    oy[r] = (S)(gg.w * (syq[r] * gg.x - sxq[r] * gg.z));
-  }
-  dq_out[2 * i] = ox;
-  dq_out[2 * i + 1] = oy;
-}
-
-template <class S, bool POWER0, int LPP>
-__device__ inline void flux_dispatch(const Geom& g, const State<S>& st, int i, double delta,
-                                     double power, double fr[4], int lane) {
-  if (i < g.nw) flux_point<WALL, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
-  else if (i < g.nw + g.no) flux_point<OUTER, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
-  else flux_point<INTERIOR, S, POWER0, LPP>(g, st, i, delta, power, fr, lane);
-  if (LPP > 1) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
-  }
-}
-
-// flux_residual.F90:14-73 cal_flux_residual (unfused: stores flux_res)
-template <class S, bool POWER0>
-__global__ __launch_bounds__(BLOCK, 2) void k_flux(Geom g, State<S> st,
-                                                const double* __restrict__ delta,
-                                                V4d* __restrict__ flux_res, double power) {
-  const int i = point_id();
-  if (i >= g.n) return;
-  double fr[4];
-  flux_dispatch<S, POWER0, 1>(g, st, i, delta[i], power, fr, 0);
-  flux_res[i] = V4d{fr[0], fr[1], fr[2], fr[3]};
-}
-
-template <class S>
-__device__ inline V4<S> update_dispatch(const Geom& g, int i, const V4<S>& p, const V4<S>& po,
-                                        const double fr[4], int rk, double euler,
-                                        const Freestream& fs, double* rs) {
-  const D2 nn = g.nrm[i];
-  if (i < g.nw) return update_point<WALL, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
-  if (i < g.nw + g.no) {
-    // the far-field state of the point's own case
-    const Freestream f = g.fs_tab ? g.fs_tab[g.case_id[i]] : fs;
-    return update_point<OUTER, S>(p, po, fr, nn.x, nn.y, rk, euler, f, rs);
-  }
-  return update_point<INTERIOR, S>(p, po, fr, nn.x, nn.y, rk, euler, fs, rs);
-}
-
-// state_update.F90:10-141 state_update(rk) (unfused: in place, partial residual sums)
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_update(Geom g, V4<S>* __restrict__ prim,
-                                                  const V4<S>* __restrict__ prim_old,
-                                                  const V4d* __restrict__ flux_res,
-                                                  double* __restrict__ partial, int rk,
-                                                  double euler, Freestream fs) {
-  const int i = point_id();
-  double rs = 0.0;
-  if (i < g.n) {
-    const V4d f = flux_res[i];
-    const double fr[4] = {f.x, f.y, f.z, f.w};
-    prim[i] = update_dispatch<S>(g, i, prim[i], prim_old[i], fr, rk, euler, fs, &rs);
-  }
-  const double t = block_sum(rs);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
-// Fused cal_flux_residual + state_update(rk) + eval_q_variables of the new state.
-// Reads the stage's (q, dq, qm, prim), writes the next (prim, q) buffers; stage 1 also
-// saves prim_old (fpi_solver.F90:20-22), the last stage the residual partial sums and
-// the wave speeds for the next iteration's func_delta.
-template <class S, bool POWER0, int LPP>
-__global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
-    Geom g, State<S> st, const V4<S>* __restrict__ prim, V4<S>* __restrict__ prim_old,
-    V4<S>* __restrict__ prim_out, V4<S>* __restrict__ q_out, double* __restrict__ cspeed,
-    const double* __restrict__ delta, double* __restrict__ partial, const Ctl* __restrict__ ctl,
-    int rk, int last, double euler, double power, Freestream fs, double* __restrict__ rs_pt) {
-  if (ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  double rs = 0.0;
-  double fr[4] = {0.0, 0.0, 0.0, 0.0};
-  // every lane reaches the group sums (no early exit before them)
-  if (i < g.n) flux_dispatch<S, POWER0, LPP>(g, st, i, delta[i], power, fr, lane);
-  else if (LPP > 1) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
-  }
-  if (i < g.n && lane == 0) {
-    const V4<S> p = prim[i];
-    if (rk == 1) prim_old[i] = p;
-    const V4<S> po = (rk == 3) ? prim_old[i] : p;
-    const V4<S> pn = update_dispatch<S>(g, i, p, po, fr, rk, euler, fs, &rs);
-    prim_out[i] = pn;
-    q_out[i] = q_of_prim(pn);
-    if (last) {
-      cspeed[i] = wave_speed(pn);
-      if (rs_pt) rs_pt[i] = rs;
-    }
-  }
-  if (last) {
-    const double tot = block_sum(rs);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-  }
-}
-
-// state_update.F90:190-236 func_delta (unfused)
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_func_delta(Geom g, const V4<S>* __restrict__ prim,
-                                                      double* __restrict__ delta, double cfl) {
-  const int i = point_id();
-  if (i >= g.n) return;
-  const D2 pi = g.xy[i];
-  double min_delt = 1.0;
-  const int nb = g.nbhs[i];
-  for (int s = 0; s < nb; ++s) {
-    const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
-    const D2 pk = g.xy[k];
-    double dist = (pk.x - pi.x) * (pk.x - pi.x) + (pk.y - pi.y) * (pk.y - pi.y);
-    dist = sqrt(dist);
-    double dt = dist / wave_speed(prim[k]);
-    dt = cfl * dt;
-    if (min_delt > dt) min_delt = dt;
-  }
-  delta[i] = min_delt;
-}
 
 // Sum of the residual partials and the fpi_solver.F90:57-70 residue logic; one block.
 __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ partial, int np,
@@ -380,50 +58,6 @@ __global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old) {
   ctl->it = 0;
   ctl->halt = 0;
   if (set_res_old) ctl->res_old = res_old;
-}
-
-// compute_force_coeffs.F90:11-102, one block per shape id (a batch numbers the shapes
-// of case c as c * shapes + s).  Shape points are grouped by shape id (sh_ptr), each
-// with its (nx, ny, ds, mx - 0.25, my) in sgeo and its ordinal among the shape points
-// in original order (sh_pos, the cp output's order); shape_par per shape id holds
-// (0.5 rho_inf M^2, cos theta, sin theta) of its case.  Cl/Cd/Cm into clcdcm[3 s + k].
-template <class S>
-__global__ __launch_bounds__(BLOCK) void k_forces(const V4<S>* __restrict__ prim,
-                                                  const int32_t* __restrict__ sh_m,
-                                                  const int32_t* __restrict__ sh_pos,
-                                                  const double* __restrict__ sgeo,
-                                                  const int32_t* __restrict__ sh_ptr,
-                                                  const double* __restrict__ shape_par, int nsid,
-                                                  double* __restrict__ cp, double* __restrict__ clcdcm,
-                                                  double* __restrict__ cd_hist,
-                                                  const Ctl* __restrict__ ctl) {
-  if (ctl && ctl->halt) return;
-  const double pr_inf = 1.0 / 1.4;
-  const int s = blockIdx.x;
-  const double cp_scale = shape_par[3 * s], ct = shape_par[3 * s + 1], st = shape_par[3 * s + 2];
-  double H = 0.0, V = 0.0, M = 0.0;
-  const int j1 = sh_ptr[s + 1];
-  for (int j = sh_ptr[s] + threadIdx.x; j < j1; j += blockDim.x) {
-    const double c0 = (double)prim[sh_m[j]].w - pr_inf;
-    const double c = -c0 / cp_scale;
-    cp[sh_pos[j]] = c;
-    const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
-    H += c * nx * ds;
-    V += c * ny * ds;
-    M += (-c * ny * ds * sgeo[5 * j + 3] + c * nx * ds * sgeo[5 * j + 4]);
-  }
-  H = block_sum(H);
-  __syncthreads();
-  V = block_sum(V);
-  __syncthreads();
-  M = block_sum(M);
-  if (threadIdx.x == 0) {
-    const double cl = V * ct - H * st, cd = H * ct + V * st;
-    clcdcm[3 * s] = cl;
-    clcdcm[3 * s + 1] = cd;
-    clcdcm[3 * s + 2] = M;
-    if (ctl && cd_hist) cd_hist[(size_t)ctl->it * nsid + s] = cd;
-  }
 }
 
 // Residual norms of a batch, one block per case: sqrt(sum over the case's wall and
@@ -614,11 +248,13 @@ static inline unsigned split_mask(int flag1, double dels, double deln) {
   return m;
 }
 
+static LaunchEnv env_of(const mf_ctx* c) {
+  return LaunchEnv{c->stream, c->g, c->n, c->power0, c->p.power, c->p.cfl, c->p.euler, c->fs};
+}
+
 template <class S>
 static int launch_q_variables(mf_ctx* c) {
-  const int nb = (c->n + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL((k_q_variables<S>), dim3(nb), dim3(BLOCK), 0, c->stream, c->n,
-                     (const V4<S>*)c->prim[c->cur], (V4<S>*)c->q[c->cur], c->cspeed);
+  launch_q_variables_any<S>(env_of(c), (const V4<S>*)c->prim[c->cur], (V4<S>*)c->q[c->cur], c->cspeed);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -1103,38 +739,13 @@ static State<S> state_of(mf_ctx* c, int qbuf, int dqbuf) {
                   (const S*)c->eps};
 }
 
-static inline dim3 lane_blocks(const mf_ctx* c, int lpp) {
-  return dim3((unsigned)(((int64_t)c->n * lpp + BLOCK - 1) / BLOCK));
-}
-
-// LPP dispatch: f.template operator()<L>() for the runtime lanes-per-point value
-template <class F>
-static void with_lpp(int lpp, F&& f) {
-  switch (lpp) {
-    case 2: f.template operator()<2>(); break;
-    case 4: f.template operator()<4>(); break;
-    case 8: f.template operator()<8>(); break;
-    default: f.template operator()<1>(); break;
-  }
-}
+static inline dim3 lane_blocks(const mf_ctx* c, int lpp) { return lane_grid(c->n, lpp); }
 
 // eval_q_derivatives (+ func_delta) of q into dq, qm (and delta)
 template <class S>
 static void launch_lsq_on(mf_ctx* c, const V4<S>* q, V4<S>* dq, V4<S>* qm, const double* cspeed,
                           double* delta, bool with_delta, const Ctl* ctl, int lpp, V4<float>* qsum = nullptr) {
-  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
-  const double cfl = c->p.cfl, pw = c->p.power;
-  const bool p0 = c->power0;
-  with_lpp(lpp, [&]<int L>() {
-    if (p0 && with_delta)
-      hipLaunchKernelGGL((k_lsq<S, true, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
-    else if (p0)
-      hipLaunchKernelGGL((k_lsq<S, true, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
-    else if (with_delta)
-      hipLaunchKernelGGL((k_lsq<S, false, true, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
-    else
-      hipLaunchKernelGGL((k_lsq<S, false, false, L>), gr, bl, 0, c->stream, c->g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
-  });
+  launch_lsq_any<S>(env_of(c), q, dq, qm, cspeed, delta, with_delta, ctl, lpp, qsum);
 }
 
 template <class S>
@@ -1147,15 +758,7 @@ static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl) {
 template <class S>
 static void launch_inner_on(mf_ctx* c, const V4<S>* q, const V4<S>* a, V4<S>* b, const Ctl* ctl, int lpp,
                             const V4<float>* qsum = nullptr) {
-  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
-  const bool p0 = c->power0;
-  const double pw = c->p.power;
-  // the mixed path gathers only dq (first-pass sums from the LSQ kernel)
-  constexpr bool SUMS = std::is_same<S, float>::value;
-  with_lpp(lpp, [&]<int L>() {
-    if (p0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl, qsum);
-    else hipLaunchKernelGGL((k_inner<S, false, L, SUMS>), gr, bl, 0, c->stream, c->g, q, a, b, pw, ctl, qsum);
-  });
+  launch_inner_any<S>(env_of(c), q, a, b, ctl, lpp, qsum);
 }
 
 template <class S>
@@ -1169,19 +772,8 @@ template <class S>
 static void launch_flux_update_on(mf_ctx* c, const State<S>& st, const V4<S>* prim, V4<S>* pold, V4<S>* pout,
                                   V4<S>* qout, double* cspeed, const double* delta, double* partial,
                                   const Ctl* ctl, int rk, bool last, int lpp, double* rs_pt = nullptr) {
-  const dim3 gr = lane_blocks(c, lpp), bl(BLOCK);
-  const bool p0 = c->power0;
-  const int lst = last ? 1 : 0;
-  const double eu = c->p.euler, pw = c->p.power;
-  const Freestream fs = c->fs;
-  with_lpp(lpp, [&]<int L>() {
-    if (p0)
-      hipLaunchKernelGGL((k_flux_update<S, true, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
-    else
-      hipLaunchKernelGGL((k_flux_update<S, false, L>), gr, bl, 0, c->stream, c->g, st, prim, pold, pout, qout,
-                         cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
-  });
+  launch_flux_update_any<S>(env_of(c), st, prim, pold, pout, qout, cspeed, delta, partial, ctl, rk, last, lpp,
+                            rs_pt);
 }
 
 template <class S>
@@ -1193,9 +785,8 @@ static void launch_flux_update(mf_ctx* c, int buf, int dqbuf, int rk, bool last)
 
 template <class S>
 static void launch_forces(mf_ctx* c, int buf, const Ctl* ctl, double* cd_hist) {
-  hipLaunchKernelGGL((k_forces<S>), dim3(c->nsid), dim3(BLOCK), 0, c->stream, (const V4<S>*)c->prim[buf],
-                     c->sh_m, c->sh_pos, c->sgeo, c->sh_ptr, c->shape_par, c->nsid, c->cp, c->clcdcm,
-                     cd_hist, ctl);
+  const ForcesArgs f{c->sh_m, c->sh_pos, c->sgeo, c->sh_ptr, c->shape_par, c->nsid, c->cp, c->clcdcm};
+  launch_forces_any<S>(env_of(c), (const V4<S>*)c->prim[buf], f, cd_hist, ctl);
 }
 
 static void launch_finalize(mf_ctx* c) {
@@ -1700,11 +1291,8 @@ int mf_eval_q_variables(mf_ctx* c) {
 
 int mf_func_delta(mf_ctx* c) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
-  const dim3 gr(nblocks(c)), bl(BLOCK);
-  if (c->ssz == 8)
-    hipLaunchKernelGGL(k_func_delta<double>, gr, bl, 0, c->stream, c->g, (const V4d*)c->prim[c->cur], c->delta, c->p.cfl);
-  else
-    hipLaunchKernelGGL(k_func_delta<float>, gr, bl, 0, c->stream, c->g, (const V4<float>*)c->prim[c->cur], c->delta, c->p.cfl);
+  if (c->ssz == 8) launch_func_delta_any<double>(env_of(c), (const V4d*)c->prim[c->cur], c->delta);
+  else launch_func_delta_any<float>(env_of(c), (const V4<float>*)c->prim[c->cur], c->delta);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -1729,16 +1317,8 @@ int mf_eval_q_inner_loop(mf_ctx* c) {
 
 int mf_cal_flux_residual(mf_ctx* c) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
-  const dim3 gr(nblocks(c)), bl(BLOCK);
-  if (c->ssz == 8) {
-    State<double> st = state_of<double>(c, c->cur, c->dq_cur);
-    if (c->power0) hipLaunchKernelGGL((k_flux<double, true>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
-    else hipLaunchKernelGGL((k_flux<double, false>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
-  } else {
-    State<float> st = state_of<float>(c, c->cur, c->dq_cur);
-    if (c->power0) hipLaunchKernelGGL((k_flux<float, true>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
-    else hipLaunchKernelGGL((k_flux<float, false>), gr, bl, 0, c->stream, c->g, st, c->delta, c->flux_res, c->p.power);
-  }
+  if (c->ssz == 8) launch_flux_any<double>(env_of(c), state_of<double>(c, c->cur, c->dq_cur), c->delta, c->flux_res);
+  else launch_flux_any<float>(env_of(c), state_of<float>(c, c->cur, c->dq_cur), c->delta, c->flux_res);
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
@@ -1746,13 +1326,11 @@ int mf_cal_flux_residual(mf_ctx* c) {
 int mf_state_update(mf_ctx* c, int32_t rk, double* sum_res_sqr) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
   if (rk < 1 || rk > 4) return set_err(MF_ERR_INVALID, "rk must be 1..4");
-  const dim3 gr(nblocks(c)), bl(BLOCK);
   if (c->ssz == 8)
-    hipLaunchKernelGGL(k_update<double>, gr, bl, 0, c->stream, c->g, (V4d*)c->prim[c->cur],
-                       (const V4d*)c->prim_old, c->flux_res, c->partial, rk, c->p.euler, c->fs);
+    launch_update_any<double>(env_of(c), (V4d*)c->prim[c->cur], (const V4d*)c->prim_old, c->flux_res, c->partial, rk);
   else
-    hipLaunchKernelGGL(k_update<float>, gr, bl, 0, c->stream, c->g, (V4<float>*)c->prim[c->cur],
-                       (const V4<float>*)c->prim_old, c->flux_res, c->partial, rk, c->p.euler, c->fs);
+    launch_update_any<float>(env_of(c), (V4<float>*)c->prim[c->cur], (const V4<float>*)c->prim_old, c->flux_res,
+                             c->partial, rk);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(BLOCK), 0, c->stream, c->partial, nblocks(c), c->scalar);
   HIPCHK(hipGetLastError());
