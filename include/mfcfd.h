@@ -164,7 +164,12 @@ int mf_batch_cases(mf_ctx *ctx, int32_t *ncases, int32_t *first);
  * residual + state update.  A point's neighbours are split over that many lanes of a
  * wavefront and the partial sums combined with shuffles; 1 (the default) keeps the
  * reference's summation order.  Replaces the thread-block shape option of the CUDA
- * build (src_cuda data_structure.F90:104 blockx/blocky/blockz, q_lskum.cuf:51). */
+ * build (src_cuda data_structure.F90:104 blockx/blocky/blockz, q_lskum.cuf:51).
+ * The LSQ and inner-iteration kernels also take MF_LPP_STAGED | 1, 2 or 4: blocks of
+ * 64 consecutive points first copy their neighbour records (each once) into LDS and
+ * walk the stencils there; same per-lane summation order, bit-identical results.
+ * mf_set_tuning rejects it when the context's halos do not fit in LDS. */
+#define MF_LPP_STAGED 16
 typedef struct mf_tuning {
   int32_t lpp_lsq, lpp_inner, lpp_flux, reserved;
   double us_lsq, us_inner, us_flux;  /* mf_autotune: mean launch time of the chosen shape */

@@ -37,11 +37,43 @@ __global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __res
   cspeed[i] = wave_speed(p);
 }
 
+// Thread-to-point map of a stencil kernel.  Global walk: point_id() over BLOCK-thread
+// blocks, LPP lanes per point, threads past the last point leave.  Staged walk: block b
+// (XCD-remapped) holds points b*STAGE_P.. with LPP lanes each; every thread stays for
+// the block's barrier, `live` tells whether its point exists.
+template <int LPP, bool STAGED>
+struct PointMap {
+  int b, i, lane;
+  bool live;
+  __device__ PointMap(int n) {
+    if constexpr (STAGED) {
+      b = xcd_block(blockIdx.x, gridDim.x);
+      i = b * STAGE_P + (int)threadIdx.x / LPP;
+      lane = (int)threadIdx.x % LPP;
+    } else {
+      const int t = point_id();
+      b = 0;
+      i = t / LPP;
+      lane = t % LPP;
+    }
+    live = i < n;
+  }
+};
+
+// LDS bytes of the staged kernels: per halo entry the records the kernel stages, plus the
+// block's slot positions (uint16 [W][STAGE_P])
+inline size_t stage_lds_lsq(int hmax, int width, size_t ssz, bool delta) {
+  return (size_t)hmax * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P * 2;
+}
+inline size_t stage_lds_inner(int hmax, int width, size_t ssz, bool sums) {
+  return (size_t)hmax * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+}
+
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
 // (state_update.F90:190-236), which gathers over the same stencil.
 // qsum (mixed path): the first-pass sums sum w (dx, dy) (q_k - q_i) are also stored, for
 // the inner iterations (q does not change between them).
-template <class S, bool POWER0, bool DELTA, int LPP>
+template <class S, bool POWER0, bool DELTA, int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__ q,
                                                V4<S>* __restrict__ dq, V4<S>* __restrict__ qm,
                                                const double* __restrict__ cspeed,
@@ -49,54 +81,72 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
                                                double power, const Ctl* __restrict__ ctl,
                                                V4<float>* __restrict__ qsum) {
   if (ctl && ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;  // whole groups: n * LPP threads are a multiple of LPP
+  const PointMap<LPP, STAGED> pm(g.n);
+  if (!STAGED && !pm.live) return;  // whole groups: n * LPP threads are a multiple of LPP
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
   const D2 pi = g.xy[i];
   const V4<S> qi = q[i];
   V4<S> qmax = qi, qmin = qi;
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
   double min_delt = 1.0;
-  const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
-  struct Nbr {
-    V4<S> q;
-    D2 p;
-    double c;
+  const int nb = pm.live ? g.nbhs[i] : 0;
+  // one edge; a slot past nb (global walk only) carries the point's own record with
+  // valid = false: neutral for the strict max/min, masked for the sums
+  auto edge = [&](const V4<S>& qk, const D2& pk, double ck, bool valid) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (qk[r] > qmax[r]) qmax[r] = qk[r];
+      if (qk[r] < qmin[r]) qmin[r] = qk[r];
+    }
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    double w = 1.0;
+    if (!POWER0 || DELTA) {
+      const double dist = sqrt(delx * delx + dely * dely);
+      if (!POWER0) w = pow(dist, power);
+      if (DELTA) {
+        double dt = dist / ck;
+        dt = cfl * dt;
+        if (valid && min_delt > dt) min_delt = dt;
+      }
+    }
+    const double wx = w * delx, wy = w * dely;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double d = (double)qk[r] - (double)qi[r];
+      sxq[r] = valid ? sxq[r] + wx * d : sxq[r];
+      syq[r] = valid ? syq[r] + wy * d : syq[r];
+    }
   };
-  walk_stencil<walk_batch<S, LPP>(), LPP>(
-      cp, g.n, nb, lane, (uint32_t)i << 4,
-      [&](uint32_t e) {
-        const int k = (int)(e >> 4);
-        return Nbr{q[k], g.xy[k], DELTA ? cspeed[k] : 0.0};
-      },
-      [&](const Nbr& nk, uint32_t, bool valid) {
-        // a slot past nb carries the point's own q: neutral for the strict max/min
-        const V4<S> qk = nk.q;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (qk[r] > qmax[r]) qmax[r] = qk[r];
-          if (qk[r] < qmin[r]) qmin[r] = qk[r];
-        }
-        const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-        double w = 1.0;
-        if (!POWER0 || DELTA) {
-          const double dist = sqrt(delx * delx + dely * dely);
-          if (!POWER0) w = pow(dist, power);
-          if (DELTA) {
-            double dt = dist / nk.c;
-            dt = cfl * dt;
-            if (valid && min_delt > dt) min_delt = dt;
-          }
-        }
-        const double wx = w * delx, wy = w * dely;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double d = (double)qk[r] - (double)qi[r];
-          sxq[r] = valid ? sxq[r] + wx * d : sxq[r];
-          syq[r] = valid ? syq[r] + wy * d : syq[r];
-        }
-      });
+  if constexpr (STAGED) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int H = g.hmax;
+    V4<S>* s_q = (V4<S>*)smem;
+    D2* s_xy = (D2*)(s_q + H);
+    double* s_c = (double*)(s_xy + H);
+    uint16_t* s_cl = (uint16_t*)(s_c + (DELTA ? H : 0));
+    stage_and_walk<LPP>(
+        g, pm.b, nb, s_cl,
+        [&](int j, int k) {
+          s_q[j] = q[k];
+          s_xy[j] = g.xy[k];
+          if (DELTA) s_c[j] = cspeed[k];
+        },
+        [&](int j) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
+    if (!pm.live) return;
+  } else {
+    struct Nbr {
+      V4<S> q;
+      D2 p;
+      double c;
+    };
+    walk_stencil<walk_batch<S, LPP>(), LPP>(
+        g.conn + i, g.n, nb, lane, (uint32_t)i << 4,
+        [&](uint32_t e) {
+          const int k = (int)(e >> 4);
+          return Nbr{q[k], g.xy[k], DELTA ? cspeed[k] : 0.0};
+        },
+        [&](const Nbr& nk, uint32_t, bool valid) { edge(nk.q, nk.p, nk.c, valid); });
+  }
   if (LPP > 1) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -131,21 +181,21 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
 // - 0.5 [dx (dq_x,k - dq_x,i) + dy (dq_y,k - dq_y,i)]) start from the LSQ kernel's
 // first-pass sums, so only the neighbours' dq and coordinates are gathered; the FP64
 // path keeps the reference's per-edge form.
-template <class S, bool POWER0, int LPP, bool SUMS>
+template <class S, bool POWER0, int LPP, bool SUMS, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict__ q,
                                                  const V4<S>* __restrict__ dq_in,
                                                  V4<S>* __restrict__ dq_out, double power,
                                                  const Ctl* __restrict__ ctl,
                                                  const V4<float>* __restrict__ qsum) {
   if (ctl && ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;
+  const PointMap<LPP, STAGED> pm(g.n);
+  if (!STAGED && !pm.live) return;
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
   const D2 pi = g.xy[i];
   const V4<S> dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
-  const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
+  const int nb = pm.live ? g.nbhs[i] : 0;
+  V4<S> qi{};
   if constexpr (SUMS) {
     if (lane == 0) {
       const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
@@ -155,54 +205,75 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
         syq[r] = b[r];
       }
     }
+  } else {
+    qi = q[i];
+  }
+  // one edge (qk unused in the sums form); valid as in k_lsq
+  auto edge = [&](const D2& pk, const V4<S>& qk, const V4<S>& dxk, const V4<S>& dyk, bool valid) {
+    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+    double w = 1.0;
+    if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+    if constexpr (SUMS) {
+      const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double d = delx * (double)(dxk[r] - dxi[r]) + dely * (double)(dyk[r] - dyi[r]);
+        sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
+        syq[r] = valid ? syq[r] - hy * d : syq[r];
+      }
+    } else {
+      const double wx = w * delx, wy = w * dely;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
+        const double temp2 = (double)qk[r] - 0.5 * (delx * (double)dxk[r] + dely * (double)dyk[r]);
+        sxq[r] = valid ? sxq[r] + (wx * (temp2 - temp1)) : sxq[r];
+        syq[r] = valid ? syq[r] + (wy * (temp2 - temp1)) : syq[r];
+      }
+    }
+  };
+  if constexpr (STAGED) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int H = g.hmax;
+    V4<S>* s_dx = (V4<S>*)smem;
+    V4<S>* s_dy = s_dx + H;
+    V4<S>* s_q = s_dy + H;  // FP64 form only
+    D2* s_xy = (D2*)(s_q + (SUMS ? 0 : H));
+    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    stage_and_walk<LPP>(
+        g, pm.b, nb, s_cl,
+        [&](int j, int k) {
+          s_dx[j] = dq_in[2 * k];
+          s_dy[j] = dq_in[2 * k + 1];
+          if (!SUMS) s_q[j] = q[k];
+          s_xy[j] = g.xy[k];
+        },
+        [&](int j) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
+    if (!pm.live) return;
+  } else if constexpr (SUMS) {
     struct Nbr {
       D2 p;
       V4<S> dx, dy;
     };
     walk_stencil<walk_batch<S, LPP>(), LPP>(
-        cp, g.n, nb, lane, (uint32_t)i << 4,
+        g.conn + i, g.n, nb, lane, (uint32_t)i << 4,
         [&](uint32_t e) {
           const int k = (int)(e >> 4);
           return Nbr{g.xy[k], dq_in[2 * k], dq_in[2 * k + 1]};
         },
-        [&](const Nbr& nk, uint32_t, bool valid) {
-          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-          double w = 1.0;
-          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-          const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double d = delx * (double)(nk.dx[r] - dxi[r]) + dely * (double)(nk.dy[r] - dyi[r]);
-            sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
-            syq[r] = valid ? syq[r] - hy * d : syq[r];
-          }
-        });
-  }
-  if constexpr (!SUMS) {
-    const V4<S> qi = q[i];
+        [&](const Nbr& nk, uint32_t, bool valid) { edge(nk.p, qi, nk.dx, nk.dy, valid); });
+  } else {
     struct Nbr {
       D2 p;
       V4<S> q, dx, dy;
     };
     walk_stencil<walk_batch<S, LPP>(), LPP>(
-        cp, g.n, nb, lane, (uint32_t)i << 4,
+        g.conn + i, g.n, nb, lane, (uint32_t)i << 4,
         [&](uint32_t e) {
           const int k = (int)(e >> 4);
           return Nbr{g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]};
         },
-        [&](const Nbr& nk, uint32_t, bool valid) {
-          const double delx = nk.p.x - pi.x, dely = nk.p.y - pi.y;
-          double w = 1.0;
-          if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
-          const double wx = w * delx, wy = w * dely;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double temp1 = (double)qi[r] - 0.5 * (delx * (double)dxi[r] + dely * (double)dyi[r]);
-            const double temp2 = (double)nk.q[r] - 0.5 * (delx * (double)nk.dx[r] + dely * (double)nk.dy[r]);
-            sxq[r] = valid ? sxq[r] + (wx * (temp2 - temp1)) : sxq[r];
-            syq[r] = valid ? syq[r] + (wy * (temp2 - temp1)) : syq[r];
-          }
-        });
+        [&](const Nbr& nk, uint32_t, bool valid) { edge(nk.p, nk.q, nk.dx, nk.dy, valid); });
   }
   if (LPP > 1) {
 #pragma unroll
@@ -412,35 +483,75 @@ inline void with_lpp(int lpp, F&& f) {
   }
 }
 
-// eval_q_derivatives (+ func_delta) of q into dq, qm (and delta)
+// Staged shapes: 1, 2 or 4 lanes per point (a block of STAGE_P points is at most BLOCK
+// threads)
+template <class F>
+inline void with_lpp_staged(int lpp, F&& f) {
+  switch (lpp) {
+    case 2: f.template operator()<2>(); break;
+    case 4: f.template operator()<4>(); break;
+    default: f.template operator()<1>(); break;
+  }
+}
+
+inline dim3 stage_grid(int n) { return dim3((unsigned)((n + STAGE_P - 1) / STAGE_P)); }
+
+// eval_q_derivatives (+ func_delta) of q into dq, qm (and delta).  lpp: lanes per point,
+// with LPP_STAGED set for the staged walk (the caller has checked that it fits in LDS).
 template <class S>
 void launch_lsq_env(const LaunchEnv& e, const V4<S>* q, V4<S>* dq, V4<S>* qm, const double* cspeed,
                     double* delta, bool with_delta, const Ctl* ctl, int lpp, V4<float>* qsum) {
-  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   const double cfl = e.cfl, pw = e.power;
+  if (lpp & LPP_STAGED) {
+    const dim3 gr = stage_grid(e.n);
+    const size_t lds = stage_lds_lsq(e.g.hmax, e.g.width, sizeof(S), with_delta);
+    with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
+      const dim3 bl(STAGE_P * L);
+      if (e.power0 && with_delta)
+        hipLaunchKernelGGL((k_lsq<S, true, true, L, true>), gr, bl, lds, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      else if (e.power0)
+        hipLaunchKernelGGL((k_lsq<S, true, false, L, true>), gr, bl, lds, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      else if (with_delta)
+        hipLaunchKernelGGL((k_lsq<S, false, true, L, true>), gr, bl, lds, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      else
+        hipLaunchKernelGGL((k_lsq<S, false, false, L, true>), gr, bl, lds, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+    });
+    return;
+  }
+  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   with_lpp(lpp, [&]<int L>() {
     if (e.power0 && with_delta)
-      hipLaunchKernelGGL((k_lsq<S, true, true, L>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      hipLaunchKernelGGL((k_lsq<S, true, true, L, false>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else if (e.power0)
-      hipLaunchKernelGGL((k_lsq<S, true, false, L>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      hipLaunchKernelGGL((k_lsq<S, true, false, L, false>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else if (with_delta)
-      hipLaunchKernelGGL((k_lsq<S, false, true, L>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      hipLaunchKernelGGL((k_lsq<S, false, true, L, false>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
     else
-      hipLaunchKernelGGL((k_lsq<S, false, false, L>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
+      hipLaunchKernelGGL((k_lsq<S, false, false, L, false>), gr, bl, 0, e.stream, e.g, q, dq, qm, cspeed, delta, cfl, pw, ctl, qsum);
   });
 }
 
 // one inner iteration: dq_in (a) -> dq_out (b); the mixed path gathers only dq (sums
-// form, first-pass sums from the LSQ kernel)
+// form, first-pass sums from the LSQ kernel).  lpp as for launch_lsq_env.
 template <class S>
 void launch_inner_env(const LaunchEnv& e, const V4<S>* q, const V4<S>* a, V4<S>* b, const Ctl* ctl, int lpp,
                       const V4<float>* qsum) {
-  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   const double pw = e.power;
   constexpr bool SUMS = std::is_same<S, float>::value;
+  if (lpp & LPP_STAGED) {
+    const dim3 gr = stage_grid(e.n);
+    const size_t lds = stage_lds_inner(e.g.hmax, e.g.width, sizeof(S), SUMS);
+    with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
+      const dim3 bl(STAGE_P * L);
+      if (e.power0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS, true>), gr, bl, lds, e.stream, e.g, q, a, b, pw, ctl, qsum);
+      else hipLaunchKernelGGL((k_inner<S, false, L, SUMS, true>), gr, bl, lds, e.stream, e.g, q, a, b, pw, ctl, qsum);
+    });
+    return;
+  }
+  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   with_lpp(lpp, [&]<int L>() {
-    if (e.power0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS>), gr, bl, 0, e.stream, e.g, q, a, b, pw, ctl, qsum);
-    else hipLaunchKernelGGL((k_inner<S, false, L, SUMS>), gr, bl, 0, e.stream, e.g, q, a, b, pw, ctl, qsum);
+    if (e.power0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS, false>), gr, bl, 0, e.stream, e.g, q, a, b, pw, ctl, qsum);
+    else hipLaunchKernelGGL((k_inner<S, false, L, SUMS, false>), gr, bl, 0, e.stream, e.g, q, a, b, pw, ctl, qsum);
   });
 }
 

@@ -386,6 +386,58 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 
 }  // extern "C"
 
+// Halos of the staged stencil walks: for each block of STAGE_P consecutive (internal)
+// points, its own points, then every other neighbour of them once (ascending), and each
+// slot's position in that list.  Leaves g.hmax = 0 (staging unavailable) when a halo
+// does not fit a 16-bit position.
+static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
+  const int n = c->n, W = c->W;
+  const int nblk = (n + STAGE_P - 1) / STAGE_P;
+  std::vector<int32_t> ptr(nblk + 1, 0), ids;
+  std::vector<uint16_t> loc((size_t)W * n, 0);
+  ids.reserve((size_t)n * 3);
+  std::vector<int32_t> other;
+  int hmax = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int p0 = b * STAGE_P, p1 = std::min(n, p0 + STAGE_P);
+    other.clear();
+    for (int i = p0; i < p1; ++i)
+      for (int s = 0; s < nbh[i]; ++s) {
+        const int k = (int)(conn[(size_t)s * n + i] >> 4);
+        if (k < p0 || k >= p1) other.push_back(k);
+      }
+    std::sort(other.begin(), other.end());
+    other.erase(std::unique(other.begin(), other.end()), other.end());
+    const int own = p1 - p0, hn = own + (int)other.size();
+    if (hn > 65535) {
+      c->g.hmax = 0;
+      return MF_OK;
+    }
+    hmax = std::max(hmax, hn);
+    for (int i = p0; i < p1; ++i) ids.push_back(i);
+    ids.insert(ids.end(), other.begin(), other.end());
+    for (int i = p0; i < p1; ++i)
+      for (int s = 0; s < nbh[i]; ++s) {
+        const int k = (int)(conn[(size_t)s * n + i] >> 4);
+        const int pos = (k >= p0 && k < p1)
+                            ? k - p0
+                            : own + (int)(std::lower_bound(other.begin(), other.end(), k) - other.begin());
+        loc[(size_t)s * n + i] = (uint16_t)pos;
+      }
+    ptr[b + 1] = (int32_t)ids.size();
+  }
+  int32_t *d_ptr, *d_ids;
+  uint16_t* d_loc;
+  int rc;
+  if ((rc = upload(c, &d_ptr, ptr)) || (rc = upload(c, &d_ids, ids)) || (rc = upload(c, &d_loc, loc))) return rc;
+  c->g.halo_ptr = d_ptr;
+  c->g.halo_ids = d_ids;
+  c->g.conn_loc = d_loc;
+  c->g.hmax = hmax;
+  c->g.width = W;
+  return MF_OK;
+}
+
 static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, const BatchSpec* bs) {
   const int n = geo->n_points, W = geo->max_nbhs;
   if (n <= 0 || W <= 0) return set_err(MF_ERR_INVALID, "n_points and max_nbhs must be > 0");
@@ -635,6 +687,7 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
       (rc = upload(c, &d_sxy, sxyv)) || (rc = upload(c, &d_obd, obd)))
     return rc;
   c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no, prm->first_order ? 1 : 0};
+  if ((rc = build_halos(c, conn_d, nbh_d))) return rc;
   if (c->ssz == 8) {
     double* e;
     if ((rc = upload(c, &e, eps_d))) return rc;
@@ -1003,6 +1056,22 @@ static void drop_graphs(mf_ctx* c) {
 
 static bool valid_lpp(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
 
+// Largest dynamic LDS of one block of a staged kernel
+static constexpr size_t kMaxStageLds = 64 * 1024;
+
+// A forward stencil kernel's shape: kind 0 LSQ, 1 inner iteration, 2 flux+update.  The
+// staged walk (LPP_STAGED | 1, 2 or 4) exists for the LSQ and inner kernels when the
+// context's halos fit in LDS.
+static bool valid_shape(const mf_ctx* c, int kind, int v) {
+  if (valid_lpp(v)) return true;
+  if (kind == 2 || !(v & LPP_STAGED) || c->g.hmax <= 0) return false;
+  const int l = v & ~LPP_STAGED;
+  if (l != 1 && l != 2 && l != 4) return false;
+  const size_t lds = kind == 0 ? stage_lds_lsq(c->g.hmax, c->g.width, c->ssz, true)
+                               : stage_lds_inner(c->g.hmax, c->g.width, c->ssz, c->ssz == 4);
+  return lds <= kMaxStageLds;
+}
+
 // mean time (us) of `reps` launches of fn on the context's stream, after 2 untimed
 template <class F>
 static int time_launches(mf_ctx* c, int reps, F&& fn, double* us) {
@@ -1036,9 +1105,16 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   const int cur = c->cur;
   auto q = (const V4<S>*)c->q[cur];
   const int cand[4] = {1, 2, 4, 8};
+  // the stencil-gather kernels also try the staged walk where it fits
+  std::vector<int> cand_st[2];
+  for (int k = 0; k < 2; ++k) {
+    cand_st[k].assign(cand, cand + 4);
+    for (int l : {1, 2, 4})
+      if (valid_shape(c, k, LPP_STAGED | l)) cand_st[k].push_back(LPP_STAGED | l);
+  }
   double best[3] = {1e300, 1e300, 1e300};
   int pick[3] = {1, 1, 1};
-  for (int lpp : cand) {
+  for (int lpp : cand_st[0]) {
     double us;
     if ((rc = time_launches(c, reps, [&] {
            launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp, c->qsum);
@@ -1047,7 +1123,7 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
     if (us < best[0]) best[0] = us, pick[0] = lpp;
   }
   launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, 1, c->qsum);
-  for (int lpp : cand) {
+  for (int lpp : cand_st[1]) {
     double us;
     if ((rc = time_launches(c, reps, [&] {
            launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp, c->qsum);
@@ -1093,8 +1169,10 @@ int mf_get_tuning(mf_ctx* c, mf_tuning* out) {
 
 int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   if (!c || !t) return set_err(MF_ERR_INVALID, "mf_set_tuning: null argument");
-  if (!valid_lpp(t->lpp_lsq) || !valid_lpp(t->lpp_inner) || !valid_lpp(t->lpp_flux))
-    return set_err(MF_ERR_INVALID, "lanes per point must be 1, 2, 4 or 8");
+  if (!valid_shape(c, 0, t->lpp_lsq) || !valid_shape(c, 1, t->lpp_inner) || !valid_shape(c, 2, t->lpp_flux))
+    return set_err(MF_ERR_INVALID,
+                   "lanes per point must be 1, 2, 4 or 8 (LSQ and inner kernels also MF_LPP_STAGED | 1, 2 or 4 "
+                   "when the context's halos fit in LDS)");
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
     if (rv[k] != 0 && !valid_lpp(rv[k]))

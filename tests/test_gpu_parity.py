@@ -196,7 +196,9 @@ def test_autotune_and_invalid_shape(small_cloud):
         a.q_lskum(5)
         before = a.prim.copy()
         t = a.autotune(reps=3)
-        assert all(t[k] in (1, 2, 4, 8) for k in ("lpp_lsq", "lpp_inner", "lpp_flux"))
+        staged = tuple(16 | v for v in (1, 2, 4))
+        assert all(t[k] in (1, 2, 4, 8) + staged for k in ("lpp_lsq", "lpp_inner"))
+        assert t["lpp_flux"] in (1, 2, 4, 8)
         assert all(t[k] > 0 for k in ("us_lsq", "us_inner", "us_flux"))
         assert np.array_equal(a.prim, before)
         # the tuned context continues the same trajectory as an untuned one
@@ -206,6 +208,27 @@ def test_autotune_and_invalid_shape(small_cloud):
         assert rel_err(a.prim, b.prim) < 1e-12
         with pytest.raises(MfError):
             a.tuning = {"lpp_flux": 3}
+        with pytest.raises(MfError):
+            a.tuning = {"lpp_flux": 16 | 2}  # no staged form of the flux kernel
+        with pytest.raises(MfError):
+            a.tuning = {"lpp_inner": 16 | 8}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("lpp", [1, 2, 4])
+def test_staged_walk_bit_identical(c1_cloud, precision, lpp):
+    """The staged walk (neighbour records copied once per 64-point block into LDS) keeps
+    each lane's slots and summation order: LSQ and inner kernels give the same bits as
+    the global walk with the same lanes per point."""
+    kw = dict(precision=precision, inner_iterations=2)
+    with _gpu(c1_cloud, **kw) as a, _gpu(c1_cloud, **kw) as b:
+        a.tuning = {"lpp_lsq": lpp, "lpp_inner": lpp, "lpp_flux": 2}
+        b.tuning = {"lpp_lsq": 16 | lpp, "lpp_inner": 16 | lpp, "lpp_flux": 2}
+        assert b.tuning["lpp_lsq"] == 16 | lpp and b.tuning["lpp_inner"] == 16 | lpp
+        ra, _, cda = a.q_lskum(25)
+        rb, _, cdb = b.q_lskum(25)
+        assert np.array_equal(a.prim, b.prim)
+        assert np.array_equal(ra, rb) and np.array_equal(cda, cdb)
 
 
 def test_nan_residue_stops_loop(small_cloud):
