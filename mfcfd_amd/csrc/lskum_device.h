@@ -89,13 +89,15 @@ struct Geom {
   const Freestream* fs_tab = nullptr;
   // Staged walks (hmax = 0: not available).  Block b of STAGE_P points has the halo
   // halo_ids[halo_ptr[b] .. halo_ptr[b + 1]): its own points first, then every other
-  // neighbour of its points once; conn_loc [W][N] is each slot's position in its
-  // block's halo.
+  // neighbour of its points once; conn_loc [W][loc_stride] is each slot's position in
+  // its block's halo (rows padded to whole blocks, so a block's row segment is 128 B
+  // and 16-byte aligned).
   const int32_t* halo_ptr = nullptr;
   const int32_t* halo_ids = nullptr;
   const uint16_t* conn_loc = nullptr;
-  int32_t hmax = 0;   // largest halo
-  int32_t width = 0;  // W, slots per point of conn and conn_loc
+  int32_t hmax = 0;        // largest halo, rounded up to a multiple of 8
+  int32_t width = 0;       // W, slots per point of conn and conn_loc
+  int32_t loc_stride = 0;  // N rounded up to a multiple of STAGE_P
 };
 
 struct Ctl {
@@ -184,11 +186,12 @@ __device__ __forceinline__ void stage_and_walk(const Geom& g, int b, int nb, uin
                                                Use&& use) {
   const int h0 = g.halo_ptr[b], hn = g.halo_ptr[b + 1] - h0;
   for (int j = threadIdx.x; j < hn; j += STAGE_P * LPP) fill(j, g.halo_ids[h0 + j]);
-  const int p0 = b * STAGE_P;
-  const int np = min(STAGE_P, g.n - p0);
-  for (int j = threadIdx.x; j < g.width * STAGE_P; j += STAGE_P * LPP) {
-    const int s = j / STAGE_P, pl = j - s * STAGE_P;
-    s_cl[j] = pl < np ? g.conn_loc[(size_t)s * g.n + p0 + pl] : (uint16_t)0;
+  // the block's slot positions: per slot a 128-byte row segment, 8 x 16-byte loads
+  constexpr int V = STAGE_P * 2 / 16;
+  const uint4* src = (const uint4*)(g.conn_loc + (size_t)b * STAGE_P);
+  for (int j = threadIdx.x; j < g.width * V; j += STAGE_P * LPP) {
+    const int s = j / V;
+    ((uint4*)s_cl)[j] = src[(size_t)s * (g.loc_stride / 8) + (j - s * V)];
   }
   __syncthreads();
   const int li = threadIdx.x / LPP;

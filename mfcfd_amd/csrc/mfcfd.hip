@@ -393,8 +393,9 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
   const int n = c->n, W = c->W;
   const int nblk = (n + STAGE_P - 1) / STAGE_P;
+  const int stride = nblk * STAGE_P;
   std::vector<int32_t> ptr(nblk + 1, 0), ids;
-  std::vector<uint16_t> loc((size_t)W * n, 0);
+  std::vector<uint16_t> loc((size_t)W * stride, 0);
   ids.reserve((size_t)n * 3);
   std::vector<int32_t> other;
   int hmax = 0;
@@ -409,7 +410,7 @@ static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::
     std::sort(other.begin(), other.end());
     other.erase(std::unique(other.begin(), other.end()), other.end());
     const int own = p1 - p0, hn = own + (int)other.size();
-    if (hn > 65535) {
+    if (hn > 65528) {
       c->g.hmax = 0;
       return MF_OK;
     }
@@ -422,7 +423,7 @@ static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::
         const int pos = (k >= p0 && k < p1)
                             ? k - p0
                             : own + (int)(std::lower_bound(other.begin(), other.end(), k) - other.begin());
-        loc[(size_t)s * n + i] = (uint16_t)pos;
+        loc[(size_t)s * stride + i] = (uint16_t)pos;
       }
     ptr[b + 1] = (int32_t)ids.size();
   }
@@ -433,8 +434,9 @@ static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::
   c->g.halo_ptr = d_ptr;
   c->g.halo_ids = d_ids;
   c->g.conn_loc = d_loc;
-  c->g.hmax = hmax;
+  c->g.hmax = (hmax + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
   c->g.width = W;
+  c->g.loc_stride = stride;
   return MF_OK;
 }
 
