@@ -326,7 +326,9 @@ def run_batch(iters: int, adjoint: bool = True):
 
 
 COPY_PROBE_BYTES = 256 << 20
-PROBE_SHAPES = {"lpp_lsq": 2, "lpp_inner": 2, "lpp_flux": 4}
+# the launch shapes the autotuner keeps for this workload (16 | L: staged walk, L lanes)
+PROBE_SHAPES = {"fp64": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 4},
+                "mixed": {"lpp_lsq": 16 | 2, "lpp_inner": 16 | 4, "lpp_flux": 4}}
 
 
 def probe():
@@ -340,8 +342,7 @@ def probe():
     p = {k: WORKLOAD[k] for k in ("mach", "aoa", "cfl", "vl_const", "inner_iterations")}
     for prec in ("fp64", "mixed"):
         with LskumSolver(cloud, precision=prec, **p) as s:
-            # the launch shapes the autotuner picks for this workload
-            s.tuning = PROBE_SHAPES
+            s.tuning = PROBE_SHAPES[prec]
             s.q_lskum(2)
     print(json.dumps({"probe": "ok", "copy_bytes": COPY_PROBE_BYTES, "copy_ms": ms.value,
                       "points": cloud.n, "mean_nbhs": float(cloud.nbhs.mean())}))
