@@ -165,10 +165,11 @@ int mf_batch_cases(mf_ctx *ctx, int32_t *ncases, int32_t *first);
  * wavefront and the partial sums combined with shuffles; 1 (the default) keeps the
  * reference's summation order.  Replaces the thread-block shape option of the CUDA
  * build (src_cuda data_structure.F90:104 blockx/blocky/blockz, q_lskum.cuf:51).
- * The LSQ and inner-iteration kernels also take MF_LPP_STAGED | 1, 2 or 4: blocks of
- * 64 consecutive points first copy their neighbour records (each once) into LDS and
- * walk the stencils there; same per-lane summation order, bit-identical results.
- * mf_set_tuning rejects it when the context's halos do not fit in LDS. */
+ * Staged shapes, MF_LPP_STAGED | lanes: blocks of consecutive points first copy their
+ * neighbours' records (each once) into LDS and walk the stencils there; same per-lane
+ * summation order, bit-identical results.  LSQ and inner-iteration kernels: 64-point
+ * blocks, 1, 2 or 4 lanes; flux + update: 32-point blocks, 2, 4 or 8 lanes.
+ * mf_set_tuning rejects a staged shape whose halos do not fit in LDS. */
 #define MF_LPP_STAGED 16
 typedef struct mf_tuning {
   int32_t lpp_lsq, lpp_inner, lpp_flux, reserved;

@@ -28,10 +28,25 @@ namespace mf {
 constexpr int WALL = 0, OUTER = 1, INTERIOR = 2;
 constexpr unsigned XPOS = 1u, XNEG = 2u, YPOS = 4u, YNEG = 8u;
 constexpr int BLOCK = 256;
-// Staged stencil walks: blocks of STAGE_P consecutive points; a lanes-per-point value
-// with the LPP_STAGED bit set selects the staged form of a stencil kernel
+// Staged stencil walks: blocks of consecutive points (STAGE_P for the LSQ and inner
+// kernels, STAGE_PF for the flux kernel, whose neighbour records are larger); a
+// lanes-per-point value with the LPP_STAGED bit set selects a kernel's staged form
 constexpr int STAGE_P = 64;
+constexpr int STAGE_PF = 32;
 constexpr int LPP_STAGED = 16;
+
+// One staging layout (blocks of P points).  Block b's halo is ids[ptr[b] .. ptr[b+1]):
+// its own points first, then every other neighbour of its points once.  loc [W][stride]
+// holds each slot's position in its block's halo (low 12 bits) and the slot's
+// half-stencil bits (high 4); rows are padded to whole blocks, so a block's row segment
+// is 16-byte aligned.
+struct Halo {
+  const int32_t* ptr = nullptr;
+  const int32_t* ids = nullptr;
+  const uint16_t* loc = nullptr;
+  int32_t hmax = 0;    // largest halo, rounded up to a multiple of 8 (0: not available)
+  int32_t stride = 0;  // N rounded up to a multiple of P
+};
 
 template <class T>
 struct alignas(4 * sizeof(T)) V4 {
@@ -87,17 +102,9 @@ struct Geom {
   // freestreams (null for a single case: the kernels' Freestream argument)
   const int32_t* case_id = nullptr;
   const Freestream* fs_tab = nullptr;
-  // Staged walks (hmax = 0: not available).  Block b of STAGE_P points has the halo
-  // halo_ids[halo_ptr[b] .. halo_ptr[b + 1]): its own points first, then every other
-  // neighbour of its points once; conn_loc [W][loc_stride] is each slot's position in
-  // its block's halo (rows padded to whole blocks, so a block's row segment is 128 B
-  // and 16-byte aligned).
-  const int32_t* halo_ptr = nullptr;
-  const int32_t* halo_ids = nullptr;
-  const uint16_t* conn_loc = nullptr;
-  int32_t hmax = 0;        // largest halo, rounded up to a multiple of 8
-  int32_t width = 0;       // W, slots per point of conn and conn_loc
-  int32_t loc_stride = 0;  // N rounded up to a multiple of STAGE_P
+  // staged walks: hl[0] in blocks of STAGE_P points, hl[1] in blocks of STAGE_PF
+  Halo hl[2];
+  int32_t width = 0;  // W, slots per point of conn and of the halos' loc
 };
 
 struct Ctl {
@@ -173,29 +180,32 @@ __device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, in
   }
 }
 
-// The staged form of a stencil walk.  A block holds STAGE_P consecutive points, LPP
-// lanes each, and first copies its halo (fill(j, k): point k's record into the caller's
-// LDS arrays at halo position j) and its points' slot positions into LDS; after one
-// barrier every lane walks its slots (lane, lane + LPP, ...) with use(j) reading halo
-// position j.  Each neighbour record then crosses from L2 once per block instead of
-// once per edge (about 2.8 instead of 23.5 record loads per point at 64 points per
-// block), and the walk's loads are LDS reads.  A lane's slots and their order are
-// those of the global walk, so its sums are bit-identical.
-template <int LPP, class Fill, class Use>
-__device__ __forceinline__ void stage_and_walk(const Geom& g, int b, int nb, uint16_t* s_cl, Fill&& fill,
-                                               Use&& use) {
-  const int h0 = g.halo_ptr[b], hn = g.halo_ptr[b + 1] - h0;
-  for (int j = threadIdx.x; j < hn; j += STAGE_P * LPP) fill(j, g.halo_ids[h0 + j]);
-  // the block's slot positions: per slot a 128-byte row segment, 8 x 16-byte loads
-  constexpr int V = STAGE_P * 2 / 16;
-  const uint4* src = (const uint4*)(g.conn_loc + (size_t)b * STAGE_P);
-  for (int j = threadIdx.x; j < g.width * V; j += STAGE_P * LPP) {
+// The staged form of a stencil walk.  A block holds P consecutive points, LPP lanes
+// each, and first copies its halo (fill(j, k): point k's records into the caller's LDS
+// arrays at halo position j) and its points' slot words into LDS; after one barrier
+// every lane walks its slots (lane, lane + LPP, ...) with use(j, m) reading halo
+// position j, m the slot's half-stencil bits.  Each neighbour record then crosses from
+// L2 once per block instead of once per edge (about 2.8 instead of 23.5 record loads
+// per point at 64 points per block), and the walk's loads are LDS reads.  A lane's
+// slots and their order are those of the global walk, so its sums are bit-identical.
+template <int P, int LPP, class Fill, class Use>
+__device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int b, int nb, uint16_t* s_cl,
+                                               Fill&& fill, Use&& use) {
+  const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
+  for (int j = threadIdx.x; j < hn; j += P * LPP) fill(j, h.ids[h0 + j]);
+  // the block's slot words: per slot a P * 2-byte row segment in 16-byte loads
+  constexpr int V = P * 2 / 16;
+  const uint4* src = (const uint4*)(h.loc + (size_t)b * P);
+  for (int j = threadIdx.x; j < g.width * V; j += P * LPP) {
     const int s = j / V;
-    ((uint4*)s_cl)[j] = src[(size_t)s * (g.loc_stride / 8) + (j - s * V)];
+    ((uint4*)s_cl)[j] = src[(size_t)s * (h.stride / 8) + (j - s * V)];
   }
   __syncthreads();
   const int li = threadIdx.x / LPP;
-  for (int s = threadIdx.x % LPP; s < nb; s += LPP) use((int)s_cl[s * STAGE_P + li]);
+  for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
+    const unsigned e = s_cl[s * P + li];
+    use((int)(e & 0xFFFu), e >> 12);
+  }
 }
 
 template <class S>
@@ -489,9 +499,68 @@ __host__ __device__ inline double coef_y(const HalfCoef& h, bool first, double d
 // (wall_fluxes.F90:20-292, outer_fluxes.F90:18-300, interior_fluxes.F90:19-404).
 // Returns flux_res(:, i) in fr; with LPP > 1, this lane's share (edges lane, lane +
 // LPP, ...), to be summed over the point's lanes.
-template <int CLS, class S, bool POWER0, int LPP = 1>
-__device__ inline void flux_point(const Geom& g, const State<S>& st, int i, double delta,
-                                  double power, double fr[4], int lane = 0) {
+// Where flux_point reads a neighbour's records (k: a point id for the global walk, a
+// halo position for the staged one) and how it walks its slots (walk(f) calls f(k, m)
+// per slot of the lane, m the slot's half-stencil bits).
+template <class S>
+struct GlobalNbrs {
+  const Geom& g;
+  const State<S>& st;
+  int i, nb, lane, lpp;
+  __device__ D2 xy(int k) const { return g.xy[k]; }
+  __device__ V4<S> q(int k) const { return st.q[k]; }
+  __device__ V4<S> dqx(int k) const { return st.dq[2 * k]; }
+  __device__ V4<S> dqy(int k) const { return st.dq[2 * k + 1]; }
+  __device__ V4<S> qmax(int k) const { return st.qm[2 * k]; }
+  __device__ V4<S> qmin(int k) const { return st.qm[2 * k + 1]; }
+  __device__ S eps(int k) const { return st.eps[k]; }
+  template <class F>
+  __device__ void walk(F&& f) const {
+    const uint32_t* cp = g.conn + i;
+    // the next slot's connectivity word is loaded while this edge is evaluated, so
+    // each edge starts with one memory round trip (its neighbour's records), not two
+    uint32_t pk_next = lane < nb ? cp[(size_t)lane * g.n] : 0u;
+    for (int s = lane; s < nb; s += lpp) {
+      const uint32_t pk = pk_next;
+      if (s + lpp < nb) pk_next = cp[(size_t)(s + lpp) * g.n];
+      f((int)(pk >> 4), pk & 15u);
+    }
+  }
+};
+
+// A block's staged halo in LDS (structure of arrays, one entry per halo position)
+template <class S>
+struct HaloRecs {
+  D2* xy;
+  V4<S> *q, *dqx, *dqy, *qmax, *qmin;
+  S* eps;
+};
+
+template <class S, int P, int LPP>
+struct StagedNbrs {
+  const Geom& g;
+  const HaloRecs<S>& r;
+  const uint16_t* s_cl;
+  int li, nb, lane;
+  __device__ D2 xy(int k) const { return r.xy[k]; }
+  __device__ V4<S> q(int k) const { return r.q[k]; }
+  __device__ V4<S> dqx(int k) const { return r.dqx[k]; }
+  __device__ V4<S> dqy(int k) const { return r.dqy[k]; }
+  __device__ V4<S> qmax(int k) const { return r.qmax[k]; }
+  __device__ V4<S> qmin(int k) const { return r.qmin[k]; }
+  __device__ S eps(int k) const { return r.eps[k]; }
+  template <class F>
+  __device__ void walk(F&& f) const {
+    for (int s = lane; s < nb; s += LPP) {
+      const unsigned e = s_cl[s * P + li];
+      f((int)(e & 0xFFFu), e >> 12);
+    }
+  }
+};
+
+template <int CLS, class S, bool POWER0, int LPP, class Nbrs>
+__device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, double delta, double power,
+                                      double fr[4], const Nbrs& nbrs) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -511,17 +580,8 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
     }
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
-  // the next slot's connectivity word is loaded while this edge is evaluated, so each
-  // edge starts with one memory round trip (its neighbour's records), not two
-  uint32_t pk_next = lane < nb ? cp[(size_t)lane * g.n] : 0u;
-  for (int s = lane; s < nb; s += LPP) {
-    const uint32_t pk = pk_next;
-    if (s + LPP < nb) pk_next = cp[(size_t)(s + LPP) * g.n];
-    const int k = (int)(pk >> 4);
-    const unsigned m = pk & 15u;
-    const D2 pkk = g.xy[k];
+  nbrs.walk([&](int k, unsigned m) {
+    const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -539,7 +599,7 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
     const S dx = (S)delx, dy = (S)dely;
     S qti[4], qtk[4];
     if (g.first_order) {
-      const V4<S> qk = st.q[k];
+      const V4<S> qk = nbrs.q(k);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         qti[r] = qi[r];
@@ -547,10 +607,10 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
       }
     } else {
       reconstruct_eval(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
-      const V4<S> qk = st.q[k];
-      const V4<S> dxk = st.dq[2 * k], dyk = st.dq[2 * k + 1];
-      const V4<S> qmaxk = st.qm[2 * k], qmink = st.qm[2 * k + 1];
-      reconstruct_eval(qk, dxk, dyk, qmaxk, qmink, st.eps[k], dx, dy, qtk);
+      const V4<S> qk = nbrs.q(k);
+      const V4<S> dxk = nbrs.dqx(k), dyk = nbrs.dqy(k);
+      const V4<S> qmaxk = nbrs.qmax(k), qmink = nbrs.qmin(k);
+      reconstruct_eval(qk, dxk, dyk, qmaxk, qmink, nbrs.eps(k), dx, dy, qtk);
     }
     // the two ends one after the other: one end's transcendental temporaries live
     // at a time
@@ -594,11 +654,19 @@ __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, doub
         if (tie_y) acc[r] = acc[r] + ((double)Hy[r] - (double)Gy[r]) * cy2;
       }
     }
-  }
+  });
   // flux_residual.F90:36 (wall: 2*delta), :50 and :68 (delta)
   const double scale = (CLS == WALL) ? 2.0 * delta : delta;
 #pragma unroll
   for (int r = 0; r < 4; ++r) fr[r] = scale * acc[r];
+}
+
+// A point's flux residual from the global walk
+template <int CLS, class S, bool POWER0, int LPP = 1>
+__device__ inline void flux_point(const Geom& g, const State<S>& st, int i, double delta, double power,
+                                  double fr[4], int lane = 0) {
+  flux_point_src<CLS, S, POWER0, LPP>(g, st, i, delta, power, fr,
+                                      GlobalNbrs<S>{g, st, i, g.nbhs[i], lane, LPP});
 }
 
 // state_update.F90:144-163 primitive_to_conserved (rotated frame)

@@ -41,14 +41,14 @@ __global__ __launch_bounds__(BLOCK) void k_q_variables(int n, const V4<S>* __res
 // blocks, LPP lanes per point, threads past the last point leave.  Staged walk: block b
 // (XCD-remapped) holds points b*STAGE_P.. with LPP lanes each; every thread stays for
 // the block's barrier, `live` tells whether its point exists.
-template <int LPP, bool STAGED>
+template <int LPP, bool STAGED, int P = STAGE_P>
 struct PointMap {
   int b, i, lane;
   bool live;
   __device__ PointMap(int n) {
     if constexpr (STAGED) {
       b = xcd_block(blockIdx.x, gridDim.x);
-      i = b * STAGE_P + (int)threadIdx.x / LPP;
+      i = b * P + (int)threadIdx.x / LPP;
       lane = (int)threadIdx.x % LPP;
     } else {
       const int t = point_id();
@@ -61,12 +61,15 @@ struct PointMap {
 };
 
 // LDS bytes of the staged kernels: per halo entry the records the kernel stages, plus the
-// block's slot positions (uint16 [W][STAGE_P])
+// block's slot words (uint16 [W][P])
 inline size_t stage_lds_lsq(int hmax, int width, size_t ssz, bool delta) {
   return (size_t)hmax * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P * 2;
 }
 inline size_t stage_lds_inner(int hmax, int width, size_t ssz, bool sums) {
   return (size_t)hmax * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+}
+inline size_t stage_lds_flux(int hmax, int width, size_t ssz) {
+  return (size_t)hmax * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
 }
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
@@ -119,19 +122,19 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   };
   if constexpr (STAGED) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hmax;
+    const int H = g.hl[0].hmax;
     V4<S>* s_q = (V4<S>*)smem;
     D2* s_xy = (D2*)(s_q + H);
     double* s_c = (double*)(s_xy + H);
     uint16_t* s_cl = (uint16_t*)(s_c + (DELTA ? H : 0));
-    stage_and_walk<LPP>(
-        g, pm.b, nb, s_cl,
+    stage_and_walk<STAGE_P, LPP>(
+        g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
           s_q[j] = q[k];
           s_xy[j] = g.xy[k];
           if (DELTA) s_c[j] = cspeed[k];
         },
-        [&](int j) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
+        [&](int j, unsigned) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
     if (!pm.live) return;
   } else {
     struct Nbr {
@@ -234,21 +237,21 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   };
   if constexpr (STAGED) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hmax;
+    const int H = g.hl[0].hmax;
     V4<S>* s_dx = (V4<S>*)smem;
     V4<S>* s_dy = s_dx + H;
     V4<S>* s_q = s_dy + H;  // FP64 form only
     D2* s_xy = (D2*)(s_q + (SUMS ? 0 : H));
     uint16_t* s_cl = (uint16_t*)(s_xy + H);
-    stage_and_walk<LPP>(
-        g, pm.b, nb, s_cl,
+    stage_and_walk<STAGE_P, LPP>(
+        g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
           s_dx[j] = dq_in[2 * k];
           s_dy[j] = dq_in[2 * k + 1];
           if (!SUMS) s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
+        [&](int j, unsigned) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
     if (!pm.live) return;
   } else if constexpr (SUMS) {
     struct Nbr {
@@ -354,24 +357,70 @@ __global__ __launch_bounds__(BLOCK) void k_update(Geom g, V4<S>* __restrict__ pr
 // Reads the stage's (q, dq, qm, prim), writes the next (prim, q) buffers; stage 1 also
 // saves prim_old (fpi_solver.F90:20-22), the last stage the residual partial sums and
 // the wave speeds for the next iteration's func_delta.
-template <class S, bool POWER0, int LPP>
+template <class S, bool POWER0, int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     Geom g, State<S> st, const V4<S>* __restrict__ prim, V4<S>* __restrict__ prim_old,
     V4<S>* __restrict__ prim_out, V4<S>* __restrict__ q_out, double* __restrict__ cspeed,
     const double* __restrict__ delta, double* __restrict__ partial, const Ctl* __restrict__ ctl,
     int rk, int last, double euler, double power, Freestream fs, double* __restrict__ rs_pt) {
   if (ctl->halt) return;
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
+  const PointMap<LPP, STAGED, STAGE_PF> pm(g.n);
+  const int i = pm.i, lane = pm.lane;
   double rs = 0.0;
   double fr[4] = {0.0, 0.0, 0.0, 0.0};
-  // every lane reaches the group sums (no early exit before them)
-  if (i < g.n) flux_dispatch<S, POWER0, LPP>(g, st, i, delta[i], power, fr, lane);
-  else if (LPP > 1) {
+  if constexpr (STAGED) {
+    // the block's halo in LDS, then the points' stencils from it (every thread reaches
+    // the barrier and the group sums)
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Halo& h = g.hl[1];
+    const int H = h.hmax;
+    HaloRecs<S> r;
+    r.q = (V4<S>*)smem;
+    r.dqx = r.q + H;
+    r.dqy = r.dqx + H;
+    r.qmax = r.dqy + H;
+    r.qmin = r.qmax + H;
+    r.xy = (D2*)(r.qmin + H);
+    r.eps = (S*)(r.xy + H);
+    uint16_t* s_cl = (uint16_t*)(r.eps + H);
+    const int h0 = h.ptr[pm.b], hn = h.ptr[pm.b + 1] - h0;
+    for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
+      const int k = h.ids[h0 + j];
+      r.q[j] = st.q[k];
+      r.dqx[j] = st.dq[2 * k];
+      r.dqy[j] = st.dq[2 * k + 1];
+      r.qmax[j] = st.qm[2 * k];
+      r.qmin[j] = st.qm[2 * k + 1];
+      r.xy[j] = g.xy[k];
+      r.eps[j] = st.eps[k];
+    }
+    constexpr int V = STAGE_PF * 2 / 16;
+    const uint4* src = (const uint4*)(h.loc + (size_t)pm.b * STAGE_PF);
+    for (int j = threadIdx.x; j < g.width * V; j += STAGE_PF * LPP) {
+      const int sl = j / V;
+      ((uint4*)s_cl)[j] = src[(size_t)sl * (h.stride / 8) + (j - sl * V)];
+    }
+    __syncthreads();
+    if (pm.live) {
+      const StagedNbrs<S, STAGE_PF, LPP> nbrs{g, r, s_cl, (int)threadIdx.x / LPP, g.nbhs[i], lane};
+      const double dl = delta[i];
+      if (i < g.nw) flux_point_src<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+      else if (i < g.nw + g.no) flux_point_src<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+      else flux_point_src<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+    }
+    if (LPP > 1) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
+      for (int r2 = 0; r2 < 4; ++r2) fr[r2] = group_sum<LPP>(fr[r2]);
+    }
+  } else {
+    // every lane reaches the group sums (no early exit before them)
+    if (pm.live) flux_dispatch<S, POWER0, LPP>(g, st, i, delta[i], power, fr, lane);
+    else if (LPP > 1) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) fr[r] = group_sum<LPP>(fr[r]);
+    }
   }
-  if (i < g.n && lane == 0) {
+  if (pm.live && lane == 0) {
     const V4<S> p = prim[i];
     if (rk == 1) prim_old[i] = p;
     const V4<S> po = (rk == 3) ? prim_old[i] : p;
@@ -504,7 +553,7 @@ void launch_lsq_env(const LaunchEnv& e, const V4<S>* q, V4<S>* dq, V4<S>* qm, co
   const double cfl = e.cfl, pw = e.power;
   if (lpp & LPP_STAGED) {
     const dim3 gr = stage_grid(e.n);
-    const size_t lds = stage_lds_lsq(e.g.hmax, e.g.width, sizeof(S), with_delta);
+    const size_t lds = stage_lds_lsq(e.g.hl[0].hmax, e.g.width, sizeof(S), with_delta);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       const dim3 bl(STAGE_P * L);
       if (e.power0 && with_delta)
@@ -540,7 +589,7 @@ void launch_inner_env(const LaunchEnv& e, const V4<S>* q, const V4<S>* a, V4<S>*
   constexpr bool SUMS = std::is_same<S, float>::value;
   if (lpp & LPP_STAGED) {
     const dim3 gr = stage_grid(e.n);
-    const size_t lds = stage_lds_inner(e.g.hmax, e.g.width, sizeof(S), SUMS);
+    const size_t lds = stage_lds_inner(e.g.hl[0].hmax, e.g.width, sizeof(S), SUMS);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       const dim3 bl(STAGE_P * L);
       if (e.power0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS, true>), gr, bl, lds, e.stream, e.g, q, a, b, pw, ctl, qsum);
@@ -555,23 +604,49 @@ void launch_inner_env(const LaunchEnv& e, const V4<S>* q, const V4<S>* a, V4<S>*
   });
 }
 
-// fused flux residual + state update (rk) + q_variables of the new state
+// fused flux residual + state update (rk) + q_variables of the new state.  lpp: lanes
+// per point, with LPP_STAGED set for the staged walk (2, 4 or 8 lanes, blocks of
+// STAGE_PF points)
 template <class S>
 void launch_flux_update_env(const LaunchEnv& e, const State<S>& st, const V4<S>* prim, V4<S>* pold, V4<S>* pout,
                             V4<S>* qout, double* cspeed, const double* delta, double* partial, const Ctl* ctl,
                             int rk, bool last, int lpp, double* rs_pt) {
-  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   const int lst = last ? 1 : 0;
   const double eu = e.euler, pw = e.power;
   const Freestream fs = e.fs;
+  if (lpp & LPP_STAGED) {
+    const dim3 gr((unsigned)((e.n + STAGE_PF - 1) / STAGE_PF));
+    const size_t lds = stage_lds_flux(e.g.hl[1].hmax, e.g.width, sizeof(S));
+    auto go = [&]<int L>() {
+      const dim3 bl(STAGE_PF * L);
+      if (e.power0)
+        hipLaunchKernelGGL((k_flux_update<S, true, L, true>), gr, bl, lds, e.stream, e.g, st, prim, pold, pout, qout,
+                           cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
+      else
+        hipLaunchKernelGGL((k_flux_update<S, false, L, true>), gr, bl, lds, e.stream, e.g, st, prim, pold, pout,
+                           qout, cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
+    };
+    switch (lpp & ~LPP_STAGED) {
+      case 2: go.template operator()<2>(); break;
+      case 4: go.template operator()<4>(); break;
+      default: go.template operator()<8>(); break;
+    }
+    return;
+  }
+  const dim3 gr = lane_grid(e.n, lpp), bl(BLOCK);
   with_lpp(lpp, [&]<int L>() {
     if (e.power0)
-      hipLaunchKernelGGL((k_flux_update<S, true, L>), gr, bl, 0, e.stream, e.g, st, prim, pold, pout, qout,
+      hipLaunchKernelGGL((k_flux_update<S, true, L, false>), gr, bl, 0, e.stream, e.g, st, prim, pold, pout, qout,
                          cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
     else
-      hipLaunchKernelGGL((k_flux_update<S, false, L>), gr, bl, 0, e.stream, e.g, st, prim, pold, pout, qout,
+      hipLaunchKernelGGL((k_flux_update<S, false, L, false>), gr, bl, 0, e.stream, e.g, st, prim, pold, pout, qout,
                          cspeed, delta, partial, ctl, rk, lst, eu, pw, fs, rs_pt);
   });
+}
+
+// Residual partials one flux+update launch writes (one per block)
+inline int flux_blocks(int n, int lpp) {
+  return (lpp & LPP_STAGED) ? (n + STAGE_PF - 1) / STAGE_PF : (int)lane_grid(n, lpp).x;
 }
 
 // Cl, Cd, Cm (and cp) per shape id

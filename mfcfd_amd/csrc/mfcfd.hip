@@ -386,21 +386,23 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 
 }  // extern "C"
 
-// Halos of the staged stencil walks: for each block of STAGE_P consecutive (internal)
-// points, its own points, then every other neighbour of them once (ascending), and each
-// slot's position in that list.  Leaves g.hmax = 0 (staging unavailable) when a halo
-// does not fit a 16-bit position.
-static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
+// Halo of the staged stencil walks in blocks of P consecutive (internal) points: each
+// block's own points, then every other neighbour of them once (ascending), and per slot
+// its position in that list with the slot's half-stencil bits above it.  Leaves
+// h.hmax = 0 (staging unavailable) when a halo outgrows the 12-bit position.
+static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh, int P,
+                      Halo& h) {
   const int n = c->n, W = c->W;
-  const int nblk = (n + STAGE_P - 1) / STAGE_P;
-  const int stride = nblk * STAGE_P;
+  const int nblk = (n + P - 1) / P;
+  const int stride = nblk * P;
   std::vector<int32_t> ptr(nblk + 1, 0), ids;
   std::vector<uint16_t> loc((size_t)W * stride, 0);
-  ids.reserve((size_t)n * 3);
+  ids.reserve((size_t)n * 4);
   std::vector<int32_t> other;
   int hmax = 0;
+  h = Halo{};
   for (int b = 0; b < nblk; ++b) {
-    const int p0 = b * STAGE_P, p1 = std::min(n, p0 + STAGE_P);
+    const int p0 = b * P, p1 = std::min(n, p0 + P);
     other.clear();
     for (int i = p0; i < p1; ++i)
       for (int s = 0; s < nbh[i]; ++s) {
@@ -410,20 +412,18 @@ static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::
     std::sort(other.begin(), other.end());
     other.erase(std::unique(other.begin(), other.end()), other.end());
     const int own = p1 - p0, hn = own + (int)other.size();
-    if (hn > 65528) {
-      c->g.hmax = 0;
-      return MF_OK;
-    }
+    if (hn > 4096 - 8) return MF_OK;
     hmax = std::max(hmax, hn);
     for (int i = p0; i < p1; ++i) ids.push_back(i);
     ids.insert(ids.end(), other.begin(), other.end());
     for (int i = p0; i < p1; ++i)
       for (int s = 0; s < nbh[i]; ++s) {
-        const int k = (int)(conn[(size_t)s * n + i] >> 4);
+        const uint32_t w = conn[(size_t)s * n + i];
+        const int k = (int)(w >> 4);
         const int pos = (k >= p0 && k < p1)
                             ? k - p0
                             : own + (int)(std::lower_bound(other.begin(), other.end(), k) - other.begin());
-        loc[(size_t)s * stride + i] = (uint16_t)pos;
+        loc[(size_t)s * stride + i] = (uint16_t)(pos | ((w & 15u) << 12));
       }
     ptr[b + 1] = (int32_t)ids.size();
   }
@@ -431,12 +431,19 @@ static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::
   uint16_t* d_loc;
   int rc;
   if ((rc = upload(c, &d_ptr, ptr)) || (rc = upload(c, &d_ids, ids)) || (rc = upload(c, &d_loc, loc))) return rc;
-  c->g.halo_ptr = d_ptr;
-  c->g.halo_ids = d_ids;
-  c->g.conn_loc = d_loc;
-  c->g.hmax = (hmax + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
-  c->g.width = W;
-  c->g.loc_stride = stride;
+  h.ptr = d_ptr;
+  h.ids = d_ids;
+  h.loc = d_loc;
+  h.hmax = (hmax + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
+  h.stride = stride;
+  return MF_OK;
+}
+
+static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
+  c->g.width = c->W;
+  int rc;
+  if ((rc = build_halo(c, conn, nbh, STAGE_P, c->g.hl[0])) || (rc = build_halo(c, conn, nbh, STAGE_PF, c->g.hl[1])))
+    return rc;
   return MF_OK;
 }
 
@@ -716,7 +723,8 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
   c->qm = qmb;
   c->prim_old = pob;
   // room for the partial sums of the largest lanes-per-point grid (8 lanes)
-  c->npartial = (int)(((int64_t)n * 8 + BLOCK - 1) / BLOCK);
+  // one residual partial per flux+update block: 8 lanes per point, or the staged walk
+  c->npartial = (int)std::max(((int64_t)n * 8 + BLOCK - 1) / BLOCK, ((int64_t)n + STAGE_PF - 1) / STAGE_PF);
   if ((rc = dev_alloc(c, &c->flux_res, n)) || (rc = dev_alloc(c, &c->delta, n)) ||
       (rc = dev_alloc(c, &c->cspeed, n)) || (rc = dev_alloc(c, &c->partial, c->npartial)) ||
       (rc = dev_alloc(c, &c->scalar, 4)) || (rc = dev_alloc(c, &c->ctl, 1)))
@@ -853,7 +861,7 @@ static void launch_finalize(mf_ctx* c) {
     return;
   }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial,
-                     (int)lane_blocks(c, c->lpp_flux).x, c->n, c->p.restart, c->graph_first_it, c->ctl,
+                     flux_blocks(c->n, c->lpp_flux), c->n, c->p.restart, c->graph_first_it, c->ctl,
                      c->res_hist, c->resnew_hist);
 }
 
@@ -1062,15 +1070,18 @@ static bool valid_lpp(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
 static constexpr size_t kMaxStageLds = 64 * 1024;
 
 // A forward stencil kernel's shape: kind 0 LSQ, 1 inner iteration, 2 flux+update.  The
-// staged walk (LPP_STAGED | 1, 2 or 4) exists for the LSQ and inner kernels when the
-// context's halos fit in LDS.
+// staged walks (LPP_STAGED | lanes) exist when the context's halos fit in LDS: 1, 2 or 4
+// lanes for the LSQ and inner kernels, 2, 4 or 8 for the flux kernel.
 static bool valid_shape(const mf_ctx* c, int kind, int v) {
   if (valid_lpp(v)) return true;
-  if (kind == 2 || !(v & LPP_STAGED) || c->g.hmax <= 0) return false;
+  if (!(v & LPP_STAGED)) return false;
   const int l = v & ~LPP_STAGED;
-  if (l != 1 && l != 2 && l != 4) return false;
-  const size_t lds = kind == 0 ? stage_lds_lsq(c->g.hmax, c->g.width, c->ssz, true)
-                               : stage_lds_inner(c->g.hmax, c->g.width, c->ssz, c->ssz == 4);
+  const Halo& h = c->g.hl[kind == 2 ? 1 : 0];
+  if (h.hmax <= 0) return false;
+  if (kind == 2 ? (l != 2 && l != 4 && l != 8) : (l != 1 && l != 2 && l != 4)) return false;
+  const size_t lds = kind == 0   ? stage_lds_lsq(h.hmax, c->g.width, c->ssz, true)
+                     : kind == 1 ? stage_lds_inner(h.hmax, c->g.width, c->ssz, c->ssz == 4)
+                                 : stage_lds_flux(h.hmax, c->g.width, c->ssz);
   return lds <= kMaxStageLds;
 }
 
@@ -1107,11 +1118,11 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   const int cur = c->cur;
   auto q = (const V4<S>*)c->q[cur];
   const int cand[4] = {1, 2, 4, 8};
-  // the stencil-gather kernels also try the staged walk where it fits
-  std::vector<int> cand_st[2];
-  for (int k = 0; k < 2; ++k) {
+  // and every staged walk that fits
+  std::vector<int> cand_st[3];
+  for (int k = 0; k < 3; ++k) {
     cand_st[k].assign(cand, cand + 4);
-    for (int l : {1, 2, 4})
+    for (int l : {1, 2, 4, 8})
       if (valid_shape(c, k, LPP_STAGED | l)) cand_st[k].push_back(LPP_STAGED | l);
   }
   double best[3] = {1e300, 1e300, 1e300};
@@ -1134,7 +1145,7 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
     if (us < best[1]) best[1] = us, pick[1] = lpp;
   }
   const State<S> st = state_of<S>(c, cur, 0);
-  for (int lpp : cand) {
+  for (int lpp : cand_st[2]) {
     double us;
     if ((rc = time_launches(c, reps, [&] {
            launch_flux_update_on<S>(c, st, (const V4<S>*)c->prim[cur], (V4<S>*)c->prim_old,
