@@ -74,3 +74,39 @@ def test_fortran_host_rejects_bad_shapes(small_cloud, tmp_path):
     pointio.write_case(str(tmp_path / "case.in"), cfl=0.2, shapes=1)
     out = subprocess.run([HOST, "--check"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert out.returncode != 0 and "shapes value wrong" in out.stdout
+
+
+@pytest.mark.parametrize("nparts", [2, 5])
+def test_partitioned_point_files_round_trip(small_cloud, tmp_path, nparts):
+    """point.h5 groups /1../N and partGridNNNN files of an N-rank run: each rank's local
+    points, ghosts (PETSc global index, x, y, min_dist) and local-numbered conn/left/right
+    reassemble into the original cloud."""
+    from mfcfd_amd import pointio
+    c = small_cloud
+    ref = pointio.local_table(c)
+    parts = pointio.partition(c, nparts)
+    assert sum(p.local.shape[0] for p in parts) == c.n
+    offs = np.concatenate([[0], np.cumsum([p.local.shape[0] for p in parts])])
+    owner_ids = np.concatenate([p.local[:, 0] for p in parts]).astype(int)  # by PETSc index
+    for r, p in enumerate(parts):
+        L = p.local.shape[0]
+        # every reference past the local block is a ghost, and every ghost is referenced
+        refs = np.concatenate([p.local[:, 12:].ravel(), p.local[:, 6], p.local[:, 7]]).astype(int)
+        refs = refs[refs > 0]
+        assert refs.max() <= p.total
+        assert set(refs[refs > L]) == set(range(L + 1, p.total + 1))
+        # ghosts carry the owner's coordinates, and no ghost is owned by this rank
+        gid = p.ghost[:, 0].astype(int)
+        assert np.all((gid - 1 < offs[r]) | (gid - 1 >= offs[r + 1]))
+        orig = owner_ids[gid - 1] - 1
+        assert np.array_equal(p.ghost[:, 1], c.x[orig]) and np.array_equal(p.ghost[:, 2], c.y[orig])
+        assert np.array_equal(p.ghost[:, 3], c.min_dist[orig])
+    back = pointio.local_table(pointio.assemble(parts))
+    assert np.array_equal(back[:, 1:], ref[:, 1:])
+    pointio.write_point_h5(str(tmp_path / "point.h5"), c, nparts=nparts)
+    h5 = pointio.assemble(pointio.read_point_h5_parts(str(tmp_path / "point.h5"), nparts))
+    assert np.array_equal(pointio.local_table(h5)[:, 1:], ref[:, 1:])
+    pointio.write_partgrid(str(tmp_path / "partGrid"), c, fmt=2, nparts=nparts)
+    assert (tmp_path / f"partGrid{nparts - 1:04d}").exists()
+    txt = pointio.assemble(pointio.read_partgrid_parts(str(tmp_path / "partGrid"), nparts, fmt=2))
+    assert np.array_equal(pointio.local_table(txt)[:, 1:], ref[:, 1:])
