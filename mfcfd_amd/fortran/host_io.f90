@@ -24,6 +24,8 @@ module host_io
         ! options with no counterpart in readcase: the device path's precision and
         ! whether to time the kernels' launch shapes before running
         integer :: precision = 0, autotune = 1
+        ! 1: options from input.nml (the CUDA build's run: output.dat, restart.dat)
+        integer :: nml = 0
         character(len=64) :: time = 'steady', tscheme = 'ssprk43', solution_accuracy = 'second'
         character(len=64) :: format_file = 'legacy', restart_solution = 'no', adjoint_mode = 'blackbox'
         character(len=64) :: limiter = 'venkat', run_option = 'normal', precision_name = 'fp64'
@@ -214,6 +216,64 @@ contains
         if (trim(o%time) /= 'steady') call fail('-time unsteady is not supported')
         if (trim(o%limiter) /= 'venkat') call fail('only -limiter venkat is supported')
         if (o%ad_mode == 1 .and. o%chkpts < 1) call fail('-adjoint_mode checkpoints needs -chkpts >= 1')
+    end subroutine
+
+    ! input.nml of the CUDA build (src_cuda/data_structure.F90:106-176): namelist
+    ! input_parameters with that build's defaults (:52-104).  blockx, blocky and blockz
+    ! (its thread-block shape) are read and have no effect: mf_autotune picks the
+    ! kernels' launch shapes.
+    subroutine read_nml(path, o)
+        character(len=*), intent(in) :: path
+        type(case_options), intent(inout) :: o
+        integer :: shapes, max_iters, blockx, blocky, blockz, nsave, interior_points_normal_flag, inner_iterations
+        real(8) :: cfl, vl_const, power, mach, aoa
+        character(len=20) :: restart_solution, solution_accuracy, format_file, tscheme
+        namelist /input_parameters/ shapes, cfl, max_iters, blockx, blocky, blockz, vl_const, power, &
+            restart_solution, solution_accuracy, format_file, nsave, interior_points_normal_flag, tscheme, &
+            mach, aoa, inner_iterations
+        integer :: u, ios
+        shapes = 1
+        cfl = 0.0d0
+        max_iters = 10000000
+        blockx = 32
+        blocky = 1
+        blockz = 1
+        vl_const = 150.0d0
+        power = 0.0d0
+        restart_solution = 'no'
+        solution_accuracy = 'second'
+        format_file = 'legacy'
+        nsave = 10000000
+        interior_points_normal_flag = 0
+        tscheme = 'first'
+        mach = 0.0d0
+        aoa = 0.0d0
+        inner_iterations = 0
+        open(newunit=u, file=path, status='old', action='read', iostat=ios)
+        if (ios /= 0) call fail('cannot open '//trim(path))
+        read(u, nml=input_parameters, iostat=ios)
+        if (ios /= 0) call fail('cannot read namelist input_parameters from '//trim(path))
+        close(u)
+        write(*, nml=input_parameters)
+        o%nml = 1
+        o%shapes = shapes
+        o%cfl = cfl
+        o%max_iters = max_iters
+        o%vl_const = vl_const
+        o%power = power
+        o%nsave = nsave
+        o%interior_points_normal_flag = interior_points_normal_flag
+        o%inner_iterations = inner_iterations
+        o%mach = mach
+        o%aoa = aoa
+        o%tscheme = tscheme
+        o%solution_accuracy = solution_accuracy
+        o%format_file = format_file
+        ! data_structure.F90:150-154: 'yes' restarts (from restart.dat)
+        o%restart_solution = merge('same', 'no  ', trim(restart_solution) == 'yes')
+        call finish_options(o)
+        ! data_structure.F90:168-172: quadtree files carry their own normals
+        if (o%format == 2) o%interior_points_normal_flag = 100000
     end subroutine
 
     subroutine print_options(o)
@@ -452,6 +512,119 @@ contains
         do i = 1, p%n
             write(u, '(3i8,6e30.20)') p%original_id(i), p%flag_1(i), p%flag_2(i), p%x(i), p%y(i), &
                 prim(1, i), prim(2, i), prim(3, i), prim(4, i)
+        end do
+        close(u)
+    end subroutine
+
+    ! output.dat of the CUDA build (src_cuda/post_processing.F90:7-36): "N it res_old",
+    ! then per point x, y, flag_1, qtdepth, prim, the Mach value as that routine
+    ! computes it (sqrt(|u|^2 / a), a the speed of sound: kept as written there), the
+    ! entropy measure (compute_entropy.F90:8-28) and the adaptation sensor
+    ! (adaptation_sensors.F90:7-101).  Both are evaluated from this prim at every
+    ! save; the CUDA build evaluates them once, after its last iteration.
+    subroutine write_output_dat(path, p, prim, it, res_old)
+        character(len=*), intent(in) :: path
+        type(point_data), intent(in) :: p
+        real(8), intent(in) :: prim(4, p%n)
+        integer, intent(in) :: it
+        real(8), intent(in) :: res_old
+        real(8), allocatable :: entropy(:), sensor(:)
+        real(8), parameter :: gamma = 1.4d0, pr_inf = 1.0d0 / 1.4d0
+        real(8) :: sos, vel_mag, mach_number
+        integer :: u, i
+        allocate(entropy(p%n), sensor(p%n))
+        call entropy_measure(p, prim, entropy)
+        call adapt_sensor(p, prim, sensor)
+        open(newunit=u, file=path)
+        write(u, *) p%n, it, res_old
+        do i = 1, p%n
+            sos = dsqrt(gamma * prim(4, i) / prim(1, i))
+            vel_mag = prim(2, i) * prim(2, i) + prim(3, i) * prim(3, i)
+            mach_number = dsqrt(vel_mag / sos)
+            write(u, '(2e30.20,2i4,7e30.20)') p%x(i), p%y(i), p%flag_1(i), p%qtdepth(i), &
+                prim(1, i), prim(2, i), prim(3, i), prim(4, i), mach_number, entropy(i), sensor(i)
+        end do
+        close(u)
+    end subroutine
+
+    ! compute_entropy.F90:8-28: |ln(p / rho^gamma) - ln(p_inf)| per point
+    subroutine entropy_measure(p, prim, entropy)
+        type(point_data), intent(in) :: p
+        real(8), intent(in) :: prim(4, p%n)
+        real(8), intent(out) :: entropy(p%n)
+        real(8), parameter :: gamma = 1.4d0, pr_inf = 1.0d0 / 1.4d0
+        real(8) :: temp1, temp2
+        integer :: k
+        temp2 = dlog(pr_inf)
+        do k = 1, p%n
+            temp1 = prim(1, k)**gamma
+            temp1 = prim(4, k) / temp1
+            temp1 = dlog(temp1)
+            entropy(k) = dabs(temp1 - temp2)
+        end do
+    end subroutine
+
+    ! adaptation_sensors.F90:7-101: per point the largest D2 distance to a neighbour's
+    ! state, normalised by the largest over all points
+    subroutine adapt_sensor(p, prim, sensor)
+        type(point_data), intent(in) :: p
+        real(8), intent(in) :: prim(4, p%n)
+        real(8), intent(out) :: sensor(p%n)
+        real(8) :: u1_i, u2_i, pr_i, rho_i, u1_j, u2_j, pr_j, rho_j
+        real(8) :: maxi, u_sqr, temp1, temp2, temp3, d2, max_sensor
+        integer :: i, j, r
+        do i = 1, p%n
+            u1_i = prim(2, i)
+            u2_i = prim(3, i)
+            rho_i = prim(1, i)
+            pr_i = prim(4, i)
+            maxi = 0.0
+            do r = 1, p%nbhs(i)
+                j = p%conn(i, r)
+                u1_j = prim(2, j)
+                u2_j = prim(3, j)
+                rho_j = prim(1, j)
+                pr_j = prim(4, j)
+                temp1 = (pr_i * pr_i * rho_j) / (pr_j * pr_j * rho_i)
+                temp1 = dlog(temp1)
+                temp1 = (rho_j - rho_i) * temp1
+                u_sqr = (u1_j - u1_i)**2 + (u2_j - u2_i)**2
+                temp2 = (rho_j / rho_i) + (pr_i * rho_j) / (pr_j * rho_i)
+                temp2 = temp2 * u_sqr
+                temp2 = (0.5 * rho_i * rho_i / pr_i) * temp2
+                temp3 = rho_i * (((pr_i * rho_j) / (pr_j * rho_i)) - 1)
+                temp3 = temp3 + rho_j * (((pr_j * rho_i) / (pr_i * rho_j)) - 1)
+                temp3 = 2 * temp3
+                d2 = temp1 + temp2 + temp3
+                if (d2 > maxi) maxi = d2
+            end do
+            sensor(i) = maxi
+        end do
+        max_sensor = 0.0d0
+        do i = 1, p%n
+            if (sensor(i) >= max_sensor) max_sensor = sensor(i)
+        end do
+        do i = 1, p%n
+            sensor(i) = sensor(i) / max_sensor
+        end do
+    end subroutine
+
+    ! restart.dat of the CUDA build (initial_conditions.F90:26-37): output.dat's layout,
+    ! prim in columns 5-8
+    subroutine read_restart_dat(path, n, prim, itr, res_old)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: n
+        real(8), intent(out) :: prim(4, n)
+        integer, intent(out) :: itr
+        real(8), intent(out) :: res_old
+        integer :: u, ios, i, nfile, f1, qt
+        real(8) :: dx, dy, m, e, sn
+        open(newunit=u, file=path, status='old', action='read', iostat=ios)
+        if (ios /= 0) call fail('cannot open '//trim(path))
+        read(u, *) nfile, itr, res_old
+        if (nfile /= n) call fail(trim(path)//': point count differs from the point file')
+        do i = 1, n
+            read(u, *) dx, dy, f1, qt, prim(1, i), prim(2, i), prim(3, i), prim(4, i), m, e, sn
         end do
         close(u)
     end subroutine

@@ -11,6 +11,11 @@
 ! Usage: run from a case directory holding case.in and point/point.h5 (or
 ! point/partGrid).  "meshfree_host --check" reads and validates the inputs, prints
 ! their summary and exits without touching the GPU.
+!
+! Without a case.in, an input.nml selects the CUDA build's run (src_cuda
+! meshfree_solver.F90:60-110): its namelist options, restart from restart.dat, and
+! output.dat (primitive state, Mach, entropy, adaptation sensor) at every nsave
+! iterations and at the end instead of solution/sol.dat.
 program meshfree_host
     use iso_c_binding
     use mfcfd_api
@@ -48,7 +53,14 @@ program meshfree_host
     write(*, *)
     write(*, *) '%%%%%%%%%%%-Reading the case file-%%%%%%%%%'
     write(*, *)
-    call read_case('case.in', opt)
+    if (file_exists('case.in')) then
+        call read_case('case.in', opt)
+    else if (file_exists('input.nml')) then
+        call read_nml('input.nml', opt)
+    else
+        write(*, '(a)') ' *** neither case.in nor input.nml in the working directory'
+        error stop 2
+    end if
     call print_options(opt)
     write(*, *)
     write(*, *) '%%%%%%%%%%%%-Reading point data-%%%%%%%%%%%'
@@ -115,7 +127,11 @@ program meshfree_host
     if (opt%restart == 1) then
         write(*, *) '%%%%%%%%%%%%-Using restart file-%%%%%%%%%%%'
         write(*, *)
-        call read_restart('restart/sol.dat', pts%n, prim, itr, res_old)
+        if (opt%nml == 1) then
+            call read_restart_dat('restart.dat', pts%n, prim, itr, res_old)
+        else
+            call read_restart('restart/sol.dat', pts%n, prim, itr, res_old)
+        end if
         prm%restart = 1
         prm%res_old = res_old
     end if
@@ -165,7 +181,11 @@ program meshfree_host
                 write(*, *) '%%%%%%%%%%%%%-Saving solution-%%%%%%%%%%%%%'
                 write(*, *)
                 call check(mf_get_prim(ctx, prim), 'mf_get_prim')
-                call write_solution('solution/sol.dat', pts, prim, it - 2, res_old)
+                if (opt%nml == 1) then
+                    call write_output_dat('output.dat', pts, prim, it - 1, res_old)
+                else
+                    call write_solution('solution/sol.dat', pts, prim, it - 2, res_old)
+                end if
             end if
         end do
         close(301)
@@ -175,9 +195,14 @@ program meshfree_host
 
     call check(mf_compute_cl_cd_cm(ctx, cl, cd, cm, cp), 'mf_compute_cl_cd_cm')
     call write_cp('cp/cp-file', pts, cp)
-    ! header "it - 1" of the last iteration run, as fpi_solver.F90:73-82 writes it
     call check(mf_get_prim(ctx, prim), 'mf_get_prim')
-    call write_solution('solution/sol.dat', pts, prim, it - 2, res_old)
+    if (opt%nml == 1) then
+        ! q_lskum.cuf:150 and meshfree_solver.F90:107: the last iteration run
+        call write_output_dat('output.dat', pts, prim, it - 1, res_old)
+    else
+        ! header "it - 1" of the last iteration run, as fpi_solver.F90:73-82 writes it
+        call write_solution('solution/sol.dat', pts, prim, it - 2, res_old)
+    end if
     write(*, *) 'Vector function is ', cd
     write(*, '(a,3es24.16)') ' Cl Cd Cm (shape 1):', cl(1), cd(1), cm(1)
     call check(mf_destroy(ctx), 'mf_destroy')

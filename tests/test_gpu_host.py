@@ -103,3 +103,61 @@ def test_host_adjoint_writes_sensitivity(small_cloud, tmp_path):
         J, dx, dy, _ = s.adjoint(12, snaps=4)
     g = np.concatenate([dx, dy])
     assert np.abs(np.concatenate([sens[:, 1], sens[:, 2]]) - g).max() <= 1e-10 * np.abs(g).max()
+
+
+def _d2_sensor(cloud, prim):
+    """adaptation_sensors.F90:7-101 restated with numpy: per point the largest D2
+    distance to a neighbour's state, over the largest of all points."""
+    rho, u1, u2, pr = prim.T
+    s = np.zeros(cloud.n)
+    for i in range(cloud.n):
+        j = cloud.conn[i, :cloud.nbhs[i]]
+        t1 = (rho[j] - rho[i]) * np.log((pr[i] * pr[i] * rho[j]) / (pr[j] * pr[j] * rho[i]))
+        usq = (u1[j] - u1[i]) ** 2 + (u2[j] - u2[i]) ** 2
+        t2 = (0.5 * rho[i] * rho[i] / pr[i]) * ((rho[j] / rho[i]) + (pr[i] * rho[j]) / (pr[j] * rho[i])) * usq
+        t3 = 2 * (rho[i] * ((pr[i] * rho[j]) / (pr[j] * rho[i]) - 1) + rho[j] * ((pr[j] * rho[i]) / (pr[i] * rho[j]) - 1))
+        s[i] = max(0.0, float((t1 + t2 + t3).max()))
+    return s / s.max()
+
+
+def test_host_input_nml_output_dat(small_cloud, tmp_path):
+    """The CUDA build's run: input.nml in, output.dat out (post_processing.F90:7-36):
+    x, y, flag_1, qtdepth, prim, the Mach value as that routine computes it, the
+    entropy measure and the normalised D2 sensor; then a restart from restart.dat."""
+    from mfcfd_amd.solver import LskumSolver
+    (tmp_path / "point").mkdir()
+    pointio.write_point_h5(str(tmp_path / "point" / "point.h5"), small_cloud)
+    nml = ("&input_parameters\n cfl = 0.2\n max_iters = 20\n nsave = 10\n mach = 0.63\n aoa = 2\n"
+           " vl_const = 5\n tscheme = 'ssprk43'\n inner_iterations = 2\n blockx = 64\n/\n")
+    (tmp_path / "input.nml").write_text(nml)
+    _run(tmp_path)
+    with open(tmp_path / "output.dat") as fh:
+        head = fh.readline().split()
+        rows = np.loadtxt(fh)
+    with LskumSolver(small_cloud, **CASE) as s:
+        s.q_lskum(20)
+        prim = s.prim
+        s.q_lskum(10, first_it=21)
+        prim30 = s.prim
+    assert (int(head[0]), int(head[1])) == (small_cloud.n, 20)
+    assert np.array_equal(rows[:, 0], small_cloud.x) and np.array_equal(rows[:, 1], small_cloud.y)
+    assert np.array_equal(rows[:, 2], small_cloud.flag_1)
+    # autotuned launch shapes reorder sums only
+    assert rel_err(rows[:, 4:8], prim) <= 1e-11
+    rho, u1, u2, pr = rows[:, 4:8].T
+    mach_as_written = np.sqrt((u1 * u1 + u2 * u2) / np.sqrt(1.4 * pr / rho))
+    assert rel_err(rows[:, 8], mach_as_written) <= 1e-14
+    entropy = np.abs(np.log(pr / rho ** 1.4) - np.log(1.0 / 1.4))
+    assert np.abs(rows[:, 9] - entropy).max() <= 1e-13
+    sensor = _d2_sensor(small_cloud, rows[:, 4:8])
+    assert rows[:, 10].max() == 1.0 and np.abs(rows[:, 10] - sensor).max() <= 1e-12
+    # restart.dat (initial_conditions.F90:26-37): 10 more iterations continue the run
+    shutil.copy(tmp_path / "output.dat", tmp_path / "restart.dat")
+    (tmp_path / "input.nml").write_text(nml.replace("max_iters = 20", "max_iters = 10")
+                                        .replace("/\n", " restart_solution = 'yes'\n/\n"))
+    _run(tmp_path)
+    with open(tmp_path / "output.dat") as fh:
+        head = fh.readline().split()
+        rows = np.loadtxt(fh)
+    assert int(head[1]) == 30
+    assert rel_err(rows[:, 4:8], prim30) <= 1e-10

@@ -110,3 +110,36 @@ def test_partitioned_point_files_round_trip(small_cloud, tmp_path, nparts):
     assert (tmp_path / f"partGrid{nparts - 1:04d}").exists()
     txt = pointio.assemble(pointio.read_partgrid_parts(str(tmp_path / "partGrid"), nparts, fmt=2))
     assert np.array_equal(pointio.local_table(txt)[:, 1:], ref[:, 1:])
+
+
+INPUT_NML = """&input_parameters
+	cfl = 0.2
+	max_iters = 7
+	mach = 0.63
+	vl_const = 650
+	format_file = 'legacy'
+	tscheme = 'ssprk43'
+	blockx = 32
+	inner_iterations = 1
+	aoa = 2
+/
+"""
+
+
+def test_fortran_host_reads_input_nml(small_cloud, tmp_path):
+    """Without a case.in the host takes the CUDA build's input.nml
+    (examples/cuda/input.nml.example's keys) and echoes the namelist."""
+    if not os.path.exists(HOST):
+        pytest.skip("meshfree_host not built")
+    (tmp_path / "point").mkdir()
+    pointio.write_point_h5(str(tmp_path / "point" / "point.h5"), small_cloud)
+    (tmp_path / "input.nml").write_text(INPUT_NML)
+    out = subprocess.run([HOST, "--check"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    echo = out.stdout.upper().replace(" ", "")
+    assert "MAX_ITERS=7" in echo and "BLOCKX=32" in echo and "VL_CONST=650" in echo
+    assert re.search(r"tscheme\s*:\s*ssprk43", out.stdout)
+    assert f"check: n={small_cloud.n}" in out.stdout
+    (tmp_path / "input.nml").write_text(INPUT_NML.replace("tscheme = 'ssprk43'", "tscheme = 'bogus'"))
+    out = subprocess.run([HOST, "--check"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "tscheme" in out.stdout
