@@ -125,12 +125,14 @@ def run_adjoint(iters: int, snaps: int):
     out = {"workload": f"NACA0012 320x125 ({cloud.n} points), 2nd order, 2 inner, M 0.8, AoA 1.25, "
                        f"J = Cd, {iters} iterations", "primal_s": primal_s,
            "tuning": {k: tune[k] for k in ("lpp_lsq", "lpp_inner", "lpp_flux")}}
-    # the configured binomial schedule, then every iterate kept (iters x 32 B per
-    # point of HBM: no recomputation)
-    for key, ns in (("binomial", snaps), ("store_all", iters)):
+    # the configured binomial schedule; every iterate kept (iters x 32 B per point of
+    # HBM: the reverse re-runs each iteration once, taped); every iteration taped in
+    # the primal sweep (about 1.3 kB per point per iteration: nothing re-run)
+    from mfcfd_amd._lib import MF_ADJ_TAPE_ALL
+    for key, ns in (("binomial", snaps), ("store_all", iters), ("tape_all", MF_ADJ_TAPE_ALL)):
         with LskumSolver(cloud, **p) as s:
             s.tuning = out["tuning"]
-            print(f"[bench] adjoint: {iters} iterations, {ns} snapshots", file=sys.stderr, flush=True)
+            print(f"[bench] adjoint: {iters} iterations, {key} ({ns})", file=sys.stderr, flush=True)
             t0 = time.perf_counter()
             J, dx, dy, st = s.adjoint(iters, snaps=ns)
             adj_s = time.perf_counter() - t0

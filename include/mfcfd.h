@@ -200,7 +200,11 @@ int mf_layout(mf_ctx *ctx, mf_layout_info *info);
  * returns J; mf_adjoint_backward runs the reverse sweep and returns dJ/dx, dJ/dy per
  * point in original order (the reference's pointb%x, pointb%y, written to
  * sensitivity/sensitivity.dat by src_mpi_adjoint/post_processing.F90:41-66).  The
- * context is left holding the final primal state.  FP64 contexts with power = 0. */
+ * context is left holding the final primal state.  FP64 contexts with power = 0.
+ * snaps = MF_ADJ_TAPE_ALL tapes every iteration's stage states during the primal
+ * sweep instead (about 1.3 kB per point per iteration at 2 inner iterations; refused
+ * when that exceeds the free device memory), so the reverse sweep re-runs nothing. */
+#define MF_ADJ_TAPE_ALL (-1)
 typedef struct mf_adjoint_stats {
   int64_t forward_steps;  /* iterations advanced: the primal sweep plus recomputation */
   int64_t taped_steps;    /* iterations re-run with their stage intermediates taped  */

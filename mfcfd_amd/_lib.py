@@ -24,6 +24,8 @@ MF_FIELD_PRIM_OLD = 6
 MF_FIELD_NX = 7
 MF_FIELD_NY = 8
 MF_NUM_KERNEL_KINDS = 5
+# mf_adjoint*: snaps value that tapes every iteration in the primal sweep
+MF_ADJ_TAPE_ALL = -1
 # mf_tuning: staged walks (MF_LPP_STAGED | lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux)
 MF_LPP_STAGED = 16
 KERNEL_KINDS = ("lsq", "inner", "flux_update", "finalize", "forces")

@@ -1497,6 +1497,7 @@ int mf_layout(mf_ctx* c, mf_layout_info* info) {
 
 static void adj_release(mf_ctx* c) {
   if (!c->adj) return;
+  if (c->adj->tape_buf) (void)hipFree(c->adj->tape_buf);
   if (c->adj->e0) (void)hipEventDestroy(c->adj->e0);
   if (c->adj->e1) (void)hipEventDestroy(c->adj->e1);
   delete c->adj;

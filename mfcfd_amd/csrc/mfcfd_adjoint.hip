@@ -839,6 +839,23 @@ struct AdjState {
   int lpp_frev = 1, lpp_lrev = 1, lpp_irev = 1, lpp_gath = 1;
   float primal_ms = 0.f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // Every iteration taped during the primal sweep (snaps = MF_ADJ_TAPE_ALL): the
+  // reverse then re-runs nothing.  tapes[t] points into tape_buf; bind_tape() points
+  // the one-iteration tape above (tprim, tq, tdq, tqm, tpold, tdelta) at iteration t,
+  // whose input prim is iteration t-1's output (base.prim0 for t = 0).
+  struct Tape {
+    std::vector<V4d*> prim, q, dq, qm;  // prim: stage outputs 1..S; q: stages 0..S-1
+    V4d* pold = nullptr;
+    double* delta = nullptr;
+  };
+  bool tape_all = false;
+  void* tape_buf = nullptr;
+  size_t tape_cap = 0;
+  std::vector<Tape> tapes;
+  // the one-iteration tape's own buffers (restored when a checkpointed schedule runs)
+  std::vector<V4d*> base_prim, base_q, base_dq, base_qm;
+  V4d* base_pold = nullptr;
+  double* base_delta = nullptr;
 };
 
 static int adj_zero(mf_ctx* c, void* p, int64_t n_doubles) {
@@ -1013,9 +1030,19 @@ static int binomial_mid(int t0, int t1, int free_slots) {
 static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   const int n = c->n;
   const int S = c->p.rks, I = c->p.inner_iterations;
-  if (A.ready && A.S == S && A.I == I && (int)A.snap.size() >= snaps) return MF_OK;
-  if (A.ready) return set_err(MF_ERR_STATE, "adjoint buffers already sized for a different schedule");
   int rc;
+  if (A.ready && (A.S != S || A.I != I))
+    return set_err(MF_ERR_STATE, "adjoint buffers already sized for a different scheme");
+  if (A.ready) {
+    // a schedule with more snapshots than any before on this context: add slots
+    while ((int)A.snap.size() < snaps) {
+      V4d* p = nullptr;
+      if ((rc = dev_alloc(c, &p, n))) return rc;
+      A.snap.push_back(p);
+      A.snap_t.push_back(-1);
+    }
+    return MF_OK;
+  }
   A.S = S;
   A.I = I;
   A.tprim.assign(S + 1, nullptr);
@@ -1026,8 +1053,14 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   for (auto& p : A.tq) if ((rc = dev_alloc(c, &p, n))) return rc;
   for (auto& p : A.tdq) if ((rc = dev_alloc(c, &p, 2 * (size_t)n))) return rc;
   for (auto& p : A.tqm) if ((rc = dev_alloc(c, &p, 2 * (size_t)n))) return rc;
-  if ((rc = dev_alloc(c, &A.tpold, n)) || (rc = dev_alloc(c, &A.tdelta, n)) ||
-      (rc = dev_alloc(c, &A.tcspeed, n)) || (rc = dev_alloc(c, &A.tpartial, c->npartial)) ||
+  if ((rc = dev_alloc(c, &A.tpold, n)) || (rc = dev_alloc(c, &A.tdelta, n))) return rc;
+  A.base_prim = A.tprim;
+  A.base_q = A.tq;
+  A.base_dq = A.tdq;
+  A.base_qm = A.tqm;
+  A.base_pold = A.tpold;
+  A.base_delta = A.tdelta;
+  if ((rc = dev_alloc(c, &A.tcspeed, n)) || (rc = dev_alloc(c, &A.tpartial, c->npartial)) ||
       (rc = dev_alloc(c, &A.tctl, 1)) || (rc = dev_alloc(c, &A.Pb, n)) ||
       (rc = dev_alloc(c, &A.Pstage, n)) || (rc = dev_alloc(c, &A.poldb, n)) ||
       (rc = dev_alloc(c, &A.frb, n)) || (rc = dev_alloc(c, &A.qb, n)) ||
@@ -1086,6 +1119,77 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   HIPCHK(hipEventCreate(&A.e0));
   HIPCHK(hipEventCreate(&A.e1));
   A.ready = true;
+  return MF_OK;
+}
+
+// The one-iteration tape back on its own buffers (checkpointed schedules)
+static void bind_base(AdjState& A) {
+  A.tprim = A.base_prim;
+  A.tq = A.base_q;
+  A.tdq = A.base_dq;
+  A.tqm = A.base_qm;
+  A.tpold = A.base_pold;
+  A.tdelta = A.base_delta;
+}
+
+// The one-iteration tape pointed at iteration t of the all-iterations tape
+static void bind_tape(AdjState& A, int t) {
+  const AdjState::Tape& T = A.tapes[t];
+  A.tprim[0] = t == 0 ? A.base_prim[0] : A.tapes[t - 1].prim[A.S - 1];
+  for (int s = 0; s < A.S; ++s) {
+    A.tprim[s + 1] = T.prim[s];
+    A.tq[s] = T.q[s];
+    A.tqm[s] = T.qm[s];
+  }
+  A.tq[A.S] = A.base_q[A.S];  // the last stage's q is not read by the reverse
+  for (size_t k = 0; k < T.dq.size(); ++k) A.tdq[k] = T.dq[k];
+  A.tpold = T.pold;
+  A.tdelta = T.delta;
+}
+
+// Room for `iters` taped iterations (one allocation, reused while large enough);
+// refuses a tape larger than the device's free memory less 1 GiB
+static int tape_alloc(mf_ctx* c, AdjState& A, int iters) {
+  const size_t n = (size_t)c->n, S = (size_t)A.S, I = (size_t)A.I;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t v4 = up(n * sizeof(V4d)), v8 = up(2 * n * sizeof(V4d)), dl = up(n * sizeof(double));
+  const size_t per = S * 2 * v4 + S * (I + 1) * v8 + S * v8 + v4 + dl;
+  const size_t need = per * (size_t)iters;
+  if (need > A.tape_cap) {
+    if (A.tape_buf) {
+      HIPCHK(hipFree(A.tape_buf));
+      A.tape_buf = nullptr;
+      A.tape_cap = 0;
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    if (need + ((size_t)1 << 30) > free_b)
+      return set_err(MF_ERR_INVALID, "taping all %d iterations needs %.1f GB; %.1f GB of device memory is free",
+                     iters, need / 1e9, free_b / 1e9);
+    HIPCHK(hipMalloc(&A.tape_buf, need));
+    A.tape_cap = need;
+  }
+  A.tapes.assign(iters, AdjState::Tape{});
+  char* p = (char*)A.tape_buf;
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += b;
+    return r;
+  };
+  for (auto& T : A.tapes) {
+    T.prim.resize(S);
+    T.q.resize(S);
+    T.qm.resize(S);
+    T.dq.resize(S * (I + 1));
+    for (size_t s = 0; s < S; ++s) {
+      T.prim[s] = (V4d*)take(v4);
+      T.q[s] = (V4d*)take(v4);
+      T.qm[s] = (V4d*)take(v8);
+    }
+    for (auto& d : T.dq) d = (V4d*)take(v8);
+    T.pold = (V4d*)take(v4);
+    T.delta = (double*)take(dl);
+  }
   return MF_OK;
 }
 
@@ -1159,12 +1263,15 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   if (!c) return set_err(MF_ERR_INVALID, "null context");
   if (c->ssz != 8) return set_err(MF_ERR_STATE, "the adjoint needs an MF_FP64 context");
   if (!c->power0) return set_err(MF_ERR_STATE, "the adjoint supports power = 0 only");
-  if (iters < 1 || snaps < 1) return set_err(MF_ERR_INVALID, "iters and snaps must be >= 1");
+  if (iters < 1 || (snaps < 1 && snaps != MF_ADJ_TAPE_ALL))
+    return set_err(MF_ERR_INVALID, "iters must be >= 1 and snaps >= 1 or MF_ADJ_TAPE_ALL");
   if (c->nshape == 0) return set_err(MF_ERR_STATE, "J = Cd needs shape points");
   if (!c->adj) c->adj = new AdjState();
   AdjState& A = *c->adj;
   int rc;
-  if ((rc = adj_alloc(c, A, snaps))) return rc;
+  const bool tape_all = snaps == MF_ADJ_TAPE_ALL;
+  if ((rc = adj_alloc(c, A, tape_all ? 1 : snaps))) return rc;
+  bind_base(A);
   const bool pinned = c->lpp_rev[0] && c->lpp_rev[1] && c->lpp_rev[2] && c->lpp_rev[3];
   if (!A.tuned && !pinned && (rc = adj_tune(c, A))) return rc;
   // shapes pinned with mf_set_tuning override the timed ones
@@ -1176,7 +1283,30 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   A.iters = iters;
   A.fwd_steps = A.taped = 0;
   A.peak_snaps = 1;
+  A.tape_all = tape_all;
   for (auto& t : A.snap_t) t = -1;
+  if (tape_all) {
+    // the primal sweep as taped iterations, each kept for the reverse
+    if ((rc = tape_alloc(c, A, iters))) return rc;
+    HIPCHK(hipEventRecord(A.e0, c->stream));
+    if ((rc = adj_copy(c, A.base_prim[0], c->prim[c->cur]))) return rc;
+    for (int t = 0; t < iters; ++t) {
+      bind_tape(A, t);
+      if ((rc = adj_record(c, A))) return rc;
+    }
+    if ((rc = adj_copy(c, c->prim[c->cur], A.tprim[A.S])) || (rc = q_variables(c))) return rc;
+    HIPCHK(hipEventRecord(A.e1, c->stream));
+    HIPCHK(hipEventSynchronize(A.e1));
+    HIPCHK(hipEventElapsedTime(&A.primal_ms, A.e0, A.e1));
+    A.fwd_steps = iters;
+    A.peak_snaps = 0;
+    std::vector<double> cl(c->nsid), cd(c->nsid), cm(c->nsid);
+    if ((rc = mf_compute_cl_cd_cm(c, cl.data(), cd.data(), cm.data(), nullptr))) return rc;
+    A.J = 0.0;
+    for (double v : cd) A.J += v;
+    if (J) *J = A.J;
+    return MF_OK;
+  }
   // snapshot of the initial state, then the primal sweep, storing the checkpoints the
   // reverse recursion's right spine will ask for
   HIPCHK(hipEventRecord(A.e0, c->stream));
@@ -1242,7 +1372,27 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
                      (const V4d*)c->prim[c->cur], c->sh_m, c->sh_id, A.sh_l, A.sh_r, c->sgeo, c->nshape,
                      (const double*)c->shape_par, c->g.xy, A.Pb, A.xyb, A.nrmb);
   HIPCHK(hipGetLastError());
-  if ((rc = adj_rev_range(c, A, 0, A.iters, 0, (int)A.snap.size() - 1))) return rc;
+  if (A.tape_all) {
+    for (int t = A.iters - 1; t >= 0; --t) {
+      bind_tape(A, t);
+      if ((rc = adj_reverse_iteration(c, A))) return rc;
+      A.taped += 1;
+      if (A.progress_every > 0 && A.taped % A.progress_every == 0) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::fprintf(stderr, "[mfcfd] adjoint: %lld of %d iterations reversed\n", (long long)A.taped, A.iters);
+        std::fflush(stderr);
+      }
+    }
+    bind_base(A);
+    // the tape is as large as the run: give it back now
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(A.tape_buf));
+    A.tape_buf = nullptr;
+    A.tape_cap = 0;
+    A.tapes.clear();
+  } else if ((rc = adj_rev_range(c, A, 0, A.iters, 0, (int)A.snap.size() - 1))) {
+    return rc;
+  }
   // normals depend on x, y of the boundary neighbours (point_normals_b.F90)
   if (A.nbd > 0)
     hipLaunchKernelGGL(k_normals_rev, dim3((A.nbd + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, A.nbd,

@@ -11,6 +11,8 @@ module mfcfd_api
     integer(c_int32_t), parameter :: MF_FP64 = 0, MF_MIXED = 1
     ! mf_tuning: staged walks (MF_LPP_STAGED + lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux)
     integer(c_int32_t), parameter :: MF_LPP_STAGED = 16
+    ! mf_adjoint*: snaps value that tapes every iteration in the primal sweep
+    integer(c_int32_t), parameter :: MF_ADJ_TAPE_ALL = -1
     integer(c_int32_t), parameter :: MF_FIELD_Q = 1, MF_FIELD_DQ = 2, MF_FIELD_QM = 3, &
         MF_FIELD_FLUX_RES = 4, MF_FIELD_DELTA = 5, MF_FIELD_PRIM_OLD = 6, MF_FIELD_NX = 7, &
         MF_FIELD_NY = 8

@@ -217,7 +217,8 @@ class LskumSolver:
     def adjoint(self, iters: int, snaps: int = 32, first_it: int = 1):
         """J = sum of Cd after `iters` iterations from the current state, and dJ/dx,
         dJ/dy per point (the reference's pointb%x, pointb%y).  Returns
-        (J, dJdx, dJdy, stats)."""
+        (J, dJdx, dJdy, stats).  snaps: prim snapshots of the binomial schedule, or
+        L.MF_ADJ_TAPE_ALL (-1) to tape every iteration in the primal sweep."""
         J = C.c_double()
         dx, dy = np.empty(self.n), np.empty(self.n)
         st = L.MfAdjointStats()

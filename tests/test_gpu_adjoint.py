@@ -101,18 +101,32 @@ def test_adjoint_fd_wall_points(small_cloud):
 
 
 def test_adjoint_schedule_invariance(small_cloud):
-    """1 snapshot (recompute from the start), 3, and one per iteration agree."""
-    out = []
-    for snaps in (1, 3, 25):
+    """1 snapshot (recompute from the start), 3, one per iteration, and every iteration
+    taped in the primal sweep (MF_ADJ_TAPE_ALL) agree; the taped sweep leaves the same
+    final state and re-runs nothing."""
+    from mfcfd_amd._lib import MF_ADJ_TAPE_ALL
+    out, final = [], []
+    for snaps in (1, 3, 25, MF_ADJ_TAPE_ALL):
         with _gpu(small_cloud) as g:
             J, dx, dy, st = g.adjoint(12, snaps=snaps)
             out.append((J, dx, dy, st))
-    ref = np.concatenate([out[-1][1], out[-1][2]])
-    for J, dx, dy, st in out[:-1]:
-        assert J == out[-1][0]
+            final.append(g.prim)
+    ref = np.concatenate([out[2][1], out[2][2]])
+    for k, (J, dx, dy, st) in enumerate(out):
+        assert J == out[2][0]
         g = np.concatenate([dx, dy])
         assert np.abs(g - ref).max() <= 1e-10 * np.abs(ref).max()
+        assert np.array_equal(final[k], final[2])
     assert out[0][3]["forward_steps"] > out[1][3]["forward_steps"] >= out[2][3]["forward_steps"]
+    assert out[3][3]["forward_steps"] == 12 and out[3][3]["taped_steps"] == 12
+    # a checkpointed run after a taped one on the same context, from the same state
+    with _gpu(small_cloud) as g:
+        start = g.prim
+        J1, _, _, _ = g.adjoint(12, snaps=MF_ADJ_TAPE_ALL)
+        g.prim = start
+        J2, dx2, dy2, _ = g.adjoint(12, snaps=3)
+        assert J1 == J2 == out[2][0]
+        assert np.abs(np.concatenate([dx2, dy2]) - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("lpp", [2, 4, 8])
