@@ -1,8 +1,10 @@
 // Device code of the LSKUM fixed-point iteration for gfx950 (CDNA4, wave64).
 //
-// One thread per point; every kernel walks the point's neighbour slots in the
-// reference's conn order so each least-squares sum accumulates in the same order as
-// the reference (q_variables.F90:71-102, wall_fluxes.F90:55-99 and friends).
+// LPP lanes per point (1 by default); a lane walks the point's neighbour slots lane,
+// lane + LPP, ... in the reference's conn order, so with one lane each least-squares
+// sum accumulates in the reference's order (q_variables.F90:71-102,
+// wall_fluxes.F90:55-99 and friends) and with more the lanes' partial sums are combined
+// by shuffles.  The staged walks read the same slots from a block's halo in LDS.
 //
 // Device layout (points renumbered on the host: wall | outer | interior, each class
 // in Hilbert-curve order):
@@ -15,6 +17,8 @@
 //                                          2x2 solve coefficients of the flux derivative
 //   prim, q        V4<S>  [N]
 //   dq, qm         V4<S>  [2N]             dq: [2i] d/dx, [2i+1] d/dy; qm: [2i] max, [2i+1] min
+//   halos (Halo)   per block of 64 (LSQ, inner) or 32 (flux) points: the halo's point
+//                  ids and uint16 [W][N] slot words (halo position | half-stencil bits)
 //
 // S is double for the FP64 path and float for the mixed path; least-squares sums and
 // solves, flux-derivative accumulations, residual norms and force sums are double in

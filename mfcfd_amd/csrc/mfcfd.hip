@@ -3,7 +3,9 @@
 // Host side: builds the device layout from the reference's point arrays (normals,
 // stencil split, Hilbert renumbering, point-fixed least-squares geometry), owns the
 // device state of one cloud, and replays one fpi_solver iteration
-// (fpi_solver.F90:13-83) as a captured HIP graph.  Device side: lskum_device.h.
+// (fpi_solver.F90:13-83) as a captured HIP graph.  Device side: lskum_device.h (point
+// and edge routines), lskum_kernels.h (stencil kernels and launchers, FP64 instances
+// here, the mixed path's in mfcfd_f32.hip), mfcfd_adjoint.hip (the reverse sweep).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
