@@ -518,6 +518,13 @@ struct GlobalNbrs {
   __device__ V4<S> qmax(int k) const { return st.qm[2 * k]; }
   __device__ V4<S> qmin(int k) const { return st.qm[2 * k + 1]; }
   __device__ S eps(int k) const { return st.eps[k]; }
+  // the point's own record index; slot s's neighbour index and half-stencil bits
+  __device__ int own() const { return i; }
+  __device__ void slot(int s, int& k, unsigned& m) const {
+    const uint32_t pk = g.conn[(size_t)s * g.n + i];
+    k = (int)(pk >> 4);
+    m = pk & 15u;
+  }
   template <class F>
   __device__ void walk(F&& f) const {
     const uint32_t* cp = g.conn + i;
@@ -553,6 +560,12 @@ struct StagedNbrs {
   __device__ V4<S> qmax(int k) const { return r.qmax[k]; }
   __device__ V4<S> qmin(int k) const { return r.qmin[k]; }
   __device__ S eps(int k) const { return r.eps[k]; }
+  __device__ int own() const { return li; }  // a block's own points lead its halo
+  __device__ void slot(int s, int& k, unsigned& m) const {
+    const unsigned e = s_cl[s * P + li];
+    k = (int)(e & 0xFFFu);
+    m = e >> 12;
+  }
   template <class F>
   __device__ void walk(F&& f) const {
     for (int s = lane; s < nb; s += LPP) {
@@ -561,6 +574,42 @@ struct StagedNbrs {
     }
   }
 };
+
+// A STAGE_PF block's flux halo into LDS (records of State st and the coordinates, then
+// the block's slot words), ended by a barrier; every thread of the block takes part.
+// The LDS layout (HaloRecs, then the slot words) is stage_lds_flux's.
+template <class S, int LPP>
+__device__ inline void stage_flux_halo(const Geom& g, const State<S>& st, int b, unsigned char* smem,
+                                       HaloRecs<S>& r, uint16_t*& s_cl) {
+  const Halo& h = g.hl[1];
+  const int H = h.hmax;
+  r.q = (V4<S>*)smem;
+  r.dqx = r.q + H;
+  r.dqy = r.dqx + H;
+  r.qmax = r.dqy + H;
+  r.qmin = r.qmax + H;
+  r.xy = (D2*)(r.qmin + H);
+  r.eps = (S*)(r.xy + H);
+  s_cl = (uint16_t*)(r.eps + H);
+  const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
+  for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
+    const int k = h.ids[h0 + j];
+    r.q[j] = st.q[k];
+    r.dqx[j] = st.dq[2 * k];
+    r.dqy[j] = st.dq[2 * k + 1];
+    r.qmax[j] = st.qm[2 * k];
+    r.qmin[j] = st.qm[2 * k + 1];
+    r.xy[j] = g.xy[k];
+    r.eps[j] = st.eps[k];
+  }
+  constexpr int V = STAGE_PF * 2 / 16;
+  const uint4* src = (const uint4*)(h.loc + (size_t)b * STAGE_PF);
+  for (int j = threadIdx.x; j < g.width * V; j += STAGE_PF * LPP) {
+    const int sl = j / V;
+    ((uint4*)s_cl)[j] = src[(size_t)sl * (h.stride / 8) + (j - sl * V)];
+  }
+  __syncthreads();
+}
 
 template <int CLS, class S, bool POWER0, int LPP, class Nbrs>
 __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, double delta, double power,

@@ -372,35 +372,9 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     // the block's halo in LDS, then the points' stencils from it (every thread reaches
     // the barrier and the group sums)
     extern __shared__ __align__(16) unsigned char smem[];
-    const Halo& h = g.hl[1];
-    const int H = h.hmax;
     HaloRecs<S> r;
-    r.q = (V4<S>*)smem;
-    r.dqx = r.q + H;
-    r.dqy = r.dqx + H;
-    r.qmax = r.dqy + H;
-    r.qmin = r.qmax + H;
-    r.xy = (D2*)(r.qmin + H);
-    r.eps = (S*)(r.xy + H);
-    uint16_t* s_cl = (uint16_t*)(r.eps + H);
-    const int h0 = h.ptr[pm.b], hn = h.ptr[pm.b + 1] - h0;
-    for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
-      const int k = h.ids[h0 + j];
-      r.q[j] = st.q[k];
-      r.dqx[j] = st.dq[2 * k];
-      r.dqy[j] = st.dq[2 * k + 1];
-      r.qmax[j] = st.qm[2 * k];
-      r.qmin[j] = st.qm[2 * k + 1];
-      r.xy[j] = g.xy[k];
-      r.eps[j] = st.eps[k];
-    }
-    constexpr int V = STAGE_PF * 2 / 16;
-    const uint4* src = (const uint4*)(h.loc + (size_t)pm.b * STAGE_PF);
-    for (int j = threadIdx.x; j < g.width * V; j += STAGE_PF * LPP) {
-      const int sl = j / V;
-      ((uint4*)s_cl)[j] = src[(size_t)sl * (h.stride / 8) + (j - sl * V)];
-    }
-    __syncthreads();
+    uint16_t* s_cl;
+    stage_flux_halo<S, LPP>(g, st, pm.b, smem, r, s_cl);
     if (pm.live) {
       const StagedNbrs<S, STAGE_PF, LPP> nbrs{g, r, s_cl, (int)threadIdx.x / LPP, g.nbhs[i], lane};
       const double dl = delta[i];

@@ -1190,8 +1190,10 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
                    "when the context's halos fit in LDS)");
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
-    if (rv[k] != 0 && !valid_lpp(rv[k]))
-      return set_err(MF_ERR_INVALID, "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8");
+    if (rv[k] != 0 && !valid_lpp(rv[k]) && !(k == 0 && valid_shape(c, 2, rv[k])))
+      return set_err(MF_ERR_INVALID,
+                     "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux reverse also a staged "
+                     "flux shape)");
   c->lpp_lsq = t->lpp_lsq;
   c->lpp_inner = t->lpp_inner;
   c->lpp_flux = t->lpp_flux;

@@ -66,14 +66,16 @@ struct Trans {
   const int32_t* own;  // [E]
 };
 
-// q-tilde of one end of an edge (the point's own q when first order)
-__device__ inline void end_qtilde(const State<double>& st, int p, double delx, double dely, int first_order,
+// q-tilde of one end of an edge (the point's own q when first order); p indexes the
+// records of nbrs (a point id, or a halo position)
+template <class Nbrs>
+__device__ inline void end_qtilde(const Nbrs& nbrs, int p, double delx, double dely, int first_order,
                                   double qt[4]) {
-  const V4d q = st.q[p];
+  const V4d q = nbrs.q(p);
   if (first_order) {
     for (int r = 0; r < 4; ++r) qt[r] = q[r];
   } else {
-    reconstruct(q, st.dq[2 * p], st.dq[2 * p + 1], st.qm[2 * p], st.qm[2 * p + 1], st.eps[p], delx, dely, qt);
+    reconstruct(q, nbrs.dqx(p), nbrs.dqy(p), nbrs.qmax(p), nbrs.qmin(p), nbrs.eps(p), delx, dely, qt);
   }
 }
 
@@ -81,10 +83,11 @@ __device__ inline void end_qtilde(const State<double>& st, int p, double delx, d
 // coefficients).  Per edge, one end at a time (its q-tilde and kinetic intermediates
 // live only while that end is processed): the end's fluxes feed the adjoints of the
 // edge's coefficients and of delta, and the end's flux adjoints are reversed into its
-// record.
-template <int CLS, int LPP>
-__device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<double>& st, int i,
-                               double delta, const V4d& frbv, const AdjOut& o, int lane) {
+// record.  nbrs gives the records of both ends (the taped state in global memory, or a
+// block's halo staged in LDS) and the point's slots.
+template <int CLS, int LPP, class Nbrs>
+__device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nbrs, int i, double delta,
+                               const V4d& frbv, const AdjOut& o, int lane) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -108,12 +111,11 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
   double odx[4] = {0, 0, 0, 0}, ody[4] = {0, 0, 0, 0};  // own dq adjoints
   double xib = 0.0, yib = 0.0, nxb = 0.0, nyb = 0.0;
   const int nb = g.nbhs[i];
-  const uint32_t* cp = g.conn + i;
   for (int s = lane; s < nb; s += LPP) {
-    const uint32_t pk = cp[(size_t)s * g.n];
-    const int k = (int)(pk >> 4);
-    const unsigned m = pk & 15u;
-    const D2 pkk = g.xy[k];
+    int k;
+    unsigned m;
+    nbrs.slot(s, k, m);
+    const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
     const double dsw = delx * tx + dely * ty;
     const double dnw = delx * nx + dely * ny;
@@ -134,9 +136,9 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     D2* ep = reinterpret_cast<D2*>(o.eb + (size_t)o.epos[(size_t)s * g.n + i] * EF_FLUX);
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
-      const int p = e ? k : i;
+      const int p = e ? k : nbrs.own();
       double qt[4];
-      end_qtilde(st, p, delx, dely, g.first_order, qt);
+      end_qtilde(nbrs, p, delx, dely, g.first_order, qt);
       KinF f;
       kin_fwd(qt, nx, ny, f);
       const double sg = e ? 1.0 : -1.0;
@@ -181,7 +183,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
       if (g.first_order) {
         for (int r = 0; r < 4; ++r) d.q[r] = qtb[r];
       } else {
-        reconstruct_rev_d(st.q[p], st.dq[2 * p], st.dq[2 * p + 1], st.qm[2 * p], st.qm[2 * p + 1], st.eps[p],
+        reconstruct_rev_d(nbrs.q(p), nbrs.dqx(p), nbrs.dqy(p), nbrs.qmax(p), nbrs.qmin(p), nbrs.eps(p),
                           delx, dely, qtb, d, delxb, delyb);
       }
       if (e) {
@@ -244,10 +246,10 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
     }
   }
   for (int s = lane; s < nb; s += LPP) {
-    const uint32_t pk = cp[(size_t)s * g.n];
-    const int k = (int)(pk >> 4);
-    const unsigned m = pk & 15u;
-    const D2 pkk = g.xy[k];
+    int k;
+    unsigned m;
+    nbrs.slot(s, k, m);
+    const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
@@ -299,16 +301,33 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const State<do
   o.deltab[i] += (CLS == WALL) ? 2.0 * dsum : dsum;  // delta enters every stage
 }
 
-template <int LPP>
+template <int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<double> st,
                                                        const double* __restrict__ delta,
                                                        const V4d* __restrict__ frb, AdjOut o) {
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;  // whole lane groups
-  if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
-  else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
-  else flux_point_rev<INTERIOR, LPP>(g, hs, st, i, delta[i], frb[i], o, lane);
+  if constexpr (STAGED) {
+    // the block's halo of the taped stage state in LDS (blocks of STAGE_PF points)
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    HaloRecs<double> r;
+    uint16_t* s_cl;
+    stage_flux_halo<double, LPP>(g, st, b, smem, r, s_cl);
+    const int li = threadIdx.x / LPP, lane = threadIdx.x % LPP;
+    const int i = b * STAGE_PF + li;
+    if (i >= g.n) return;  // whole lane groups
+    const StagedNbrs<double, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
+    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+  } else {
+    const int t = point_id();
+    const int i = t / LPP, lane = t % LPP;
+    if (i >= g.n) return;  // whole lane groups
+    const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
+    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+  }
 }
 
 // reverse of the state update of stage rk: from the adjoint of the stage's new prim
@@ -890,8 +909,23 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
   const HalfSums hs{A.hxx, A.hyy, A.hxy};
   const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
   const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
+  if (lpp & LPP_STAGED) {
+    // staged: blocks of STAGE_PF points, 2, 4 or 8 lanes each
+    const dim3 gr((unsigned)((c->n + STAGE_PF - 1) / STAGE_PF));
+    const size_t lds = stage_lds_flux(c->g.hl[1].hmax, c->g.width, sizeof(double));
+    auto go = [&]<int L>() {
+      hipLaunchKernelGGL((k_flux_rev<L, true>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
+                         (const double*)A.tdelta, (const V4d*)A.frb, o);
+    };
+    switch (lpp & ~LPP_STAGED) {
+      case 2: go.template operator()<2>(); break;
+      case 4: go.template operator()<4>(); break;
+      default: go.template operator()<8>(); break;
+    }
+    return;
+  }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL(k_flux_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
+    hipLaunchKernelGGL((k_flux_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
                        (const double*)A.tdelta, (const V4d*)A.frb, o);
   });
 }
@@ -989,6 +1023,14 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
   const int reps = 5, cand[4] = {1, 2, 4, 8};
   double best[4] = {1e300, 1e300, 1e300, 1e300};
   int pick[4] = {1, 1, 1, 1};
+  // the flux reverse also tries its staged form
+  for (int l : {2, 4, 8}) {
+    const int lpp = LPP_STAGED | l;
+    if (!valid_shape(c, 2, lpp)) continue;
+    double us = 0;
+    if ((rc = time_launches(c, reps, [&] { launch_flux_rev(c, A, 0, A.dqb[A.I & 1], lpp); }, &us))) return rc;
+    if (us < best[0]) best[0] = us, pick[0] = lpp;
+  }
   for (int lpp : cand) {
     double us[4] = {0, 0, 0, 0};
     if ((rc = time_launches(c, reps, [&] { launch_flux_rev(c, A, 0, A.dqb[A.I & 1], lpp); }, &us[0])) ||

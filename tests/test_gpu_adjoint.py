@@ -180,3 +180,20 @@ def test_adjoint_matches_tangent_c3():
     print(f"J {J:.15e} vs {ref['J']:.15e}; directional {adj:.14e} vs tangent {ref['dJ']:.14e}")
     assert abs(J - ref["J"]) <= 1e-12 * abs(ref["J"])
     assert abs(adj - ref["dJ"]) <= 1e-10 * abs(ref["dJ"])
+
+
+@pytest.mark.parametrize("lpp", [2, 4, 8])
+def test_adjoint_staged_flux_reverse(small_cloud, lpp):
+    """The flux reverse's staged form (each 32-point block's halo of the taped stage state
+    in LDS) walks the same slots in the same order as the global form: the same gradient
+    up to the order of the few floating-point atomics of the sweep (func_delta's argmin,
+    the force integral, the normals)."""
+    pin = {"lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
+    with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
+        a.tuning = {"lpp_flux_rev": lpp, **pin}
+        b.tuning = {"lpp_flux_rev": 16 | lpp, **pin}
+        Ja, xa, ya, sa = a.adjoint(10, snaps=3)
+        Jb, xb, yb, sb = b.adjoint(10, snaps=3)
+    assert sa["lpp_flux_rev"] == lpp and sb["lpp_flux_rev"] == 16 | lpp
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
