@@ -187,8 +187,8 @@ __device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, in
 // The staged form of a stencil walk.  A block holds P consecutive points, LPP lanes
 // each, and first copies its halo (fill(j, k): point k's records into the caller's LDS
 // arrays at halo position j) and its points' slot words into LDS; after one barrier
-// every lane walks its slots (lane, lane + LPP, ...) with use(j, m) reading halo
-// position j, m the slot's half-stencil bits.  Each neighbour record then crosses from
+// every lane walks its slots (lane, lane + LPP, ...) with use(j, m, s) reading halo
+// position j, m the slot's half-stencil bits, s the slot.  Each neighbour record then crosses from
 // L2 once per block instead of once per edge (about 2.8 instead of 23.5 record loads
 // per point at 64 points per block), and the walk's loads are LDS reads.  A lane's
 // slots and their order are those of the global walk, so its sums are bit-identical.
@@ -208,7 +208,7 @@ __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int
   const int li = threadIdx.x / LPP;
   for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
     const unsigned e = s_cl[s * P + li];
-    use((int)(e & 0xFFFu), e >> 12);
+    use((int)(e & 0xFFFu), e >> 12, s);
   }
 }
 

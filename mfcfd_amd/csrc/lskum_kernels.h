@@ -68,6 +68,10 @@ inline size_t stage_lds_lsq(int hmax, int width, size_t ssz, bool delta) {
 inline size_t stage_lds_inner(int hmax, int width, size_t ssz, bool sums) {
   return (size_t)hmax * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
 }
+// the adjoint's inner reverse (FP64): q, dq_x, dq_y and xy per halo entry
+inline size_t stage_lds_inner_rev(int hmax, int width) {
+  return (size_t)hmax * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+}
 inline size_t stage_lds_flux(int hmax, int width, size_t ssz) {
   return (size_t)hmax * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
 }
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
           s_xy[j] = g.xy[k];
           if (DELTA) s_c[j] = cspeed[k];
         },
-        [&](int j, unsigned) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
+        [&](int j, unsigned, int) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
     if (!pm.live) return;
   } else {
     struct Nbr {
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
           if (!SUMS) s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
+        [&](int j, unsigned, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
     if (!pm.live) return;
   } else if constexpr (SUMS) {
     struct Nbr {

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_rev(Geom g, const V4d* __restr
 // Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
 // adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
 // neighbour's dq adjoint being -0.5 (delx, dely) times its q adjoint (k_gather_lsq).
-template <int LPP>
+template <int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restrict__ q,
                                                      const V4d* __restrict__ dq_in,
                                                      const V4d* __restrict__ dq_out,
@@ -376,9 +376,9 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ qb,
                                                      D2* __restrict__ xyb, double* __restrict__ eb,
                                                      const int32_t* __restrict__ epos) {
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;
+  const PointMap<LPP, STAGED> pm(g.n);
+  if (!STAGED && !pm.live) return;
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
   const D2 pi = g.xy[i];
   const V4d gg = g.lsq_geo[i];
   const double sxx = gg.x, syy = gg.y, sxy = gg.z, obd = gg.w;
@@ -403,12 +403,10 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   const V4d qi = q[i], dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
   double qib[4] = {0, 0, 0, 0}, dxib[4] = {0, 0, 0, 0}, dyib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
-  const int nb = g.nbhs[i];
-  for (int s = lane; s < nb; s += LPP) {
-    const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
-    const D2 pk = g.xy[k];
+  const int nb = pm.live ? g.nbhs[i] : 0;
+  // edge in slot s to the neighbour with coordinates pk and records qk, dxk, dyk
+  auto edge = [&](int s, const D2& pk, const V4d& qk, const V4d& dxk, const V4d& dyk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    const V4d qk = q[k], dxk = dq_in[2 * k], dyk = dq_in[2 * k + 1];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
     double Dbv[4];
     for (int r = 0; r < 4; ++r) {
@@ -429,6 +427,31 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
     ep[2] = D2{dxb, dyb};
     xib -= dxb;
     yib -= dyb;
+  };
+  if constexpr (STAGED) {
+    // the block's halo (q, dq_in, xy) in LDS
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int H = g.hl[0].hmax;
+    V4d* s_q = (V4d*)smem;
+    V4d* s_dx = s_q + H;
+    V4d* s_dy = s_dx + H;
+    D2* s_xy = (D2*)(s_dy + H);
+    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    stage_and_walk<STAGE_P, LPP>(
+        g, g.hl[0], pm.b, nb, s_cl,
+        [&](int j, int k) {
+          s_q[j] = q[k];
+          s_dx[j] = dq_in[2 * k];
+          s_dy[j] = dq_in[2 * k + 1];
+          s_xy[j] = g.xy[k];
+        },
+        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j], s_dx[j], s_dy[j]); });
+    if (!pm.live) return;
+  } else {
+    for (int s = lane; s < nb; s += LPP) {
+      const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
+      edge(s, g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]);
+    }
   }
   if (LPP > 1) {
     for (int r = 0; r < 4; ++r) {
@@ -941,9 +964,19 @@ static void launch_gather_flux(mf_ctx* c, AdjState& A, V4d* dqb_last, int lpp) {
 // inner iteration j of stage s: dq^(j) adjoint (dqb[j & 1]) -> dq^(j-1)
 static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
   const int cur = j & 1, prv = (j - 1) & 1;
+  const V4d* q = (const V4d*)A.tq[s];
+  const V4d* din = (const V4d*)A.tdq[s * (A.I + 1) + j - 1];
+  const V4d* dout = (const V4d*)A.tdq[s * (A.I + 1) + j];
+  if (lpp & LPP_STAGED) {
+    const size_t lds = stage_lds_inner_rev(c->g.hl[0].hmax, c->g.width);
+    with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
+      hipLaunchKernelGGL((k_inner_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, din,
+                         dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+    });
+    return;
+  }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL(k_inner_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, (const V4d*)A.tq[s],
-                       (const V4d*)A.tdq[s * (A.I + 1) + j - 1], (const V4d*)A.tdq[s * (A.I + 1) + j],
+    hipLaunchKernelGGL((k_inner_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, din, dout,
                        (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
   });
 }
@@ -1023,6 +1056,15 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
   const int reps = 5, cand[4] = {1, 2, 4, 8};
   double best[4] = {1e300, 1e300, 1e300, 1e300};
   int pick[4] = {1, 1, 1, 1};
+  // the inner reverse tries its staged form (64-point blocks, 1, 2 or 4 lanes)
+  if (A.I > 0)
+    for (int l : {1, 2, 4}) {
+      const int lpp = LPP_STAGED | l;
+      if (!valid_inner_rev_staged(c, lpp)) continue;
+      double us = 0;
+      if ((rc = time_launches(c, reps, [&] { launch_inner_rev(c, A, 0, 1, lpp); }, &us))) return rc;
+      if (us < best[2]) best[2] = us, pick[2] = lpp;
+    }
   // the flux reverse also tries its staged form
   for (int l : {2, 4, 8}) {
     const int lpp = LPP_STAGED | l;

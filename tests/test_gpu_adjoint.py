@@ -182,18 +182,21 @@ def test_adjoint_matches_tangent_c3():
     assert abs(adj - ref["dJ"]) <= 1e-10 * abs(ref["dJ"])
 
 
-@pytest.mark.parametrize("lpp", [2, 4, 8])
-def test_adjoint_staged_flux_reverse(small_cloud, lpp):
-    """The flux reverse's staged form (each 32-point block's halo of the taped stage state
-    in LDS) walks the same slots in the same order as the global form: the same gradient
-    up to the order of the few floating-point atomics of the sweep (func_delta's argmin,
-    the force integral, the normals)."""
-    pin = {"lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
+@pytest.mark.parametrize("kernel,lpp", [("flux", 2), ("flux", 4), ("flux", 8), ("inner", 1), ("inner", 2),
+                                        ("inner", 4)])
+def test_adjoint_staged_reverse(small_cloud, kernel, lpp):
+    """The staged forms of the flux reverse (32-point blocks) and of the inner-iteration
+    reverse (64-point blocks) read the taped stage state from an LDS halo and walk the
+    same slots in the same order as the global forms: the same gradient up to the order
+    of the few floating-point atomics of the sweep (func_delta's argmin, the force
+    integral, the normals)."""
+    key = {"flux": "lpp_flux_rev", "inner": "lpp_inner_rev"}[kernel]
+    pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
     with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
-        a.tuning = {"lpp_flux_rev": lpp, **pin}
-        b.tuning = {"lpp_flux_rev": 16 | lpp, **pin}
+        a.tuning = {**pin, key: lpp}
+        b.tuning = {**pin, key: 16 | lpp}
         Ja, xa, ya, sa = a.adjoint(10, snaps=3)
         Jb, xb, yb, sb = b.adjoint(10, snaps=3)
-    assert sa["lpp_flux_rev"] == lpp and sb["lpp_flux_rev"] == 16 | lpp
+    assert sa[key] == lpp and sb[key] == 16 | lpp
     ref = np.concatenate([xa, ya])
     assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
