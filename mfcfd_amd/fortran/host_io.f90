@@ -205,7 +205,11 @@ contains
         o%restart = merge(1, 0, trim(o%restart_solution) == 'same')
         o%first_order = merge(1, 0, trim(o%solution_accuracy) == 'first')
         o%format = merge(2, 1, trim(o%format_file) == 'quadtree')
-        o%ad_mode = merge(1, 0, trim(o%adjoint_mode) == 'checkpoints')
+        ! -adjoint_mode checkpoints -chkpts S: binomial schedule of S snapshots
+        ! (q_lskum_chkpts_b); -adjoint_mode tape (no counterpart in the reference): every
+        ! iteration taped during the primal sweep (MF_ADJ_TAPE_ALL), when it fits in HBM
+        o%ad_mode = merge(1, 0, trim(o%adjoint_mode) == 'checkpoints' .or. trim(o%adjoint_mode) == 'tape')
+        if (trim(o%adjoint_mode) == 'tape') o%chkpts = -1
         if (trim(o%precision_name) == 'mixed') then
             o%precision = 1
         else if (trim(o%precision_name) == 'fp64') then
@@ -215,7 +219,8 @@ contains
         end if
         if (trim(o%time) /= 'steady') call fail('-time unsteady is not supported')
         if (trim(o%limiter) /= 'venkat') call fail('only -limiter venkat is supported')
-        if (o%ad_mode == 1 .and. o%chkpts < 1) call fail('-adjoint_mode checkpoints needs -chkpts >= 1')
+        if (o%ad_mode == 1 .and. o%chkpts < 1 .and. trim(o%adjoint_mode) /= 'tape') &
+            call fail('-adjoint_mode checkpoints needs -chkpts >= 1')
     end subroutine
 
     ! input.nml of the CUDA build (src_cuda/data_structure.F90:106-176): namelist
@@ -294,7 +299,7 @@ contains
         write(*, *) 'inner_iterations    :', o%inner_iterations
         write(*, *) 'restart         :', trim(o%restart_solution)
         write(*, *) 'precision       :', trim(o%precision_name)
-        if (o%ad_mode == 1) write(*, *) 'checkpoints         :', o%chkpts
+        if (o%ad_mode == 1) write(*, *) 'checkpoints         :', o%chkpts, '(-1: every iteration taped)'
     end subroutine
 
     ! -------------------------------------------------------------------------

@@ -4,7 +4,8 @@
 ! with q_lskum.F90:15-78), one GPU instead of MPI ranks: read case.in, read the point
 ! file, set the freestream or restart state, iterate, and write the residue history,
 ! cp/cp-file, solution/sol.dat and (with -adjoint_mode checkpoints -chkpts S, as
-! src_mpi_adjoint/meshfree_solver.F90 runs q_lskum_chkpts_b) the Cd sensitivities in
+! src_mpi_adjoint/meshfree_solver.F90 runs q_lskum_chkpts_b, or -adjoint_mode tape to
+! tape every iteration in the primal sweep) the Cd sensitivities in
 ! sensitivity/sensitivity.dat.  Normals, connectivity splits, func_delta, the stages
 ! and the force coefficients all run on the device.
 !

@@ -90,9 +90,13 @@ def test_host_restart_continues(small_cloud, tmp_path):
     assert rel_err(sol[:, 5:9], prim) <= 1e-14
 
 
-def test_host_adjoint_writes_sensitivity(small_cloud, tmp_path):
+@pytest.mark.parametrize("mode", ["checkpoints", "tape"])
+def test_host_adjoint_writes_sensitivity(small_cloud, tmp_path, mode):
+    """-adjoint_mode checkpoints -chkpts 4 (binomial), and -adjoint_mode tape (every
+    iteration taped in the primal sweep): the same sensitivities as mf_adjoint."""
     from mfcfd_amd.solver import LskumSolver
-    _case(tmp_path, small_cloud, max_iters=12, adjoint_mode="checkpoints", chkpts=4)
+    opts = dict(chkpts=4) if mode == "checkpoints" else {}
+    _case(tmp_path, small_cloud, max_iters=12, adjoint_mode=mode, **opts)
     out = _run(tmp_path)
     assert "Objective" in out
     with open(tmp_path / "sensitivity" / "sensitivity.dat") as fh:
