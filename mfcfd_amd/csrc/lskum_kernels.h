@@ -68,6 +68,10 @@ inline size_t stage_lds_lsq(int hmax, int width, size_t ssz, bool delta) {
 inline size_t stage_lds_inner(int hmax, int width, size_t ssz, bool sums) {
   return (size_t)hmax * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
 }
+// the adjoint's LSQ reverse (FP64): q and xy per halo entry
+inline size_t stage_lds_lsq_rev(int hmax, int width) {
+  return (size_t)hmax * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+}
 // the adjoint's inner reverse (FP64): q, dq_x, dq_y and xy per halo entry
 inline size_t stage_lds_inner_rev(int hmax, int width) {
   return (size_t)hmax * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;

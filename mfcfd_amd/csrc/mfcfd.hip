@@ -1087,11 +1087,14 @@ static bool valid_shape(const mf_ctx* c, int kind, int v) {
   return lds <= kMaxStageLds;
 }
 
-// the adjoint's inner reverse: staged shapes with 1, 2 or 4 lanes, halo in LDS
-static bool valid_inner_rev_staged(const mf_ctx* c, int lpp) {
+// staged shapes of the adjoint's LSQ reverse (kind 1) and inner reverse (kind 2): 1, 2
+// or 4 lanes, 64-point blocks, halo in LDS
+static bool valid_rev_staged(const mf_ctx* c, int kind, int lpp) {
   const int l = lpp & ~LPP_STAGED;
-  return (lpp & LPP_STAGED) && (l == 1 || l == 2 || l == 4) && c->g.hl[0].hmax > 0 &&
-         stage_lds_inner_rev(c->g.hl[0].hmax, c->g.width) <= kMaxStageLds;
+  if (!(lpp & LPP_STAGED) || (l != 1 && l != 2 && l != 4) || c->g.hl[0].hmax <= 0) return false;
+  const size_t lds = kind == 1 ? stage_lds_lsq_rev(c->g.hl[0].hmax, c->g.width)
+                               : stage_lds_inner_rev(c->g.hl[0].hmax, c->g.width);
+  return lds <= kMaxStageLds;
 }
 
 // mean time (us) of `reps` launches of fn on the context's stream, after 2 untimed
@@ -1198,10 +1201,10 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
     if (rv[k] != 0 && !valid_lpp(rv[k]) && !(k == 0 && valid_shape(c, 2, rv[k])) &&
-        !(k == 2 && valid_inner_rev_staged(c, rv[k])))
+        !((k == 1 || k == 2) && valid_rev_staged(c, k, rv[k])))
       return set_err(MF_ERR_INVALID,
-                     "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux and inner reverses also "
-                     "their staged shapes)");
+                     "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux, LSQ and inner reverses "
+                     "also their staged shapes)");
   c->lpp_lsq = t->lpp_lsq;
   c->lpp_inner = t->lpp_inner;
   c->lpp_flux = t->lpp_flux;

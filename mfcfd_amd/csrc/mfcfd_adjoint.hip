@@ -475,16 +475,16 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
 // contributions added to qb, xyb; per edge, the neighbour's q adjoint (with the qm
 // adjoint of the components it maximises or minimises) and the edge's (delx, dely)
 // adjoint go to the edge buffer (fields 0-5, k_gather_lsq)
-template <int LPP>
+template <int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict__ q,
                                                    const V4d* __restrict__ dq,
                                                    const V4d* __restrict__ dqb,
                                                    const V4d* __restrict__ qmb, V4d* __restrict__ qb,
                                                    D2* __restrict__ xyb, double* __restrict__ eb,
                                                    const int32_t* __restrict__ epos) {
-  const int t = point_id();
-  const int i = t / LPP, lane = t % LPP;
-  if (i >= g.n) return;
+  const PointMap<LPP, STAGED> pm(g.n);
+  if (!STAGED && !pm.live) return;
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
   const D2 pi = g.xy[i];
   const V4d gg = g.lsq_geo[i];
   const double sxx = gg.x, syy = gg.y, sxy = gg.z, obd = gg.w;
@@ -514,12 +514,10 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   int smax[4] = {-1, -1, -1, -1}, smin[4] = {-1, -1, -1, -1};
   double qib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
-  const int nb = g.nbhs[i];
-  for (int s = lane; s < nb; s += LPP) {
-    const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
-    const D2 pk = g.xy[k];
+  const int nb = pm.live ? g.nbhs[i] : 0;
+  // edge in slot s to the neighbour with coordinates pk and q qk
+  auto edge = [&](int s, const D2& pk, const V4d& qk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    const V4d qk = q[k];
     double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
     double Dbv[4];
     for (int r = 0; r < 4; ++r) {
@@ -544,6 +542,27 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
     ep[2] = D2{dxb, dyb};
     xib -= dxb;
     yib -= dyb;
+  };
+  if constexpr (STAGED) {
+    // the block's halo (q, xy) in LDS
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int H = g.hl[0].hmax;
+    V4d* s_q = (V4d*)smem;
+    D2* s_xy = (D2*)(s_q + H);
+    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    stage_and_walk<STAGE_P, LPP>(
+        g, g.hl[0], pm.b, nb, s_cl,
+        [&](int j, int k) {
+          s_q[j] = q[k];
+          s_xy[j] = g.xy[k];
+        },
+        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j]); });
+    if (!pm.live) return;
+  } else {
+    for (int s = lane; s < nb; s += LPP) {
+      const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
+      edge(s, g.xy[k], q[k]);
+    }
   }
   if (LPP > 1) {
     // the point's extremes over its lanes: the larger (smaller) value, on ties the
@@ -991,10 +1010,19 @@ static void launch_gather_lsq(mf_ctx* c, AdjState& A, V4d* dqb, int lpp) {
 }
 
 static void launch_lsq_rev(mf_ctx* c, AdjState& A, int s, int lpp) {
+  const V4d* q = (const V4d*)A.tq[s];
+  const V4d* dq0 = (const V4d*)A.tdq[s * (A.I + 1)];
+  if (lpp & LPP_STAGED) {
+    const size_t lds = stage_lds_lsq_rev(c->g.hl[0].hmax, c->g.width);
+    with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
+      hipLaunchKernelGGL((k_lsq_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, dq0,
+                         (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+    });
+    return;
+  }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL(k_lsq_rev<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, (const V4d*)A.tq[s],
-                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb,
-                       A.eb, (const int32_t*)A.epos);
+    hipLaunchKernelGGL((k_lsq_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, dq0,
+                       (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
   });
 }
 
@@ -1056,11 +1084,19 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
   const int reps = 5, cand[4] = {1, 2, 4, 8};
   double best[4] = {1e300, 1e300, 1e300, 1e300};
   int pick[4] = {1, 1, 1, 1};
+  // the LSQ reverse tries its staged form (64-point blocks, 1, 2 or 4 lanes)
+  for (int l : {1, 2, 4}) {
+    const int lpp = LPP_STAGED | l;
+    if (!valid_rev_staged(c, 1, lpp)) continue;
+    double us = 0;
+    if ((rc = time_launches(c, reps, [&] { launch_lsq_rev(c, A, 0, lpp); }, &us))) return rc;
+    if (us < best[1]) best[1] = us, pick[1] = lpp;
+  }
   // the inner reverse tries its staged form (64-point blocks, 1, 2 or 4 lanes)
   if (A.I > 0)
     for (int l : {1, 2, 4}) {
       const int lpp = LPP_STAGED | l;
-      if (!valid_inner_rev_staged(c, lpp)) continue;
+      if (!valid_rev_staged(c, 2, lpp)) continue;
       double us = 0;
       if ((rc = time_launches(c, reps, [&] { launch_inner_rev(c, A, 0, 1, lpp); }, &us))) return rc;
       if (us < best[2]) best[2] = us, pick[2] = lpp;

@@ -183,14 +183,15 @@ def test_adjoint_matches_tangent_c3():
 
 
 @pytest.mark.parametrize("kernel,lpp", [("flux", 2), ("flux", 4), ("flux", 8), ("inner", 1), ("inner", 2),
-                                        ("inner", 4)])
+                                        ("inner", 4), ("lsq", 1), ("lsq", 2), ("lsq", 4)])
 def test_adjoint_staged_reverse(small_cloud, kernel, lpp):
-    """The staged forms of the flux reverse (32-point blocks) and of the inner-iteration
-    reverse (64-point blocks) read the taped stage state from an LDS halo and walk the
+    """The staged forms of the flux reverse (32-point blocks) and of the LSQ and
+    inner-iteration reverses (64-point blocks) read the taped stage state from an LDS
+    halo and walk the
     same slots in the same order as the global forms: the same gradient up to the order
     of the few floating-point atomics of the sweep (func_delta's argmin, the force
     integral, the normals)."""
-    key = {"flux": "lpp_flux_rev", "inner": "lpp_inner_rev"}[kernel]
+    key = {"flux": "lpp_flux_rev", "inner": "lpp_inner_rev", "lsq": "lpp_lsq_rev"}[kernel]
     pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
     with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
         a.tuning = {**pin, key: lpp}
