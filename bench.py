@@ -330,7 +330,7 @@ def run_batch(iters: int, adjoint: bool = True):
 COPY_PROBE_BYTES = 256 << 20
 # the launch shapes the autotuner keeps for this workload (16 | L: staged walk, L lanes)
 PROBE_SHAPES = {"fp64": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 4},
-                "mixed": {"lpp_lsq": 16 | 2, "lpp_inner": 16 | 4, "lpp_flux": 4}}
+                "mixed": {"lpp_lsq": 16 | 2, "lpp_inner": 16 | 4, "lpp_flux": 16 | 8}}
 
 
 def probe():
@@ -419,7 +419,7 @@ def main():
     fu_ms = best["kernels"]["flux_update"][0]
     per_launch = flux_update_bytes_per_point(sb, mean_nbhs) * cloud.n
     achieved = per_launch / (fu_ms * 1e-3) / 1e9
-    traffic = valu_busy = l2_hit = None
+    traffic = valu_busy = l2_hit = traffic_kernel = None
     tpath = os.path.join(ROOT, "profiles", "traffic_flux_update.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
@@ -428,6 +428,7 @@ def main():
         traffic = t.get(key)
         valu_busy = t.get(f"{key}_valu_busy")
         l2_hit = t.get(f"{key}_l2_hit")
+        traffic_kernel = t.get("kernels", {}).get(key)
 
     it_bytes = iteration_bytes_per_point(sb, mean_nbhs, WORKLOAD["inner_iterations"])
     line = {
@@ -452,7 +453,10 @@ def main():
                      "kernel": "k_flux_update", "kernel_ms": fu_ms,
                      "algorithmic_bytes_per_launch": per_launch,
                      # counters of the same kernel (profiles/): the VALU, not HBM, binds it
-                     "valu_busy_pct": valu_busy, "l2_hit": l2_hit},
+                     "valu_busy_pct": valu_busy, "l2_hit": l2_hit,
+                     # the launch shape timed here (lanes per point, 16 | L: staged) and the
+                     # kernel instance the counters were taken on
+                     "shape": best["tuning"]["lpp_flux"], "counters_kernel": traffic_kernel},
         "iteration_hbm_gbs": it_bytes * best["point_iters_per_s"] / 1e9,
         "fp64_point_iters_per_s": fp64["point_iters_per_s"],
         "mixed_point_iters_per_s": mixed["point_iters_per_s"],
