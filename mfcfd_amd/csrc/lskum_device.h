@@ -688,8 +688,11 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double dgx = (double)gxk[r] - (double)gxi[r];
-      const double dgy = (double)gyk[r] - (double)gyi[r];
+      // the end fluxes' difference in the state precision (for FP64 the same as after
+      // widening; for FP32 its rounding is at most that of the FP32 fluxes themselves),
+      // accumulated in FP64
+      const double dgx = (double)(gxk[r] - gxi[r]);
+      const double dgy = (double)(gyk[r] - gyi[r]);
       acc[r] = acc[r] + dgx * cx + dgy * cy;
     }
     if (tie_x || tie_y) {
@@ -703,8 +706,8 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
       end_fluxes<CLS>(kin_eval(qtk, nxs, nys), false, false, Hx, Hy);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        if (tie_x) acc[r] = acc[r] + ((double)Hx[r] - (double)Gx[r]) * cx2;
-        if (tie_y) acc[r] = acc[r] + ((double)Hy[r] - (double)Gy[r]) * cy2;
+        if (tie_x) acc[r] = acc[r] + (double)(Hx[r] - Gx[r]) * cx2;
+        if (tie_y) acc[r] = acc[r] + (double)(Hy[r] - Gy[r]) * cy2;
       }
     }
   });
