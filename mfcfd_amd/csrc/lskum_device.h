@@ -341,14 +341,18 @@ __device__ inline Kin<float> kin_fast(const float qt[4], float nx, float ny) {
   return k;
 }
 
-// venkat with a reciprocal in place of the division (mixed path)
+// venkat for the mixed path.  The reference's ratio has the factor del_neg in both
+// numerator and denominator (limiters.F90:91-97); past the threshold del_neg != 0, so
+// it is divided out: phi = min(1, (p^2 + e^2 + 2 d p) / (p^2 + e^2 + d (2 d + p))) with
+// d = del_neg, p = del_pos; the denominator equals (p + d/2)^2 + 7/4 d^2 + e^2 > 0.  A
+// reciprocal replaces the division.
 __device__ inline float venkat_fast(float qt, float qk, float qmax, float qmin, float epsi) {
   const float del_neg = qt - qk;
   if (fabsf(del_neg) <= Consts<float>::THRESH) return 1.0f;
   const float del_pos = (del_neg > 0.0f) ? qmax - qk : qmin - qk;
-  const float e2 = epsi * epsi;
-  const float num = (del_pos * del_pos + e2) * del_neg + 2.0f * del_neg * del_neg * del_pos;
-  const float den = (del_pos * del_pos + 2.0f * del_neg * del_neg + del_neg * del_pos + e2) * del_neg;
+  const float a = del_pos * del_pos + epsi * epsi;
+  const float num = a + 2.0f * del_neg * del_pos;
+  const float den = a + del_neg * (2.0f * del_neg + del_pos);
   const float t = num * frcp(den);
   return (t < 1.0f) ? t : 1.0f;
 }
