@@ -296,3 +296,22 @@ def test_profile_kernels_runs(small_cloud):
         t = g.profile_kernels(3)
         assert t["flux_update"][0] > 0 and t["flux_update"][1] == 4
         assert t["lsq"][1] == 4 and t["inner"][1] == 8
+
+
+@pytest.mark.parametrize("power", [-2.0, 1.0])
+def test_distance_weights_power(small_cloud, power):
+    """-power p weights each edge of the least-squares sums and of the flux derivatives by
+    dist**p (q_variables.F90:91,169; interior_fluxes.F90:71,170,270,368): a 20-iteration
+    trajectory against the oracle, and the staged walks against the global ones."""
+    o = _oracle(small_cloud, power=power)
+    o.run(20)
+    with _gpu(small_cloud, power=power) as a, _gpu(small_cloud, power=power) as b:
+        b.tuning = {"lpp_lsq": 16 | 2, "lpp_inner": 16 | 2, "lpp_flux": 16 | 4}
+        a.tuning = {"lpp_lsq": 2, "lpp_inner": 2, "lpp_flux": 4}
+        ra, _, cda = a.q_lskum(20)
+        b.q_lskum(20)
+        assert rel_err(a.prim, o.field("prim")) < 1e-10
+        assert np.array_equal(a.prim, b.prim)
+    with _gpu(small_cloud, power=power) as c:
+        c.q_lskum(20)
+        assert rel_err(c.prim, o.field("prim")) < 1e-10
