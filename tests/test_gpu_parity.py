@@ -315,3 +315,68 @@ def test_distance_weights_power(small_cloud, power):
     with _gpu(small_cloud, power=power) as c:
         c.q_lskum(20)
         assert rel_err(c.prim, o.field("prim")) < 1e-10
+
+
+def _relabelled(cloud, **fields):
+    import dataclasses
+    return dataclasses.replace(cloud, **{k: v.copy() for k, v in fields.items()})
+
+
+@pytest.mark.parametrize("case", ["normal_flag_1", "format_2"])
+def test_interior_normal_options(small_cloud, case):
+    """point_normals.F90:90-103: interior normals (1, 0) with
+    -interior_points_normal_flag 1, and the file's normals with -format_file quadtree
+    (format 2; here seeded random unit vectors), which put every interior point's
+    split fluxes in a rotated frame.  20 iterations against the oracle."""
+    c = small_cloud
+    kw = {}
+    if case == "normal_flag_1":
+        kw["interior_points_normal_flag"] = 1
+    else:
+        rng = np.random.default_rng(7)
+        ang = rng.uniform(0.0, 2.0 * np.pi, c.n)
+        c = _relabelled(c, nx=np.cos(ang), ny=np.sin(ang))
+        kw["format"] = 2
+    o = _oracle(c, **kw)
+    o.run(20)
+    o.compute_cl_cd_cm()
+    with _gpu(c, **kw) as g:
+        # A 90-degree turn of the interior frame only reassigns the split stencils, so
+        # normal flag 1 leaves the trajectory unchanged up to rounding: check the
+        # normals themselves as well.
+        assert np.array_equal(g.field("nx"), o.field("nx"))
+        assert np.array_equal(g.field("ny"), o.field("ny"))
+        interior = c.flag_1 == 1
+        if case == "normal_flag_1":
+            assert np.all(g.field("nx")[interior] == 1.0) and np.all(g.field("ny")[interior] == 0.0)
+        else:
+            assert np.array_equal(g.field("nx")[interior], c.nx[interior])
+        g.q_lskum(20)
+        assert rel_err(g.prim, o.field("prim")) < 1e-10
+        _, cd, _, _ = g.compute_cl_cd_cm()
+        assert abs(cd[0] - o.field("Cd")[0]) <= 1e-10 * abs(o.field("Cd")[0])
+
+
+def test_two_shapes(small_cloud):
+    """-shapes 2 with the lower surface's wall points relabelled flag_2 = 2: per-shape
+    Cl, Cd, Cm (compute_force_coeffs.F90:11-102) against the oracle, and the two shapes'
+    sums equal the one-shape forces (the integral is split, not changed)."""
+    c = small_cloud
+    f2 = c.flag_2.copy()
+    lower = (c.flag_1 == 0) & (c.y < 0.0)
+    f2[lower] = 2
+    c2 = _relabelled(c, flag_2=f2)
+    o = _oracle(c2, shapes=2)
+    o.run(15)
+    o.compute_cl_cd_cm()
+    with _gpu(c2, shapes=2) as g, _gpu(c) as h:
+        g.q_lskum(15)
+        h.q_lskum(15)
+        assert np.array_equal(g.prim, h.prim)  # the shapes only label force sums
+        cl, cd, cm, _ = g.compute_cl_cd_cm()
+        cl1, cd1, cm1, _ = h.compute_cl_cd_cm()
+        assert cl.shape == (2,) and cl1.shape == (1,)
+        for got, ref in ((cl, o.field("Cl")), (cd, o.field("Cd")), (cm, o.field("Cm"))):
+            assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+        for two, one in ((cl, cl1), (cd, cd1), (cm, cm1)):
+            assert abs(two.sum() - one[0]) <= 1e-12 * abs(one[0]) + 1e-15

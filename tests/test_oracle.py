@@ -151,6 +151,70 @@ def test_live_reference_routines():
         assert ((ref_phi > 0) & (ref_phi <= 1)).all()
 
 
+@pytest.mark.skipif(not RefRoutines.available(), reason="reference sources not built here")
+@pytest.mark.parametrize("case", ["normal_flag_1", "format_2"])
+def test_live_reference_normal_options(case):
+    """point_normals.F90:90-103 under -interior_points_normal_flag 1 (interior normal
+    (1, 0)) and format 2 (the file's interior normals kept; here seeded random unit
+    vectors), then the flux residual in those frames (flux_residual.F90), against the
+    reference's compiled routines."""
+    import dataclasses
+    from mfcfd_amd.cloud import naca_ogrid
+    c = naca_ogrid(80, 30, seed=6)
+    if case == "normal_flag_1":
+        kw = dict(interior_points_normal_flag=1)
+    else:
+        ang = np.random.default_rng(11).uniform(0.0, 2.0 * np.pi, c.n)
+        c = dataclasses.replace(c, nx=np.cos(ang), ny=np.sin(ang))
+        kw = dict(format=2)
+    o = Oracle(c, cfl=0.2, mach=0.63, aoa_deg=2.0, vl_const=5.0, inner_iterations=0,
+               **kw).setup()
+    o.run(8)
+    ref = RefRoutines(c, vl_const=5.0, **kw)
+    ref.set("nx", c.nx)
+    ref.set("ny", c.ny)
+    ref.call("compute_normals")
+    assert np.array_equal(ref.get("nx"), o.field("nx"))
+    assert np.array_equal(ref.get("ny"), o.field("ny"))
+    interior = c.flag_1 == 1
+    if case == "normal_flag_1":
+        assert (o.field("nx")[interior] == 1.0).all() and (o.field("ny")[interior] == 0.0).all()
+    else:
+        assert np.array_equal(o.field("nx")[interior], c.nx[interior])
+    ref.set_split(o)
+    ref.set("prim", o.field("prim"))
+    for r in ("eval_q_variables", "eval_q_derivatives"):
+        ref.call(r)
+        getattr(o, r)()
+    o.func_delta()
+    ref.set("delta", o.field("delta"))
+    ref.call("cal_flux_residual")
+    o.cal_flux_residual()
+    assert np.array_equal(ref.get("flux_res"), o.field("flux_res"))
+
+
+def test_force_coefficients_split_over_shapes(cloud):
+    """compute_force_coeffs.F90:11-102 accumulates each wall point into the shape its
+    flag_2 names: relabelling the lower surface as shape 2 splits Cl, Cd, Cm into two
+    parts whose sums are the one-shape values."""
+    import dataclasses
+    o1 = Oracle(cloud, mach=0.63, aoa_deg=2.0, vl_const=5.0, inner_iterations=0).setup()
+    o1.run(6)
+    f2 = cloud.flag_2.copy()
+    lower = (cloud.flag_1 == 0) & (cloud.y < 0.0)
+    f2[lower] = 2
+    o2 = Oracle(dataclasses.replace(cloud, flag_2=f2), mach=0.63, aoa_deg=2.0,
+                vl_const=5.0, inner_iterations=0, shapes=2).setup()
+    o2.run(6)
+    assert np.array_equal(o1.field("prim"), o2.field("prim"))
+    o1.compute_cl_cd_cm()
+    o2.compute_cl_cd_cm()
+    for f in ("Cl", "Cd", "Cm"):
+        two, one = o2.field(f), o1.field(f)
+        assert two.shape == (2,) and (two != 0).all()
+        assert abs(two.sum() - one[0]) <= 1e-13 * max(abs(one[0]), abs(two).max())
+
+
 # ---------------------------------------------------------------------------------
 # closed forms
 
