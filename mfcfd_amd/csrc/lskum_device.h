@@ -305,56 +305,46 @@ __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // erf(s) given ex = exp(-s^2): Abramowitz & Stegun 7.1.26, erf(x) = 1 - t (a1 + t (a2 +
-// t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x), x >= 0, |error| <= 1.5e-7; odd in s
-__device__ inline float ferf_from_exp(float s, float ex) {
+// t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x), x >= 0, |error| <= 1.5e-7; odd in s.
+// With s given as c x (c > 0), pass p / c.
+__device__ inline float ferf_from_exp(float s, float ex, float p = 0.3275911f) {
   const float x = fabsf(s);
-  const float t = frcp(1.0f + 0.3275911f * x);
+  const float t = frcp(1.0f + p * x);
   const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
   const float e = 1.0f - poly * ex;
   return s < 0.0f ? -e : e;
 }
 
 // The flux preliminaries (q-tilde -> primitive -> split-flux pieces) of the mixed
-// path: kin_of_qtilde's quantities with the identities b = rho / (2 p) = beta and
-// p / rho = 1 / (2 beta) used instead of recomputing them
+// path: kin_of_qtilde's quantities with the identities b = rho / (2 p) = beta,
+// p / rho = 1 / (2 beta) = rsq(beta)^2 / 2 and ut^2 + un^2 = u1^2 + u2^2 used instead of
+// recomputing them.  The exponentials run in base 2: S1, S2 are scaled by sqrt(log2 e),
+// so exp(-s^2) = exp2(-S^2) and erf's t = 1 / (1 + p s) takes p / sqrt(log2 e).
 __device__ inline Kin<float> kin_fast(const float qt[4], float nx, float ny) {
+  constexpr float LOG2E = 1.44269504f, SQRT_LOG2E = 1.20112241f;
   const float beta = -qt[3] * 0.5f;
-  const float tmp = 0.5f * frcp(beta);
+  const float rs = frsq(beta);
+  const float tmp = (0.5f * rs) * rs;  // 1 / (2 beta)
   const float u1 = qt[1] * tmp, u2 = qt[2] * tmp;
-  const float temp1 = qt[0] + beta * (u1 * u1 + u2 * u2);
-  const float rho = fexp(temp1 - flog(beta) * 2.5f);  // 1 / (gamma - 1) = 2.5
+  const float uu = u1 * u1 + u2 * u2;
+  const float temp1 = qt[0] + beta * uu;
+  // rho = exp(temp1 - log(beta) / (gamma - 1)), 1 / (gamma - 1) = 2.5
+  const float rho = __builtin_amdgcn_exp2f(temp1 * LOG2E - 2.5f * __builtin_amdgcn_logf(beta));
   Kin<float> k;
   k.ut = u1 * ny - u2 * nx;
   k.un = u1 * nx + u2 * ny;
-  const float rs = frsq(beta);
-  const float sb = beta * rs;
-  const float S1 = k.ut * sb, S2 = k.un * sb;
-  const float ex1 = fexp(-S1 * S1), ex2 = fexp(-S2 * S2);
+  const float sbl = (beta * rs) * SQRT_LOG2E;  // sqrt(beta log2 e)
+  const float S1 = k.ut * sbl, S2 = k.un * sbl;
+  const float ex1 = __builtin_amdgcn_exp2f(-S1 * S1), ex2 = __builtin_amdgcn_exp2f(-S2 * S2);
   const float half_inv_spb = 0.282094792f * rs;  // 0.5 / sqrt(pi beta)
   k.B1 = ex1 * half_inv_spb;
   k.B2 = ex2 * half_inv_spb;
-  k.e1 = ferf_from_exp(S1, ex1);
-  k.e2 = ferf_from_exp(S2, ex2);
+  k.e1 = ferf_from_exp(S1, ex1, 0.3275911f / SQRT_LOG2E);
+  k.e2 = ferf_from_exp(S2, ex2, 0.3275911f / SQRT_LOG2E);
   k.pr_by_rho = tmp;
-  k.u_sqr = k.ut * k.ut + k.un * k.un;
+  k.u_sqr = uu;
   k.rho = rho;
   return k;
-}
-
-// venkat for the mixed path.  The reference's ratio has the factor del_neg in both
-// numerator and denominator (limiters.F90:91-97); past the threshold del_neg != 0, so
-// it is divided out: phi = min(1, (p^2 + e^2 + 2 d p) / (p^2 + e^2 + d (2 d + p))) with
-// d = del_neg, p = del_pos; the denominator equals (p + d/2)^2 + 7/4 d^2 + e^2 > 0.  A
-// reciprocal replaces the division.
-__device__ inline float venkat_fast(float qt, float qk, float qmax, float qmin, float epsi) {
-  const float del_neg = qt - qk;
-  if (fabsf(del_neg) <= Consts<float>::THRESH) return 1.0f;
-  const float del_pos = (del_neg > 0.0f) ? qmax - qk : qmin - qk;
-  const float a = del_pos * del_pos + epsi * epsi;
-  const float num = a + 2.0f * del_neg * del_pos;
-  const float den = a + del_neg * (2.0f * del_neg + del_pos);
-  const float t = num * frcp(den);
-  return (t < 1.0f) ? t : 1.0f;
 }
 
 // The stencil kernels' entry points: FP64 keeps the reference's routines, the mixed
@@ -450,22 +440,59 @@ __host__ __device__ inline void reconstruct_eval(const V4d& q, const V4d& dx, co
                                                  double qt[4]) {
   reconstruct(q, dx, dy, qmax, qmin, eps, delx, dely, qt);
 }
+// The mixed path's limited reconstruction.  The reference's limiter ratio has the
+// factor del_neg in both numerator and denominator (limiters.F90:91-97); past the
+// threshold del_neg != 0, so it is divided out: phi = min(1, num / den) with
+// a = p^2 + e^2, num = a + 2 del_neg p, den = a + del_neg (2 del_neg + p), p = del_pos;
+// den = (p + del_neg/2)^2 + 7/4 del_neg^2 + e^2 > 0.  del_neg = qtilde - q is taken as
+// -d/2 exactly, so num = a - d p, den = a + del_neg (p - d) and qtilde = q + phi del_neg;
+// a reciprocal replaces the division.
 __device__ inline void reconstruct_eval(const V4<float>& q, const V4<float>& dx, const V4<float>& dy,
                                         const V4<float>& qmax, const V4<float>& qmin, float eps, float delx,
                                         float dely, float qt[4]) {
+  const float e2 = eps * eps;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const float d = delx * dx[r] + dely * dy[r];
-    const float phi = venkat_fast(q[r] - 0.5f * d, q[r], qmax[r], qmin[r], eps);
-    qt[r] = q[r] - 0.5f * phi * d;
+    const float dn = -0.5f * d;
+    const float p = (dn > 0.0f ? qmax[r] : qmin[r]) - q[r];
+    const float a = p * p + e2;
+    const float num = a - d * p;
+    const float den = a + dn * (p - d);
+    const float t = fminf(num * frcp(den), 1.0f);
+    const float phi = (fabsf(dn) <= Consts<float>::THRESH) ? 1.0f : t;
+    qt[r] = q[r] + phi * dn;
   }
+}
+
+// The mixed path's interior split fluxes (split_fluxes.F90:8-190 regrouped): with
+// rA = rho A and rB = +-rho B of a direction, Gx = (ut rA1 + rB1, (p/rho + ut^2) rA1 +
+// ut rB1, un Gx(1), ut (7/2 p/rho + u^2/2) rA1 + (3 p/rho + u^2/2) rB1), and Gy the same
+// with ut and un exchanged.  sx, sy = +1 / -1 for the + / - halves.
+__device__ inline void interior_fluxes_fast(const Kin<float>& k, float sx, float sy, float Gx[4], float Gy[4]) {
+  const float hu = 0.5f * k.u_sqr;
+  const float c7 = 3.5f * k.pr_by_rho + hu, c6 = 3.0f * k.pr_by_rho + hu;
+  const float rA1 = k.rho * (0.5f * sx * k.e1 + 0.5f);
+  const float rB1 = (sx * k.rho) * k.B1;
+  Gx[0] = k.ut * rA1 + rB1;
+  Gx[1] = (k.ut * k.ut + k.pr_by_rho) * rA1 + k.ut * rB1;
+  Gx[2] = k.un * Gx[0];
+  Gx[3] = (k.ut * c7) * rA1 + c6 * rB1;
+  const float rA2 = k.rho * (0.5f * sy * k.e2 + 0.5f);
+  const float rB2 = (sy * k.rho) * k.B2;
+  Gy[0] = k.un * rA2 + rB2;
+  Gy[2] = (k.un * k.un + k.pr_by_rho) * rA2 + k.un * rB2;
+  Gy[1] = k.ut * Gy[0];
+  Gy[3] = (k.un * c7) * rA2 + c6 * rB2;
 }
 
 // Split fluxes of one end for one edge: the x flux of the neighbour's x half and
 // the y flux of its y half.  `xp`/`yp` pick the + or - half.
 template <int CLS, class S>
 __host__ __device__ inline void end_fluxes(const Kin<S>& k, bool xp, bool yp, S Gx[4], S Gy[4]) {
-  if (CLS == INTERIOR) {
+  if constexpr (CLS == INTERIOR && sizeof(S) == 4) {
+    interior_fluxes_fast(k, xp ? 1.0f : -1.0f, yp ? 1.0f : -1.0f, Gx, Gy);
+  } else if (CLS == INTERIOR) {
     flux_gx(k, xp, Gx);
     flux_gy(k, yp, Gy);
   } else {
@@ -500,6 +527,24 @@ __host__ __device__ inline double coef_y(const HalfCoef& h, bool first, double d
   const double sxy = first ? h.sxy[2] : h.sxy[3];
   const double obd = first ? h.obd[2] : h.obd[3];
   return (dnw * c1 - dsw * sxy) * obd;
+}
+
+// The mixed path's form of the same coefficients: a = c1 * obd and b = sxy * obd per
+// half in FP32 (8 registers instead of 24), so an edge's coefficient is dsw * a - dnw * b
+// (x) or dnw * a - dsw * b (y).  Their rounding (~1e-7 relative) is that of the FP32
+// flux differences they multiply.
+struct HalfCoefF {
+  float a[4], b[4];
+};
+__device__ inline float coef_x(const HalfCoefF& h, bool first, float dsw, float dnw) {
+  const float a = first ? h.a[0] : h.a[1];
+  const float b = first ? h.b[0] : h.b[1];
+  return dsw * a - dnw * b;
+}
+__device__ inline float coef_y(const HalfCoefF& h, bool first, float dsw, float dnw) {
+  const float a = first ? h.a[2] : h.a[3];
+  const float b = first ? h.b[2] : h.b[3];
+  return dnw * a - dsw * b;
 }
 
 // Flux residual of point i (flux_residual.F90:14-73) for class CLS: the sum over the
@@ -626,32 +671,48 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
   const V4<S> dxi = st.dq[2 * i], dyi = st.dq[2 * i + 1];
   const V4<S> qmaxi = st.qm[2 * i], qmini = st.qm[2 * i + 1];
   const S epsi = st.eps[i];
-  HalfCoef hc;
+  // FP64: the reference's coefficient arithmetic; the mixed path: FP32 edge geometry
+  // and coefficients (HalfCoefF)
+  constexpr bool F32 = sizeof(S) == 4;
+  using R = std::conditional_t<F32, float, double>;
+  std::conditional_t<F32, HalfCoefF, HalfCoef> hc;
   {
     const V4d a = g.dg_c1[i], b = g.dg_sxy[i], c = g.dg_obd[i];
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
-      hc.c1[d] = a[d];
-      hc.sxy[d] = b[d];
-      hc.obd[d] = c[d];
+      if constexpr (F32) {
+        hc.a[d] = (float)(a[d] * c[d]);
+        hc.b[d] = (float)(b[d] * c[d]);
+      } else {
+        hc.c1[d] = a[d];
+        hc.sxy[d] = b[d];
+        hc.obd[d] = c[d];
+      }
     }
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   nbrs.walk([&](int k, unsigned m) {
     const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
-    const double dels = delx * tx + dely * ty;
-    const double deln = delx * nx + dely * ny;
-    double w = 1.0;
-    if (!POWER0) w = pow(sqrt(dels * dels + deln * deln), power);
-    const double dsw = dels * w, dnw = deln * w;
+    R dels, deln;
+    if constexpr (F32) {
+      const float fx = (float)delx, fy = (float)dely;
+      dels = fx * nys - fy * nxs;
+      deln = fx * nxs + fy * nys;
+    } else {
+      dels = delx * tx + dely * ty;
+      deln = delx * nx + dely * ny;
+    }
+    R w = 1.0;
+    if (!POWER0) w = (R)pow(sqrt((double)dels * dels + (double)deln * deln), power);
+    const R dsw = dels * w, dnw = deln * w;
     const bool xp = (m & XPOS) != 0u;
     const bool tie_x = xp && (m & XNEG) != 0u;
     // interior: y half from the mask; wall: yneg (slot 3); outer: ypos (slot 2)
     const bool yp = (CLS == INTERIOR) ? (m & YPOS) != 0u : (CLS == OUTER);
     const bool tie_y = (CLS == INTERIOR) && yp && (m & YNEG) != 0u;
-    const double cx = coef_x(hc, xp, dsw, dnw);
-    const double cy = coef_y(hc, yp, dsw, dnw);
+    const R cx = coef_x(hc, xp, dsw, dnw);
+    const R cy = coef_y(hc, yp, dsw, dnw);
 
     const S dx = (S)delx, dy = (S)dely;
     S qti[4], qtk[4];
@@ -669,9 +730,15 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
       const V4<S> qmaxk = nbrs.qmax(k), qmink = nbrs.qmin(k);
       reconstruct_eval(qk, dxk, dyk, qmaxk, qmink, nbrs.eps(k), dx, dy, qtk);
     }
-    // the two ends one after the other: one end's transcendental temporaries live
-    // at a time
     S gxi[4], gyi[4], gxk[4], gyk[4];
+    if constexpr (sizeof(S) == 4) {
+      // the mixed path's short sequences: both ends in straight-line code (a loop
+      // over the ends moves its inputs and results between iterations)
+      end_fluxes<CLS>(kin_eval(qti, nxs, nys), xp, yp, gxi, gyi);
+      end_fluxes<CLS>(kin_eval(qtk, nxs, nys), xp, yp, gxk, gyk);
+    } else {
+    // FP64: the two ends one after the other, so one end's library-transcendental
+    // temporaries live at a time
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
       S qt[4];
@@ -689,6 +756,7 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
           gyi[r] = Gy[r];
         }
       }
+    }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
