@@ -48,7 +48,9 @@ struct Halo {
   const int32_t* ptr = nullptr;
   const int32_t* ids = nullptr;
   const uint16_t* loc = nullptr;
-  int32_t hmax = 0;    // largest halo, rounded up to a multiple of 8 (0: not available)
+  // halo entries a block holds in LDS, a multiple of 8 (0: staging not available).
+  // A block whose halo is larger reads its entries past hcap from global memory.
+  int32_t hcap = 0;
   int32_t stride = 0;  // N rounded up to a multiple of P
 };
 
@@ -192,11 +194,18 @@ __device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, in
 // L2 once per block instead of once per edge (about 2.8 instead of 23.5 record loads
 // per point at 64 points per block), and the walk's loads are LDS reads.  A lane's
 // slots and their order are those of the global walk, so its sums are bit-identical.
-template <int P, int LPP, class Fill, class Use>
+//
+// LDS holds the first h.hcap halo entries.  The few blocks with larger halos (at 64
+// points per block, those of wall and outer points, whose neighbours lie along a curve
+// rather than around the block) read their entries past it from global memory: the
+// slot then goes to use_global(k, m, s) with k the neighbour's point id.  The check is
+// made per block, so the other blocks walk exactly as before.
+template <int P, int LPP, class Fill, class Use, class UseGlobal>
 __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int b, int nb, uint16_t* s_cl,
-                                               Fill&& fill, Use&& use) {
+                                               Fill&& fill, Use&& use, UseGlobal&& use_global) {
   const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
-  for (int j = threadIdx.x; j < hn; j += P * LPP) fill(j, h.ids[h0 + j]);
+  const int hl = hn < h.hcap ? hn : h.hcap;
+  for (int j = threadIdx.x; j < hl; j += P * LPP) fill(j, h.ids[h0 + j]);
   // the block's slot words: per slot a P * 2-byte row segment in 16-byte loads
   constexpr int V = P * 2 / 16;
   const uint4* src = (const uint4*)(h.loc + (size_t)b * P);
@@ -206,9 +215,18 @@ __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int
   }
   __syncthreads();
   const int li = threadIdx.x / LPP;
-  for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
-    const unsigned e = s_cl[s * P + li];
-    use((int)(e & 0xFFFu), e >> 12, s);
+  if (hn <= h.hcap) {
+    for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
+      const unsigned e = s_cl[s * P + li];
+      use((int)(e & 0xFFFu), e >> 12, s);
+    }
+  } else {
+    for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
+      const unsigned e = s_cl[s * P + li];
+      const int j = (int)(e & 0xFFFu);
+      if (j < h.hcap) use(j, e >> 12, s);
+      else use_global(h.ids[h0 + j], e >> 12, s);
+    }
   }
 }
 
@@ -631,7 +649,7 @@ template <class S, int LPP>
 __device__ inline void stage_flux_halo(const Geom& g, const State<S>& st, int b, unsigned char* smem,
                                        HaloRecs<S>& r, uint16_t*& s_cl) {
   const Halo& h = g.hl[1];
-  const int H = h.hmax;
+  const int H = h.hcap;
   r.q = (V4<S>*)smem;
   r.dqx = r.q + H;
   r.dqy = r.dqx + H;

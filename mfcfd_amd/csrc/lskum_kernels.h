@@ -62,22 +62,22 @@ struct PointMap {
 
 // LDS bytes of the staged kernels: per halo entry the records the kernel stages, plus the
 // block's slot words (uint16 [W][P])
-inline size_t stage_lds_lsq(int hmax, int width, size_t ssz, bool delta) {
-  return (size_t)hmax * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P * 2;
+inline size_t stage_lds_lsq(int hcap, int width, size_t ssz, bool delta) {
+  return (size_t)hcap * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P * 2;
 }
-inline size_t stage_lds_inner(int hmax, int width, size_t ssz, bool sums) {
-  return (size_t)hmax * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+inline size_t stage_lds_inner(int hcap, int width, size_t ssz, bool sums) {
+  return (size_t)hcap * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
 }
 // the adjoint's LSQ reverse (FP64): q and xy per halo entry
-inline size_t stage_lds_lsq_rev(int hmax, int width) {
-  return (size_t)hmax * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+inline size_t stage_lds_lsq_rev(int hcap, int width) {
+  return (size_t)hcap * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
 }
 // the adjoint's inner reverse (FP64): q, dq_x, dq_y and xy per halo entry
-inline size_t stage_lds_inner_rev(int hmax, int width) {
-  return (size_t)hmax * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+inline size_t stage_lds_inner_rev(int hcap, int width) {
+  return (size_t)hcap * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
 }
-inline size_t stage_lds_flux(int hmax, int width, size_t ssz) {
-  return (size_t)hmax * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
+inline size_t stage_lds_flux(int hcap, int width, size_t ssz) {
+  return (size_t)hcap * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
 }
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   };
   if constexpr (STAGED) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hl[0].hmax;
+    const int H = g.hl[0].hcap;
     V4<S>* s_q = (V4<S>*)smem;
     D2* s_xy = (D2*)(s_q + H);
     double* s_c = (double*)(s_xy + H);
@@ -142,7 +142,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
           s_xy[j] = g.xy[k];
           if (DELTA) s_c[j] = cspeed[k];
         },
-        [&](int j, unsigned, int) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); });
+        [&](int j, unsigned, int) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); },
+        [&](int k, unsigned, int) { edge(q[k], g.xy[k], DELTA ? cspeed[k] : 0.0, true); });
     if (!pm.live) return;
   } else {
     struct Nbr {
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   };
   if constexpr (STAGED) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hl[0].hmax;
+    const int H = g.hl[0].hcap;
     V4<S>* s_dx = (V4<S>*)smem;
     V4<S>* s_dy = s_dx + H;
     V4<S>* s_q = s_dy + H;  // FP64 form only
@@ -259,7 +260,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
           if (!SUMS) s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); });
+        [&](int j, unsigned, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); },
+        [&](int k, unsigned, int) { edge(g.xy[k], SUMS ? qi : q[k], dq_in[2 * k], dq_in[2 * k + 1], true); });
     if (!pm.live) return;
   } else if constexpr (SUMS) {
     struct Nbr {
@@ -535,7 +537,7 @@ void launch_lsq_env(const LaunchEnv& e, const V4<S>* q, V4<S>* dq, V4<S>* qm, co
   const double cfl = e.cfl, pw = e.power;
   if (lpp & LPP_STAGED) {
     const dim3 gr = stage_grid(e.n);
-    const size_t lds = stage_lds_lsq(e.g.hl[0].hmax, e.g.width, sizeof(S), with_delta);
+    const size_t lds = stage_lds_lsq(e.g.hl[0].hcap, e.g.width, sizeof(S), with_delta);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       const dim3 bl(STAGE_P * L);
       if (e.power0 && with_delta)
@@ -571,7 +573,7 @@ void launch_inner_env(const LaunchEnv& e, const V4<S>* q, const V4<S>* a, V4<S>*
   constexpr bool SUMS = std::is_same<S, float>::value;
   if (lpp & LPP_STAGED) {
     const dim3 gr = stage_grid(e.n);
-    const size_t lds = stage_lds_inner(e.g.hl[0].hmax, e.g.width, sizeof(S), SUMS);
+    const size_t lds = stage_lds_inner(e.g.hl[0].hcap, e.g.width, sizeof(S), SUMS);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       const dim3 bl(STAGE_P * L);
       if (e.power0) hipLaunchKernelGGL((k_inner<S, true, L, SUMS, true>), gr, bl, lds, e.stream, e.g, q, a, b, pw, ctl, qsum);
@@ -598,7 +600,7 @@ void launch_flux_update_env(const LaunchEnv& e, const State<S>& st, const V4<S>*
   const Freestream fs = e.fs;
   if (lpp & LPP_STAGED) {
     const dim3 gr((unsigned)((e.n + STAGE_PF - 1) / STAGE_PF));
-    const size_t lds = stage_lds_flux(e.g.hl[1].hmax, e.g.width, sizeof(S));
+    const size_t lds = stage_lds_flux(e.g.hl[1].hcap, e.g.width, sizeof(S));
     auto go = [&]<int L>() {
       const dim3 bl(STAGE_PF * L);
       if (e.power0)

@@ -391,16 +391,20 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 // Halo of the staged stencil walks in blocks of P consecutive (internal) points: each
 // block's own points, then every other neighbour of them once (ascending), and per slot
 // its position in that list with the slot's half-stencil bits above it.  Leaves
-// h.hmax = 0 (staging unavailable) when a halo outgrows the 12-bit position.
+// h.hcap = 0 (staging unavailable) when a halo outgrows the 12-bit position.
+// capped: LDS holds the 99th percentile of the blocks' halo sizes rather than the
+// largest (the largest belong to the few blocks of wall and outer points), so more
+// blocks fit on a CU; the larger halos' remaining entries are read from global memory.
 static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh, int P,
-                      Halo& h) {
+                      bool capped, Halo& h) {
   const int n = c->n, W = c->W;
   const int nblk = (n + P - 1) / P;
   const int stride = nblk * P;
   std::vector<int32_t> ptr(nblk + 1, 0), ids;
   std::vector<uint16_t> loc((size_t)W * stride, 0);
   ids.reserve((size_t)n * 4);
-  std::vector<int32_t> other;
+  std::vector<int32_t> other, sizes;
+  sizes.reserve(nblk);
   int hmax = 0;
   h = Halo{};
   for (int b = 0; b < nblk; ++b) {
@@ -416,6 +420,7 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
     const int own = p1 - p0, hn = own + (int)other.size();
     if (hn > 4096 - 8) return MF_OK;
     hmax = std::max(hmax, hn);
+    sizes.push_back(hn);
     for (int i = p0; i < p1; ++i) ids.push_back(i);
     ids.insert(ids.end(), other.begin(), other.end());
     for (int i = p0; i < p1; ++i)
@@ -436,7 +441,12 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
   h.ptr = d_ptr;
   h.ids = d_ids;
   h.loc = d_loc;
-  h.hmax = (hmax + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
+  int cap = hmax;
+  if (capped) {
+    std::sort(sizes.begin(), sizes.end());
+    cap = sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))];
+  }
+  h.hcap = (cap + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
   h.stride = stride;
   return MF_OK;
 }
@@ -444,7 +454,10 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
 static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
   c->g.width = c->W;
   int rc;
-  if ((rc = build_halo(c, conn, nbh, STAGE_P, c->g.hl[0])) || (rc = build_halo(c, conn, nbh, STAGE_PF, c->g.hl[1])))
+  // the flux kernel's halos stay whole: its staging reads every entry from LDS, and its
+  // occupancy is set by registers, not LDS
+  if ((rc = build_halo(c, conn, nbh, STAGE_P, true, c->g.hl[0])) ||
+      (rc = build_halo(c, conn, nbh, STAGE_PF, false, c->g.hl[1])))
     return rc;
   return MF_OK;
 }
@@ -1079,11 +1092,11 @@ static bool valid_shape(const mf_ctx* c, int kind, int v) {
   if (!(v & LPP_STAGED)) return false;
   const int l = v & ~LPP_STAGED;
   const Halo& h = c->g.hl[kind == 2 ? 1 : 0];
-  if (h.hmax <= 0) return false;
+  if (h.hcap <= 0) return false;
   if (kind == 2 ? (l != 2 && l != 4 && l != 8) : (l != 1 && l != 2 && l != 4)) return false;
-  const size_t lds = kind == 0   ? stage_lds_lsq(h.hmax, c->g.width, c->ssz, true)
-                     : kind == 1 ? stage_lds_inner(h.hmax, c->g.width, c->ssz, c->ssz == 4)
-                                 : stage_lds_flux(h.hmax, c->g.width, c->ssz);
+  const size_t lds = kind == 0   ? stage_lds_lsq(h.hcap, c->g.width, c->ssz, true)
+                     : kind == 1 ? stage_lds_inner(h.hcap, c->g.width, c->ssz, c->ssz == 4)
+                                 : stage_lds_flux(h.hcap, c->g.width, c->ssz);
   return lds <= kMaxStageLds;
 }
 
@@ -1091,9 +1104,9 @@ static bool valid_shape(const mf_ctx* c, int kind, int v) {
 // or 4 lanes, 64-point blocks, halo in LDS
 static bool valid_rev_staged(const mf_ctx* c, int kind, int lpp) {
   const int l = lpp & ~LPP_STAGED;
-  if (!(lpp & LPP_STAGED) || (l != 1 && l != 2 && l != 4) || c->g.hl[0].hmax <= 0) return false;
-  const size_t lds = kind == 1 ? stage_lds_lsq_rev(c->g.hl[0].hmax, c->g.width)
-                               : stage_lds_inner_rev(c->g.hl[0].hmax, c->g.width);
+  if (!(lpp & LPP_STAGED) || (l != 1 && l != 2 && l != 4) || c->g.hl[0].hcap <= 0) return false;
+  const size_t lds = kind == 1 ? stage_lds_lsq_rev(c->g.hl[0].hcap, c->g.width)
+                               : stage_lds_inner_rev(c->g.hl[0].hcap, c->g.width);
   return lds <= kMaxStageLds;
 }
 

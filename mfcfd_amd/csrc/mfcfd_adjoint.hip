@@ -431,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   if constexpr (STAGED) {
     // the block's halo (q, dq_in, xy) in LDS
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hl[0].hmax;
+    const int H = g.hl[0].hcap;
     V4d* s_q = (V4d*)smem;
     V4d* s_dx = s_q + H;
     V4d* s_dy = s_dx + H;
@@ -445,7 +445,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
           s_dy[j] = dq_in[2 * k + 1];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j], s_dx[j], s_dy[j]); });
+        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j], s_dx[j], s_dy[j]); },
+        [&](int k, unsigned, int s) { edge(s, g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]); });
     if (!pm.live) return;
   } else {
     for (int s = lane; s < nb; s += LPP) {
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   if constexpr (STAGED) {
     // the block's halo (q, xy) in LDS
     extern __shared__ __align__(16) unsigned char smem[];
-    const int H = g.hl[0].hmax;
+    const int H = g.hl[0].hcap;
     V4d* s_q = (V4d*)smem;
     D2* s_xy = (D2*)(s_q + H);
     uint16_t* s_cl = (uint16_t*)(s_xy + H);
@@ -556,7 +557,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
           s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j]); });
+        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j]); },
+        [&](int k, unsigned, int s) { edge(s, g.xy[k], q[k]); });
     if (!pm.live) return;
   } else {
     for (int s = lane; s < nb; s += LPP) {
@@ -954,7 +956,7 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
   if (lpp & LPP_STAGED) {
     // staged: blocks of STAGE_PF points, 2, 4 or 8 lanes each
     const dim3 gr((unsigned)((c->n + STAGE_PF - 1) / STAGE_PF));
-    const size_t lds = stage_lds_flux(c->g.hl[1].hmax, c->g.width, sizeof(double));
+    const size_t lds = stage_lds_flux(c->g.hl[1].hcap, c->g.width, sizeof(double));
     auto go = [&]<int L>() {
       hipLaunchKernelGGL((k_flux_rev<L, true>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
                          (const double*)A.tdelta, (const V4d*)A.frb, o);
@@ -987,7 +989,7 @@ static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
   const V4d* din = (const V4d*)A.tdq[s * (A.I + 1) + j - 1];
   const V4d* dout = (const V4d*)A.tdq[s * (A.I + 1) + j];
   if (lpp & LPP_STAGED) {
-    const size_t lds = stage_lds_inner_rev(c->g.hl[0].hmax, c->g.width);
+    const size_t lds = stage_lds_inner_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       hipLaunchKernelGGL((k_inner_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, din,
                          dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
@@ -1013,7 +1015,7 @@ static void launch_lsq_rev(mf_ctx* c, AdjState& A, int s, int lpp) {
   const V4d* q = (const V4d*)A.tq[s];
   const V4d* dq0 = (const V4d*)A.tdq[s * (A.I + 1)];
   if (lpp & LPP_STAGED) {
-    const size_t lds = stage_lds_lsq_rev(c->g.hl[0].hmax, c->g.width);
+    const size_t lds = stage_lds_lsq_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       hipLaunchKernelGGL((k_lsq_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, dq0,
                          (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos);

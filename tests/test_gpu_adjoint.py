@@ -201,3 +201,20 @@ def test_adjoint_staged_reverse(small_cloud, kernel, lpp):
     assert sa[key] == lpp and sb[key] == 16 | lpp
     ref = np.concatenate([xa, ya])
     assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("kernel", ["inner", "lsq"])
+def test_adjoint_staged_reverse_overflow(c1_cloud, kernel):
+    """On the 9,600-point cloud the largest 64-point halo (529 entries, a block of wall
+    points) exceeds the LDS capacity (the 99th percentile of the halo sizes, 504): that
+    block reads its entries past the capacity from global memory.  The staged reverses
+    still give the global walk's gradient."""
+    key = {"inner": "lpp_inner_rev", "lsq": "lpp_lsq_rev"}[kernel]
+    pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
+    with _gpu(c1_cloud) as a, _gpu(c1_cloud) as b:
+        a.tuning = {**pin, key: 2}
+        b.tuning = {**pin, key: 16 | 2}
+        Ja, xa, ya, _ = a.adjoint(6, snaps=2)
+        Jb, xb, yb, _ = b.adjoint(6, snaps=2)
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
