@@ -12,13 +12,15 @@
 //   eps            S       [N]             Venkatakrishnan epsilon (VL_CONST*min_dist)**3
 //   conn           uint32  [W][N]          slot-major: (neighbour << 4) | direction bits
 //   lsq_geo        double4 [N]             (sum dx^2, sum dy^2, sum dx dy, 1/det) of the
-//                                          full stencil, point-fixed geometry
+//                                          full stencil, point-fixed geometry (mixed
+//                                          path: float4 lsq_geo_f, the solve's products)
 //   dg_c1/sxy/obd  double4 [N]             per half-stencil (xpos, xneg, ypos, yneg) the
 //                                          2x2 solve coefficients of the flux derivative
 //   prim, q        V4<S>  [N]
 //   dq, qm         V4<S>  [2N]             dq: [2i] d/dx, [2i+1] d/dy; qm: [2i] max, [2i+1] min
 //   halos (Halo)   per block of 64 (LSQ, inner) or 32 (flux) points: the halo's point
-//                  ids and uint16 [W][N] slot words (halo position | half-stencil bits)
+//                  ids and [W][N] slot words (uint8 halo positions in 64-point blocks;
+//                  uint16 position | half-stencil bits in the flux kernel's)
 //
 // S is double for the FP64 path and float for the mixed path; least-squares sums and
 // solves, flux-derivative accumulations, residual norms and force sums are double in
@@ -47,7 +49,8 @@ constexpr int LPP_STAGED = 16;
 struct Halo {
   const int32_t* ptr = nullptr;
   const int32_t* ids = nullptr;
-  const uint16_t* loc = nullptr;
+  const uint16_t* loc = nullptr;  // [W][stride] position | half-stencil bits << 12
+  const uint8_t* loc8 = nullptr;  // [W][stride] position, 255 past hcap (capped halos)
   // halo entries a block holds in LDS, a multiple of 8 (0: staging not available).
   // A block whose halo is larger reads its entries past hcap from global memory.
   int32_t hcap = 0;
@@ -111,6 +114,9 @@ struct Geom {
   // staged walks: hl[0] in blocks of STAGE_P points, hl[1] in blocks of STAGE_PF
   Halo hl[2];
   int32_t width = 0;  // W, slots per point of conn and of the halos' loc
+  // mixed path: lsq_geo as the FP32 solve coefficients (sum dy^2 / det, sum dx dy / det,
+  // sum dx^2 / det): d/dx = a Sx - b Sy, d/dy = c Sy - b Sx for the sums Sx, Sy
+  const V4<float>* lsq_geo_f = nullptr;
 };
 
 struct Ctl {
@@ -188,44 +194,42 @@ __device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, in
 
 // The staged form of a stencil walk.  A block holds P consecutive points, LPP lanes
 // each, and first copies its halo (fill(j, k): point k's records into the caller's LDS
-// arrays at halo position j) and its points' slot words into LDS; after one barrier
-// every lane walks its slots (lane, lane + LPP, ...) with use(j, m, s) reading halo
-// position j, m the slot's half-stencil bits, s the slot.  Each neighbour record then crosses from
-// L2 once per block instead of once per edge (about 2.8 instead of 23.5 record loads
-// per point at 64 points per block), and the walk's loads are LDS reads.  A lane's
-// slots and their order are those of the global walk, so its sums are bit-identical.
+// arrays at halo position j) and its points' 8-bit slot words into LDS; after one
+// barrier every lane walks its slots (lane, lane + LPP, ...) with use(j, s) reading halo
+// position j for slot s.  Each neighbour record then crosses from L2 once per block
+// instead of once per edge (about 2.8 instead of 23.5 record loads per point at 64
+// points per block), and the walk's loads are LDS reads.  A lane's slots and their order
+// are those of the global walk, so its sums are bit-identical.
 //
 // LDS holds the first h.hcap halo entries.  The few blocks with larger halos (at 64
 // points per block, those of wall and outer points, whose neighbours lie along a curve
-// rather than around the block) read their entries past it from global memory: the
-// slot then goes to use_global(k, m, s) with k the neighbour's point id.  The check is
-// made per block, so the other blocks walk exactly as before.
+// rather than around the block) read the neighbours past it from global memory: such a
+// slot's word is 255 and it goes to use_global(k, s) with k the neighbour's point id from
+// the point's connectivity.  The check is made per block, so the other blocks walk
+// exactly as before.
 template <int P, int LPP, class Fill, class Use, class UseGlobal>
-__device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int b, int nb, uint16_t* s_cl,
+__device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int b, int nb, uint8_t* s_cl,
                                                Fill&& fill, Use&& use, UseGlobal&& use_global) {
   const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
   const int hl = hn < h.hcap ? hn : h.hcap;
   for (int j = threadIdx.x; j < hl; j += P * LPP) fill(j, h.ids[h0 + j]);
-  // the block's slot words: per slot a P * 2-byte row segment in 16-byte loads
-  constexpr int V = P * 2 / 16;
-  const uint4* src = (const uint4*)(h.loc + (size_t)b * P);
+  // the block's slot words: per slot a P-byte row segment in 16-byte loads
+  constexpr int V = P / 16;
+  const uint4* src = (const uint4*)(h.loc8 + (size_t)b * P);
   for (int j = threadIdx.x; j < g.width * V; j += P * LPP) {
     const int s = j / V;
-    ((uint4*)s_cl)[j] = src[(size_t)s * (h.stride / 8) + (j - s * V)];
+    ((uint4*)s_cl)[j] = src[(size_t)s * (h.stride / 16) + (j - s * V)];
   }
   __syncthreads();
   const int li = threadIdx.x / LPP;
   if (hn <= h.hcap) {
-    for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
-      const unsigned e = s_cl[s * P + li];
-      use((int)(e & 0xFFFu), e >> 12, s);
-    }
+    for (int s = threadIdx.x % LPP; s < nb; s += LPP) use((int)s_cl[s * P + li], s);
   } else {
+    const int i = b * P + li;
     for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
-      const unsigned e = s_cl[s * P + li];
-      const int j = (int)(e & 0xFFFu);
-      if (j < h.hcap) use(j, e >> 12, s);
-      else use_global(h.ids[h0 + j], e >> 12, s);
+      const int j = s_cl[s * P + li];
+      if (j < h.hcap) use(j, s);
+      else use_global((int)(g.conn[(size_t)s * g.n + i] >> 4), s);
     }
   }
 }

@@ -61,23 +61,47 @@ struct PointMap {
 };
 
 // LDS bytes of the staged kernels: per halo entry the records the kernel stages, plus the
-// block's slot words (uint16 [W][P])
+// block's slot words (uint8 [W][P] in 64-point blocks, uint16 [W][P] in the flux kernel's)
 inline size_t stage_lds_lsq(int hcap, int width, size_t ssz, bool delta) {
-  return (size_t)hcap * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P * 2;
+  return (size_t)hcap * (4 * ssz + sizeof(D2) + (delta ? sizeof(double) : 0)) + (size_t)width * STAGE_P;
 }
 inline size_t stage_lds_inner(int hcap, int width, size_t ssz, bool sums) {
-  return (size_t)hcap * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+  return (size_t)hcap * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P;
 }
 // the adjoint's LSQ reverse (FP64): q and xy per halo entry
 inline size_t stage_lds_lsq_rev(int hcap, int width) {
-  return (size_t)hcap * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+  return (size_t)hcap * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P;
 }
 // the adjoint's inner reverse (FP64): q, dq_x, dq_y and xy per halo entry
 inline size_t stage_lds_inner_rev(int hcap, int width) {
-  return (size_t)hcap * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P * 2;
+  return (size_t)hcap * (3 * 4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P;
 }
 inline size_t stage_lds_flux(int hcap, int width, size_t ssz) {
   return (size_t)hcap * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
+}
+
+// The 2x2 least-squares solve of point i (q_variables.F90:104-110 and :198-208):
+// d/dx = (Sx sum dy^2 - Sy sum dx dy) / det, d/dy = (Sy sum dx^2 - Sx sum dx dy) / det
+// from the edge sums Sx, Sy.  FP64: the reference's operations; mixed path: the FP32
+// products with 1/det of lsq_geo_f.
+template <class S>
+__device__ inline void lsq_solve(const Geom& g, int i, const double sx[4], const double sy[4], V4<S>& ox,
+                                 V4<S>& oy) {
+  if constexpr (sizeof(S) == 4) {
+    const V4<float> f = g.lsq_geo_f[i];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ox[r] = (float)(sx[r] * f.x - sy[r] * f.y);
+      oy[r] = (float)(sy[r] * f.z - sx[r] * f.y);
+    }
+  } else {
+    const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ox[r] = (sx[r] * gg.y - sy[r] * gg.z) * gg.w;
+      oy[r] = (sy[r] * gg.x - sx[r] * gg.z) * gg.w;
+    }
+  }
 }
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
@@ -134,7 +158,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
     V4<S>* s_q = (V4<S>*)smem;
     D2* s_xy = (D2*)(s_q + H);
     double* s_c = (double*)(s_xy + H);
-    uint16_t* s_cl = (uint16_t*)(s_c + (DELTA ? H : 0));
+    uint8_t* s_cl = (uint8_t*)(s_c + (DELTA ? H : 0));
     stage_and_walk<STAGE_P, LPP>(
         g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
@@ -142,8 +166,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
           s_xy[j] = g.xy[k];
           if (DELTA) s_c[j] = cspeed[k];
         },
-        [&](int j, unsigned, int) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); },
-        [&](int k, unsigned, int) { edge(q[k], g.xy[k], DELTA ? cspeed[k] : 0.0, true); });
+        [&](int j, int) { edge(s_q[j], s_xy[j], DELTA ? s_c[j] : 0.0, true); },
+        [&](int k, int) { edge(q[k], g.xy[k], DELTA ? cspeed[k] : 0.0, true); });
     if (!pm.live) return;
   } else {
     struct Nbr {
@@ -170,13 +194,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
     if (DELTA) min_delt = group_min<LPP>(min_delt);
     if (lane != 0) return;
   }
-  const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
   V4<S> ox, oy;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    ox[r] = (S)((sxq[r] * gg.y - syq[r] * gg.z) * gg.w);
-    oy[r] = (S)((syq[r] * gg.x - sxq[r] * gg.z) * gg.w);
-  }
+  lsq_solve(g, i, sxq, syq, ox, oy);
   dq[2 * i] = ox;
   dq[2 * i + 1] = oy;
   qm[2 * i] = qmax;
@@ -251,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
     V4<S>* s_dy = s_dx + H;
     V4<S>* s_q = s_dy + H;  // FP64 form only
     D2* s_xy = (D2*)(s_q + (SUMS ? 0 : H));
-    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    uint8_t* s_cl = (uint8_t*)(s_xy + H);
     stage_and_walk<STAGE_P, LPP>(
         g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
@@ -260,8 +279,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
           if (!SUMS) s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); },
-        [&](int k, unsigned, int) { edge(g.xy[k], SUMS ? qi : q[k], dq_in[2 * k], dq_in[2 * k + 1], true); });
+        [&](int j, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); },
+        [&](int k, int) { edge(g.xy[k], SUMS ? qi : q[k], dq_in[2 * k], dq_in[2 * k + 1], true); });
     if (!pm.live) return;
   } else if constexpr (SUMS) {
     struct Nbr {
@@ -296,13 +315,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
     }
     if (lane != 0) return;
   }
-  const V4d gg = g.lsq_geo[i];
   V4<S> ox, oy;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    ox[r] = (S)(gg.w * (sxq[r] * gg.y - syq[r] * gg.z));
-    oy[r] = (S)(gg.w * (syq[r] * gg.x - sxq[r] * gg.z));
-  }
+  lsq_solve(g, i, sxq, syq, ox, oy);
   dq_out[2 * i] = ox;
   dq_out[2 * i + 1] = oy;
 }

@@ -392,9 +392,13 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 // block's own points, then every other neighbour of them once (ascending), and per slot
 // its position in that list with the slot's half-stencil bits above it.  Leaves
 // h.hcap = 0 (staging unavailable) when a halo outgrows the 12-bit position.
-// capped: LDS holds the 99th percentile of the blocks' halo sizes rather than the
-// largest (the largest belong to the few blocks of wall and outer points), so more
-// blocks fit on a CU; the larger halos' remaining entries are read from global memory.
+// capped (the 64-point halos of the LSQ and inner kernels and their reverses): LDS holds
+// the 99th percentile of the blocks' halo sizes, at most 248, rather than the largest
+// (the largest belong to the few blocks of wall and outer points), so more blocks fit
+// on a CU; the slot words are 8-bit positions (loc8), with 255 for a position past
+// the capacity, whose neighbour the kernel then reads through the point's own
+// connectivity.  Otherwise (the flux kernel's 32-point halos) every entry is in LDS and
+// the 16-bit slot words (loc) carry the half-stencil bits.
 static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh, int P,
                       bool capped, Halo& h) {
   const int n = c->n, W = c->W;
@@ -434,19 +438,32 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
       }
     ptr[b + 1] = (int32_t)ids.size();
   }
-  int32_t *d_ptr, *d_ids;
-  uint16_t* d_loc;
-  int rc;
-  if ((rc = upload(c, &d_ptr, ptr)) || (rc = upload(c, &d_ids, ids)) || (rc = upload(c, &d_loc, loc))) return rc;
-  h.ptr = d_ptr;
-  h.ids = d_ids;
-  h.loc = d_loc;
   int cap = hmax;
   if (capped) {
     std::sort(sizes.begin(), sizes.end());
-    cap = sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))];
+    cap = std::min(sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))], 248);
   }
-  h.hcap = (cap + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
+  const int hcap = (cap + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
+  int32_t *d_ptr, *d_ids;
+  int rc;
+  if ((rc = upload(c, &d_ptr, ptr)) || (rc = upload(c, &d_ids, ids))) return rc;
+  if (capped) {
+    std::vector<uint8_t> loc8(loc.size());
+    for (size_t e = 0; e < loc.size(); ++e) {
+      const int pos = loc[e] & 0xFFF;
+      loc8[e] = (uint8_t)(pos < hcap ? pos : 255);
+    }
+    uint8_t* d_loc8;
+    if ((rc = upload(c, &d_loc8, loc8))) return rc;
+    h.loc8 = d_loc8;
+  } else {
+    uint16_t* d_loc;
+    if ((rc = upload(c, &d_loc, loc))) return rc;
+    h.loc = d_loc;
+  }
+  h.ptr = d_ptr;
+  h.ids = d_ids;
+  h.hcap = hcap;
   h.stride = stride;
   return MF_OK;
 }
@@ -712,6 +729,15 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
     return rc;
   c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no, prm->first_order ? 1 : 0};
   if ((rc = build_halos(c, conn_d, nbh_d))) return rc;
+  if (c->ssz == 4) {
+    // the mixed path's solve coefficients (sum dy^2, sum dx dy, sum dx^2) / det in FP32
+    std::vector<V4<float>> lf(n);
+    for (int j = 0; j < n; ++j)
+      lf[j] = V4<float>{(float)(lsq[j].y * lsq[j].w), (float)(lsq[j].z * lsq[j].w), (float)(lsq[j].x * lsq[j].w), 0.0f};
+    V4<float>* d_lf;
+    if ((rc = upload(c, &d_lf, lf))) return rc;
+    c->g.lsq_geo_f = d_lf;
+  }
   if (c->ssz == 8) {
     double* e;
     if ((rc = upload(c, &e, eps_d))) return rc;
@@ -1094,6 +1120,7 @@ static bool valid_shape(const mf_ctx* c, int kind, int v) {
   const Halo& h = c->g.hl[kind == 2 ? 1 : 0];
   if (h.hcap <= 0) return false;
   if (kind == 2 ? (l != 2 && l != 4 && l != 8) : (l != 1 && l != 2 && l != 4)) return false;
+  if ((kind == 2 ? STAGE_PF : STAGE_P) * l > BLOCK) return false;
   const size_t lds = kind == 0   ? stage_lds_lsq(h.hcap, c->g.width, c->ssz, true)
                      : kind == 1 ? stage_lds_inner(h.hcap, c->g.width, c->ssz, c->ssz == 4)
                                  : stage_lds_flux(h.hcap, c->g.width, c->ssz);
@@ -1105,6 +1132,7 @@ static bool valid_shape(const mf_ctx* c, int kind, int v) {
 static bool valid_rev_staged(const mf_ctx* c, int kind, int lpp) {
   const int l = lpp & ~LPP_STAGED;
   if (!(lpp & LPP_STAGED) || (l != 1 && l != 2 && l != 4) || c->g.hl[0].hcap <= 0) return false;
+  if (STAGE_P * l > BLOCK) return false;
   const size_t lds = kind == 1 ? stage_lds_lsq_rev(c->g.hl[0].hcap, c->g.width)
                                : stage_lds_inner_rev(c->g.hl[0].hcap, c->g.width);
   return lds <= kMaxStageLds;

@@ -436,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
     V4d* s_dx = s_q + H;
     V4d* s_dy = s_dx + H;
     D2* s_xy = (D2*)(s_dy + H);
-    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    uint8_t* s_cl = (uint8_t*)(s_xy + H);
     stage_and_walk<STAGE_P, LPP>(
         g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
@@ -445,8 +445,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
           s_dy[j] = dq_in[2 * k + 1];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j], s_dx[j], s_dy[j]); },
-        [&](int k, unsigned, int s) { edge(s, g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]); });
+        [&](int j, int s) { edge(s, s_xy[j], s_q[j], s_dx[j], s_dy[j]); },
+        [&](int k, int s) { edge(s, g.xy[k], q[k], dq_in[2 * k], dq_in[2 * k + 1]); });
     if (!pm.live) return;
   } else {
     for (int s = lane; s < nb; s += LPP) {
@@ -550,15 +550,15 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
     const int H = g.hl[0].hcap;
     V4d* s_q = (V4d*)smem;
     D2* s_xy = (D2*)(s_q + H);
-    uint16_t* s_cl = (uint16_t*)(s_xy + H);
+    uint8_t* s_cl = (uint8_t*)(s_xy + H);
     stage_and_walk<STAGE_P, LPP>(
         g, g.hl[0], pm.b, nb, s_cl,
         [&](int j, int k) {
           s_q[j] = q[k];
           s_xy[j] = g.xy[k];
         },
-        [&](int j, unsigned, int s) { edge(s, s_xy[j], s_q[j]); },
-        [&](int k, unsigned, int s) { edge(s, g.xy[k], q[k]); });
+        [&](int j, int s) { edge(s, s_xy[j], s_q[j]); },
+        [&](int k, int s) { edge(s, g.xy[k], q[k]); });
     if (!pm.live) return;
   } else {
     for (int s = lane; s < nb; s += LPP) {
