@@ -137,6 +137,14 @@ int mf_sync(mf_ctx *ctx);
  * [4] forces.  Count of launches of each kind per iteration in `launches`. */
 #define MF_NUM_KERNEL_KINDS 5
 int mf_profile_kernels(mf_ctx *ctx, int32_t iters, double *mean_ms, int32_t *launches);
+/* Steady launch time: the LSQ, inner-sweep and flux+update kernels at the context's
+ * current shapes (mf_set_tuning / mf_autotune), each launched `reps` times back to back
+ * between two HIP events on the context's stream, as they run inside an iteration;
+ * us[0..2] receives the mean microseconds per launch.  Writes only what an iteration
+ * recomputes before reading (dq, qm and the next state buffers), so the state and the
+ * next mf_run are unchanged.  Single-case contexts.  No counterpart in the reference:
+ * a measurement entry point beside mf_profile_kernels. */
+int mf_time_kernels(mf_ctx *ctx, int32_t reps, double *us);
 
 /* A batch: several independent flow cases (each its own cloud, of any size, with its
  * own Mach number and angle of attack; the other options shared) advanced together by

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("mf_run_async", C.c_int, [_VP, C.c_int32, C.c_int32]),
     ("mf_sync", C.c_int, [_VP]),
     ("mf_profile_kernels", C.c_int, [_VP, C.c_int32, _DP, _IP]),
+    ("mf_time_kernels", C.c_int, [_VP, C.c_int32, _DP]),
     ("mf_layout", C.c_int, [_VP, C.POINTER(MfLayout)]),
     ("mf_create_batch", C.c_int, [C.POINTER(MfCase), C.c_int32, C.POINTER(MfParams), C.POINTER(C.c_void_p)]),
     ("mf_batch_run", C.c_int, [_VP, C.c_int32, C.c_int32, _DP, _DP, _DP]),

@@ -1161,15 +1161,48 @@ static int time_launches(mf_ctx* c, int reps, F&& fn, double* us) {
 
 // The timed launches write only scratch between iterations: dq, qm, and the next
 // buffers of prim and q (stage 2: no prim_old, wave speed or residual writes).
-template <class S>
-static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
+static int prepare_timing(mf_ctx* c) {
   int rc;
   if ((rc = q_variables(c))) return rc;
   if (!c->tune_ctl && (rc = dev_alloc(c, &c->tune_ctl, 1))) return rc;
   const Ctl z{0, 0, 1.0};
   HIPCHK(hipMemcpy(c->tune_ctl, &z, sizeof(Ctl), hipMemcpyHostToDevice));
+  return MF_OK;
+}
+
+// Mean time of one launch of kernel `kind` (0 LSQ, 1 inner sweep, 2 flux+update) with
+// lanes-per-point shape lpp over reps back-to-back launches.  The inner sweep reads the
+// dq the LSQ kernel last wrote to dq[0].
+template <class S>
+static int time_shape(mf_ctx* c, int kind, int lpp, int reps, double* us) {
   const int cur = c->cur;
   auto q = (const V4<S>*)c->q[cur];
+  if (kind == 0)
+    return time_launches(c, reps, [&] {
+      launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp, c->qsum);
+    }, us);
+  if (kind == 1)
+    return time_launches(c, reps, [&] {
+      launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp, c->qsum);
+    }, us);
+  const State<S> st = state_of<S>(c, cur, 0);
+  return time_launches(c, reps, [&] {
+    launch_flux_update_on<S>(c, st, (const V4<S>*)c->prim[cur], (V4<S>*)c->prim_old, (V4<S>*)c->prim[cur ^ 1],
+                             (V4<S>*)c->q[cur ^ 1], c->cspeed, c->delta, c->partial, c->tune_ctl, 2, false, lpp);
+  }, us);
+}
+
+// the LSQ kernel at the context's own shape, so that the inner sweeps read its dq
+template <class S>
+static void refresh_dq(mf_ctx* c, int lpp) {
+  launch_lsq_on<S>(c, (const V4<S>*)c->q[c->cur], (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false,
+                   nullptr, lpp, c->qsum);
+}
+
+template <class S>
+static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
+  int rc;
+  if ((rc = prepare_timing(c))) return rc;
   const int cand[4] = {1, 2, 4, 8};
   // and every staged walk that fits
   std::vector<int> cand_st[3];
@@ -1180,33 +1213,13 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   }
   double best[3] = {1e300, 1e300, 1e300};
   int pick[3] = {1, 1, 1};
-  for (int lpp : cand_st[0]) {
-    double us;
-    if ((rc = time_launches(c, reps, [&] {
-           launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp, c->qsum);
-         }, &us)))
-      return rc;
-    if (us < best[0]) best[0] = us, pick[0] = lpp;
-  }
-  launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, 1, c->qsum);
-  for (int lpp : cand_st[1]) {
-    double us;
-    if ((rc = time_launches(c, reps, [&] {
-           launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp, c->qsum);
-         }, &us)))
-      return rc;
-    if (us < best[1]) best[1] = us, pick[1] = lpp;
-  }
-  const State<S> st = state_of<S>(c, cur, 0);
-  for (int lpp : cand_st[2]) {
-    double us;
-    if ((rc = time_launches(c, reps, [&] {
-           launch_flux_update_on<S>(c, st, (const V4<S>*)c->prim[cur], (V4<S>*)c->prim_old,
-                                    (V4<S>*)c->prim[cur ^ 1], (V4<S>*)c->q[cur ^ 1], c->cspeed, c->delta,
-                                    c->partial, c->tune_ctl, 2, false, lpp);
-         }, &us)))
-      return rc;
-    if (us < best[2]) best[2] = us, pick[2] = lpp;
+  for (int k = 0; k < 3; ++k) {
+    if (k == 1) refresh_dq<S>(c, 1);
+    for (int lpp : cand_st[k]) {
+      double us;
+      if ((rc = time_shape<S>(c, k, lpp, reps, &us))) return rc;
+      if (us < best[k]) best[k] = us, pick[k] = lpp;
+    }
   }
   c->lpp_lsq = pick[0];
   c->lpp_inner = pick[1];
@@ -1218,12 +1231,31 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   return MF_OK;
 }
 
+template <class S>
+static int time_kernels_impl(mf_ctx* c, int reps, double* us) {
+  int rc;
+  if ((rc = prepare_timing(c))) return rc;
+  const int lpp[3] = {c->lpp_lsq, c->lpp_inner, c->lpp_flux};
+  for (int k = 0; k < 3; ++k) {
+    if (k == 1) refresh_dq<S>(c, lpp[0]);
+    if ((rc = time_shape<S>(c, k, lpp[k], reps, &us[k]))) return rc;
+  }
+  return MF_OK;
+}
+
 extern "C" {
 
 int mf_autotune(mf_ctx* c, int32_t reps, mf_tuning* out) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
   if (reps <= 0) reps = 10;
   return c->ssz == 8 ? autotune_impl<double>(c, reps, out) : autotune_impl<float>(c, reps, out);
+}
+
+int mf_time_kernels(mf_ctx* c, int32_t reps, double* us) {
+  if (!c || !us) return set_err(MF_ERR_INVALID, "mf_time_kernels: null argument");
+  if (c->ncases > 0) return set_err(MF_ERR_STATE, "mf_time_kernels times a single-case context");
+  if (reps <= 0) reps = 20;
+  return c->ssz == 8 ? time_kernels_impl<double>(c, reps, us) : time_kernels_impl<float>(c, reps, us);
 }
 
 int mf_get_tuning(mf_ctx* c, mf_tuning* out) {

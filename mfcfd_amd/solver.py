@@ -182,6 +182,13 @@ class LskumSolver:
                 "mf_profile_kernels")
         return {k: (float(ms[j]), int(cnt[j])) for j, k in enumerate(L.KERNEL_KINDS)}
 
+    def time_kernels(self, reps: int = 20) -> dict:
+        """Mean launch time (us) of the LSQ, inner-sweep and flux+update kernels at the
+        current shapes, each over `reps` back-to-back launches; the state is unchanged."""
+        us = np.zeros(3)
+        L.check(L.lib().mf_time_kernels(self.h, int(reps), _dp(us)), "mf_time_kernels")
+        return {"lsq": float(us[0]), "inner": float(us[1]), "flux_update": float(us[2])}
+
     # ---- launch shapes (lanes per point of the stencil kernels)
     def autotune(self, reps: int = 10) -> dict:
         """Time every lanes-per-point shape on the current state and keep the fastest

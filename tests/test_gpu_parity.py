@@ -380,3 +380,20 @@ def test_two_shapes(small_cloud):
             assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
         for two, one in ((cl, cl1), (cd, cd1), (cm, cm1)):
             assert abs(two.sum() - one[0]) <= 1e-12 * abs(one[0]) + 1e-15
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_time_kernels_keeps_state(small_cloud, precision):
+    """mf_time_kernels launches the LSQ, inner and flux+update kernels at the current
+    shapes on the current state and writes only what an iteration recomputes: the
+    iterations after it give the same bits as without it."""
+    kw = dict(precision=precision)
+    with _gpu(small_cloud, **kw) as a, _gpu(small_cloud, **kw) as b:
+        a.q_lskum(5)
+        b.q_lskum(5)
+        us = b.time_kernels(5)
+        assert all(v > 0.0 for v in us.values())
+        ra, _, cda = a.q_lskum(10, first_it=6)
+        rb, _, cdb = b.q_lskum(10, first_it=6)
+        assert np.array_equal(a.prim, b.prim)
+        assert np.array_equal(ra, rb) and np.array_equal(cda, cdb)
