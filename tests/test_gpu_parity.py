@@ -317,6 +317,26 @@ def test_distance_weights_power(small_cloud, power):
         assert rel_err(c.prim, o.field("prim")) < 1e-10
 
 
+@pytest.mark.parametrize("power", [-2.0, 1.0])
+def test_mixed_distance_weights(small_cloud, power):
+    """The mixed path's distance weights (FP32 edge geometry, weight from the FP64
+    distance): after 60 iterations |dCl|, |dCd| <= 1e-4 of the FP64 path at the same
+    -power, and its staged walks give the global walks' bits."""
+    kw = dict(power=power)
+    with _gpu(small_cloud, **kw) as a, _gpu(small_cloud, precision="mixed", **kw) as b, \
+            _gpu(small_cloud, precision="mixed", **kw) as c:
+        b.tuning = {"lpp_lsq": 2, "lpp_inner": 2, "lpp_flux": 4}
+        c.tuning = {"lpp_lsq": 16 | 2, "lpp_inner": 16 | 2, "lpp_flux": 16 | 4}
+        a.q_lskum(60)
+        b.q_lskum(60)
+        c.q_lskum(60)
+        cla, cda, _, _ = a.compute_cl_cd_cm()
+        clb, cdb, _, _ = b.compute_cl_cd_cm()
+        assert abs(cla[0] - clb[0]) <= 1e-4 and abs(cda[0] - cdb[0]) <= 1e-4
+        assert rel_err(b.prim, a.prim) < 1e-4
+        assert np.array_equal(b.prim, c.prim)
+
+
 def _relabelled(cloud, **fields):
     import dataclasses
     return dataclasses.replace(cloud, **{k: v.copy() for k, v in fields.items()})
