@@ -16,6 +16,8 @@
 namespace mf {
 
 constexpr double TWO_BY_SQRT_PI = 1.1283791670955126;
+constexpr double INV_SQRT_PI = 0.5641895835477563;
+constexpr double INV_GAMMA_M1 = 1.0 / Consts<double>::GAMMA_M1;
 
 // ---------------------------------------------------------------------------------
 // q-tilde -> primitive -> flux preliminaries, with every intermediate kept
@@ -176,26 +178,31 @@ __host__ __device__ inline void flux_quad_rev(const KinF& k, bool a1p, bool a2p,
 // reverse of kin_fwd: adds d/d(qt) into qtb and d/d(nx, ny) into nxb, nyb
 __host__ __device__ inline void kin_rev(const KinF& k, const KinB& kb, double nx, double ny,
                                         double qtb[4], double& nxb, double& nyb) {
+  // the derivatives' divisions as products with three reciprocals: 1/rho and 1/sb
+  // divided, 1/p = 2 beta / rho (p = rho tmp, tmp = 1/(2 beta)), 1/spb = (1/sb)/sqrt(pi)
+  // and 1/beta = 2 tmp; the forward values above keep the primal's own operations
+  const double irho = 1.0 / k.rho, isb = 1.0 / k.sb;
+  const double ip = 2.0 * k.beta * irho, ispb = INV_SQRT_PI * isb, ibeta = 2.0 * k.tmp;
   double utb = kb.ut, unb = kb.un, rhob = kb.rho;
   // usq = ut^2 + un^2
   utb += 2.0 * k.ut * kb.usq;
   unb += 2.0 * k.un * kb.usq;
   // prr = p / rho
-  double pb = kb.prr / k.rho;
-  rhob -= kb.prr * k.prr / k.rho;
+  double pb = kb.prr * irho;
+  rhob -= kb.prr * k.prr * irho;
   // e = erf(S), B = 0.5 exp(-S^2) / spb
   double S1b = kb.e1 * TWO_BY_SQRT_PI * k.ex1 - 2.0 * k.S1 * k.B1 * kb.B1;
   double S2b = kb.e2 * TWO_BY_SQRT_PI * k.ex2 - 2.0 * k.S2 * k.B2 * kb.B2;
-  const double spbb = -(kb.B1 * k.B1 + kb.B2 * k.B2) / k.spb;
+  const double spbb = -(kb.B1 * k.B1 + kb.B2 * k.B2) * ispb;
   // S1 = ut sb, S2 = un sb
   utb += S1b * k.sb;
   unb += S2b * k.sb;
   const double sbb = S1b * k.ut + S2b * k.un;
   // sb = sqrt(b), spb = sqrt(pi b)
-  const double bb = sbb / (2.0 * k.sb) + spbb * Consts<double>::PI / (2.0 * k.spb);
+  const double bb = 0.5 * sbb * isb + 0.5 * Consts<double>::PI * spbb * ispb;
   // b = 0.5 rho / p
-  rhob += bb * 0.5 / k.p;
-  pb -= bb * k.b / k.p;
+  rhob += 0.5 * bb * ip;
+  pb -= bb * k.b * ip;
   // ut = u1 ny - u2 nx, un = u1 nx + u2 ny
   double u1b = utb * ny + unb * nx;
   double u2b = -utb * nx + unb * ny;
@@ -207,7 +214,7 @@ __host__ __device__ inline void kin_rev(const KinF& k, const KinB& kb, double nx
   // rho = exp(T2), T2 = T1 - log(beta)/(gamma-1)
   const double T2b = rhob * k.rho;
   const double T1b = T2b;
-  double betab = -T2b / (Consts<double>::GAMMA_M1 * k.beta);
+  double betab = -T2b * ibeta * INV_GAMMA_M1;
   // T1 = qt1 + beta (u1^2 + u2^2)
   qtb[0] += T1b;
   betab += T1b * (k.u1 * k.u1 + k.u2 * k.u2);
@@ -218,7 +225,7 @@ __host__ __device__ inline void kin_rev(const KinF& k, const KinB& kb, double nx
   qtb[2] += u2b * k.tmp;
   tmpb += u1b * k.qt[1] + u2b * k.qt[2];
   // tmp = 0.5 / beta, beta = -0.5 qt4
-  betab -= tmpb * k.tmp / k.beta;
+  betab -= tmpb * k.tmp * ibeta;
   qtb[3] += -0.5 * betab;
 }
 
@@ -234,10 +241,11 @@ __host__ __device__ inline void venkat_rev(double qt0, double q, double qmax, do
   const double num = (dp * dp + e2) * dn + 2.0 * dn * dn * dp;
   const double D = dp * dp + 2.0 * dn * dn + dn * dp + e2;
   const double den = D * dn;
-  const double t = num / den;
+  const double t = num / den;  // the primal's value, so the branch is the primal's
   if (!(t < 1.0)) return;
-  const double numb = phib / den;
-  const double denb = -phib * t / den;
+  const double iden = 1.0 / den;
+  const double numb = phib * iden;
+  const double denb = -phib * t * iden;
   const double dnb = numb * ((dp * dp + e2) + 4.0 * dn * dp) + denb * (D + dn * (4.0 * dn + dp));
   const double dpb = numb * (2.0 * dp * dn + 2.0 * dn * dn) + denb * (dn * (2.0 * dp + dn));
   qt0b += dnb;
