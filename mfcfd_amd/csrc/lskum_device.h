@@ -207,9 +207,15 @@ __device__ __forceinline__ void walk_stencil(const uint32_t* __restrict__ cp, in
 // slot's word is 255 and it goes to use_global(k, s) with k the neighbour's point id from
 // the point's connectivity.  The check is made per block, so the other blocks walk
 // exactly as before.
-template <int P, int LPP, class Fill, class Use, class UseGlobal>
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
+// ready() runs after the barrier, before the walk (the block's records are in LDS)
+template <int P, int LPP, class Fill, class Use, class UseGlobal, class Ready = NoHook>
 __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int b, int nb, uint8_t* s_cl,
-                                               Fill&& fill, Use&& use, UseGlobal&& use_global) {
+                                               Fill&& fill, Use&& use, UseGlobal&& use_global,
+                                               Ready&& ready = Ready{}) {
   const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
   const int hl = hn < h.hcap ? hn : h.hcap;
   for (int j = threadIdx.x; j < hl; j += P * LPP) fill(j, h.ids[h0 + j]);
@@ -221,6 +227,7 @@ __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int
     ((uint4*)s_cl)[j] = src[(size_t)s * (h.stride / 16) + (j - s * V)];
   }
   __syncthreads();
+  ready();
   const int li = threadIdx.x / LPP;
   if (hn <= h.hcap) {
     for (int s = threadIdx.x % LPP; s < nb; s += LPP) use((int)s_cl[s * P + li], s);

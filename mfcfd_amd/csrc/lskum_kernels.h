@@ -222,21 +222,19 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   const PointMap<LPP, STAGED> pm(g.n);
   if (!STAGED && !pm.live) return;
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
-  const D2 pi = g.xy[i];
-  const V4<S> dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
+  // the point's own coordinates and dq: from LDS after the staged walk's barrier (own
+  // points lead the halo), else from global memory
+  D2 pi{};
+  V4<S> dxi{}, dyi{};
+  if constexpr (!STAGED) {
+    pi = g.xy[i];
+    dxi = dq_in[2 * i];
+    dyi = dq_in[2 * i + 1];
+  }
   double sxq[4] = {0.0, 0.0, 0.0, 0.0}, syq[4] = {0.0, 0.0, 0.0, 0.0};
   const int nb = pm.live ? g.nbhs[i] : 0;
   V4<S> qi{};
-  if constexpr (SUMS) {
-    if (lane == 0) {
-      const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        sxq[r] = a[r];
-        syq[r] = b[r];
-      }
-    }
-  } else {
+  if constexpr (!SUMS) {
     qi = q[i];
   }
   // one edge (qk unused in the sums form); valid as in k_lsq
@@ -280,7 +278,13 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
           s_xy[j] = g.xy[k];
         },
         [&](int j, int) { edge(s_xy[j], SUMS ? qi : s_q[j], s_dx[j], s_dy[j], true); },
-        [&](int k, int) { edge(g.xy[k], SUMS ? qi : q[k], dq_in[2 * k], dq_in[2 * k + 1], true); });
+        [&](int k, int) { edge(g.xy[k], SUMS ? qi : q[k], dq_in[2 * k], dq_in[2 * k + 1], true); },
+        [&] {
+          const int li = (int)threadIdx.x / LPP;
+          pi = s_xy[li];
+          dxi = s_dx[li];
+          dyi = s_dy[li];
+        });
     if (!pm.live) return;
   } else if constexpr (SUMS) {
     struct Nbr {
@@ -314,6 +318,15 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
       syq[r] = group_sum<LPP>(syq[r]);
     }
     if (lane != 0) return;
+  }
+  if constexpr (SUMS) {
+    // the first-pass sums join after the edge terms
+    const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sxq[r] = (double)a[r] + sxq[r];
+      syq[r] = (double)b[r] + syq[r];
+    }
   }
   V4<S> ox, oy;
   lsq_solve(g, i, sxq, syq, ox, oy);
