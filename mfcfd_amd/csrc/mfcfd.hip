@@ -28,6 +28,21 @@ using namespace mf;
 // Kernels
 // =================================================================================
 
+// This thread's share of p[begin, end) (elements begin + threadIdx.x + k * blockDim.x):
+// eight independent accumulators, so eight loads are in flight at a time instead of one
+// dependent load per add; the grouping is fixed, so the result is deterministic.
+__device__ inline double strided_sum(const double* __restrict__ p, int begin, int end) {
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int st = (int)blockDim.x;
+  int j = begin + (int)threadIdx.x;
+  for (; j + 7 * st < end; j += 8 * st) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] += p[j + u * st];
+  }
+  for (; j < end; j += st) v[0] += p[j];
+  return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
 // Sum of the residual partials and the fpi_solver.F90:57-70 residue logic; one block.
 __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ partial, int np,
                                                     int n, int restart, int first_it,
@@ -35,9 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ p
                                                     double* __restrict__ res_hist,
                                                     double* __restrict__ resnew_hist) {
   if (ctl->halt) return;
-  double v = 0.0;
-  for (int j = threadIdx.x; j < np; j += blockDim.x) v += partial[j];
-  const double tot = block_sum(v);
+  const double tot = block_sum(strided_sum(partial, 0, np));
   if (threadIdx.x == 0) {
     const int it = ctl->it;
     const int t = first_it + it;
@@ -75,10 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_batch(const double* __restri
                                                           double* __restrict__ resnew_hist) {
   const int c = blockIdx.x;
   const int* r = rng + 5 * c;
-  double v = 0.0;
-  for (int j = r[0] + threadIdx.x; j < r[1]; j += blockDim.x) v += rs_pt[j];
-  for (int j = r[2] + threadIdx.x; j < r[3]; j += blockDim.x) v += rs_pt[j];
-  const double tot = block_sum(v);
+  const double tot = block_sum(strided_sum(rs_pt, r[0], r[1]) + strided_sum(rs_pt, r[2], r[3]));
   if (threadIdx.x == 0) {
     const int it = ctl->it;
     const double res_new = sqrt(tot) / (double)r[4];
@@ -99,9 +109,7 @@ __global__ void k_tick(Ctl* ctl) { ctl->it += 1; }
 // Sum of partials into one scalar (unfused state_update's sum_res_sqr)
 __global__ __launch_bounds__(BLOCK) void k_sum(const double* __restrict__ partial, int np,
                                                double* __restrict__ out) {
-  double v = 0.0;
-  for (int j = threadIdx.x; j < np; j += blockDim.x) v += partial[j];
-  const double t = block_sum(v);
+  const double t = block_sum(strided_sum(partial, 0, np));
   if (threadIdx.x == 0) *out = t;
 }
 
