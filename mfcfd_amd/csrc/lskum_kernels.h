@@ -492,15 +492,29 @@ __global__ __launch_bounds__(BLOCK) void k_forces(const V4<S>* __restrict__ prim
   const double cp_scale = shape_par[3 * s], ct = shape_par[3 * s + 1], st = shape_par[3 * s + 2];
   double H = 0.0, V = 0.0, M = 0.0;
   const int j1 = sh_ptr[s + 1];
-  for (int j = sh_ptr[s] + threadIdx.x; j < j1; j += blockDim.x) {
-    const double c0 = (double)prim[sh_m[j]].w - pr_inf;
+  auto point = [&](int j, double pw) {
+    const double c0 = pw - pr_inf;
     const double c = -c0 / cp_scale;
     cp[sh_pos[j]] = c;
     const double nx = sgeo[5 * j], ny = sgeo[5 * j + 1], ds = sgeo[5 * j + 2];
     H += c * nx * ds;
     V += c * ny * ds;
     M += (-c * ny * ds * sgeo[5 * j + 3] + c * nx * ds * sgeo[5 * j + 4]);
+  };
+  // four points' gathers (index, then pressure) issued together, then used in order
+  const int bs = (int)blockDim.x;
+  int j = sh_ptr[s] + (int)threadIdx.x;
+  for (; j + 3 * bs < j1; j += 4 * bs) {
+    int m[4];
+    double pw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) m[u] = sh_m[j + u * bs];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pw[u] = (double)prim[m[u]].w;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) point(j + u * bs, pw[u]);
   }
+  for (; j < j1; j += bs) point(j, (double)prim[sh_m[j]].w);
   H = block_sum(H);
   __syncthreads();
   V = block_sum(V);
