@@ -337,6 +337,20 @@ def test_mixed_distance_weights(small_cloud, power):
         assert np.array_equal(b.prim, c.prim)
 
 
+@pytest.mark.parametrize("kw", [dict(first_order=1, inner_iterations=0), dict(tscheme="first", inner_iterations=1)],
+                         ids=["first_order", "tscheme_first"])
+def test_mixed_other_schemes(small_cloud, kw):
+    """The mixed path with q-tilde = q (first order) and with the one-stage time scheme
+    (rks 1, euler 2): after 60 iterations |dCl|, |dCd| <= 1e-4 of the FP64 path."""
+    with _gpu(small_cloud, **kw) as a, _gpu(small_cloud, precision="mixed", **kw) as b:
+        a.q_lskum(60)
+        b.q_lskum(60)
+        cla, cda, _, _ = a.compute_cl_cd_cm()
+        clb, cdb, _, _ = b.compute_cl_cd_cm()
+        assert abs(cla[0] - clb[0]) <= 1e-4 and abs(cda[0] - cdb[0]) <= 1e-4
+        assert rel_err(b.prim, a.prim) < 1e-4
+
+
 def _relabelled(cloud, **fields):
     import dataclasses
     return dataclasses.replace(cloud, **{k: v.copy() for k, v in fields.items()})
