@@ -135,7 +135,14 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
     }
     const double delx = pk.x - pi.x, dely = pk.y - pi.y;
     double w = 1.0;
-    if (!POWER0 || DELTA) {
+    // the mixed path takes func_delta's time step in FP32 (it scales the pseudo-time
+    // update, not the discrete equations the iteration converges to)
+    constexpr bool DT32 = DELTA && POWER0 && sizeof(S) == 4;
+    if (DT32) {
+      const float dist = __builtin_amdgcn_sqrtf((float)(delx * delx + dely * dely));
+      const double dt = (double)((float)cfl * (dist * __builtin_amdgcn_rcpf((float)ck)));
+      if (valid && min_delt > dt) min_delt = dt;
+    } else if (!POWER0 || DELTA) {
       const double dist = sqrt(delx * delx + dely * dely);
       if (!POWER0) w = pow(dist, power);
       if (DELTA) {
