@@ -656,8 +656,12 @@ struct StagedNbrs {
 // A STAGE_PF block's flux halo into LDS (records of State st and the coordinates, then
 // the block's slot words), ended by a barrier; every thread of the block takes part.
 // The LDS layout (HaloRecs, then the slot words) is stage_lds_flux's.
+// Returns false, without staging, for a block whose halo exceeds the LDS capacity (the
+// few blocks of wall and outer points): its points then walk the global records with
+// the same lanes and slots (GlobalNbrs), which gives the same bits.  Every thread
+// reaches the barrier either way.
 template <class S, int LPP>
-__device__ inline void stage_flux_halo(const Geom& g, const State<S>& st, int b, unsigned char* smem,
+__device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b, unsigned char* smem,
                                        HaloRecs<S>& r, uint16_t*& s_cl) {
   const Halo& h = g.hl[1];
   const int H = h.hcap;
@@ -670,6 +674,10 @@ __device__ inline void stage_flux_halo(const Geom& g, const State<S>& st, int b,
   r.eps = (S*)(r.xy + H);
   s_cl = (uint16_t*)(r.eps + H);
   const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
+  if (hn > H) {
+    __syncthreads();
+    return false;
+  }
   for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
     const int k = h.ids[h0 + j];
     r.q[j] = st.q[k];
@@ -687,6 +695,7 @@ __device__ inline void stage_flux_halo(const Geom& g, const State<S>& st, int b,
     ((uint4*)s_cl)[j] = src[(size_t)sl * (h.stride / 8) + (j - sl * V)];
   }
   __syncthreads();
+  return true;
 }
 
 template <int CLS, class S, bool POWER0, int LPP, class Nbrs>

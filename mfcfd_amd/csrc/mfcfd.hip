@@ -400,15 +400,16 @@ int mf_create(const mf_geometry* geo, const mf_params* prm, mf_ctx** out) {
 // block's own points, then every other neighbour of them once (ascending), and per slot
 // its position in that list with the slot's half-stencil bits above it.  Leaves
 // h.hcap = 0 (staging unavailable) when a halo outgrows the 12-bit position.
-// capped (the 64-point halos of the LSQ and inner kernels and their reverses): LDS holds
-// the 99th percentile of the blocks' halo sizes, at most 248, rather than the largest
-// (the largest belong to the few blocks of wall and outer points), so more blocks fit
-// on a CU; the slot words are 8-bit positions (loc8), with 255 for a position past
-// the capacity, whose neighbour the kernel then reads through the point's own
-// connectivity.  Otherwise (the flux kernel's 32-point halos) every entry is in LDS and
-// the 16-bit slot words (loc) carry the half-stencil bits.
+// LDS holds the 99th percentile of the blocks' halo sizes rather than the largest (the
+// largest belong to the few blocks of wall and outer points), so more blocks fit on a
+// CU.  byte_words (the 64-point halos of the LSQ and inner kernels and their reverses):
+// the slot words are 8-bit positions (loc8), the capacity at most 248, and 255 marks a
+// position past it, whose neighbour the kernel reads through the point's own
+// connectivity.  Otherwise (the flux kernels' 32-point halos) the 16-bit slot words
+// (loc) carry the half-stencil bits, and a block whose halo exceeds the capacity walks
+// the global records instead.
 static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh, int P,
-                      bool capped, Halo& h) {
+                      bool byte_words, Halo& h) {
   const int n = c->n, W = c->W;
   const int nblk = (n + P - 1) / P;
   const int stride = nblk * P;
@@ -417,7 +418,6 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
   ids.reserve((size_t)n * 4);
   std::vector<int32_t> other, sizes;
   sizes.reserve(nblk);
-  int hmax = 0;
   h = Halo{};
   for (int b = 0; b < nblk; ++b) {
     const int p0 = b * P, p1 = std::min(n, p0 + P);
@@ -431,7 +431,6 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
     other.erase(std::unique(other.begin(), other.end()), other.end());
     const int own = p1 - p0, hn = own + (int)other.size();
     if (hn > 4096 - 8) return MF_OK;
-    hmax = std::max(hmax, hn);
     sizes.push_back(hn);
     for (int i = p0; i < p1; ++i) ids.push_back(i);
     ids.insert(ids.end(), other.begin(), other.end());
@@ -446,16 +445,14 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
       }
     ptr[b + 1] = (int32_t)ids.size();
   }
-  int cap = hmax;
-  if (capped) {
-    std::sort(sizes.begin(), sizes.end());
-    cap = std::min(sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))], 248);
-  }
+  std::sort(sizes.begin(), sizes.end());
+  int cap = sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))];
+  if (byte_words) cap = std::min(cap, 248);
   const int hcap = (cap + 7) / 8 * 8;  // keeps every LDS array of the staged kernels 16-byte aligned
   int32_t *d_ptr, *d_ids;
   int rc;
   if ((rc = upload(c, &d_ptr, ptr)) || (rc = upload(c, &d_ids, ids))) return rc;
-  if (capped) {
+  if (byte_words) {
     std::vector<uint8_t> loc8(loc.size());
     for (size_t e = 0; e < loc.size(); ++e) {
       const int pos = loc[e] & 0xFFF;
@@ -479,8 +476,6 @@ static int build_halo(mf_ctx* c, const std::vector<uint32_t>& conn, const std::v
 static int build_halos(mf_ctx* c, const std::vector<uint32_t>& conn, const std::vector<int32_t>& nbh) {
   c->g.width = c->W;
   int rc;
-  // the flux kernel's halos stay whole: its staging reads every entry from LDS, and its
-  // occupancy is set by registers, not LDS
   if ((rc = build_halo(c, conn, nbh, STAGE_P, true, c->g.hl[0])) ||
       (rc = build_halo(c, conn, nbh, STAGE_PF, false, c->g.hl[1])))
     return rc;

@@ -311,14 +311,21 @@ __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, Stat
     const int b = xcd_block(blockIdx.x, gridDim.x);
     HaloRecs<double> r;
     uint16_t* s_cl;
-    stage_flux_halo<double, LPP>(g, st, b, smem, r, s_cl);
+    const bool staged = stage_flux_halo<double, LPP>(g, st, b, smem, r, s_cl);
     const int li = threadIdx.x / LPP, lane = threadIdx.x % LPP;
     const int i = b * STAGE_PF + li;
     if (i >= g.n) return;  // whole lane groups
-    const StagedNbrs<double, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
-    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    if (staged) {
+      const StagedNbrs<double, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
+      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    } else {
+      const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
+      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    }
   } else {
     const int t = point_id();
     const int i = t / LPP, lane = t % LPP;

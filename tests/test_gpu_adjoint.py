@@ -203,13 +203,14 @@ def test_adjoint_staged_reverse(small_cloud, kernel, lpp):
     assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("kernel", ["inner", "lsq"])
+@pytest.mark.parametrize("kernel", ["inner", "lsq", "flux"])
 def test_adjoint_staged_reverse_overflow(c1_cloud, kernel):
-    """On the 9,600-point cloud the largest 64-point halo (529 entries, a block of wall
-    points) exceeds the LDS capacity (the 99th percentile of the halo sizes, 504): that
-    block reads its entries past the capacity from global memory.  The staged reverses
-    still give the global walk's gradient."""
-    key = {"inner": "lpp_inner_rev", "lsq": "lpp_lsq_rev"}[kernel]
+    """On the 9,600-point cloud the largest halos exceed the LDS capacity (the 99th
+    percentile of the blocks' halo sizes; at most 248 entries for the 64-point halos):
+    64-point blocks of wall points read their neighbours past the capacity from global
+    memory, and 32-point blocks whose halo does not fit walk the global records.  The
+    staged reverses still give the global walk's gradient."""
+    key = {"inner": "lpp_inner_rev", "lsq": "lpp_lsq_rev", "flux": "lpp_flux_rev"}[kernel]
     pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
     with _gpu(c1_cloud) as a, _gpu(c1_cloud) as b:
         a.tuning = {**pin, key: 2}
