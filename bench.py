@@ -333,8 +333,8 @@ def run_batch(iters: int, adjoint: bool = True):
 
 COPY_PROBE_BYTES = 256 << 20
 # the launch shapes the autotuner keeps for this workload (16 | L: staged walk, L lanes)
-PROBE_SHAPES = {"fp64": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 4},
-                "mixed": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 4}}
+PROBE_SHAPES = {"fp64": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 16 | 4},
+                "mixed": {"lpp_lsq": 16 | 4, "lpp_inner": 16 | 4, "lpp_flux": 16 | 4}}
 
 
 def probe():
