@@ -40,3 +40,10 @@ def rel_err(a, b, floor=1e-300):
     b = np.asarray(b, dtype=np.float64)
     scale = np.maximum(np.abs(b).max(), floor)
     return float(np.abs(a - b).max() / scale)
+
+
+@pytest.fixture(scope="session")
+def c2_cloud():
+    """The 640 x 250 NACA0012 cloud of the 160,000-point second-order workload."""
+    from mfcfd_amd.cloud import naca_ogrid
+    return naca_ogrid(640, 250, seed=0)
