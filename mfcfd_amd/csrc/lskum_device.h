@@ -117,6 +117,9 @@ struct Geom {
   // mixed path: lsq_geo as the FP32 solve coefficients (sum dy^2 / det, sum dx dy / det,
   // sum dx^2 / det): d/dx = a Sx - b Sy, d/dy = c Sy - b Sx for the sums Sx, Sy
   const V4<float>* lsq_geo_f = nullptr;
+  // mixed path: the half-stencil coefficients in FP32, [2i] c1 * obd and [2i+1] sxy * obd
+  // of the four halves (HalfCoefF)
+  const V4<float>* dg_f = nullptr;
 };
 
 struct Ctl {
@@ -714,18 +717,20 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
   constexpr bool F32 = sizeof(S) == 4;
   using R = std::conditional_t<F32, float, double>;
   std::conditional_t<F32, HalfCoefF, HalfCoef> hc;
-  {
+  if constexpr (F32) {
+    const V4<float> a = g.dg_f[2 * i], b = g.dg_f[2 * i + 1];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      hc.a[d] = a[d];
+      hc.b[d] = b[d];
+    }
+  } else {
     const V4d a = g.dg_c1[i], b = g.dg_sxy[i], c = g.dg_obd[i];
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
-      if constexpr (F32) {
-        hc.a[d] = (float)(a[d] * c[d]);
-        hc.b[d] = (float)(b[d] * c[d]);
-      } else {
-        hc.c1[d] = a[d];
-        hc.sxy[d] = b[d];
-        hc.obd[d] = c[d];
-      }
+      hc.c1[d] = a[d];
+      hc.sxy[d] = b[d];
+      hc.obd[d] = c[d];
     }
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};

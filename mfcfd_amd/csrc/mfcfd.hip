@@ -740,6 +740,16 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
     V4<float>* d_lf;
     if ((rc = upload(c, &d_lf, lf))) return rc;
     c->g.lsq_geo_f = d_lf;
+    // and the flux kernel's half-stencil coefficients (c1 * obd, sxy * obd) in FP32
+    std::vector<V4<float>> df((size_t)2 * n);
+    for (int j = 0; j < n; ++j)
+      for (int d = 0; d < 4; ++d) {
+        df[2 * j][d] = (float)(c1[j][d] * obd[j][d]);
+        df[2 * j + 1][d] = (float)(sxyv[j][d] * obd[j][d]);
+      }
+    V4<float>* d_df;
+    if ((rc = upload(c, &d_df, df))) return rc;
+    c->g.dg_f = d_df;
   }
   if (c->ssz == 8) {
     double* e;
