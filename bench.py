@@ -40,11 +40,13 @@ def flux_update_bytes_per_point(state_bytes: int, mean_nbhs: float) -> float:
 
     Reads of the point's own records (q 4, dq 8, qm 8, prim 4, prim_old 4 words of
     state; eps 1 state word; xy, nrm 4 doubles; half-stencil solve coefficients 12
-    doubles; delta 1 double; nbhs 4 B; conn 4 B per slot) and writes (prim, q: 8
-    state words).  Neighbour gathers are served from cache and not counted.
+    doubles, or 8 floats in the mixed path; delta 1 double; nbhs 4 B; conn 4 B per
+    slot) and writes (prim, q: 8 state words).  Neighbour gathers are served from cache
+    and not counted.
     """
     w = state_bytes
-    reads = (4 + 8 + 8 + 4 + 4 + 1) * w + (4 + 12 + 1) * 8 + 4 + 4 * mean_nbhs
+    coef = 12 * 8 if w == 8 else 8 * 4
+    reads = (4 + 8 + 8 + 4 + 4 + 1) * w + (4 + 1) * 8 + coef + 4 + 4 * mean_nbhs
     writes = 8 * w
     return reads + writes
 
