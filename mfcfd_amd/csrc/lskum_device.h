@@ -126,6 +126,8 @@ struct Ctl {
   int32_t it;      // iterations completed in this run (index into the histories)
   int32_t halt;    // set once a NaN residue is seen (q_lskum.F90:71)
   double res_old;  // fpi_solver.F90:65-67
+  int32_t first_it;  // iteration number of the run's first iteration (residue rule t <= 2)
+  int32_t reserved;
 };
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so each XCD (own L2) walks one

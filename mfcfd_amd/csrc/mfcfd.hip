@@ -45,7 +45,7 @@ __device__ inline double strided_sum(const double* __restrict__ p, int begin, in
 
 // Sum of the residual partials and the fpi_solver.F90:57-70 residue logic; one block.
 __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ partial, int np,
-                                                    int n, int restart, int first_it,
+                                                    int n, int restart,
                                                     Ctl* __restrict__ ctl,
                                                     double* __restrict__ res_hist,
                                                     double* __restrict__ resnew_hist) {
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ p
   const double tot = block_sum(strided_sum(partial, 0, np));
   if (threadIdx.x == 0) {
     const int it = ctl->it;
-    const int t = first_it + it;
+    const int t = ctl->first_it + it;
     const double res_new = sqrt(tot) / (double)n;
     double residue;
     if (t <= 2 && restart == 0) {
@@ -69,9 +69,10 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ p
   }
 }
 
-__global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old) {
+__global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old, int first_it) {
   ctl->it = 0;
   ctl->halt = 0;
+  ctl->first_it = first_it;
   if (set_res_old) ctl->res_old = res_old;
 }
 
@@ -81,7 +82,7 @@ __global__ void k_begin_run(Ctl* ctl, int set_res_old, double res_old) {
 // Cases are independent: a NaN in one does not stop the others.
 __global__ __launch_bounds__(BLOCK) void k_finalize_batch(const double* __restrict__ rs_pt,
                                                           const int32_t* __restrict__ rng, int ncases,
-                                                          int restart, int first_it,
+                                                          int restart,
                                                           const Ctl* __restrict__ ctl,
                                                           double* __restrict__ res_old,
                                                           double* __restrict__ res_hist,
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_batch(const double* __restri
     const int it = ctl->it;
     const double res_new = sqrt(tot) / (double)r[4];
     double residue;
-    if (first_it + it <= 2 && restart == 0) {
+    if (ctl->first_it + it <= 2 && restart == 0) {
       res_old[c] = res_new;
       residue = 0.0;
     } else {
@@ -184,7 +185,6 @@ struct mf_ctx {
   int32_t *sh_m = nullptr, *sh_id = nullptr;
   double *sgeo = nullptr, *cp = nullptr, *clcdcm = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_first_it = 0;
   // kept for the adjoint: internal ids of left/right (-1 for interior points), the
   // half-stencil sums, the shape points
   std::vector<int32_t> left_i, right_i, h_sh_m;
@@ -361,6 +361,7 @@ static void adj_forget_tuning(mf_ctx* c);
 
 int mf_destroy(mf_ctx* c) {
   if (!c) return MF_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   adj_release(c);
   for (auto& gexec : c->graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -909,13 +910,13 @@ static void launch_forces(mf_ctx* c, int buf, const Ctl* ctl, double* cd_hist) {
 static void launch_finalize(mf_ctx* c) {
   if (c->ncases > 0) {
     hipLaunchKernelGGL(k_finalize_batch, dim3(c->ncases), dim3(BLOCK), 0, c->stream, c->rs_pt, c->case_rng,
-                       c->ncases, c->p.restart, c->graph_first_it, c->ctl, c->res_old_case, c->res_hist,
+                       c->ncases, c->p.restart, c->ctl, c->res_old_case, c->res_hist,
                        c->resnew_hist);
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, c->stream, c->ctl);
     return;
   }
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BLOCK), 0, c->stream, c->partial,
-                     flux_blocks(c->n, c->lpp_flux), c->n, c->p.restart, c->graph_first_it, c->ctl,
+                     flux_blocks(c->n, c->lpp_flux), c->n, c->p.restart, c->ctl,
                      c->res_hist, c->resnew_hist);
 }
 
@@ -982,7 +983,9 @@ static int record(mf_ctx* c, int phase, Timer* tm) {
 
 static int ensure_history(mf_ctx* c, int iters) {
   if (iters <= c->hist_cap && c->res_hist) return MF_OK;
-  // histories are captured into the graphs: reallocate and drop the graphs
+  // histories are captured into the graphs: reallocate and drop the graphs, once the
+  // launches already queued from them have finished
+  HIPCHK(hipStreamSynchronize(c->stream));
   for (auto& gexec : c->graph)
     if (gexec) {
       HIPCHK(hipGraphExecDestroy(gexec));
@@ -1015,17 +1018,10 @@ static int run_queue(mf_ctx* c, int32_t first_it, int32_t iters) {
   if ((rc = ensure_history(c, iters))) return rc;
   // refresh q and wave speeds from prim: the reference routines may have changed prim
   if ((rc = q_variables(c))) return rc;
-  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old);
+  // first_it lives in the control block, so the captured graphs serve every first_it
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old,
+                     (int)first_it);
   HIPCHK(hipGetLastError());
-  // first_it is a kernel argument of k_finalize: a new value needs new graphs
-  if (c->graph_first_it != first_it) {
-    for (auto& gexec : c->graph)
-      if (gexec) {
-        HIPCHK(hipGraphExecDestroy(gexec));
-        gexec = nullptr;
-      }
-    c->graph_first_it = first_it;
-  }
   int phase = c->cur;
   for (int it = 0; it < iters; ++it) {
     if ((rc = build_graph(c, phase))) return rc;
@@ -1080,7 +1076,7 @@ int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launc
   int rc;
   if ((rc = ensure_history(c, iters))) return rc;
   if ((rc = q_variables(c))) return rc;
-  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old);
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, c->stream, c->ctl, c->p.restart ? 1 : 0, c->p.res_old, 1);
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
   Timer tm{c->stream, &ev, &kind};
@@ -1111,6 +1107,7 @@ int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launc
 }  // extern "C"
 
 static void drop_graphs(mf_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
   for (auto& gexec : c->graph)
     if (gexec) {
       (void)hipGraphExecDestroy(gexec);

@@ -171,7 +171,8 @@ program meshfree_host
             chunk = min(maxchunk, last - it + 1, opt%nsave - mod(it - 1, opt%nsave))
             call check(mf_run(ctx, int(it, c_int32_t), int(chunk, c_int32_t), res, rnew, cdh, done), 'mf_run')
             do k = 1, done
-                if (it + k - 1 == 2 .and. opt%restart == 0) res_old = rnew(k)
+                ! fpi_solver.F90:63-66: res_old follows res_new at every t <= 2
+                if (it + k - 1 <= 2 .and. opt%restart == 0) res_old = rnew(k)
                 write(*, '(a12,i8,a15,e30.20)') 'iterations:', it + k - 1, 'residue:', res(k)
                 write(301, *) it + k - 1, res(k)
             end do

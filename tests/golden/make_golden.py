@@ -8,8 +8,10 @@ Run in a container that has the reference sources (``make -C oracle ref`` first)
 Contents (all float64 / int32, no pickles):
   cloud_*          a 32 x 10 NACA0012 O-cloud (320 points, seed 11, first spacing
                    0.02, far field at 4 chords)
-  split_*          the half-stencil lists the stencil split produces (input to the
-                   reference flux routines; generate_connectivity needs PETSc)
+  split_*          the oracle's half-stencil lists (handed to the reference flux
+                   routines as input: the reference's generate_connectivity needs
+                   PETSc, so these are no reference output; the tests pin them
+                   against a brute-force split instead)
   prim, delta      a developed state (25 oracle iterations from freestream, M 0.63,
                    AoA 2, CFL 0.2, VL_CONST 5, 1 inner iteration) and its time steps
   nx, ny           reference compute_normals

@@ -3,8 +3,8 @@ differences of the FP64 oracle.
 
 * Any direction (vx, vy): sum(dJ/dx vx + dJ/dy vy) from the GPU reverse sweep equals
   the oracle's forward-mode derivative to 1e-6 relative (the bar the reference's
-  Tapenade adjoint is held to).  The tangent is itself checked against finite
-  differences in tests/test_oracle.py.
+  Tapenade adjoint is held to).  The tangent is itself checked against central
+  differences in tests/test_oracle.py (test_dual_tangent_central_difference).
 * 16 seeded wall points: dJ/dx, dJ/dy against central differences of the oracle with
   a 1e-7 step (stencil split held fixed, normals recomputed), to 1e-5 relative.
 * The checkpoint schedule changes the recomputation, not the result.

@@ -11,7 +11,16 @@
    stage weights, symmetric-pressure force identities and brute-force time steps.
 
 Routines with no reference build here (state_update, func_delta,
-generate_connectivity, compute_cl_cd_cm: they need PETSc) are pinned by 3 only.
+generate_connectivity, compute_cl_cd_cm: they need PETSc) are pinned by 3 only:
+   - generate_connectivity by a brute-force split, on a jittered cloud and on a
+     lattice whose stencils hold exact ties;
+   - state_update by a non-zero flux residual chosen so that every stage lands on a
+     known state (textbook conserved variables), with the wall's normal momentum
+     removed and the residual sum checked;
+   - compute_cl_cd_cm by a polygon wall with alternating side lengths under a
+     harmonic pressure, whose H, V and moment sums have closed forms;
+   - func_delta by brute force;
+   - the dual-number tangent (oracle/dual.h) by central differences.
 """
 import os
 
@@ -43,19 +52,19 @@ def _gold_oracle(g):
 
 
 def test_golden_normals_and_split(gold):
+    """Normals against the reference's compute_normals; the split against the
+    brute-force split of the golden cloud (the split_* fixtures are the oracle's own
+    lists, kept as a regression record: the reference's generate_connectivity needs
+    PETSc, so no reference output of it exists)."""
     o = _gold_oracle(gold)
     assert np.array_equal(o.field("nx"), gold["nx"])
     assert np.array_equal(o.field("ny"), gold["ny"])
-    for f in ("xpos_nbhs", "xneg_nbhs", "ypos_nbhs", "yneg_nbhs", "xpos_conn", "xneg_conn",
-              "ypos_conn", "yneg_conn"):
-        a = o.ifield(f)
-        b = gold["split_" + f]
-        if f.endswith("conn"):
-            n = o.ifield(f.replace("conn", "nbhs"))
-            for i in range(len(n)):
-                assert np.array_equal(a[i, :n[i]], b[i, :n[i]])
-        else:
-            assert np.array_equal(a, b)
+    c = _gold_cloud(gold)
+    halves = _brute_force_split(c.x, c.y, gold["nx"], gold["ny"], c.flag_1, c.conn, c.nbhs)
+    _check_split(o, halves)
+    for h in halves:
+        n = gold["split_" + h + "_nbhs"]
+        assert np.array_equal(o.ifield(h + "_nbhs"), n)
 
 
 def test_golden_q_lsq_inner(gold):
@@ -405,3 +414,282 @@ def test_first_order_is_lsq_of_point_fluxes(cloud):
         sol = np.linalg.solve(A, np.vstack([ds @ dG, dn @ dG]))
         total += sol[1] if ydir else sol[0]
     assert np.allclose(o.field("flux_res")[i], o.field("delta")[i] * total, rtol=1e-9, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------------
+# pins of the routines the reference cannot build here (they include PETSc headers)
+
+
+def _brute_force_split(x, y, nx, ny, flag_1, conn, nbhs):
+    """generate_connectivity.F90:8-256 from its definition, one point at a time: the
+    neighbour offset d in the point's frame (tangent t = (ny, -nx), normal n), xpos
+    takes d.t <= 0, xneg d.t >= 0 (a tie joins both); interior points split by d.n the
+    same way; wall points put every neighbour in yneg (:171-174), outer points every
+    neighbour in ypos (:241-244).  Lists keep connectivity order."""
+    halves = {h: [] for h in ("xpos", "xneg", "ypos", "yneg")}
+    for i in range(len(x)):
+        t = np.array([ny[i], -nx[i]])
+        nv = np.array([nx[i], ny[i]])
+        lists = {h: [] for h in halves}
+        for k in conn[i, :nbhs[i]]:
+            d = np.array([x[k] - x[i], y[k] - y[i]])
+            s_ = d[0] * t[0] + d[1] * t[1]
+            n_ = d[0] * nv[0] + d[1] * nv[1]
+            if s_ <= 0:
+                lists["xpos"].append(k)
+            if s_ >= 0:
+                lists["xneg"].append(k)
+            if flag_1[i] == 0:
+                lists["yneg"].append(k)
+            elif flag_1[i] == 2:
+                lists["ypos"].append(k)
+            else:
+                if n_ <= 0:
+                    lists["ypos"].append(k)
+                if n_ >= 0:
+                    lists["yneg"].append(k)
+        for h in halves:
+            halves[h].append(lists[h])
+    return halves
+
+
+def _check_split(o, halves):
+    ties = 0
+    for h, rows in halves.items():
+        cnt = o.ifield(h + "_nbhs")
+        lst = o.ifield(h + "_conn")
+        for i, row in enumerate(rows):
+            assert cnt[i] == len(row), (h, i)
+            assert list(lst[i, :cnt[i]]) == row, (h, i)
+    for i in range(len(halves["xpos"])):
+        ties += len(set(halves["xpos"][i]) & set(halves["xneg"][i]))
+    return ties
+
+
+def test_stencil_split_brute_force(cloud, gold):
+    """The oracle's stencil split equals the brute-force split on the 672-point
+    jittered cloud and on the golden cloud (with the reference's own normals)."""
+    for c in (cloud, _gold_cloud(gold)):
+        o = Oracle(c).setup()
+        halves = _brute_force_split(c.x, c.y, o.field("nx"), o.field("ny"), c.flag_1,
+                                    c.conn, c.nbhs)
+        _check_split(o, halves)
+    assert np.array_equal(o.field("nx"), gold["nx"])
+
+
+def _lattice_cloud(m=7, h=0.25):
+    """An m x m lattice: bottom row wall (shape 1, left to right), top row outer,
+    the rest interior; each point's neighbours are the lattice points within 1.5 h.
+    Interior normals are (0, 1), so every vertical or horizontal neighbour is an
+    exact tie of delta_s or delta_n."""
+    from mfcfd_amd.cloud import PointCloud
+    ii, jj = np.meshgrid(np.arange(m), np.arange(m))
+    x = (ii.ravel() * h).astype(np.float64)
+    y = (jj.ravel() * h).astype(np.float64)
+    n = m * m
+    row = jj.ravel()
+    flag_1 = np.where(row == 0, 0, np.where(row == m - 1, 2, 1)).astype(np.int32)
+    flag_2 = (row == 0).astype(np.int32)
+    left = np.full(n, -1, np.int32)
+    right = np.full(n, -1, np.int32)
+    for r, step in ((0, 1), (m - 1, -1)):  # wall left to right, outer right to left
+        idx = np.where(row == r)[0]
+        idx = idx[np.argsort(x[idx] * step)]
+        for a, k in enumerate(idx):
+            left[k] = idx[max(a - 1, 0)]
+            right[k] = idx[min(a + 1, len(idx) - 1)]
+    conn = np.full((n, 24), -1, np.int32)
+    nbhs = np.zeros(n, np.int32)
+    for i in range(n):
+        d = np.hypot(x - x[i], y - y[i])
+        nb = [k for k in np.argsort(d, kind="stable") if k != i and d[k] <= 1.5 * h]
+        conn[i, :len(nb)] = nb
+        nbhs[i] = len(nb)
+    return PointCloud(x=x, y=y, nx=np.zeros(n), ny=np.ones(n), min_dist=np.full(n, h),
+                      left=left, right=right, flag_1=flag_1, flag_2=flag_2, nbhs=nbhs,
+                      conn=conn)
+
+
+def test_stencil_split_exact_ties():
+    """Exact ties (delta_s = 0 or delta_n = 0) land in both halves, in connectivity
+    order; the wall and outer rows take every neighbour in yneg and ypos."""
+    c = _lattice_cloud()
+    o = Oracle(c).setup()
+    nx, ny = o.field("nx"), o.field("ny")
+    wall = c.flag_1 == 0
+    assert np.array_equal(nx[wall], np.zeros(wall.sum())) and (ny[wall] == 1.0).all()
+    halves = _brute_force_split(c.x, c.y, nx, ny, c.flag_1, c.conn, c.nbhs)
+    ties = _check_split(o, halves)
+    assert ties >= c.n  # every point has a vertical neighbour (delta_s = 0)
+    i = int(np.where((c.flag_1 == 1) & (c.x == 0.75) & (c.y == 0.75))[0][0])
+    assert o.ifield("xpos_nbhs")[i] == 5 and o.ifield("xneg_nbhs")[i] == 5
+    assert o.ifield("ypos_nbhs")[i] == 5 and o.ifield("yneg_nbhs")[i] == 5
+    for k in np.where(wall)[0]:
+        assert o.ifield("yneg_nbhs")[k] == c.nbhs[k] and o.ifield("ypos_nbhs")[k] == 0
+    for k in np.where(c.flag_1 == 2)[0]:
+        assert o.ifield("ypos_nbhs")[k] == c.nbhs[k] and o.ifield("yneg_nbhs")[k] == 0
+
+
+def _textbook_conserved(p, nx, ny):
+    """(rho, rho u.t, rho u.n, p/(gamma-1) + rho |u|^2 / 2), t = (ny, -nx)."""
+    rho, u1, u2, pr = p.T
+    ut = u1 * ny - u2 * nx
+    un = u1 * nx + u2 * ny
+    return np.column_stack([rho, rho * ut, rho * un, pr / 0.4 + 0.5 * rho * (u1 * u1 + u2 * u2)])
+
+
+def _textbook_primitive(U, nx, ny):
+    rho = U[:, 0]
+    ut, un = U[:, 1] / rho, U[:, 2] / rho
+    u1 = ut * ny + un * nx
+    u2 = -ut * nx + un * ny
+    return np.column_stack([rho, u1, u2, 0.4 * (U[:, 3] - 0.5 * rho * (u1 * u1 + u2 * u2))])
+
+
+@pytest.mark.parametrize("rk,first", [(1, False), (2, False), (3, False), (4, False), (1, True)])
+def test_state_update_nonzero_residual(cloud, rk, first):
+    """state_update.F90:10-141 with a non-zero flux residual chosen from a target
+    state P*: flux_res = 2 (U(P) - U(P*)) / euler in textbook conserved variables of
+    the point's frame.  Stages rk != 3 must land on P* (interior points), stage 3 on
+    the 2/3 U_old + 1/3 U* blend; wall points land on P* with the normal momentum
+    removed (:44); the residual sums (rho* - rho)^2 over wall and interior points."""
+    kw = dict(cfl=0.2, euler=2.0, rks=1) if first else dict(cfl=0.2)
+    o = Oracle(cloud, **kw).setup()
+    euler = 2.0 if first else 1.0
+    rng = np.random.default_rng(rk)
+    n = cloud.n
+    nx, ny = o.field("nx").copy(), o.field("ny").copy()
+
+    def state():
+        return np.column_stack([rng.uniform(0.7, 1.3, n), rng.uniform(-0.6, 0.6, n),
+                                rng.uniform(-0.6, 0.6, n), rng.uniform(0.5, 0.9, n)])
+    P, Pstar, Pold = state(), state(), state()
+    U, Ustar, Uold = (_textbook_conserved(s, nx, ny) for s in (P, Pstar, Pold))
+    o.field("prim")[:] = P
+    o.field("prim_old")[:] = Pold
+    o.field("flux_res")[:] = 2.0 * (U - Ustar) / euler
+    o.state_update(rk)
+    got = o.field("prim")
+    if rk == 3:
+        expect_U = 2.0 / 3.0 * Uold + 1.0 / 3.0 * (U - 0.5 * 2.0 * (U - Ustar))
+    else:
+        expect_U = Ustar.copy()
+    wall, inter = cloud.flag_1 == 0, cloud.flag_1 == 1
+    expect_U[wall, 2] = 0.0
+    expect = _textbook_primitive(expect_U, nx, ny)
+    sel = wall | inter
+    assert np.allclose(got[sel], expect[sel], rtol=1e-12, atol=1e-13)
+    un_wall = got[wall, 1] * nx[wall] + got[wall, 2] * ny[wall]
+    assert np.abs(un_wall).max() < 1e-14
+    res = ((expect_U[sel, 0] - U[sel, 0]) ** 2).sum()
+    assert np.isclose(o.scalar("sum_res_sqr"), res, rtol=1e-12)
+
+
+def _ngon_cloud(M=20, R=0.5, cx=0.3, cy=0.2, frac=0.6):
+    """A wall of N = 2M points (shape 1) on a circle of radius R centred on (cx, cy),
+    running clockwise: the even points form a regular M-gon at angles -2 pi k/M, each
+    odd point follows its even point by the angle a = frac 2 pi/M (then b = 2 pi/M - a
+    to the next), so the sides alternate between two lengths.  A clockwise regular
+    outer ring of radius 4R surrounds it; neighbours are left, right and the matching
+    point of the other ring."""
+    from mfcfd_amd.cloud import PointCloud
+    N = 2 * M
+    a = frac * 2.0 * np.pi / M
+    k = np.arange(N) // 2
+    th = -2.0 * np.pi * k / M - np.where(np.arange(N) % 2 == 1, a, 0.0)
+    tho = -2.0 * np.pi * np.arange(N) / N
+    x = np.concatenate([cx + R * np.cos(th), cx + 4 * R * np.cos(tho)])
+    y = np.concatenate([cy + R * np.sin(th), cy + 4 * R * np.sin(tho)])
+    n = 2 * N
+    j = np.arange(N)
+    left = np.concatenate([(j - 1) % N, N + (j - 1) % N]).astype(np.int32)
+    right = np.concatenate([(j + 1) % N, N + (j + 1) % N]).astype(np.int32)
+    other = np.concatenate([N + j, j])
+    conn = np.full((n, 24), -1, np.int32)
+    conn[:, 0], conn[:, 1], conn[:, 2] = left, right, other
+    flag_1 = np.concatenate([np.zeros(N), np.full(N, 2)]).astype(np.int32)
+    flag_2 = np.concatenate([np.ones(N), np.zeros(N)]).astype(np.int32)
+    return PointCloud(x=x, y=y, nx=np.zeros(n), ny=np.ones(n), min_dist=np.full(n, 0.1),
+                      left=left, right=right, flag_1=flag_1, flag_2=flag_2,
+                      nbhs=np.full(n, 3, np.int32), conn=conn), th, a
+
+
+def test_force_quadrature_polygon():
+    """compute_force_coeffs.F90:11-102 on a polygon inscribed in a circle whose sides
+    alternate between the chords of angles a and b = 2 pi/M - a.  Every point's ds is
+    R (sin(a/2) + sin(b/2)) and its normal (rot90(right - left), point_normals.F90)
+    is the radius at angle theta + d, d = +-(b - a)/2 alternating.  With
+    p = p_inf + A cos(theta) + B sin(theta) on the even points and p_inf on the odd
+    ones, q = rho_inf M^2/2 and cp = -(p - p_inf)/q (the reference's sign), the sums
+    over the regular M-gon of even points are exact:
+      H = -(ds/q) (M/2) (A cos d - B sin d),  V = -(ds/q) (M/2) (A sin d + B cos d),
+      Cm = cy H - (cx - 1/4) V,
+    and Cl = V cos(aoa) - H sin(aoa), Cd = H cos(aoa) + V sin(aoa).  A ds taken from
+    one side only, a wrong normal orientation, moment arm or sign fails it."""
+    M, R, cx, cy = 20, 0.5, 0.3, 0.2
+    c, th, a = _ngon_cloud(M, R, cx, cy)
+    b = 2.0 * np.pi / M - a
+    d = 0.5 * (b - a)
+    mach, aoa = 0.7, 3.0
+    o = Oracle(c, mach=mach, aoa_deg=aoa).setup()
+    N = 2 * M
+    even = np.arange(N) % 2 == 0
+    dd = np.where(even, d, -d)
+    assert np.allclose(o.field("nx")[:N], np.cos(th + dd), atol=1e-15)
+    assert np.allclose(o.field("ny")[:N], np.sin(th + dd), atol=1e-15)
+    A, B = 0.05, -0.03
+    p = o.field("prim")
+    p[:N, 3] = np.where(even, 1 / 1.4 + A * np.cos(th) + B * np.sin(th), 1 / 1.4)
+    o.compute_cl_cd_cm()
+    q = 0.5 * mach * mach
+    ds = R * (np.sin(a / 2) + np.sin(b / 2))
+    H = -(ds / q) * (M / 2) * (A * np.cos(d) - B * np.sin(d))
+    V = -(ds / q) * (M / 2) * (A * np.sin(d) + B * np.cos(d))
+    al = np.deg2rad(aoa)
+    assert np.isclose(o.field("Cd")[0], H * np.cos(al) + V * np.sin(al), rtol=1e-12)
+    assert np.isclose(o.field("Cl")[0], V * np.cos(al) - H * np.sin(al), rtol=1e-12)
+    assert np.isclose(o.field("Cm")[0], cy * H - (cx - 0.25) * V, rtol=1e-12)
+
+
+def test_dual_tangent_central_difference(cloud):
+    """The oracle's forward-mode (dual number) tangent of J = Cd after 20 iterations
+    against central differences of the oracle's own J, re-run from freestream with the
+    coordinates moved by +-h v (normals recomputed; a step that changes a stencil
+    split is not used).  min_dist is point-file data, not a function of x and y."""
+    from oracle.oracle import tangent_cd
+    import dataclasses
+    params = dict(cfl=0.2, mach=0.63, aoa_deg=2.0, vl_const=5.0, inner_iterations=1)
+    iters = 20
+    rng = np.random.default_rng(5)
+    # a smooth direction with a seeded random part, on the interior points and the
+    # wall points ahead of x = 0.9: the wall points at the trailing edge sit on a
+    # branch switch of J (their one-sided differences differ by ~1%), where a central
+    # difference averages two branches and the tangent follows one
+    vx = np.sin(2.0 * cloud.x + 1.0) * np.cos(cloud.y) + 0.2 * rng.standard_normal(cloud.n)
+    vy = np.cos(3.0 * cloud.y) * np.sin(cloud.x + 0.5) + 0.2 * rng.standard_normal(cloud.n)
+    moved = (cloud.flag_1 == 1) | ((cloud.flag_1 == 0) & (cloud.x < 0.9))
+    vx, vy = np.where(moved, vx, 0.0), np.where(moved, vy, 0.0)
+    J, dJ = tangent_cd(cloud, vx, vy, iters, **params)
+
+    def run(h):
+        c = dataclasses.replace(cloud, x=cloud.x + h * vx, y=cloud.y + h * vy)
+        o = Oracle(c, **params).setup()
+        o.run(iters)
+        o.compute_cl_cd_cm()
+        return float(o.field("Cd").sum()), o
+
+    J0, o0 = run(0.0)
+    assert abs(J - J0) <= 1e-13 * abs(J0)
+    split0 = [o0.ifield(f + "_nbhs").copy() for f in ("xpos", "xneg", "ypos", "yneg")]
+    best = np.inf
+    for h in (1e-7, 1e-8, 1e-9):
+        (Jp, op), (Jm, om) = run(h), run(-h)
+        same = all(np.array_equal(s, oo.ifield(f + "_nbhs"))
+                   for oo in (op, om) for s, f in zip(split0, ("xpos", "xneg", "ypos", "yneg")))
+        if not same:
+            continue
+        fd = (Jp - Jm) / (2 * h)
+        best = min(best, abs(fd - dJ) / abs(dJ))
+        print(f"h {h:g}: FD {fd:.12e} tangent {dJ:.12e} rel {abs(fd - dJ) / abs(dJ):.2e}")
+    assert abs(dJ) > 1e-6
+    assert best < 1e-6, best
