@@ -31,8 +31,8 @@ def small_cloud():
 @pytest.fixture(scope="session")
 def c1_cloud():
     """The 160 x 60 NACA0012 cloud of the first-order FP64 workload (9,600 points)."""
-    from mfcfd_amd.cloud import naca_ogrid
-    return naca_ogrid(160, 60, seed=0)
+    from mfcfd_amd import workloads as W
+    return W.cloud_of(W.C1)
 
 
 def rel_err(a, b, floor=1e-300):
@@ -40,10 +40,3 @@ def rel_err(a, b, floor=1e-300):
     b = np.asarray(b, dtype=np.float64)
     scale = np.maximum(np.abs(b).max(), floor)
     return float(np.abs(a - b).max() / scale)
-
-
-@pytest.fixture(scope="session")
-def c2_cloud():
-    """The 640 x 250 NACA0012 cloud of the 160,000-point second-order workload."""
-    from mfcfd_amd.cloud import naca_ogrid
-    return naca_ogrid(640, 250, seed=0)

@@ -250,21 +250,6 @@ def test_nan_residue_stops_loop(small_cloud):
     assert np.isnan(res_g[-1]) and np.isnan(res_o[-1])
 
 
-def test_c1_first_order_fp64(c1_cloud):
-    """The 9,600-point first-order FP64 case (q-tilde = q, no limiter, no inner
-    iterations, M 0.63, AoA 2, CFL 0.2), 200 iterations: GPU and oracle agree."""
-    o = _oracle(c1_cloud, inner_iterations=0, first_order=1)
-    res_o, _ = o.run(200)
-    o.compute_cl_cd_cm()
-    with _gpu(c1_cloud, inner_iterations=0, first_order=1) as g:
-        res_g, _, cd_g = g.q_lskum(200)
-        ep = rel_err(g.prim, o.field("prim"))
-        print(f"C1 prim rel err after 200 its: {ep:.3e}; cd {cd_g[-1]:.12e} vs {o.field('Cd')[0]:.12e}")
-        assert ep < 1e-10
-        assert np.abs(res_g - res_o).max() < 1e-8
-        assert abs(cd_g[-1] - o.field("Cd")[0]) <= 1e-10 * abs(o.field("Cd")[0])
-
-
 def test_second_order_vl150_fp64(c1_cloud):
     """Same cloud, second order with the default limiter constant (VL_CONST 150):
     100 iterations agree."""
