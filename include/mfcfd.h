@@ -229,6 +229,27 @@ int mf_adjoint_backward(mf_ctx *ctx, double *dJdx, double *dJdy, mf_adjoint_stat
 int mf_adjoint(mf_ctx *ctx, int32_t first_it, int32_t iters, int32_t snaps, double *J,
                double *dJdx, double *dJdy, mf_adjoint_stats *stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc): the
+ * force coefficients of mf_compute_cl_cd_cm (compute_force_coeffs.F90:11-102) on
+ * caller-owned DEVICE arrays, with no context and no allocation.  Inputs in the
+ * reference's layouts and original point order: x, y, nx, ny [n_points]; prim (4,N)
+ * column-major; left, right [n_points] 0-based; flag_2 [n_points] (shape id 1..shapes
+ * of wall points); shape_idx [n_shape], 0-based ids of the shape points in the order
+ * cp is wanted.  Outputs: clcdcm[3 * shapes] (Cl, Cd, Cm of each shape) and
+ * cp[n_shape].  Work is queued on `stream` (a hipStream_t; NULL: the null stream)
+ * and the call returns without waiting.  Partial sums go to a fixed device array of
+ * the library (256 blocks x MF_NOALLOC_MAX_SHAPES shapes x 3 doubles per GPU, held
+ * until the process ends); calls that overlap on one GPU share it and race.  Rejects
+ * shapes above MF_NOALLOC_MAX_SHAPES.  A shape point whose ids are out of range gets
+ * cp = NaN and adds nothing. */
+#define MF_NOALLOC_MAX_SHAPES 8
+int mf_compute_cl_cd_cm_noalloc(int32_t n_points, int32_t n_shape, int32_t shapes, double mach,
+                                double aoa, const double *x, const double *y, const double *nx,
+                                const double *ny, const double *prim, const int32_t *left,
+                                const int32_t *right, const int32_t *flag_2,
+                                const int32_t *shape_idx, double *clcdcm, double *cp, void *stream);
+
 /* Measurement aid with no counterpart in the reference: a plain device copy of
  * n_bytes (each byte read once and written once), `reps` times, timed with HIP
  * events; the known-size kernel against which HBM traffic counters are calibrated. */

@@ -12,6 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmfcfd_hip.so")
 
 MF_OK = 0
+MF_ERR_HIP = -1
+MF_ERR_INVALID = -2
+MF_ERR_STATE = -3
 MF_FP64 = 0
 MF_MIXED = 1
 
@@ -26,6 +29,7 @@ MF_FIELD_NY = 8
 MF_NUM_KERNEL_KINDS = 5
 # mf_adjoint*: snaps value that tapes every iteration in the primal sweep
 MF_ADJ_TAPE_ALL = -1
+MF_NOALLOC_MAX_SHAPES = 8
 # mf_tuning: staged walks (MF_LPP_STAGED | lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux)
 MF_LPP_STAGED = 16
 KERNEL_KINDS = ("lsq", "inner", "flux_update", "finalize", "forces")
@@ -105,6 +109,9 @@ SIGNATURES = [
     ("mf_get_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
     ("mf_set_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
     ("mf_copy_probe", C.c_int, [C.c_int64, C.c_int32, _DP]),
+    # device pointers and the stream travel as plain addresses
+    ("mf_compute_cl_cd_cm_noalloc", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double]
+     + [_VP] * 12),
     ("mf_adjoint_forward", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _DP]),
     ("mf_adjoint_backward", C.c_int, [_VP, _DP, _DP, C.POINTER(MfAdjointStats)]),
     ("mf_adjoint", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _DP, _DP, _DP,

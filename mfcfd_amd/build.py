@@ -31,7 +31,8 @@ HIPCC_FLAGS = [
 # FP64 kernels, adjoint) without contraction; mfcfd_f32.hip (the mixed path's FP32
 # kernels, held to a tolerance against FP64, not to rounding parity) with FMA
 # contraction.
-UNITS = [("mfcfd.hip", "-ffp-contract=off"), ("mfcfd_f32.hip", "-ffp-contract=fast")]
+UNITS = [("mfcfd.hip", "-ffp-contract=off"), ("mfcfd_f32.hip", "-ffp-contract=fast"),
+         ("mfcfd_noalloc.hip", "-ffp-contract=off")]
 
 
 def hipcc() -> str:

@@ -138,6 +138,9 @@ static int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
+// Error path of the allocation-free entry point (mfcfd_noalloc.hip): same message slot
+int mf_noalloc_fail(int code, const char* msg) { return set_err(code, "%s", msg); }
+
 #define HIPCHK(call)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \
