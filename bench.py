@@ -46,9 +46,6 @@ sys.path.insert(0, ROOT)
 from mfcfd_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# one wave64 VALU instruction per 4 cycles on each of 1,024 SIMDs (256 CUs x 4) at
-# 2.4 GHz, packed FP32 disabled: the issue rate the VALU-bound kernel is held to
-VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 
 
 def flux_phase_bytes_per_point(state_bytes: int, mean_nbhs: float) -> float:
@@ -438,6 +435,9 @@ def main():
     achieved = per_launch / fu_s / 1e9
     ctr = _counters("naca0012_2560x1000", prec)
     valu_insts = ctr.get("valu_insts")
+    # wave64 VALU instructions per SIMD per cycle (1,024 SIMDs at 2.4 GHz) at this
+    # launch time; the nominal rate is one per 4 cycles with packed FP32 off
+    issue = valu_insts / (1024 * 2.4e9 * fu_s) if valu_insts else None
     it_bytes = iteration_bytes_per_point(sb, mean_nbhs, params["inner_iterations"])
     line = {
         "metric": "point-iters/s + HBM GB/s, FP32-mixed vs FP64 at dCl<=1e-4; adjoint/primal cost",
@@ -459,10 +459,10 @@ def main():
                      "kernel": "k_flux_update", "kernel_us": best["steady_us"]["flux_update"],
                      "algorithmic_bytes_per_point": per_point, "algorithmic_bytes_per_launch": per_launch,
                      # the VALU binds this kernel: its issued VALU instructions (counters,
-                     # profiles/) at one wave64 instruction per 4 cycles per SIMD against
-                     # the launch time measured here
+                     # profiles/) per SIMD-cycle at the launch time measured here
                      "valu": {"busy_pct": ctr.get("valu_busy"), "insts_per_launch": valu_insts,
-                              "issue_frac": (valu_insts / VALU_ISSUE_PER_S / fu_s) if valu_insts else None},
+                              "insts_per_simd_cycle": issue, "nominal_per_simd_cycle": 0.25,
+                              "frac_of_nominal": issue / 0.25 if issue else None},
                      "l2_hit": ctr.get("l2_hit"), "shape": best["tuning"]["lpp_flux"],
                      "counters_kernel": ctr.get("kernel")},
         "iteration_hbm_gbs": it_bytes * best["point_iters_per_s"] / 1e9,

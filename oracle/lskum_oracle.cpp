@@ -850,4 +850,53 @@ void orc_dual_cd(void* h, double* J, double* dJ) {
 
 void orc_dual_destroy(void* h) { delete static_cast<SolverT*>(h); }
 
+// Routine-level tangents (dot-product checks of single reverse routines): one field's
+// values and tangents in or out (the layouts of orc_field), and one routine called.
+static std::vector<orc::Dual>* dual_field(SolverT* s, const std::string& f) {
+  if (f == "x") return &s->x;
+  if (f == "y") return &s->y;
+  if (f == "nx") return &s->nx;
+  if (f == "ny") return &s->ny;
+  if (f == "prim") return &s->prim;
+  if (f == "q") return &s->q;
+  if (f == "dq") return &s->dq;
+  if (f == "qm") return &s->qm;
+  if (f == "temp") return &s->temp;
+  if (f == "delta") return &s->delta;
+  if (f == "flux_res") return &s->flux_res;
+  return nullptr;
+}
+
+int64_t orc_dual_set(void* h, const char* name, const double* v, const double* d) {
+  auto* a = dual_field(static_cast<SolverT*>(h), name);
+  if (!a) return -1;
+  for (size_t j = 0; j < a->size(); ++j) (*a)[j] = orc::Dual(v[j], d ? d[j] : 0.0);
+  return (int64_t)a->size();
+}
+
+int64_t orc_dual_get(void* h, const char* name, double* v, double* d) {
+  auto* a = dual_field(static_cast<SolverT*>(h), name);
+  if (!a) return -1;
+  for (size_t j = 0; j < a->size(); ++j) {
+    if (v) v[j] = (*a)[j].v;
+    if (d) d[j] = (*a)[j].d;
+  }
+  return (int64_t)a->size();
+}
+
+int32_t orc_dual_call(void* h, const char* name) {
+  auto* s = static_cast<SolverT*>(h);
+  std::string f(name);
+  if (f == "compute_normals") s->compute_normals();
+  else if (f == "generate_connectivity") s->generate_connectivity();
+  else if (f == "eval_q_variables") s->eval_q_variables();
+  else if (f == "eval_q_derivatives") s->eval_q_derivatives();
+  else if (f == "eval_q_inner_loop") s->eval_q_inner_loop();
+  else if (f == "eval_update_innerloop") s->eval_update_innerloop();
+  else if (f == "cal_flux_residual") s->cal_flux_residual();
+  else if (f == "func_delta") s->func_delta();
+  else return -1;
+  return 0;
+}
+
 }  // extern "C"

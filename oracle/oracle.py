@@ -9,6 +9,10 @@ touches it.
 * :class:`RefRoutines` drives ``oracle/_ref/libmfcfd_ref.so``, the reference's own
   Fortran routines compiled unmodified with amdflang (built only where the reference
   sources are present).
+* :class:`TapRoutines` drives ``oracle/_tap/libmfcfd_tap.so``, the reference's
+  Tapenade-generated reverse routines compiled the same way.
+* :class:`DualRoutines` runs single routines of the oracle in dual numbers (the
+  tangents a reverse routine is checked against).
 """
 from __future__ import annotations
 
@@ -38,6 +42,8 @@ def build(force: bool = False) -> None:
         subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
     if os.path.isdir("/root/reference/src_mpi_primal") and (force or not os.path.exists(REF_LIB_PATH)):
         subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+    if os.path.isdir("/root/reference/src_mpi_adjoint") and (force or not os.path.exists(TAP_LIB_PATH)):
+        subprocess.check_call(["make", "-s", "-C", HERE, "tap"])
 
 
 _lib = None
@@ -80,6 +86,12 @@ def _load():
         lib.orc_dual_run.argtypes = [vp, C.c_int32, C.c_int32]
         lib.orc_dual_cd.argtypes = [vp, dp, dp]
         lib.orc_dual_destroy.argtypes = [vp]
+        lib.orc_dual_set.argtypes = [vp, C.c_char_p, dp, dp]
+        lib.orc_dual_set.restype = C.c_int64
+        lib.orc_dual_get.argtypes = [vp, C.c_char_p, dp, dp]
+        lib.orc_dual_get.restype = C.c_int64
+        lib.orc_dual_call.argtypes = [vp, C.c_char_p]
+        lib.orc_dual_call.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -308,3 +320,151 @@ class RefRoutines:
         phi = np.zeros(4)
         self.lib.ref_venkat(k0 + 1, qt.ctypes.data, phi.ctypes.data)
         return phi
+
+
+# ---------------------------------------------------------------------------------
+class DualRoutines:
+    """The oracle in dual numbers, driven one routine at a time: set values and
+    tangents of fields, call a routine, read the tangents of its outputs (the forward
+    side of a dot-product check of a single reverse routine)."""
+
+    def __init__(self, cloud, **params):
+        lib = _load()
+        prm = OrcParams(**default_params(**params))
+        self.n, self.w = cloud.n, cloud.max_nbhs
+        self._keep = [np.ascontiguousarray(a) for a in (
+            cloud.x.astype(np.float64), cloud.y.astype(np.float64),
+            cloud.left.astype(np.int32), cloud.right.astype(np.int32),
+            cloud.flag_1.astype(np.int32), cloud.flag_2.astype(np.int32),
+            cloud.min_dist.astype(np.float64), cloud.nbhs.astype(np.int32),
+            np.where(cloud.conn >= 0, cloud.conn, 0).astype(np.int32),
+            cloud.nx.astype(np.float64), cloud.ny.astype(np.float64))]
+        x, y, l, r, f1, f2, md, nb, cn, nx, ny = self._keep
+        self.h = lib.orc_dual_create(self.n, self.w, _dp(x), _dp(y), _ip(l), _ip(r), _ip(f1), _ip(f2),
+                                     _dp(md), _ip(nb), _ip(cn), _dp(nx), _dp(ny), C.byref(prm))
+        lib.orc_dual_setup(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _load().orc_dual_destroy(self.h)
+            self.h = None
+
+    def set(self, name: str, value, tangent=None):
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
+        t = None if tangent is None else np.ascontiguousarray(tangent, dtype=np.float64).reshape(-1)
+        got = _load().orc_dual_set(self.h, name.encode(), _dp(v), None if t is None else _dp(t))
+        if got != v.size:
+            raise ValueError(f"{name}: size {v.size}, field holds {got}")
+
+    def get(self, name: str):
+        size = {"dq": 8, "qm": 8, "temp": 8, "prim": 4, "q": 4, "flux_res": 4}.get(name, 1) * self.n
+        v, t = np.zeros(size), np.zeros(size)
+        _load().orc_dual_get(self.h, name.encode(), _dp(v), _dp(t))
+        return v, t
+
+    def call(self, routine: str):
+        if _load().orc_dual_call(self.h, routine.encode()) != 0:
+            raise KeyError(routine)
+
+
+TAP_LIB_PATH = os.path.join(HERE, "_tap", "libmfcfd_tap.so")
+
+
+class TapRoutines:
+    """The reference's Tapenade-generated reverse routines (oracle/_tap), driven on one
+    cloud: point fields in, adjoint seeds in pointb, a reverse routine called, input
+    adjoints read back.  Built only where the reference sources are present."""
+
+    FIELD_IDS = dict(prim=1, q=2, dq=3, qm=4, delta=5, nx=6, ny=7, flux_res=8, x=10, y=11)
+    SIZES = dict(prim=4, q=4, dq=8, qm=8, flux_res=4)
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(TAP_LIB_PATH)
+
+    def __init__(self, cloud, power=0.0, vl_const=150.0, interior_points_normal_flag=0, format=1):
+        self.lib = lib = C.CDLL(TAP_LIB_PATH)
+        lib.tap_setup.argtypes = [C.c_int32, C.c_int32] + [C.c_void_p] * 9 + \
+            [C.c_double, C.c_double, C.c_int32, C.c_int32]
+        lib.tap_set_split.argtypes = [C.c_int32] + [C.c_void_p] * 8
+        lib.tap_field.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        lib.tap_flux_b.argtypes = [C.c_int32] + [C.c_double] * 6 + [C.c_void_p, C.c_void_p]
+        lib.tap_qtilde_to_primitive_b.argtypes = [C.c_void_p] * 3
+        lib.tap_qtilde_to_primitive.argtypes = [C.c_void_p] * 2
+        lib.tap_venkat_b.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        self.n, self.w = cloud.n, cloud.max_nbhs
+        self._keep = [np.ascontiguousarray(a) for a in (
+            cloud.x.astype(np.float64), cloud.y.astype(np.float64),
+            cloud.left.astype(np.int32), cloud.right.astype(np.int32),
+            cloud.flag_1.astype(np.int32), cloud.flag_2.astype(np.int32),
+            cloud.min_dist.astype(np.float64), cloud.nbhs.astype(np.int32),
+            np.where(cloud.conn >= 0, cloud.conn, 0).astype(np.int32))]
+        lib.tap_setup(self.n, self.w, *[a.ctypes.data for a in self._keep], power, vl_const,
+                      interior_points_normal_flag, format)
+
+    def set_split(self, orc: Oracle):
+        arrs = [np.ascontiguousarray(orc.ifield(f).astype(np.int32)) for f in (
+            "xpos_nbhs", "xneg_nbhs", "ypos_nbhs", "yneg_nbhs",
+            "xpos_conn", "xneg_conn", "ypos_conn", "yneg_conn")]
+        arrs = [np.where(a >= 0, a, 0).astype(np.int32) for a in arrs]
+        self._split = arrs
+        self.lib.tap_set_split(self.w, *[a.ctypes.data for a in arrs])
+
+    def _io(self, name, adj, arr=None):
+        size = self.SIZES.get(name, 1) * self.n
+        if arr is None:
+            a = np.zeros(size)
+            self.lib.tap_field(self.FIELD_IDS[name], adj, 1, a.ctypes.data)
+            return a
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        assert a.size == size, (name, a.size, size)
+        self.lib.tap_field(self.FIELD_IDS[name], adj, 0, a.ctypes.data)
+
+    def set(self, name, arr):
+        self._io(name, 0, arr)
+
+    def get(self, name):
+        return self._io(name, 0)
+
+    def set_adj(self, name, arr):
+        self._io(name, 1, arr)
+
+    def get_adj(self, name):
+        return self._io(name, 1)
+
+    def zero_adjoints(self):
+        self.lib.tap_zero_adjoints()
+
+    def call(self, routine: str):
+        getattr(self.lib, "tap_" + routine)()
+
+    def flux_b(self, kind, nx, ny, u1, u2, rho, pr, gb):
+        """Adjoints (nx, ny, u1, u2, rho, pr) of one flux kind for output adjoint gb."""
+        gb = np.ascontiguousarray(gb, dtype=np.float64)
+        out = np.zeros(6)
+        self.lib.tap_flux_b(kind, nx, ny, u1, u2, rho, pr, gb.ctypes.data, out.ctypes.data)
+        return out
+
+    def qtilde_to_primitive(self, qt):
+        """(rho, u1, u2, pr) of q-tilde, by the differentiated module's forward."""
+        qt = np.ascontiguousarray(qt, dtype=np.float64)
+        out = np.zeros(4)
+        self.lib.tap_qtilde_to_primitive(qt.ctypes.data, out.ctypes.data)
+        return out
+
+    def qtilde_to_primitive_b(self, qt, pb):
+        """q-tilde adjoint for adjoints pb = (rho, u1, u2, pr) of the primitives."""
+        qt = np.ascontiguousarray(qt, dtype=np.float64)
+        pb = np.ascontiguousarray(pb, dtype=np.float64)
+        out = np.zeros(4)
+        self.lib.tap_qtilde_to_primitive_b(qt.ctypes.data, pb.ctypes.data, out.ctypes.data)
+        return out
+
+    def venkat_b(self, k0: int, qt, phib):
+        """q-tilde adjoint of venkat_limiter at 0-based point k0; the q and qm adjoints
+        of the point are added into pointb."""
+        qt = np.ascontiguousarray(qt, dtype=np.float64)
+        phib = np.ascontiguousarray(phib, dtype=np.float64)
+        out = np.zeros(4)
+        self.lib.tap_venkat_b(k0 + 1, qt.ctypes.data, phib.ctypes.data, out.ctypes.data)
+        return out

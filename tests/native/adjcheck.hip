@@ -127,6 +127,15 @@ void ac_qprim_rev(const double* p, const double* qb, double* pb) {
   q_of_prim_rev(V4d{p[0], p[1], p[2], p[3]}, qb, pb);
 }
 
+// venkat_limiter reverse at one point for all four components: in qt[4] q[4] qmax[4]
+// qmin[4] eps, phib[4] -> out qtb[4] qb[4] qmaxb[4] qminb[4] (16 values)
+void ac_venkat_rev(const double* in, const double* phib, double* out) {
+  for (int j = 0; j < 16; ++j) out[j] = 0.0;
+  for (int r = 0; r < 4; ++r)
+    venkat_rev(in[r], in[4 + r], in[8 + r], in[12 + r], in[16], phib[r], out[r], out[4 + r], out[8 + r],
+               out[12 + r]);
+}
+
 double ac_wave_fwd(const double* p) { return wave_speed<double>(V4d{p[0], p[1], p[2], p[3]}); }
 
 void ac_wave_rev(const double* p, double cb, double* pb) {
