@@ -208,7 +208,10 @@ int mf_layout(mf_ctx *ctx, mf_layout_info *info);
  * returns J; mf_adjoint_backward runs the reverse sweep and returns dJ/dx, dJ/dy per
  * point in original order (the reference's pointb%x, pointb%y, written to
  * sensitivity/sensitivity.dat by src_mpi_adjoint/post_processing.F90:41-66).  The
- * context is left holding the final primal state.  FP64 contexts with power = 0.
+ * context is left holding the final primal state.  Contexts with power = 0.  On an
+ * MF_MIXED context the primal sweep and the taped re-runs run the mixed path's FP32
+ * kernels; their records are widened to FP64 and the reverse sweep and every adjoint
+ * sum are FP64 (the derivative of the FP64 iteration along the mixed trajectory).
  * snaps = MF_ADJ_TAPE_ALL tapes every iteration's stage states during the primal
  * sweep instead (about 1.3 kB per point per iteration at 2 inner iterations; refused
  * when that exceeds the free device memory), so the reverse sweep re-runs nothing. */
@@ -249,6 +252,18 @@ int mf_compute_cl_cd_cm_noalloc(int32_t n_points, int32_t n_shape, int32_t shape
                                 const double *ny, const double *prim, const int32_t *left,
                                 const int32_t *right, const int32_t *flag_2,
                                 const int32_t *shape_idx, double *clcdcm, double *cp, void *stream);
+
+/* Tangent (forward) mode of the iteration (replaces the Tapenade forward mode of
+ * src_mpi_forward: q_lskum_d.F90:28-60 seeds pointd%x/y, fpi_solver_d.F90:27-85 carries
+ * the tangents through each iteration).  From the context's current state, `iters`
+ * iterations with the point coordinates moving along (vx, vy) (original order; the
+ * reference seeds one coordinate, pointd%x(1) = 1): *J = sum over shapes of Cd after
+ * them and *dJ its derivative along (vx, vy); prim_d (4,N), if not NULL, receives the
+ * tangent of the final prim.  FP64 dual numbers on the device (a mixed context's state
+ * is widened first); any power; single-case contexts.  The context's state is left as
+ * it was. */
+int mf_tangent(mf_ctx *ctx, int32_t iters, const double *vx, const double *vy, double *J, double *dJ,
+               double *prim_d);
 
 /* Measurement aid with no counterpart in the reference: a plain device copy of
  * n_bytes (each byte read once and written once), `reps` times, timed with HIP

@@ -108,6 +108,7 @@ SIGNATURES = [
     ("mf_autotune", C.c_int, [_VP, C.c_int32, C.POINTER(MfTuning)]),
     ("mf_get_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
     ("mf_set_tuning", C.c_int, [_VP, C.POINTER(MfTuning)]),
+    ("mf_tangent", C.c_int, [_VP, C.c_int32, _DP, _DP, _DP, _DP, _DP]),
     ("mf_copy_probe", C.c_int, [C.c_int64, C.c_int32, _DP]),
     # device pointers and the stream travel as plain addresses
     ("mf_compute_cl_cd_cm_noalloc", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double]

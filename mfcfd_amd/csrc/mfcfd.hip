@@ -173,6 +173,7 @@ struct mf_ctx {
   // device arrays
   std::vector<void*> allocs;
   void* eps = nullptr;
+  const double* eps64 = nullptr;  // epsilon in FP64 (the adjoint's reverse kernels)
   void* prim[2] = {nullptr, nullptr};
   void* q[2] = {nullptr, nullptr};
   void* dq[2] = {nullptr, nullptr};
@@ -193,6 +194,7 @@ struct mf_ctx {
   std::vector<int32_t> left_i, right_i, h_sh_m;
   std::vector<V4d> h_hxx, h_hyy, h_hxy;
   struct AdjState* adj = nullptr;
+  struct TanState* tan = nullptr;  // mf_tangent's dual-number state
   // lanes per point of the stencil kernels (1, 2, 4 or 8; mf_autotune picks them)
   int lpp_lsq = 1, lpp_inner = 1, lpp_flux = 1;
   int lpp_rev[4] = {0, 0, 0, 0};  // adjoint: flux, lsq, inner, gather (0: timed)
@@ -361,11 +363,13 @@ void mf_default_params(mf_params* p) {
 struct AdjState;
 static void adj_release(mf_ctx* c);
 static void adj_forget_tuning(mf_ctx* c);
+static void tan_release(mf_ctx* c);
 
 int mf_destroy(mf_ctx* c) {
   if (!c) return MF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   adj_release(c);
+  tan_release(c);
   for (auto& gexec : c->graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
   for (void* a : c->allocs) (void)hipFree(a);
@@ -755,11 +759,13 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
     if ((rc = upload(c, &d_df, df))) return rc;
     c->g.dg_f = d_df;
   }
-  if (c->ssz == 8) {
+  {
     double* e;
     if ((rc = upload(c, &e, eps_d))) return rc;
+    c->eps64 = e;
     c->eps = e;
-  } else {
+  }
+  if (c->ssz == 4) {
     std::vector<float> ef(eps_d.begin(), eps_d.end());
     float* e;
     if ((rc = upload(c, &e, ef))) return rc;
@@ -869,10 +875,12 @@ static void launch_lsq_on(mf_ctx* c, const V4<S>* q, V4<S>* dq, V4<S>* qm, const
   launch_lsq_any<S>(env_of(c), q, dq, qm, cspeed, delta, with_delta, ctl, lpp, qsum);
 }
 
+// The mixed path's LSQ kernel also stores its first-pass sums (qsum) for the inner
+// sweeps; an iteration without inner sweeps skips that store (32 bytes per point).
 template <class S>
-static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl) {
+static void launch_lsq(mf_ctx* c, int qbuf, bool with_delta, const Ctl* ctl, bool sums = true) {
   launch_lsq_on<S>(c, (const V4<S>*)c->q[qbuf], (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, with_delta,
-                   ctl, c->lpp_lsq, c->qsum);
+                   ctl, c->lpp_lsq, sums ? c->qsum : nullptr);
 }
 
 // one inner iteration: dq_in -> dq_out
@@ -955,7 +963,7 @@ static int record_iteration(mf_ctx* c, int phase, Timer* tm) {
   const int I = c->p.inner_iterations;
   for (int rk = 1; rk <= c->p.rks; ++rk) {
     if (tm) tm->begin(KK_LSQ);
-    launch_lsq<S>(c, buf, rk == 1, c->ctl);
+    launch_lsq<S>(c, buf, rk == 1, c->ctl, I > 0);
     if (tm) tm->end(KK_LSQ);
     int d = 0;
     for (int it = 0; it < I; ++it) {
@@ -1192,7 +1200,8 @@ static int time_shape(mf_ctx* c, int kind, int lpp, int reps, double* us) {
   auto q = (const V4<S>*)c->q[cur];
   if (kind == 0)
     return time_launches(c, reps, [&] {
-      launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp, c->qsum);
+      launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp,
+                       c->p.inner_iterations > 0 ? c->qsum : nullptr);
     }, us);
   if (kind == 1)
     return time_launches(c, reps, [&] {
@@ -1595,6 +1604,7 @@ int mf_layout(mf_ctx* c, mf_layout_info* info) {
 }  // extern "C"
 
 #include "mfcfd_adjoint.hip"
+#include "mfcfd_tangent.hip"
 
 static void adj_release(mf_ctx* c) {
   if (!c->adj) return;

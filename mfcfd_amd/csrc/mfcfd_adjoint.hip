@@ -857,6 +857,14 @@ __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, int
     y[j] += x[j];
 }
 
+// FP32 records into FP64 (the mixed path's tape, read by the FP64 reverse kernels)
+__global__ void k_widen(const V4<float>* __restrict__ a, V4d* __restrict__ b, int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const V4<float> v = a[j];
+    b[j] = V4d{(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+  }
+}
+
 __global__ void k_zero(double* __restrict__ p, int64_t n) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
     p[j] = 0.0;
@@ -926,7 +934,21 @@ struct AdjState {
   std::vector<V4d*> base_prim, base_q, base_dq, base_qm;
   V4d* base_pold = nullptr;
   double* base_delta = nullptr;
+  // Mixed-precision contexts: the taped re-run runs the mixed path's FP32 kernels into
+  // FP32 records (fprim, fq, fdq, fqm, fpold, laid out as tprim ... tpold), which are
+  // then widened into the FP64 tape above; the reverse kernels and every adjoint sum
+  // stay FP64.  With every iteration taped, the kept tapes are FP32 (half the memory)
+  // and each is widened just before its reverse.
+  bool mixed = false;
+  std::vector<V4f*> fprim, fq, fdq, fqm;
+  V4f* fpold = nullptr;
+  std::vector<V4f*> fbase_prim, fbase_q, fbase_dq, fbase_qm;
+  V4f* fbase_pold = nullptr;
 };
+
+// Input and output state records of the taped iteration in the context's precision
+static void* tape_in(AdjState& A) { return A.mixed ? (void*)A.fprim[0] : (void*)A.tprim[0]; }
+static void* tape_out(AdjState& A) { return A.mixed ? (void*)A.fprim[A.S] : (void*)A.tprim[A.S]; }
 
 static int adj_zero(mf_ctx* c, void* p, int64_t n_doubles) {
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(BLOCK), 0, c->stream, (double*)p, n_doubles);
@@ -934,8 +956,52 @@ static int adj_zero(mf_ctx* c, void* p, int64_t n_doubles) {
   return MF_OK;
 }
 
-// one taped forward iteration from A.tprim[0]
+static void adj_widen_n(mf_ctx* c, const V4f* a, V4d* b, int64_t n) {
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)std::min<int64_t>((n + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK), 0,
+                     c->stream, a, b, n);
+}
+
+// the FP32 records of the taped iteration into the FP64 tape
+static int adj_widen(mf_ctx* c, AdjState& A) {
+  const int64_t n = c->n;
+  for (int s = 0; s <= A.S; ++s) adj_widen_n(c, A.fprim[s], A.tprim[s], n);
+  for (int s = 0; s < A.S; ++s) {
+    adj_widen_n(c, A.fq[s], A.tq[s], n);
+    adj_widen_n(c, A.fqm[s], A.tqm[s], 2 * n);
+  }
+  for (size_t k = 0; k < A.fdq.size(); ++k) adj_widen_n(c, A.fdq[k], A.tdq[k], 2 * n);
+  adj_widen_n(c, A.fpold, A.tpold, n);
+  HIPCHK(hipGetLastError());
+  return MF_OK;
+}
+
+// one taped forward iteration of a mixed-precision context from A.fprim[0], by the
+// mixed path's own kernels (the same launches as an iteration of the context), into
+// the FP32 records; widened into the FP64 tape when `widen`
+static int adj_record_mixed(mf_ctx* c, AdjState& A, bool widen) {
+  // q and the wave speeds by the mixed path's own unit (compiled with contraction, as
+  // the kernels that produce them inside a graph-replayed iteration)
+  launch_q_variables_any<float>(env_of(c), (const V4f*)A.fprim[0], A.fq[0], A.tcspeed);
+  V4<float>* sums = A.I > 0 ? c->qsum : nullptr;
+  for (int s = 0; s < A.S; ++s) {
+    V4f* dq0 = A.fdq[s * (A.I + 1)];
+    launch_lsq_on<float>(c, (const V4f*)A.fq[s], dq0, A.fqm[s], A.tcspeed, A.tdelta, s == 0, nullptr, c->lpp_lsq,
+                         sums);
+    for (int j = 1; j <= A.I; ++j)
+      launch_inner_on<float>(c, (const V4f*)A.fq[s], (const V4f*)A.fdq[s * (A.I + 1) + j - 1],
+                             A.fdq[s * (A.I + 1) + j], nullptr, c->lpp_inner, sums);
+    State<float> st{A.fq[s], A.fdq[s * (A.I + 1) + A.I], A.fqm[s], (const float*)c->eps};
+    launch_flux_update_on<float>(c, st, (const V4f*)A.fprim[s], A.fpold, A.fprim[s + 1], A.fq[s + 1], A.tcspeed,
+                                 (const double*)A.tdelta, A.tpartial, (const Ctl*)A.tctl, s + 1, s + 1 == A.S,
+                                 c->lpp_flux);
+  }
+  HIPCHK(hipGetLastError());
+  return widen ? adj_widen(c, A) : MF_OK;
+}
+
+// one taped forward iteration from the tape's input record
 static int adj_record(mf_ctx* c, AdjState& A) {
+  if (A.mixed) return adj_record_mixed(c, A, true);
   const int n = c->n;
   const dim3 bl(BLOCK);
   hipLaunchKernelGGL((k_q_variables<double>), dim3(nblocks(c)), bl, 0, c->stream, n,
@@ -946,7 +1012,7 @@ static int adj_record(mf_ctx* c, AdjState& A) {
     for (int j = 1; j <= A.I; ++j)
       launch_inner_on<double>(c, (const V4d*)A.tq[s], (const V4d*)A.tdq[s * (A.I + 1) + j - 1],
                               A.tdq[s * (A.I + 1) + j], nullptr, c->lpp_inner);
-    State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
+    State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], c->eps64};
     launch_flux_update_on<double>(c, st, (const V4d*)A.tprim[s], A.tpold, A.tprim[s + 1], A.tq[s + 1], A.tcspeed,
                                   (const double*)A.tdelta, A.tpartial, (const Ctl*)A.tctl, s + 1, s + 1 == A.S,
                                   c->lpp_flux);
@@ -958,7 +1024,7 @@ static int adj_record(mf_ctx* c, AdjState& A) {
 // launchers of the reverse kernels at a given lanes-per-point shape
 static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lpp) {
   const HalfSums hs{A.hxx, A.hyy, A.hxy};
-  const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], (const double*)c->eps};
+  const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], c->eps64};
   const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
   if (lpp & LPP_STAGED) {
     // staged: blocks of STAGE_PF points, 2, 4 or 8 lanes each
@@ -1084,7 +1150,7 @@ static void adj_forget_tuning(mf_ctx* c) {
 static int adj_tune(mf_ctx* c, AdjState& A) {
   int rc;
   const int n = c->n;
-  if ((rc = adj_copy(c, A.tprim[0], c->prim[c->cur])) || (rc = adj_record(c, A))) return rc;
+  if ((rc = adj_copy(c, tape_in(A), c->prim[c->cur])) || (rc = adj_record(c, A))) return rc;
   if ((rc = adj_zero(c, A.frb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.dqb[0], 8 * (int64_t)n)) ||
       (rc = adj_zero(c, A.dqb[1], 8 * (int64_t)n)) || (rc = adj_zero(c, A.qmb, 8 * (int64_t)n)) ||
       (rc = adj_zero(c, A.qb, 4 * (int64_t)n)) || (rc = adj_zero(c, A.deltab, n)) ||
@@ -1162,15 +1228,34 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
   int rc;
   if (A.ready && (A.S != S || A.I != I))
     return set_err(MF_ERR_STATE, "adjoint buffers already sized for a different scheme");
+  // snapshots hold the context's state records (FP64, or FP32 for a mixed context)
+  const size_t snap_v4 = c->ssz == 8 ? (size_t)n : ((size_t)n + 1) / 2;
   if (A.ready) {
     // a schedule with more snapshots than any before on this context: add slots
     while ((int)A.snap.size() < snaps) {
       V4d* p = nullptr;
-      if ((rc = dev_alloc(c, &p, n))) return rc;
+      if ((rc = dev_alloc(c, &p, snap_v4))) return rc;
       A.snap.push_back(p);
       A.snap_t.push_back(-1);
     }
     return MF_OK;
+  }
+  A.mixed = c->ssz == 4;
+  if (A.mixed) {
+    A.fprim.assign(S + 1, nullptr);
+    A.fq.assign(S + 1, nullptr);
+    A.fdq.assign((size_t)S * (I + 1), nullptr);
+    A.fqm.assign(S, nullptr);
+    for (auto& p : A.fprim) if ((rc = dev_alloc(c, &p, n))) return rc;
+    for (auto& p : A.fq) if ((rc = dev_alloc(c, &p, n))) return rc;
+    for (auto& p : A.fdq) if ((rc = dev_alloc(c, &p, 2 * (size_t)n))) return rc;
+    for (auto& p : A.fqm) if ((rc = dev_alloc(c, &p, 2 * (size_t)n))) return rc;
+    if ((rc = dev_alloc(c, &A.fpold, n))) return rc;
+    A.fbase_prim = A.fprim;
+    A.fbase_q = A.fq;
+    A.fbase_dq = A.fdq;
+    A.fbase_qm = A.fqm;
+    A.fbase_pold = A.fpold;
   }
   A.S = S;
   A.I = I;
@@ -1243,7 +1328,7 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
     if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1)))) return rc;
   }
   A.snap.assign(snaps, nullptr);
-  for (auto& p : A.snap) if ((rc = dev_alloc(c, &p, n))) return rc;
+  for (auto& p : A.snap) if ((rc = dev_alloc(c, &p, snap_v4))) return rc;
   A.snap_t.assign(snaps, -1);
   HIPCHK(hipEventCreate(&A.e0));
   HIPCHK(hipEventCreate(&A.e1));
@@ -1259,11 +1344,33 @@ static void bind_base(AdjState& A) {
   A.tqm = A.base_qm;
   A.tpold = A.base_pold;
   A.tdelta = A.base_delta;
+  if (A.mixed) {
+    A.fprim = A.fbase_prim;
+    A.fq = A.fbase_q;
+    A.fdq = A.fbase_dq;
+    A.fqm = A.fbase_qm;
+    A.fpold = A.fbase_pold;
+  }
 }
 
 // The one-iteration tape pointed at iteration t of the all-iterations tape
 static void bind_tape(AdjState& A, int t) {
   const AdjState::Tape& T = A.tapes[t];
+  if (A.mixed) {
+    // the kept records are FP32: point the FP32 records at them; the FP64 tape stays
+    // on its own buffers and is filled by widening
+    A.fprim[0] = t == 0 ? A.fbase_prim[0] : (V4f*)A.tapes[t - 1].prim[A.S - 1];
+    for (int s = 0; s < A.S; ++s) {
+      A.fprim[s + 1] = (V4f*)T.prim[s];
+      A.fq[s] = (V4f*)T.q[s];
+      A.fqm[s] = (V4f*)T.qm[s];
+    }
+    A.fq[A.S] = A.fbase_q[A.S];
+    for (size_t k = 0; k < T.dq.size(); ++k) A.fdq[k] = (V4f*)T.dq[k];
+    A.fpold = (V4f*)T.pold;
+    A.tdelta = T.delta;
+    return;
+  }
   A.tprim[0] = t == 0 ? A.base_prim[0] : A.tapes[t - 1].prim[A.S - 1];
   for (int s = 0; s < A.S; ++s) {
     A.tprim[s + 1] = T.prim[s];
@@ -1281,7 +1388,9 @@ static void bind_tape(AdjState& A, int t) {
 static int tape_alloc(mf_ctx* c, AdjState& A, int iters) {
   const size_t n = (size_t)c->n, S = (size_t)A.S, I = (size_t)A.I;
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  const size_t v4 = up(n * sizeof(V4d)), v8 = up(2 * n * sizeof(V4d)), dl = up(n * sizeof(double));
+  // records in the context's precision (FP32 for a mixed context)
+  const size_t rec = 4 * (size_t)c->ssz;
+  const size_t v4 = up(n * rec), v8 = up(2 * n * rec), dl = up(n * sizeof(double));
   const size_t per = S * 2 * v4 + S * (I + 1) * v8 + S * v8 + v4 + dl;
   const size_t need = per * (size_t)iters;
   if (need > A.tape_cap) {
@@ -1322,8 +1431,10 @@ static int tape_alloc(mf_ctx* c, AdjState& A, int iters) {
   return MF_OK;
 }
 
+// copy of one state record array in the context's precision (prim, a snapshot, the
+// taped iteration's input or output)
 static int adj_copy(mf_ctx* c, void* dst, const void* src) {
-  HIPCHK(hipMemcpyAsync(dst, src, (size_t)c->n * sizeof(V4d), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)c->n * 4 * c->ssz, hipMemcpyDeviceToDevice, c->stream));
   return MF_OK;
 }
 
@@ -1351,13 +1462,13 @@ static int adj_taped_reverse(mf_ctx* c, AdjState& A) {
 static int adj_rev_range(mf_ctx* c, AdjState& A, int t0, int t1, int slot0, int free_slots) {
   int rc;
   if (t1 - t0 == 1) {
-    if ((rc = adj_copy(c, A.tprim[0], A.snap[slot0]))) return rc;
+    if ((rc = adj_copy(c, tape_in(A), A.snap[slot0]))) return rc;
     return adj_taped_reverse(c, A);
   }
   if (free_slots == 0) {
     for (int t = t1 - 1; t >= t0; --t) {
       if ((rc = adj_copy(c, c->prim[c->cur], A.snap[slot0])) || (rc = adj_advance(c, A, t0, t - t0)) ||
-          (rc = adj_copy(c, A.tprim[0], c->prim[c->cur])) || (rc = adj_taped_reverse(c, A)))
+          (rc = adj_copy(c, tape_in(A), c->prim[c->cur])) || (rc = adj_taped_reverse(c, A)))
         return rc;
     }
     return MF_OK;
@@ -1390,7 +1501,6 @@ extern "C" {
 
 int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps, double* J) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
-  if (c->ssz != 8) return set_err(MF_ERR_STATE, "the adjoint needs an MF_FP64 context");
   if (!c->power0) return set_err(MF_ERR_STATE, "the adjoint supports power = 0 only");
   if (iters < 1 || (snaps < 1 && snaps != MF_ADJ_TAPE_ALL))
     return set_err(MF_ERR_INVALID, "iters must be >= 1 and snaps >= 1 or MF_ADJ_TAPE_ALL");
@@ -1418,12 +1528,13 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
     // the primal sweep as taped iterations, each kept for the reverse
     if ((rc = tape_alloc(c, A, iters))) return rc;
     HIPCHK(hipEventRecord(A.e0, c->stream));
-    if ((rc = adj_copy(c, A.base_prim[0], c->prim[c->cur]))) return rc;
+    if ((rc = adj_copy(c, A.mixed ? (void*)A.fbase_prim[0] : (void*)A.base_prim[0], c->prim[c->cur]))) return rc;
     for (int t = 0; t < iters; ++t) {
       bind_tape(A, t);
-      if ((rc = adj_record(c, A))) return rc;
+      // a mixed context keeps the FP32 records; each is widened before its reverse
+      if ((rc = A.mixed ? adj_record_mixed(c, A, false) : adj_record(c, A))) return rc;
     }
-    if ((rc = adj_copy(c, c->prim[c->cur], A.tprim[A.S])) || (rc = q_variables(c))) return rc;
+    if ((rc = adj_copy(c, c->prim[c->cur], tape_out(A))) || (rc = q_variables(c))) return rc;
     HIPCHK(hipEventRecord(A.e1, c->stream));
     HIPCHK(hipEventSynchronize(A.e1));
     HIPCHK(hipEventElapsedTime(&A.primal_ms, A.e0, A.e1));
@@ -1487,23 +1598,30 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
   HIPCHK(hipEventCreate(&b1));
   HIPCHK(hipEventRecord(b0, c->stream));
   // keep the final state; the recursion reuses the context's prim as its work buffer
-  V4d* fin = A.Pstage;  // borrowed until the sweep starts
-  (void)fin;
-  std::vector<V4d> final_state(n);
+  const size_t state_bytes = (size_t)n * 4 * c->ssz;
+  std::vector<char> final_state(state_bytes);
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(final_state.data(), c->prim[c->cur], n * sizeof(V4d), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(final_state.data(), c->prim[c->cur], state_bytes, hipMemcpyDeviceToHost));
+  // the final state in FP64 for the force integral's reverse (a mixed context's is
+  // widened into the tape's first record, free until the sweep starts)
+  const V4d* fin = (const V4d*)c->prim[c->cur];
+  if (A.mixed) {
+    adj_widen_n(c, (const V4f*)c->prim[c->cur], A.base_prim[0], n);
+    fin = A.base_prim[0];
+  }
   const int cur0 = c->cur;
   if ((rc = adj_zero(c, A.xyb, 2 * (int64_t)n)) || (rc = adj_zero(c, A.nrmb, 2 * (int64_t)n)) ||
       (rc = adj_zero(c, A.Pb, 4 * (int64_t)n)))
     return rc;
   // seed: J = sum of Cd over shapes on the final state (objective_function_b.F90:22-23)
   hipLaunchKernelGGL(k_forces_rev, dim3((c->nshape + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream,
-                     (const V4d*)c->prim[c->cur], c->sh_m, c->sh_id, A.sh_l, A.sh_r, c->sgeo, c->nshape,
+                     fin, c->sh_m, c->sh_id, A.sh_l, A.sh_r, c->sgeo, c->nshape,
                      (const double*)c->shape_par, c->g.xy, A.Pb, A.xyb, A.nrmb);
   HIPCHK(hipGetLastError());
   if (A.tape_all) {
     for (int t = A.iters - 1; t >= 0; --t) {
       bind_tape(A, t);
+      if (A.mixed && (rc = adj_widen(c, A))) return rc;
       if ((rc = adj_reverse_iteration(c, A))) return rc;
       A.taped += 1;
       if (A.progress_every > 0 && A.taped % A.progress_every == 0) {
@@ -1540,7 +1658,7 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
     if (dJdy) dJdy[c->perm[j]] = h[j].y;
   }
   c->cur = cur0;
-  HIPCHK(hipMemcpy(c->prim[c->cur], final_state.data(), n * sizeof(V4d), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->prim[c->cur], final_state.data(), state_bytes, hipMemcpyHostToDevice));
   if (q_variables(c)) return MF_ERR_HIP;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (stats) {

@@ -233,6 +233,18 @@ class LskumSolver:
                                    _dp(dx), _dp(dy), C.byref(st)), "mf_adjoint")
         return J.value, dx, dy, {f: getattr(st, f) for f, _ in L.MfAdjointStats._fields_}
 
+    def tangent(self, iters: int, vx, vy, prim_tangent: bool = False):
+        """Forward mode: J = sum of Cd after `iters` iterations from the current state
+        and its derivative along the coordinate direction (vx, vy); with prim_tangent,
+        also the final prim's tangent (N, 4).  The state is left unchanged."""
+        vx = np.ascontiguousarray(vx, dtype=np.float64)
+        vy = np.ascontiguousarray(vy, dtype=np.float64)
+        J, dJ = C.c_double(), C.c_double()
+        pd = np.empty((self.n, 4)) if prim_tangent else None
+        L.check(L.lib().mf_tangent(self.h, int(iters), _dp(vx), _dp(vy), C.byref(J), C.byref(dJ),
+                                   _dp(pd) if pd is not None else None), "mf_tangent")
+        return (J.value, dJ.value, pd) if prim_tangent else (J.value, dJ.value)
+
     def adjoint_forward(self, iters: int, snaps: int = 32, first_it: int = 1) -> float:
         J = C.c_double()
         L.check(L.lib().mf_adjoint_forward(self.h, int(first_it), int(iters), int(snaps),

@@ -173,13 +173,18 @@ def test_adjoint_matches_tangent_c3():
     from mfcfd_amd.solver import LskumSolver
     iters = ref["iterations"]
     print(f"[adjoint] c3: {c.n} points, {iters} iterations", flush=True)
-    with LskumSolver(c, **p) as g:
-        J, dJdx, dJdy, st = g.adjoint(iters, snaps=iters)
     vx, vy = M.direction(c.n)
+    with LskumSolver(c, **p) as g:
+        Jt, dJt = g.tangent(iters, vx, vy)
+        J, dJdx, dJdy, st = g.adjoint(iters, snaps=iters)
     adj = float(dJdx @ vx + dJdy @ vy)
-    print(f"J {J:.15e} vs {ref['J']:.15e}; directional {adj:.14e} vs tangent {ref['dJ']:.14e}")
+    print(f"J {J:.15e} vs {ref['J']:.15e}; directional {adj:.14e} vs oracle tangent {ref['dJ']:.14e}; "
+          f"device tangent {dJt:.14e} (J {Jt:.15e})")
     assert abs(J - ref["J"]) <= 1e-12 * abs(ref["J"])
     assert abs(adj - ref["dJ"]) <= 1e-10 * abs(ref["dJ"])
+    # the device tangent mode against the adjoint: w (J v) = (J^T w) v with w = 1
+    assert abs(Jt - J) <= 1e-10 * abs(J)
+    assert abs(dJt - adj) <= 1e-10 * abs(adj)
 
 
 @pytest.mark.parametrize("kernel,lpp", [("flux", 2), ("flux", 4), ("flux", 8), ("inner", 1), ("inner", 2),
@@ -219,3 +224,83 @@ def test_adjoint_staged_reverse_overflow(c1_cloud, kernel):
         Jb, xb, yb, _ = b.adjoint(6, snaps=2)
     ref = np.concatenate([xa, ya])
     assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("snaps", [4, -1])
+def test_mixed_adjoint_close_to_fp64(small_cloud, snaps):
+    """Mixed-precision adjoint: the primal sweep and the taped re-runs run the mixed
+    path's FP32 kernels, their records widened into the FP64 tape, and the reverse
+    kernels and every adjoint sum stay FP64.  Against the FP64 adjoint over the same
+    20 iterations: J within 1e-5 relative, the gradient within 1e-3 of its largest
+    entry, and a directional derivative within 1e-3.  The binomial (4 snapshots) and
+    tape-everything schedules give the same J bits (the re-runs repeat the primal's
+    launches) and the same gradient up to the order of the sweep's few FP64 atomics."""
+    iters = 20
+    with _gpu(small_cloud) as a:
+        J0, gx0, gy0, _ = a.adjoint(iters, snaps=4)
+    with _gpu(small_cloud, precision="mixed") as b:
+        J1, gx1, gy1, st = b.adjoint(iters, snaps=snaps)
+    g0 = np.concatenate([gx0, gy0])
+    g1 = np.concatenate([gx1, gy1])
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(g0.size)
+    rel_g = float(np.abs(g1 - g0).max() / np.abs(g0).max())
+    rel_d = abs(float(g1 @ v - g0 @ v)) / abs(float(g0 @ v))
+    print(f"[adjoint] mixed ({snaps}): J {J1:.10e} vs {J0:.10e}; gradient rel {rel_g:.2e}; "
+          f"directional rel {rel_d:.2e}; stats {st}", flush=True)
+    assert abs(J1 - J0) <= 1e-5 * abs(J0)
+    assert rel_g <= 1e-3 and rel_d <= 1e-3
+    assert np.all(np.isfinite(g1))
+    if snaps == -1:
+        with _gpu(small_cloud, precision="mixed") as c:
+            J2, gx2, gy2, _ = c.adjoint(iters, snaps=4)
+        g2 = np.concatenate([gx2, gy2])
+        assert J2 == J1 and np.abs(g2 - g1).max() <= 1e-12 * np.abs(g1).max()
+
+
+@pytest.mark.parametrize("variant", ["second_inner2", "first_order", "tscheme_first", "power_m2"])
+def test_device_tangent_matches_oracle(small_cloud, variant):
+    """mf_tangent (device forward mode, dual numbers) against the oracle's dual-number
+    tangent on 20 iterations: J and the directional derivative to 1e-10 relative; the
+    final prim's tangent against the oracle's is not exposed by the oracle, so it is
+    checked by linearity (twice the direction, twice the tangent).  power = -2 has no
+    adjoint (the reverse supports power = 0); the tangent covers it."""
+    from oracle.oracle import tangent_cd
+    kw = {"second_inner2": {}, "first_order": dict(first_order=1, inner_iterations=0),
+          "tscheme_first": dict(inner_iterations=1), "power_m2": dict(power=-2.0)}[variant]
+    okw = dict(kw)
+    gkw = dict(kw)
+    if variant == "tscheme_first":
+        okw.update(rks=1, euler=2.0)
+        gkw["tscheme"] = "first"
+    iters = 20
+    rng = np.random.default_rng(17)
+    vx, vy = rng.standard_normal(small_cloud.n), rng.standard_normal(small_cloud.n)
+    print(f"[tangent] {variant}: {small_cloud.n} points, {iters} iterations", flush=True)
+    with _gpu(small_cloud, **gkw) as g:
+        p0 = g.prim
+        J, dJ, pd = g.tangent(iters, vx, vy, prim_tangent=True)
+        J2, dJ2, pd2 = g.tangent(iters, 2 * vx, 2 * vy, prim_tangent=True)
+        assert np.array_equal(g.prim, p0)  # the context's state is left as it was
+    Jo, dJo = tangent_cd(small_cloud, vx, vy, iters, **_oparams(**okw))
+    print(f"J {J:.14e} vs {Jo:.14e}; dJ {dJ:.12e} vs oracle {dJo:.12e}", flush=True)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert abs(dJ - dJo) <= 1e-10 * abs(dJo)
+    assert J2 == J and abs(dJ2 - 2 * dJ) <= 1e-13 * abs(dJ)
+    assert np.abs(pd2 - 2 * pd).max() <= 1e-13 * np.abs(pd).max() and np.abs(pd).max() > 0
+
+
+def test_device_tangent_dot_adjoint(small_cloud):
+    """The device tangent against the device adjoint on the same 20 iterations:
+    dJ(v) = sum(dJ/dx vx + dJ/dy vy) to 1e-10, for three directions."""
+    iters = 20
+    with _gpu(small_cloud) as g:
+        J, dJdx, dJdy, _ = g.adjoint(iters, snaps=4)
+    rng = np.random.default_rng(23)
+    for _ in range(3):
+        vx, vy = rng.standard_normal(small_cloud.n), rng.standard_normal(small_cloud.n)
+        with _gpu(small_cloud) as g:
+            Jt, dJt = g.tangent(iters, vx, vy)
+        adj = float(dJdx @ vx + dJdy @ vy)
+        assert abs(Jt - J) <= 1e-10 * abs(J)
+        assert abs(dJt - adj) <= 1e-10 * abs(adj), (dJt, adj)
