@@ -208,7 +208,8 @@ int mf_layout(mf_ctx *ctx, mf_layout_info *info);
  * returns J; mf_adjoint_backward runs the reverse sweep and returns dJ/dx, dJ/dy per
  * point in original order (the reference's pointb%x, pointb%y, written to
  * sensitivity/sensitivity.dat by src_mpi_adjoint/post_processing.F90:41-66).  The
- * context is left holding the final primal state.  Contexts with power = 0.  On an
+ * context is left holding the final primal state.  Any power (the LSQ weights'
+ * derivatives included, q_variables_b.F90).  On an
  * MF_MIXED context the primal sweep and the taped re-runs run the mixed path's FP32
  * kernels; their records are widened to FP64 and the reverse sweep and every adjoint
  * sum are FP64 (the derivative of the FP64 iteration along the mixed trajectory).

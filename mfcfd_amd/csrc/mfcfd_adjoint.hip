@@ -66,6 +66,20 @@ struct Trans {
   const int32_t* own;  // [E]
 };
 
+// Least-squares weight w = dist^power of an edge with offset (a, b) (q_variables.F90:89-91,
+// interior_fluxes.F90:89-90) and f = power dist^(power - 2), so dw/da = f a and
+// dw/db = f b; power = 0 gives w = 1 and f = 0 exactly
+__device__ inline void lsq_weight(double a, double b, double power, double& w, double& f) {
+  if (power == 0.0) {
+    w = 1.0;
+    f = 0.0;
+    return;
+  }
+  const double d2 = a * a + b * b;
+  w = pow(sqrt(d2), power);
+  f = power * w / d2;
+}
+
 // q-tilde of one end of an edge (the point's own q when first order); p indexes the
 // records of nbrs (a point id, or a halo position)
 template <class Nbrs>
@@ -87,7 +101,7 @@ __device__ inline void end_qtilde(const Nbrs& nbrs, int p, double delx, double d
 // block's halo staged in LDS) and the point's slots.
 template <int CLS, int LPP, class Nbrs>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nbrs, int i, double delta,
-                               const V4d& frbv, const AdjOut& o, int lane) {
+                               const V4d& frbv, const AdjOut& o, int lane, double power) {
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -117,8 +131,11 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     nbrs.slot(s, k, m);
     const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
-    const double dsw = delx * tx + dely * ty;
-    const double dnw = delx * nx + dely * ny;
+    const double dels = delx * tx + dely * ty;
+    const double deln = delx * nx + dely * ny;
+    double w, wf;
+    lsq_weight(dels, deln, power, w, wf);
+    const double dsw = dels * w, dnw = deln * w;
     const bool xp = (m & XPOS) != 0u;
     const bool tie_x = xp && (m & XNEG) != 0u;
     const bool yp = (CLS == INTERIOR) ? (m & YPOS) != 0u : (CLS == OUTER);
@@ -211,11 +228,17 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     coef_rev(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
     if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
     if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
-    // dsw = delx ny - dely nx, dnw = delx nx + dely ny
-    delxb += dswb * ny + dnwb * nx;
-    delyb += -dswb * nx + dnwb * ny;
-    nxb += -dswb * dely + dnwb * delx;
-    nyb += dswb * delx + dnwb * dely;
+    // dsw = dels w, dnw = deln w; dels = delx ny - dely nx, deln = delx nx + dely ny
+    double delsb = dswb * w, delnb = dnwb * w;
+    if (wf != 0.0) {
+      const double wb = dswb * dels + dnwb * deln;
+      delsb += wb * wf * dels;
+      delnb += wb * wf * deln;
+    }
+    delxb += delsb * ny + delnb * nx;
+    delyb += -delsb * nx + delnb * ny;
+    nxb += -delsb * dely + delnb * delx;
+    nyb += delsb * delx + delnb * dely;
     xib -= delxb;
     yib -= delyb;
     ep[8] = D2{delxb, delyb};
@@ -253,11 +276,19 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
-    double delsb = 0.0, delnb = 0.0;
+    double w, wf;
+    lsq_weight(dels, deln, power, w, wf);
+    // the half sums dels^2 w, deln^2 w, dels deln w
+    double delsb = 0.0, delnb = 0.0, wb = 0.0;
     for (int d = 0; d < 4; ++d) {
       if (!(m & (1u << d))) continue;
-      delsb += 2.0 * dels * hxxb[d] + deln * hxyb[d];
-      delnb += 2.0 * deln * hyyb[d] + dels * hxyb[d];
+      delsb += w * (2.0 * dels * hxxb[d] + deln * hxyb[d]);
+      delnb += w * (2.0 * deln * hyyb[d] + dels * hxyb[d]);
+      wb += dels * dels * hxxb[d] + deln * deln * hyyb[d] + dels * deln * hxyb[d];
+    }
+    if (wf != 0.0) {
+      delsb += wb * wf * dels;
+      delnb += wb * wf * deln;
     }
     const double delxb = delsb * ny + delnb * nx;
     const double delyb = -delsb * nx + delnb * ny;
@@ -304,7 +335,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
 template <int LPP, bool STAGED>
 __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<double> st,
                                                        const double* __restrict__ delta,
-                                                       const V4d* __restrict__ frb, AdjOut o) {
+                                                       const V4d* __restrict__ frb, AdjOut o, double power) {
   if constexpr (STAGED) {
     // the block's halo of the taped stage state in LDS (blocks of STAGE_PF points)
     extern __shared__ __align__(16) unsigned char smem[];
@@ -317,23 +348,23 @@ __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, Stat
     if (i >= g.n) return;  // whole lane groups
     if (staged) {
       const StagedNbrs<double, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
-      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     } else {
       const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
-      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     }
   } else {
     const int t = point_id();
     const int i = t / LPP, lane = t % LPP;
     if (i >= g.n) return;  // whole lane groups
     const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
-    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
-    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane);
+    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
   }
 }
 
@@ -382,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
                                                      const V4d* __restrict__ dqb_out,
                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ qb,
                                                      D2* __restrict__ xyb, double* __restrict__ eb,
-                                                     const int32_t* __restrict__ epos) {
+                                                     const int32_t* __restrict__ epos, double power) {
   const PointMap<LPP, STAGED> pm(g.n);
   if (!STAGED && !pm.live) return;
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
@@ -414,19 +445,28 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   // edge in slot s to the neighbour with coordinates pk and records qk, dxk, dyk
   auto edge = [&](int s, const D2& pk, const V4d& qk, const V4d& dxk, const V4d& dyk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
+    double w, wf;
+    lsq_weight(dx, dy, power, w, wf);
+    // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
+    double dxb = w * (2.0 * sxxb * dx + sxyb * dy), dyb = w * (2.0 * syyb * dy + sxyb * dx);
+    double wb = sxxb * dx * dx + syyb * dy * dy + sxyb * dx * dy;
     double Dbv[4];
     for (int r = 0; r < 4; ++r) {
       const double t1 = qi[r] - 0.5 * (dx * dxi[r] + dy * dyi[r]);
       const double t2 = qk[r] - 0.5 * (dx * dxk[r] + dy * dyk[r]);
       const double D = t2 - t1;
-      const double Db = sqb[r] * dx + syqb[r] * dy;
-      dxb += sqb[r] * D - 0.5 * Db * (dxk[r] - dxi[r]);
-      dyb += syqb[r] * D - 0.5 * Db * (dyk[r] - dyi[r]);
+      const double Db = w * (sqb[r] * dx + syqb[r] * dy);
+      dxb += w * sqb[r] * D - 0.5 * Db * (dxk[r] - dxi[r]);
+      dyb += w * syqb[r] * D - 0.5 * Db * (dyk[r] - dyi[r]);
+      wb += D * (sqb[r] * dx + syqb[r] * dy);
       Dbv[r] = Db;
       qib[r] -= Db;
       dxib[r] += 0.5 * dx * Db;
       dyib[r] += 0.5 * dy * Db;
+    }
+    if (wf != 0.0) {
+      dxb += wb * wf * dx;
+      dyb += wb * wf * dy;
     }
     D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
     ep[0] = D2{Dbv[0], Dbv[1]};
@@ -489,7 +529,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
                                                    const V4d* __restrict__ dqb,
                                                    const V4d* __restrict__ qmb, V4d* __restrict__ qb,
                                                    D2* __restrict__ xyb, double* __restrict__ eb,
-                                                   const int32_t* __restrict__ epos) {
+                                                   const int32_t* __restrict__ epos, double power) {
   const PointMap<LPP, STAGED> pm(g.n);
   if (!STAGED && !pm.live) return;
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
@@ -526,7 +566,11 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   // edge in slot s to the neighbour with coordinates pk and q qk
   auto edge = [&](int s, const D2& pk, const V4d& qk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    double dxb = 2.0 * sxxb * dx + sxyb * dy, dyb = 2.0 * syyb * dy + sxyb * dx;
+    double w, wf;
+    lsq_weight(dx, dy, power, w, wf);
+    // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
+    double dxb = w * (2.0 * sxxb * dx + sxyb * dy), dyb = w * (2.0 * syyb * dy + sxyb * dx);
+    double wb = sxxb * dx * dx + syyb * dy * dy + sxyb * dx * dy;
     double Dbv[4];
     for (int r = 0; r < 4; ++r) {
       if (qk[r] > qmax[r]) {
@@ -538,11 +582,16 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
         smin[r] = s;
       }
       const double D = qk[r] - qi[r];
-      const double Db = sqb[r] * dx + syqb[r] * dy;
-      dxb += sqb[r] * D;
-      dyb += syqb[r] * D;
+      const double Db = w * (sqb[r] * dx + syqb[r] * dy);
+      dxb += w * sqb[r] * D;
+      dyb += w * syqb[r] * D;
+      wb += D * (sqb[r] * dx + syqb[r] * dy);
       Dbv[r] = Db;
       qib[r] -= Db;
+    }
+    if (wf != 0.0) {
+      dxb += wb * wf * dx;
+      dyb += wb * wf * dy;
     }
     D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
     ep[0] = D2{Dbv[0], Dbv[1]};
@@ -1032,7 +1081,7 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
     const size_t lds = stage_lds_flux(c->g.hl[1].hcap, c->g.width, sizeof(double));
     auto go = [&]<int L>() {
       hipLaunchKernelGGL((k_flux_rev<L, true>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
-                         (const double*)A.tdelta, (const V4d*)A.frb, o);
+                         (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
     };
     switch (lpp & ~LPP_STAGED) {
       case 2: go.template operator()<2>(); break;
@@ -1043,7 +1092,7 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
   }
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL((k_flux_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
-                       (const double*)A.tdelta, (const V4d*)A.frb, o);
+                       (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
   });
 }
 
@@ -1065,13 +1114,13 @@ static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
     const size_t lds = stage_lds_inner_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       hipLaunchKernelGGL((k_inner_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, din,
-                         dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+                         dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
     });
     return;
   }
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL((k_inner_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, din, dout,
-                       (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+                       (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
   });
 }
 
@@ -1091,13 +1140,13 @@ static void launch_lsq_rev(mf_ctx* c, AdjState& A, int s, int lpp) {
     const size_t lds = stage_lds_lsq_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
       hipLaunchKernelGGL((k_lsq_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, dq0,
-                         (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+                         (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
     });
     return;
   }
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL((k_lsq_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, dq0,
-                       (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos);
+                       (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
   });
 }
 
@@ -1501,7 +1550,6 @@ extern "C" {
 
 int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps, double* J) {
   if (!c) return set_err(MF_ERR_INVALID, "null context");
-  if (!c->power0) return set_err(MF_ERR_STATE, "the adjoint supports power = 0 only");
   if (iters < 1 || (snaps < 1 && snaps != MF_ADJ_TAPE_ALL))
     return set_err(MF_ERR_INVALID, "iters must be >= 1 and snaps >= 1 or MF_ADJ_TAPE_ALL");
   if (c->nshape == 0) return set_err(MF_ERR_STATE, "J = Cd needs shape points");

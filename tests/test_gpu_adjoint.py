@@ -34,11 +34,14 @@ def _oparams(**kw):
     return p
 
 
-@pytest.mark.parametrize("variant", ["second_inner2", "first_order", "tscheme_first"])
+@pytest.mark.parametrize("variant", ["second_inner2", "first_order", "tscheme_first", "power_m2", "power_1"])
 def test_adjoint_matches_tangent(small_cloud, variant):
+    """Also with distance-weighted least squares (power -2 and 1: the weights'
+    derivatives in the LSQ, inner-iteration and flux reverses, q_variables_b.F90)."""
     from oracle.oracle import tangent_cd
     kw = {"second_inner2": {}, "first_order": dict(first_order=1, inner_iterations=0),
-          "tscheme_first": dict(inner_iterations=1)}[variant]
+          "tscheme_first": dict(inner_iterations=1), "power_m2": dict(power=-2.0),
+          "power_1": dict(power=1.0)}[variant]
     okw = dict(kw)
     gkw = dict(kw)
     if variant == "tscheme_first":
@@ -263,8 +266,7 @@ def test_device_tangent_matches_oracle(small_cloud, variant):
     """mf_tangent (device forward mode, dual numbers) against the oracle's dual-number
     tangent on 20 iterations: J and the directional derivative to 1e-10 relative; the
     final prim's tangent against the oracle's is not exposed by the oracle, so it is
-    checked by linearity (twice the direction, twice the tangent).  power = -2 has no
-    adjoint (the reverse supports power = 0); the tangent covers it."""
+    checked by linearity (twice the direction, twice the tangent)."""
     from oracle.oracle import tangent_cd
     kw = {"second_inner2": {}, "first_order": dict(first_order=1, inner_iterations=0),
           "tscheme_first": dict(inner_iterations=1), "power_m2": dict(power=-2.0)}[variant]
