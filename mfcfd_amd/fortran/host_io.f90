@@ -210,6 +210,9 @@ contains
         ! iteration taped during the primal sweep (MF_ADJ_TAPE_ALL), when it fits in HBM
         o%ad_mode = merge(1, 0, trim(o%adjoint_mode) == 'checkpoints' .or. trim(o%adjoint_mode) == 'tape')
         if (trim(o%adjoint_mode) == 'tape') o%chkpts = -1
+        ! -adjoint_mode forward: the forward (tangent) mode of src_mpi_forward, seeded as
+        ! q_lskum_d.F90:34-39 seeds it (x of the point with original id 1)
+        if (trim(o%adjoint_mode) == 'forward') o%ad_mode = 2
         if (trim(o%precision_name) == 'mixed') then
             o%precision = 1
         else if (trim(o%precision_name) == 'fp64') then

@@ -6,7 +6,8 @@
 ! cp/cp-file, solution/sol.dat and (with -adjoint_mode checkpoints -chkpts S, as
 ! src_mpi_adjoint/meshfree_solver.F90 runs q_lskum_chkpts_b, or -adjoint_mode tape to
 ! tape every iteration in the primal sweep) the Cd sensitivities in
-! sensitivity/sensitivity.dat.  Normals, connectivity splits, func_delta, the stages
+! sensitivity/sensitivity.dat; with -adjoint_mode forward, src_mpi_forward's tangent of
+! Cd along x of point 1.  Normals, connectivity splits, func_delta, the stages
 ! and the force coefficients all run on the device.
 !
 ! Usage: run from a case directory holding case.in and point/point.h5 (or
@@ -34,7 +35,7 @@ program meshfree_host
     real(c_double), allocatable :: cl(:), cd(:), cm(:), cp(:)
     real(c_double), allocatable, target :: fx(:), fy(:)
     integer(c_int32_t), allocatable, target :: left(:), right(:), flag_1(:), flag_2(:), nbhs(:), conn(:, :)
-    real(c_double) :: J
+    real(c_double) :: J, Jd
     integer :: itr, it, last, chunk, done, k, nshape, maxchunk, nw, no, ni
     integer(c_int) :: rc
     integer(8) :: c0, c1, c2, rate
@@ -159,6 +160,18 @@ program meshfree_host
         write(*, '(a,i0,a,i0,a,i0)') ' forward steps ', ast%forward_steps, ', taped ', ast%taped_steps, &
             ', snapshots ', ast%snapshots
         call write_sensitivity('sensitivity/sensitivity.dat', pts, dJdx, dJdy, it - 1)
+    else if (opt%ad_mode == 2) then
+        ! src_mpi_forward (q_lskum_d.F90:28-60): pointd%x = 0 except 1 at the point with
+        ! original id 1; every iteration carries the tangents; vector_cost_funcd = dCd
+        write(*, *) '%%%%%%%%%%%%%-<<X Value Set To 1>>-%%%%%%%%%%%%'
+        allocate(dJdx(pts%n), dJdy(pts%n))
+        dJdx = 0.0d0
+        dJdy = 0.0d0
+        dJdx(1) = 1.0d0
+        call check(mf_tangent(ctx, int(opt%max_iters, c_int32_t), dJdx, dJdy, J, Jd, c_null_ptr), 'mf_tangent')
+        it = itr + opt%max_iters + 1
+        write(*, '(a,es24.16)') ' Objective (sum of Cd):', J
+        write(*, '(a,es24.16)') ' Vector function derivative is ', Jd
     else
         ! q_lskum.F90:59-73: per-iteration residue line and history file
         open(unit=301, file='residue', form='formatted', status='replace', action='write')

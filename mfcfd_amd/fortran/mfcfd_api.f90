@@ -9,8 +9,9 @@ module mfcfd_api
 
     integer(c_int), parameter :: MF_OK = 0, MF_ERR_HIP = -1, MF_ERR_INVALID = -2, MF_ERR_STATE = -3
     integer(c_int32_t), parameter :: MF_FP64 = 0, MF_MIXED = 1
-    ! mf_tuning: staged walks (MF_LPP_STAGED + lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux)
-    integer(c_int32_t), parameter :: MF_LPP_STAGED = 16
+    ! mf_tuning: staged walks (MF_LPP_STAGED + lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux);
+    ! resident inner sweeps (MF_LPP_RESIDENT + 1, 2 or 4, mixed contexts)
+    integer(c_int32_t), parameter :: MF_LPP_STAGED = 16, MF_LPP_RESIDENT = 32
     ! mf_adjoint*: snaps value that tapes every iteration in the primal sweep
     integer(c_int32_t), parameter :: MF_ADJ_TAPE_ALL = -1
     integer(c_int32_t), parameter :: MF_FIELD_Q = 1, MF_FIELD_DQ = 2, MF_FIELD_QM = 3, &
@@ -29,7 +30,7 @@ module mfcfd_api
     end type
 
     type, bind(C) :: mf_tuning
-        integer(c_int32_t) :: lpp_lsq, lpp_inner, lpp_flux, reserved
+        integer(c_int32_t) :: lpp_lsq, lpp_inner, lpp_flux, barrier_polls
         real(c_double) :: us_lsq, us_inner, us_flux
         integer(c_int32_t) :: lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather
     end type
@@ -155,6 +156,24 @@ module mfcfd_api
             integer(c_int32_t), value :: first_it, iters, snaps
             real(c_double), intent(out) :: J, dJdx(*), dJdy(*)
             type(mf_adjoint_stats), intent(out) :: stats
+        end function
+        ! forward mode (src_mpi_forward q_lskum_d): J and its derivative along (vx, vy)
+        integer(c_int) function mf_tangent(ctx, iters, vx, vy, J, dJ, prim_d) bind(C, name="mf_tangent")
+            import
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: iters
+            real(c_double), intent(in) :: vx(*), vy(*)
+            real(c_double), intent(out) :: J, dJ
+            type(c_ptr), value :: prim_d
+        end function
+        ! force coefficients from device arrays, no allocation (no reference counterpart)
+        integer(c_int) function mf_compute_cl_cd_cm_noalloc(n_points, n_shape, shapes, mach, aoa, x, y, &
+                nx, ny, prim, left, right, flag_2, shape_idx, clcdcm, cp, stream) &
+                bind(C, name="mf_compute_cl_cd_cm_noalloc")
+            import
+            integer(c_int32_t), value :: n_points, n_shape, shapes
+            real(c_double), value :: mach, aoa
+            type(c_ptr), value :: x, y, nx, ny, prim, left, right, flag_2, shape_idx, clcdcm, cp, stream
         end function
     end interface
 

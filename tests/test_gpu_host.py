@@ -109,6 +109,26 @@ def test_host_adjoint_writes_sensitivity(small_cloud, tmp_path, mode):
     assert np.abs(np.concatenate([sens[:, 1], sens[:, 2]]) - g).max() <= 1e-10 * np.abs(g).max()
 
 
+def test_host_forward_mode(small_cloud, tmp_path):
+    """-adjoint_mode forward: src_mpi_forward's tangent (q_lskum_d.F90:34-39 seeds x of
+    the point with original id 1), the printed derivative equal to mf_tangent's and to
+    the adjoint's dJ/dx of that point."""
+    import re
+    from mfcfd_amd.solver import LskumSolver
+    _case(tmp_path, small_cloud, max_iters=12, adjoint_mode="forward")
+    out = _run(tmp_path)
+    m = re.search(r"Vector function derivative is\s+(\S+)", out)
+    assert m, out[-2000:]
+    dJ = float(m.group(1))
+    vx = np.zeros(small_cloud.n)
+    vx[0] = 1.0
+    with LskumSolver(small_cloud, **CASE) as s:
+        J, dJt = s.tangent(12, vx, np.zeros(small_cloud.n))
+        _, gx, _, _ = s.adjoint(12, snaps=4)
+    assert abs(dJ - dJt) <= 1e-14 * abs(dJt) + 1e-300
+    assert abs(dJ - gx[0]) <= 1e-10 * abs(gx[0])
+
+
 def _d2_sensor(cloud, prim):
     """adaptation_sensors.F90:7-101 restated with numpy: per point the largest D2
     distance to a neighbour's state, over the largest of all points."""
