@@ -207,6 +207,7 @@ def run_adjoint(iters: int, snaps: int):
     # HBM: the reverse re-runs each iteration once, taped); every iteration taped in
     # the primal sweep (about 1.3 kB per point per iteration: nothing re-run)
     from mfcfd_amd._lib import MF_ADJ_TAPE_ALL
+    grads = {}
     for key, ns in (("binomial", snaps), ("store_all", iters), ("tape_all", MF_ADJ_TAPE_ALL)):
         with LskumSolver(cloud, **p) as s:
             s.tuning = out["tuning"]
@@ -214,9 +215,40 @@ def run_adjoint(iters: int, snaps: int):
             t0 = time.perf_counter()
             J, dx, dy, st = s.adjoint(iters, snaps=ns)
             adj_s = time.perf_counter() - t0
+        grads[key] = (J, np.concatenate([dx, dy]))
         out[key] = {"snapshots": ns, "J": J, "adjoint_total_s": adj_s,
                     "adjoint_over_primal": adj_s / primal_s, "stats": st,
                     "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+    # the mixed-precision adjoint (FP32 primal and taped re-runs, FP64 reverse), every
+    # iteration taped, against the FP64 one; its cost against the mixed primal too
+    with LskumSolver(cloud, precision="mixed", **p) as s:
+        tm = s.autotune()
+        mixed_primal_s = _timed(s, iters, 1)
+    with LskumSolver(cloud, precision="mixed", **p) as s:
+        s.tuning = {k: tm[k] for k in ("lpp_lsq", "lpp_inner", "lpp_flux")}
+        print(f"[bench] adjoint: {iters} iterations, mixed, tape_all", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        Jm, mx, my, st = s.adjoint(iters, snaps=MF_ADJ_TAPE_ALL)
+        adj_s = time.perf_counter() - t0
+    J0, g0 = grads["tape_all"]
+    gm = np.concatenate([mx, my])
+    v = np.random.default_rng(123).standard_normal(g0.size)
+    out["mixed_tape_all"] = {"J": Jm, "adjoint_total_s": adj_s, "mixed_primal_s": mixed_primal_s,
+                             "adjoint_over_fp64_primal": adj_s / primal_s,
+                             "adjoint_over_mixed_primal": adj_s / mixed_primal_s, "stats": st,
+                             "vs_fp64": {"J_rel": abs(Jm - J0) / abs(J0),
+                                         "grad_max_rel": float(np.abs(gm - g0).max() / np.abs(g0).max()),
+                                         "grad_l2_rel": float(np.linalg.norm(gm - g0) / np.linalg.norm(g0)),
+                                         "directional_rel": abs(float((gm - g0) @ v)) / abs(float(g0 @ v))}}
+    # the device tangent along one seeded direction, against the FP64 adjoint's
+    vx, vy = v[:cloud.n], v[cloud.n:]
+    with LskumSolver(cloud, **p) as s:
+        t0 = time.perf_counter()
+        Jt, dJt = s.tangent(iters, vx, vy)
+        tan_s = time.perf_counter() - t0
+    out["tangent"] = {"J": Jt, "dJ": dJt, "seconds": tan_s, "over_primal": tan_s / primal_s,
+                      "adjoint_directional": float(g0 @ v),
+                      "rel_vs_adjoint": abs(dJt - float(g0 @ v)) / abs(float(g0 @ v))}
     return out
 
 
@@ -296,8 +328,8 @@ def run_adjoint_fd(iters: int, n_points: int = 16, h: float = 1e-6, save: str | 
 def run_batch(iters: int, adjoint: bool = True, warmup: int = 20):
     """All 64 cases in one context: every graph-replayed iteration is one launch per
     kernel over all cases.  Mixed and FP64 paths from freestream: `warmup` iterations,
-    then `iters` timed; then (optionally) the FP64 Cd adjoint of the batch over the
-    same iterations."""
+    then `iters` timed; then (optionally) the Cd adjoint of the batch over the same
+    iterations in both precisions, the mixed one's per-case gradients against FP64."""
     from mfcfd_amd.solver import LskumBatch
     cases, th = W.c5_cases()
     p = W.C5["params"]
@@ -326,18 +358,30 @@ def run_batch(iters: int, adjoint: bool = True, warmup: int = 20):
     out["per_case_dCd_mixed_vs_fp64"] = {"max": float(dcd.max()), "median": float(np.median(dcd))}
     out["cd_fp64"] = [round(float(v), 8) for v in cd_of["fp64"]]
     if adjoint:
-        # every iterate kept when it fits comfortably in HBM (32 B per point per iterate)
+        # the Cd adjoint of the batch in both precisions (every iterate kept when it fits
+        # comfortably in HBM: 32 B per point per iterate), per case against FP64
         snaps = int(min(iters + warmup, max(32, 150e9 // (npts * 32))))
         os.environ["MFCFD_PROGRESS"] = "1"
-        with LskumBatch(cases, **p) as b:
-            b.run(0)
-            t0 = time.perf_counter()
-            J, dx, dy, st = b.adjoint(iters + warmup, snaps=snaps)
-            adt = time.perf_counter() - t0
-        fp64_iter_s = out["fp64"]["ms_per_step"] * 1e-3
-        out["adjoint"] = {"iterations": iters + warmup, "snapshots": snaps, "J_sum_cd": J,
-                          "seconds": adt, "over_primal": adt / (fp64_iter_s * (iters + warmup)),
-                          "stats": st, "grad_norm": float(np.sqrt((dx * dx + dy * dy).sum()))}
+        grads = {}
+        out["adjoint"] = {"iterations": iters + warmup, "snapshots": snaps}
+        for prec in ("fp64", "mixed"):
+            with LskumBatch(cases, precision=prec, **p) as b:
+                b.run(0)
+                t0 = time.perf_counter()
+                J, dx, dy, st = b.adjoint(iters + warmup, snaps=snaps)
+                adt = time.perf_counter() - t0
+                slices = [b.case_slice(k) for k in range(len(cases))]
+            grads[prec] = (dx, dy)
+            it_s = out[prec]["ms_per_step"] * 1e-3
+            out["adjoint"][prec] = {"J_sum_cd": J, "seconds": adt,
+                                    "over_primal": adt / (it_s * (iters + warmup)), "stats": st}
+            print(f"[bench] batch adjoint {prec}: {adt:.1f} s", file=sys.stderr, flush=True)
+        errs = []
+        for sl in slices:
+            g0 = np.concatenate([grads["fp64"][0][sl], grads["fp64"][1][sl]])
+            g1 = np.concatenate([grads["mixed"][0][sl], grads["mixed"][1][sl]])
+            errs.append(float(np.abs(g1 - g0).max() / np.abs(g0).max()))
+        out["adjoint"]["per_case_dCd_dxy_mixed_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
     return out
 
 
@@ -364,6 +408,36 @@ def probe(workload: str):
     print(json.dumps({"probe": "ok", "workload": workload, "copy_bytes": COPY_PROBE_BYTES,
                       "copy_ms": ms.value, "points": cloud.n, "shapes": shapes,
                       "mean_nbhs": float(cloud.nbhs.mean())}))
+
+
+def run_inner_shapes():
+    """Every launch shape of the mixed path's inner sweeps on the 40,000- and the
+    160,000-point workloads: the staged kernel (one launch per sweep) with 1, 2, 4 lanes
+    per point and the resident sweeps (every sweep of a stage in one launch, a grid
+    barrier between sweeps) where their blocks fit.  Per shape: the time per sweep over
+    20 back-to-back launches, and 100 iterations timed whole."""
+    from mfcfd_amd._lib import MfError
+    from mfcfd_amd.solver import LskumSolver
+    out = {}
+    for name, w in (("c3", W.C3), ("c2", W.C2)):
+        cloud, p = W.cloud_of(w), w["params"]
+        res = {"workload": W.DESCRIPTION[name], "inner_iterations": p["inner_iterations"]}
+        for kind, flag in (("staged", 16), ("resident", 32)):
+            for lanes in (1, 2, 4):
+                key = f"{kind}_{lanes}"
+                with LskumSolver(cloud, precision="mixed", **p) as s:
+                    try:
+                        s.tuning = {"lpp_lsq": 16 | 2, "lpp_flux": 16 | 4, "lpp_inner": flag | lanes}
+                    except MfError:
+                        res[key] = None
+                        continue
+                    s.q_lskum(5)
+                    us = s.time_kernels(20)["inner"]
+                    dt = _timed(s, 100, 6)
+                    res[key] = {"us_per_sweep": us, "ms_per_iter": 1e3 * dt / 100}
+                print(f"[bench] inner shapes {name} {key}: {res[key]}", file=sys.stderr, flush=True)
+        out[name] = res
+    return out
 
 
 def _counters(key: str, prec: str):
@@ -395,6 +469,9 @@ def main():
     ap.add_argument("--fd-step", type=float, default=1e-6, help="step of that check (chord)")
     ap.add_argument("--save-gradient", default=None, help="write the adjoint's gradient (npz)")
     ap.add_argument("--batch-iters", type=int, default=None, help="iterations of the batched run")
+    ap.add_argument("--inner-shapes", action="store_true",
+                    help="time every inner-sweep shape of the mixed path on the 40,000- and "
+                         "160,000-point workloads and print that block only")
     ap.add_argument("--probe", choices=("c2", "c4"), default=None,
                     help="counter-calibration pass on a workload (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -403,6 +480,9 @@ def main():
               "(multi-GPU halo exchange is design-only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.inner_shapes:
+        print(json.dumps({"inner_sweep_shapes": run_inner_shapes()}))
+        return
     if args.probe:
         probe(args.probe)
         return

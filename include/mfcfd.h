@@ -177,10 +177,20 @@ int mf_batch_cases(mf_ctx *ctx, int32_t *ncases, int32_t *first);
  * neighbours' records (each once) into LDS and walk the stencils there; same per-lane
  * summation order, bit-identical results.  LSQ and inner-iteration kernels: 64-point
  * blocks, 1, 2 or 4 lanes; flux + update: 32-point blocks, 2, 4 or 8 lanes.
- * mf_set_tuning rejects a staged shape whose halos do not fit in LDS. */
+ * mf_set_tuning rejects a staged shape whose halos do not fit in LDS.
+ * Resident inner sweeps, lpp_inner = MF_LPP_RESIDENT | lanes (1, 2 or 4; MF_MIXED
+ * contexts): every inner iteration of a stage in one launch of the staged inner kernel's
+ * blocks, all resident on the GPU at once, with a grid barrier between sweeps; the
+ * staged shape's results bit for bit.  Offered when the occupancy calculator fits every
+ * block on the device (about 180,000 points).  A wait at the barrier gives up after
+ * barrier_polls polls (0: the default, about a second): the run then stops,
+ * mf_run/mf_sync return MF_ERR_HIP and the context falls back to the staged shape with
+ * the same lanes.  us_inner is per sweep. */
 #define MF_LPP_STAGED 16
+#define MF_LPP_RESIDENT 32
 typedef struct mf_tuning {
-  int32_t lpp_lsq, lpp_inner, lpp_flux, reserved;
+  int32_t lpp_lsq, lpp_inner, lpp_flux;
+  int32_t barrier_polls;
   double us_lsq, us_inner, us_flux;  /* mf_autotune: mean launch time of the chosen shape */
   /* reverse kernels of the adjoint (flux, LSQ, inner iteration, edge gathers): 0 lets
    * the first mf_adjoint_forward time the shapes and keep the fastest */

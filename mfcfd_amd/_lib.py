@@ -30,8 +30,10 @@ MF_NUM_KERNEL_KINDS = 5
 # mf_adjoint*: snaps value that tapes every iteration in the primal sweep
 MF_ADJ_TAPE_ALL = -1
 MF_NOALLOC_MAX_SHAPES = 8
-# mf_tuning: staged walks (MF_LPP_STAGED | lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux)
+# mf_tuning: staged walks (MF_LPP_STAGED | lanes: 1, 2, 4 for LSQ and inner, 2, 4, 8 for flux);
+# resident inner sweeps (MF_LPP_RESIDENT | 1, 2 or 4, mixed contexts)
 MF_LPP_STAGED = 16
+MF_LPP_RESIDENT = 32
 KERNEL_KINDS = ("lsq", "inner", "flux_update", "finalize", "forces")
 
 
@@ -72,7 +74,7 @@ class MfCase(C.Structure):
 
 class MfTuning(C.Structure):
     _fields_ = [("lpp_lsq", C.c_int32), ("lpp_inner", C.c_int32), ("lpp_flux", C.c_int32),
-                ("reserved", C.c_int32), ("us_lsq", C.c_double), ("us_inner", C.c_double),
+                ("barrier_polls", C.c_int32), ("us_lsq", C.c_double), ("us_inner", C.c_double),
                 ("us_flux", C.c_double), ("lpp_flux_rev", C.c_int32), ("lpp_lsq_rev", C.c_int32),
                 ("lpp_inner_rev", C.c_int32), ("lpp_gather", C.c_int32)]
 

@@ -122,12 +122,28 @@ struct Geom {
   const V4<float>* dg_f = nullptr;
 };
 
+// halt codes of Ctl
+constexpr int32_t HALT_NAN = 1;      // a NaN residue (q_lskum.F90:71): the run stops, no error
+constexpr int32_t HALT_BARRIER = 2;  // the resident sweeps' grid barrier gave up: an error
+
 struct Ctl {
   int32_t it;      // iterations completed in this run (index into the histories)
-  int32_t halt;    // set once a NaN residue is seen (q_lskum.F90:71)
+  int32_t halt;    // HALT_NAN or HALT_BARRIER once set; every later kernel returns at once
   double res_old;  // fpi_solver.F90:65-67
   int32_t first_it;  // iteration number of the run's first iteration (residue rule t <= 2)
   int32_t reserved;
+};
+
+// Grid barrier of the resident inner sweeps.  Blocks arrive at one of SWEEP_GROUPS
+// counters (block b at counter b mod groups, each on its own 128-byte line, so arrivals
+// of different groups do not contend); each group's last arrival arrives at `top`, and
+// the last of those advances `gen`, which the waiting blocks poll.  Counters return to
+// zero at every episode, so the state serves any number of launches without a reset;
+// `abort` is set when a wait gives up, and then the state needs a reset (zero fill).
+constexpr int SWEEP_GROUPS = 64;
+struct SweepBar {
+  uint32_t group[SWEEP_GROUPS * 32];
+  uint32_t top, gen, abort, pad[29];
 };
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so each XCD (own L2) walks one

@@ -68,6 +68,12 @@ inline size_t stage_lds_lsq(int hcap, int width, size_t ssz, bool delta) {
 inline size_t stage_lds_inner(int hcap, int width, size_t ssz, bool sums) {
   return (size_t)hcap * ((sums ? 8 : 12) * ssz + sizeof(D2)) + (size_t)width * STAGE_P;
 }
+// the resident inner sweeps (mixed path): halo dq and xy, slot words, and (more than
+// one lane per point) the block's first-pass sums and solve factors
+inline size_t sweeps_lds(int hcap, int width, int lanes) {
+  return (size_t)hcap * (2 * sizeof(V4<float>) + sizeof(D2)) + (size_t)width * STAGE_P +
+         (lanes > 1 ? 3 * STAGE_P * sizeof(V4<float>) : 0);
+}
 // the adjoint's LSQ reverse (FP64): q and xy per halo entry
 inline size_t stage_lds_lsq_rev(int hcap, int width) {
   return (size_t)hcap * (4 * sizeof(double) + sizeof(D2)) + (size_t)width * STAGE_P;
@@ -80,28 +86,49 @@ inline size_t stage_lds_flux(int hcap, int width, size_t ssz) {
   return (size_t)hcap * (21 * ssz + sizeof(D2)) + (size_t)width * STAGE_PF * 2;
 }
 
+// The mixed path's 2x2 solve with the point's FP32 factors f = (sum dy^2, sum dx dy,
+// sum dx^2) / det
+__device__ inline void lsq_apply_f(const V4<float>& f, const double sx[4], const double sy[4], V4<float>& ox,
+                                   V4<float>& oy) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    ox[r] = (float)(sx[r] * f.x - sy[r] * f.y);
+    oy[r] = (float)(sy[r] * f.z - sx[r] * f.y);
+  }
+}
+
 // The 2x2 least-squares solve of point i (q_variables.F90:104-110 and :198-208):
 // d/dx = (Sx sum dy^2 - Sy sum dx dy) / det, d/dy = (Sy sum dx^2 - Sx sum dx dy) / det
 // from the edge sums Sx, Sy.  FP64: the reference's operations; mixed path: the FP32
 // products with 1/det of lsq_geo_f.
 template <class S>
-__device__ inline void lsq_solve(const Geom& g, int i, const double sx[4], const double sy[4], V4<S>& ox,
+using LsqGeo = std::conditional_t<sizeof(S) == 4, V4<float>, V4d>;
+
+template <class S>
+__device__ inline LsqGeo<S> lsq_geo_of(const Geom& g, int i) {
+  if constexpr (sizeof(S) == 4) return g.lsq_geo_f[i];
+  else return g.lsq_geo[i];
+}
+
+template <class S>
+__device__ inline void lsq_apply(const LsqGeo<S>& gg, const double sx[4], const double sy[4], V4<S>& ox,
                                  V4<S>& oy) {
   if constexpr (sizeof(S) == 4) {
-    const V4<float> f = g.lsq_geo_f[i];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      ox[r] = (float)(sx[r] * f.x - sy[r] * f.y);
-      oy[r] = (float)(sy[r] * f.z - sx[r] * f.y);
-    }
+    lsq_apply_f(gg, sx, sy, ox, oy);
   } else {
-    const V4d gg = g.lsq_geo[i];  // (sum dx^2, sum dy^2, sum dxdy, 1/det)
+    // gg = (sum dx^2, sum dy^2, sum dxdy, 1/det)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       ox[r] = (sx[r] * gg.y - sy[r] * gg.z) * gg.w;
       oy[r] = (sy[r] * gg.x - sx[r] * gg.z) * gg.w;
     }
   }
+}
+
+template <class S>
+__device__ inline void lsq_solve(const Geom& g, int i, const double sx[4], const double sy[4], V4<S>& ox,
+                                 V4<S>& oy) {
+  lsq_apply<S>(lsq_geo_of<S>(g, i), sx, sy, ox, oy);
 }
 
 // q_variables.F90:41-112 eval_q_derivatives; with DELTA also func_delta
@@ -115,7 +142,8 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
                                                double* __restrict__ delta, double cfl,
                                                double power, const Ctl* __restrict__ ctl,
                                                V4<float>* __restrict__ qsum) {
-  if (ctl && ctl->halt) return;
+  // read now, tested before the stores: the load does not delay the gathers
+  const int32_t halted = ctl ? ctl->halt : 0;
   const PointMap<LPP, STAGED> pm(g.n);
   if (!STAGED && !pm.live) return;  // whole groups: n * LPP threads are a multiple of LPP
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
@@ -201,6 +229,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
     if (DELTA) min_delt = group_min<LPP>(min_delt);
     if (lane != 0) return;
   }
+  if (halted) return;
   V4<S> ox, oy;
   lsq_solve(g, i, sxq, syq, ox, oy);
   dq[2 * i] = ox;
@@ -211,6 +240,26 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
   if (qsum) {
     qsum[2 * i] = V4<float>{(float)sxq[0], (float)sxq[1], (float)sxq[2], (float)sxq[3]};
     qsum[2 * i + 1] = V4<float>{(float)syq[0], (float)syq[1], (float)syq[2], (float)syq[3]};
+  }
+}
+
+// One edge of an inner sweep in the sums form (mixed path): the terms
+// -0.5 w (dx, dy) [dx (dq_x,k - dq_x,i) + dy (dq_y,k - dq_y,i)] of the edge sums, whose
+// (q_k - q_i) part the LSQ kernel's first pass has already summed.  Shared by the
+// per-sweep and the resident kernels so that both round alike.
+template <class S, bool POWER0>
+__device__ __forceinline__ void inner_edge_sums(const D2& pi, const V4<S>& dxi, const V4<S>& dyi, const D2& pk,
+                                                const V4<S>& dxk, const V4<S>& dyk, double power, bool valid,
+                                                double sxq[4], double syq[4]) {
+  const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+  double w = 1.0;
+  if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
+  const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double d = delx * (double)(dxk[r] - dxi[r]) + dely * (double)(dyk[r] - dyi[r]);
+    sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
+    syq[r] = valid ? syq[r] - hy * d : syq[r];
   }
 }
 
@@ -225,7 +274,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
                                                  V4<S>* __restrict__ dq_out, double power,
                                                  const Ctl* __restrict__ ctl,
                                                  const V4<float>* __restrict__ qsum) {
-  if (ctl && ctl->halt) return;
+  // read now, tested before the stores: the load does not delay the gathers
+  const int32_t halted = ctl ? ctl->halt : 0;
   const PointMap<LPP, STAGED> pm(g.n);
   if (!STAGED && !pm.live) return;
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
@@ -246,18 +296,12 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   }
   // one edge (qk unused in the sums form); valid as in k_lsq
   auto edge = [&](const D2& pk, const V4<S>& qk, const V4<S>& dxk, const V4<S>& dyk, bool valid) {
-    const double delx = pk.x - pi.x, dely = pk.y - pi.y;
-    double w = 1.0;
-    if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
     if constexpr (SUMS) {
-      const double hx = 0.5 * w * delx, hy = 0.5 * w * dely;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double d = delx * (double)(dxk[r] - dxi[r]) + dely * (double)(dyk[r] - dyi[r]);
-        sxq[r] = valid ? sxq[r] - hx * d : sxq[r];
-        syq[r] = valid ? syq[r] - hy * d : syq[r];
-      }
+      inner_edge_sums<S, POWER0>(pi, dxi, dyi, pk, dxk, dyk, power, valid, sxq, syq);
     } else {
+      const double delx = pk.x - pi.x, dely = pk.y - pi.y;
+      double w = 1.0;
+      if (!POWER0) w = pow(sqrt(delx * delx + dely * dely), power);
       const double wx = w * delx, wy = w * dely;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -326,13 +370,14 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
     }
     if (lane != 0) return;
   }
+  if (halted) return;
   if constexpr (SUMS) {
     // the first-pass sums join after the edge terms
-    const V4<float> a = qsum[2 * i], b = qsum[2 * i + 1];
+    const V4<float> sa = qsum[2 * i], sb = qsum[2 * i + 1];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      sxq[r] = (double)a[r] + sxq[r];
-      syq[r] = (double)b[r] + syq[r];
+      sxq[r] = (double)sa[r] + sxq[r];
+      syq[r] = (double)sb[r] + syq[r];
     }
   }
   V4<S> ox, oy;
@@ -740,6 +785,10 @@ void launch_inner_f32(const LaunchEnv& e, const V4f* q, const V4f* a, V4f* b, co
 void launch_flux_update_f32(const LaunchEnv& e, const State<float>& st, const V4f* prim, V4f* pold, V4f* pout,
                             V4f* qout, double* cspeed, const double* delta, double* partial, const Ctl* ctl,
                             int rk, bool last, int lpp, double* rs_pt);
+// the resident inner sweeps (lskum_sweeps.h): launch, and how many blocks fit at once
+void launch_sweeps_f32(const LaunchEnv& e, V4f* dq0, V4f* dq1, int sweeps, Ctl* ctl, const V4f* qsum, SweepBar* bar,
+                       uint32_t spin_limit, int lpp);
+int sweeps_capacity_f32(int lpp, bool power0, size_t lds);
 void launch_forces_f32(const LaunchEnv& e, const V4f* prim, const ForcesArgs& f, double* cd_hist, const Ctl* ctl);
 void launch_q_variables_f32(const LaunchEnv& e, const V4f* prim, V4f* q, double* cspeed);
 void launch_func_delta_f32(const LaunchEnv& e, const V4f* prim, double* delta);

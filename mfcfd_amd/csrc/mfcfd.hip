@@ -64,7 +64,7 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(const double* __restrict__ p
     }
     res_hist[it] = residue;
     resnew_hist[it] = res_new;
-    if (isnan(residue)) ctl->halt = 1;
+    if (isnan(residue)) ctl->halt = HALT_NAN;
     ctl->it = it + 1;
   }
 }
@@ -151,6 +151,10 @@ int mf_noalloc_fail(int code, const char* msg) { return set_err(code, "%s", msg)
 
 static const double kPi = 4.0 * std::atan(1.0);  // parameter.F90:11
 
+// Polls of the resident sweeps' grid barrier before a wait gives up (about a second at
+// a poll per microsecond; a wait normally ends within microseconds)
+static constexpr uint32_t kDefaultBarrierPolls = 1u << 20;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -210,6 +214,10 @@ struct mf_ctx {
   double *rs_pt = nullptr, *res_old_case = nullptr;
   V4<float>* qsum = nullptr;  // mixed path: the LSQ first-pass sums per point [2N]
   Ctl* tune_ctl = nullptr;
+  // mixed path, lpp_inner with MF_LPP_RESIDENT: the resident sweeps' grid barrier and
+  // the polls before a wait at it gives up
+  SweepBar* sbar = nullptr;
+  uint32_t barrier_polls = kDefaultBarrierPolls;
 };
 
 template <class T>
@@ -890,10 +898,22 @@ static void launch_inner_on(mf_ctx* c, const V4<S>* q, const V4<S>* a, V4<S>* b,
   launch_inner_any<S>(env_of(c), q, a, b, ctl, lpp, qsum);
 }
 
+// the per-sweep shape of an inner-iteration shape (the resident sweeps' staged twin)
+static int sweep_shape(int lpp) {
+  return (lpp & MF_LPP_RESIDENT) ? (LPP_STAGED | (lpp & ~MF_LPP_RESIDENT)) : lpp;
+}
+
 template <class S>
 static void launch_inner(mf_ctx* c, int qbuf, int din, const Ctl* ctl) {
   launch_inner_on<S>(c, (const V4<S>*)c->q[qbuf], (const V4<S>*)c->dq[din], (V4<S>*)c->dq[din ^ 1], ctl,
-                     c->lpp_inner, c->qsum);
+                     sweep_shape(c->lpp_inner), c->qsum);
+}
+
+// every inner sweep of a stage in one launch (lpp_inner with MF_LPP_RESIDENT): dq[0]
+// in, the last sweep's dq in dq[sweeps & 1], as after `sweeps` launch_inner calls
+static void launch_sweeps(mf_ctx* c, int sweeps, Ctl* ctl) {
+  launch_sweeps_f32(env_of(c), (V4f*)c->dq[0], (V4f*)c->dq[1], sweeps, ctl, c->qsum, c->sbar, c->barrier_polls,
+                    c->lpp_inner & ~MF_LPP_RESIDENT);
 }
 
 // fused flux residual + state update (rk) + q_variables of the new state
@@ -966,11 +986,18 @@ static int record_iteration(mf_ctx* c, int phase, Timer* tm) {
     launch_lsq<S>(c, buf, rk == 1, c->ctl, I > 0);
     if (tm) tm->end(KK_LSQ);
     int d = 0;
-    for (int it = 0; it < I; ++it) {
+    if (is_f32<S> && I > 0 && (c->lpp_inner & MF_LPP_RESIDENT)) {
       if (tm) tm->begin(KK_INNER);
-      launch_inner<S>(c, buf, d, c->ctl);
+      launch_sweeps(c, I, c->ctl);
       if (tm) tm->end(KK_INNER);
-      d ^= 1;
+      d = I & 1;
+    } else {
+      for (int it = 0; it < I; ++it) {
+        if (tm) tm->begin(KK_INNER);
+        launch_inner<S>(c, buf, d, c->ctl);
+        if (tm) tm->end(KK_INNER);
+        d ^= 1;
+      }
     }
     if (tm) tm->begin(KK_FLUX);
     launch_flux_update<S>(c, buf, d, rk, rk == c->p.rks);
@@ -1043,6 +1070,8 @@ static int run_queue(mf_ctx* c, int32_t first_it, int32_t iters) {
   return MF_OK;
 }
 
+static int check_sweeps(mf_ctx* c);
+
 extern "C" {
 
 int mf_run_async(mf_ctx* c, int32_t first_it, int32_t iters) {
@@ -1055,8 +1084,9 @@ int mf_run_async(mf_ctx* c, int32_t first_it, int32_t iters) {
 }
 
 int mf_sync(mf_ctx* c) {
+  if (!c) return set_err(MF_ERR_INVALID, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
-  return MF_OK;
+  return check_sweeps(c);
 }
 
 int mf_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue_hist,
@@ -1071,6 +1101,8 @@ int mf_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue_hist,
   HIPCHK(hipMemcpy(&ctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   const int done = ctl.it;
   c->cur = start ^ ((done * c->p.rks) & 1);
+  if (iters_done) *iters_done = done;
+  if ((rc = check_sweeps(c))) return rc;
   if (residue_hist && done) HIPCHK(hipMemcpy(residue_hist, c->res_hist, done * sizeof(double), hipMemcpyDeviceToHost));
   if (res_new_hist && done) HIPCHK(hipMemcpy(res_new_hist, c->resnew_hist, done * sizeof(double), hipMemcpyDeviceToHost));
   if (cd_hist && done && c->nshape > 0) {
@@ -1112,7 +1144,7 @@ int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launc
   HIPCHK(hipMemcpy(&ctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   c->cur = c->cur ^ ((ctl.it * c->p.rks) & 1);
   c->dq_cur = c->p.inner_iterations & 1;
-  return MF_OK;
+  return check_sweeps(c);
 }
 
 }  // extern "C"
@@ -1131,11 +1163,25 @@ static bool valid_lpp(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
 // Largest dynamic LDS of one block of a staged kernel
 static constexpr size_t kMaxStageLds = 64 * 1024;
 
+// The resident inner sweeps with `lanes` lanes per point: mixed path, halos in LDS, and
+// every 64-point block of the cloud resident at once by the occupancy calculator (single
+// cases: a batch's residue logic does not stop at a halt)
+static bool valid_resident(const mf_ctx* c, int lanes) {
+  if (c->ssz != 4 || c->ncases > 0 || (lanes != 1 && lanes != 2 && lanes != 4) || c->g.hl[0].hcap <= 0) return false;
+  const size_t lds = sweeps_lds(c->g.hl[0].hcap, c->g.width, lanes);
+  if (lds > kMaxStageLds) return false;
+  // a tenth of the calculator's capacity held back: the dispatcher does not fill every
+  // slot of every CU
+  const int64_t blocks = (c->n + STAGE_P - 1) / STAGE_P;
+  return 10 * blocks <= 9 * (int64_t)sweeps_capacity_f32(lanes, c->power0, lds);
+}
+
 // A forward stencil kernel's shape: kind 0 LSQ, 1 inner iteration, 2 flux+update.  The
 // staged walks (LPP_STAGED | lanes) exist when the context's halos fit in LDS: 1, 2 or 4
 // lanes for the LSQ and inner kernels, 2, 4 or 8 for the flux kernel.
 static bool valid_shape(const mf_ctx* c, int kind, int v) {
   if (valid_lpp(v)) return true;
+  if (kind == 1 && (v & MF_LPP_RESIDENT)) return valid_resident(c, v & ~MF_LPP_RESIDENT);
   if (!(v & LPP_STAGED)) return false;
   const int l = v & ~LPP_STAGED;
   const Halo& h = c->g.hl[kind == 2 ? 1 : 0];
@@ -1180,6 +1226,36 @@ static int time_launches(mf_ctx* c, int reps, F&& fn, double* us) {
   return MF_OK;
 }
 
+// The resident sweeps' barrier state, allocated and zeroed on first use
+static int ensure_sweep_bar(mf_ctx* c) {
+  if (c->sbar) return MF_OK;
+  int rc;
+  if ((rc = dev_alloc(c, &c->sbar, 1))) return rc;
+  HIPCHK(hipMemset(c->sbar, 0, sizeof(SweepBar)));
+  return MF_OK;
+}
+
+// After a run: a wait at the resident sweeps' barrier gave up (Ctl::halt ==
+// HALT_BARRIER).  The barrier state is zeroed, the context goes back to the staged
+// inner kernel with the same lanes (its graphs are rebuilt), and the run reports an error.
+static int barrier_gave_up(mf_ctx* c) {
+  drop_graphs(c);
+  if (c->sbar) HIPCHK(hipMemset(c->sbar, 0, sizeof(SweepBar)));
+  c->lpp_inner = sweep_shape(c->lpp_inner);
+  return set_err(MF_ERR_HIP,
+                 "resident inner sweeps: a wait at the grid barrier gave up (blocks not all resident); the run "
+                 "stopped and the context now launches one sweep at a time");
+}
+
+// After queued work has finished: an error if a wait at the resident sweeps' barrier has
+// given up since the last check (the barrier's abort flag stays set until this resets it)
+static int check_sweeps(mf_ctx* c) {
+  if (!c->sbar) return MF_OK;
+  uint32_t abort_flag = 0;
+  HIPCHK(hipMemcpy(&abort_flag, &c->sbar->abort, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return abort_flag ? barrier_gave_up(c) : MF_OK;
+}
+
 // The timed launches write only scratch between iterations: dq, qm, and the next
 // buffers of prim and q (stage 2: no prim_old, wave speed or residual writes).
 static int prepare_timing(mf_ctx* c) {
@@ -1203,9 +1279,31 @@ static int time_shape(mf_ctx* c, int kind, int lpp, int reps, double* us) {
       launch_lsq_on<S>(c, q, (V4<S>*)c->dq[0], (V4<S>*)c->qm, c->cspeed, c->delta, false, nullptr, lpp,
                        c->p.inner_iterations > 0 ? c->qsum : nullptr);
     }, us);
+  if (kind == 1 && (lpp & MF_LPP_RESIDENT)) {
+    // one launch runs every inner sweep of a stage: the time per sweep
+    const int I = std::max(c->p.inner_iterations, 1);
+    int rc;
+    if ((rc = ensure_sweep_bar(c))) return rc;
+    const int keep = c->lpp_inner;
+    c->lpp_inner = lpp;
+    rc = time_launches(c, reps, [&] { launch_sweeps(c, I, c->tune_ctl); }, us);
+    c->lpp_inner = keep;
+    if (rc) return rc;
+    *us /= I;
+    Ctl tc;
+    HIPCHK(hipMemcpy(&tc, c->tune_ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (tc.halt) {
+      // a wait gave up: not a candidate
+      *us = 1e300;
+      const Ctl z{0, 0, 1.0};
+      HIPCHK(hipMemset(c->sbar, 0, sizeof(SweepBar)));
+      HIPCHK(hipMemcpy(c->tune_ctl, &z, sizeof(Ctl), hipMemcpyHostToDevice));
+    }
+    return MF_OK;
+  }
   if (kind == 1)
     return time_launches(c, reps, [&] {
-      launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, lpp, c->qsum);
+      launch_inner_on<S>(c, q, (const V4<S>*)c->dq[0], (V4<S>*)c->dq[1], nullptr, sweep_shape(lpp), c->qsum);
     }, us);
   const State<S> st = state_of<S>(c, cur, 0);
   return time_launches(c, reps, [&] {
@@ -1233,6 +1331,10 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
     for (int l : {1, 2, 4, 8})
       if (valid_shape(c, k, LPP_STAGED | l)) cand_st[k].push_back(LPP_STAGED | l);
   }
+  // and the resident inner sweeps
+  if (c->p.inner_iterations > 0)
+    for (int l : {1, 2, 4})
+      if (valid_shape(c, 1, MF_LPP_RESIDENT | l)) cand_st[1].push_back(MF_LPP_RESIDENT | l);
   double best[3] = {1e300, 1e300, 1e300};
   int pick[3] = {1, 1, 1};
   for (int k = 0; k < 3; ++k) {
@@ -1248,7 +1350,7 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   c->lpp_flux = pick[2];
   drop_graphs(c);
   if (t)
-    *t = mf_tuning{pick[0], pick[1], pick[2], 0, best[0], best[1], best[2],
+    *t = mf_tuning{pick[0], pick[1], pick[2], (int32_t)c->barrier_polls, best[0], best[1], best[2],
                    c->lpp_rev[0], c->lpp_rev[1], c->lpp_rev[2], c->lpp_rev[3]};
   return MF_OK;
 }
@@ -1282,7 +1384,7 @@ int mf_time_kernels(mf_ctx* c, int32_t reps, double* us) {
 
 int mf_get_tuning(mf_ctx* c, mf_tuning* out) {
   if (!c || !out) return set_err(MF_ERR_INVALID, "mf_get_tuning: null argument");
-  *out = mf_tuning{c->lpp_lsq, c->lpp_inner, c->lpp_flux, 0, 0.0, 0.0, 0.0,
+  *out = mf_tuning{c->lpp_lsq, c->lpp_inner, c->lpp_flux, (int32_t)c->barrier_polls, 0.0, 0.0, 0.0,
                    c->lpp_rev[0], c->lpp_rev[1], c->lpp_rev[2], c->lpp_rev[3]};
   return MF_OK;
 }
@@ -1292,7 +1394,9 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   if (!valid_shape(c, 0, t->lpp_lsq) || !valid_shape(c, 1, t->lpp_inner) || !valid_shape(c, 2, t->lpp_flux))
     return set_err(MF_ERR_INVALID,
                    "lanes per point must be 1, 2, 4 or 8 (LSQ and inner kernels also MF_LPP_STAGED | 1, 2 or 4 "
-                   "when the context's halos fit in LDS)");
+                   "when the context's halos fit in LDS; the inner kernel MF_LPP_RESIDENT | 1, 2 or 4 on a "
+                   "mixed context whose blocks all fit on the device)");
+  if (t->barrier_polls < 0) return set_err(MF_ERR_INVALID, "barrier_polls must be >= 0");
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
     if (rv[k] != 0 && !valid_lpp(rv[k]) && !(k == 0 && valid_shape(c, 2, rv[k])) &&
@@ -1300,9 +1404,12 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
       return set_err(MF_ERR_INVALID,
                      "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux, LSQ and inner reverses "
                      "also their staged shapes)");
+  int rc;
+  if ((t->lpp_inner & MF_LPP_RESIDENT) && (rc = ensure_sweep_bar(c))) return rc;
   c->lpp_lsq = t->lpp_lsq;
   c->lpp_inner = t->lpp_inner;
   c->lpp_flux = t->lpp_flux;
+  c->barrier_polls = t->barrier_polls > 0 ? (uint32_t)t->barrier_polls : kDefaultBarrierPolls;
   for (int k = 0; k < 4; ++k) c->lpp_rev[k] = rv[k];
   adj_forget_tuning(c);
   drop_graphs(c);

@@ -1038,7 +1038,7 @@ static int adj_record_mixed(mf_ctx* c, AdjState& A, bool widen) {
                          sums);
     for (int j = 1; j <= A.I; ++j)
       launch_inner_on<float>(c, (const V4f*)A.fq[s], (const V4f*)A.fdq[s * (A.I + 1) + j - 1],
-                             A.fdq[s * (A.I + 1) + j], nullptr, c->lpp_inner, sums);
+                             A.fdq[s * (A.I + 1) + j], nullptr, sweep_shape(c->lpp_inner), sums);
     State<float> st{A.fq[s], A.fdq[s * (A.I + 1) + A.I], A.fqm[s], (const float*)c->eps};
     launch_flux_update_on<float>(c, st, (const V4f*)A.fprim[s], A.fpold, A.fprim[s + 1], A.fq[s + 1], A.tcspeed,
                                  (const double*)A.tdelta, A.tpartial, (const Ctl*)A.tctl, s + 1, s + 1 == A.S,
@@ -1060,7 +1060,7 @@ static int adj_record(mf_ctx* c, AdjState& A) {
     launch_lsq_on<double>(c, (const V4d*)A.tq[s], dq0, A.tqm[s], A.tcspeed, A.tdelta, s == 0, nullptr, c->lpp_lsq);
     for (int j = 1; j <= A.I; ++j)
       launch_inner_on<double>(c, (const V4d*)A.tq[s], (const V4d*)A.tdq[s * (A.I + 1) + j - 1],
-                              A.tdq[s * (A.I + 1) + j], nullptr, c->lpp_inner);
+                              A.tdq[s * (A.I + 1) + j], nullptr, sweep_shape(c->lpp_inner));
     State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], c->eps64};
     launch_flux_update_on<double>(c, st, (const V4d*)A.tprim[s], A.tpold, A.tprim[s + 1], A.tq[s + 1], A.tcspeed,
                                   (const double*)A.tdelta, A.tpartial, (const Ctl*)A.tctl, s + 1, s + 1 == A.S,
@@ -1621,6 +1621,7 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   HIPCHK(hipEventRecord(A.e1, c->stream));
   HIPCHK(hipEventSynchronize(A.e1));
   HIPCHK(hipEventElapsedTime(&A.primal_ms, A.e0, A.e1));
+  if ((rc = check_sweeps(c))) return rc;
   A.peak_snaps = (int)spine.size() + 1;
   std::vector<double> cl(c->nsid), cd(c->nsid), cm(c->nsid);
   if ((rc = mf_compute_cl_cd_cm(c, cl.data(), cd.data(), cm.data(), nullptr))) return rc;
@@ -1699,6 +1700,8 @@ int mf_adjoint_backward(mf_ctx* c, double* dJdx, double* dJdy, mf_adjoint_stats*
   HIPCHK(hipEventElapsedTime(&bms, b0, b1));
   (void)hipEventDestroy(b0);
   (void)hipEventDestroy(b1);
+  // the recomputed primal stretches ran the iteration's graphs
+  if ((rc = check_sweeps(c))) return rc;
   std::vector<D2> h(n);
   HIPCHK(hipMemcpy(h.data(), A.xyb, n * sizeof(D2), hipMemcpyDeviceToHost));
   for (int j = 0; j < n; ++j) {

@@ -2,6 +2,7 @@
 // instantiates the launchers of lskum_kernels.h for S = float and is compiled with FMA
 // contraction (the FP64 path, in mfcfd.hip, is not).
 #include "lskum_kernels.h"
+#include "lskum_sweeps.h"
 
 void launch_lsq_f32(const LaunchEnv& e, const V4f* q, V4f* dq, V4f* qm, const double* cspeed, double* delta,
                     bool with_delta, const Ctl* ctl, int lpp, V4f* qsum) {
@@ -12,6 +13,13 @@ void launch_inner_f32(const LaunchEnv& e, const V4f* q, const V4f* a, V4f* b, co
                       const V4f* qsum) {
   launch_inner_env<float>(e, q, a, b, ctl, lpp, qsum);
 }
+
+void launch_sweeps_f32(const LaunchEnv& e, V4f* dq0, V4f* dq1, int sweeps, Ctl* ctl, const V4f* qsum, SweepBar* bar,
+                       uint32_t spin_limit, int lpp) {
+  launch_sweeps_env(e, dq0, dq1, sweeps, ctl, qsum, bar, spin_limit, lpp);
+}
+
+int sweeps_capacity_f32(int lpp, bool power0, size_t lds) { return sweeps_capacity(lpp, power0, lds); }
 
 void launch_flux_update_f32(const LaunchEnv& e, const State<float>& st, const V4f* prim, V4f* pold, V4f* pout,
                             V4f* qout, double* cspeed, const double* delta, double* partial, const Ctl* ctl,

@@ -195,15 +195,16 @@ class LskumSolver:
         of each stencil kernel; returns the choice and the measured launch times."""
         t = L.MfTuning()
         L.check(L.lib().mf_autotune(self.h, int(reps), C.byref(t)), "mf_autotune")
-        return {f: getattr(t, f) for f, _ in L.MfTuning._fields_ if f != "reserved"}
+        return {f: getattr(t, f) for f, _ in L.MfTuning._fields_}
 
     _SHAPES = ("lpp_lsq", "lpp_inner", "lpp_flux", "lpp_flux_rev", "lpp_lsq_rev", "lpp_inner_rev",
-               "lpp_gather")
+               "lpp_gather", "barrier_polls")
 
     @property
     def tuning(self) -> dict:
         """Lanes per point of each stencil kernel (the reverse ones 0 = timed by the
-        first adjoint run)."""
+        first adjoint run), and the polls before a wait at the resident inner sweeps'
+        grid barrier gives up."""
         t = L.MfTuning()
         L.check(L.lib().mf_get_tuning(self.h, C.byref(t)), "mf_get_tuning")
         return {k: getattr(t, k) for k in self._SHAPES}

@@ -197,7 +197,9 @@ def test_autotune_and_invalid_shape(small_cloud):
         before = a.prim.copy()
         t = a.autotune(reps=3)
         staged = tuple(16 | v for v in (1, 2, 4))
-        assert all(t[k] in (1, 2, 4, 8) + staged for k in ("lpp_lsq", "lpp_inner"))
+        resident = tuple(32 | v for v in (1, 2, 4))
+        assert t["lpp_lsq"] in (1, 2, 4, 8) + staged
+        assert t["lpp_inner"] in (1, 2, 4, 8) + staged + resident
         assert t["lpp_flux"] in (1, 2, 4, 8, 16 | 2, 16 | 4, 16 | 8)
         assert all(t[k] > 0 for k in ("us_lsq", "us_inner", "us_flux"))
         assert np.array_equal(a.prim, before)
