@@ -189,7 +189,57 @@ def cpu_baseline_reference(cloud, params: dict, prim: np.ndarray):
                 seconds=times)
 
 
-def run_adjoint(iters: int, snaps: int):
+def cpu_adjoint_reference(cloud, params: dict, prim: np.ndarray):
+    """The reference's Tapenade reverse routines (src_mpi_adjoint's *_b modules built
+    unmodified by oracle/Makefile into oracle/_tap; single-threaded) on the adjoint
+    workload's cloud from its final state: one reversed SSPRK stage's
+    cal_flux_residual_b, inner iterations (eval_update_innerloop_b + eval_q_inner_loop_b
+    each), eval_q_derivatives_b and eval_q_variables_b, each timed.  A reversed
+    iteration is 4 such stages plus state_update_b and func_delta_b, which include PETSc
+    headers and are not built."""
+    from oracle.oracle import Oracle, TapRoutines
+    if not TapRoutines.available():
+        return None
+    o = Oracle(cloud, **W.oracle_params(params)).setup()
+    t = TapRoutines(cloud, vl_const=params["vl_const"])
+    t.set_split(o)
+    del o
+    t.call("compute_normals")
+    t.set("prim", prim.reshape(-1))
+    t.call("eval_q_variables")
+    t.call("eval_q_derivatives")
+    t.set("delta", np.full(cloud.n, 1e-3))
+    rng = np.random.default_rng(7)
+    n, inner = cloud.n, params["inner_iterations"]
+    times = {}
+
+    def timed(name, routines, seeds):
+        t.zero_adjoints()
+        for f, size in seeds:
+            t.set_adj(f, rng.standard_normal(size * n))
+        print(f"[bench] cpu adjoint (Tapenade routines): {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for r in routines:
+            t.call(r)
+        times[name] = time.perf_counter() - t0
+
+    timed("cal_flux_residual_b", ["cal_flux_residual_b"], [("flux_res", 4)])
+    timed("inner_iteration_b", ["eval_update_innerloop_b", "eval_q_inner_loop_b"], [("dq", 8)])
+    timed("eval_q_derivatives_b", ["eval_q_derivatives_b"], [("dq", 8), ("qm", 8)])
+    timed("eval_q_variables_b", ["eval_q_variables_b"], [("q", 4)])
+    stage = times["cal_flux_residual_b"] + inner * times["inner_iteration_b"] + \
+        times["eval_q_derivatives_b"] + times["eval_q_variables_b"]
+    return dict(seconds_per_reversed_iteration=4 * stage, cores=1, kind="reference",
+                sample=f"one reversed SSPRK stage of the reference's Tapenade routines (src_mpi_adjoint "
+                       f"*_b modules, amdflang -O2, 1 core) on the {n}-point cloud from the adjoint's "
+                       f"final state, {inner} inner iterations: "
+                       + ", ".join(f"{k} {v:.3f} s" for k, v in times.items())
+                       + "; a reversed iteration = 4 stages (state_update_b and func_delta_b, "
+                         "PETSc-bound, not built, excluded)",
+                seconds=times)
+
+
+def run_adjoint(iters: int, snaps: int, cpu: bool = True):
     """Cd adjoint of the 40,000-point workload: `iters` primal iterations with `snaps`
     binomial checkpoints, then the reverse sweep; also every iterate kept and every
     iteration taped.  Cost is reported against a plain primal run of the same
@@ -215,6 +265,7 @@ def run_adjoint(iters: int, snaps: int):
             t0 = time.perf_counter()
             J, dx, dy, st = s.adjoint(iters, snaps=ns)
             adj_s = time.perf_counter() - t0
+            final_prim = s.prim
         grads[key] = (J, np.concatenate([dx, dy]))
         out[key] = {"snapshots": ns, "J": J, "adjoint_total_s": adj_s,
                     "adjoint_over_primal": adj_s / primal_s, "stats": st,
@@ -249,6 +300,14 @@ def run_adjoint(iters: int, snaps: int):
     out["tangent"] = {"J": Jt, "dJ": dJt, "seconds": tan_s, "over_primal": tan_s / primal_s,
                       "adjoint_directional": float(g0 @ v),
                       "rel_vs_adjoint": abs(dJt - float(g0 @ v)) / abs(float(g0 @ v))}
+    if cpu:
+        ref = cpu_adjoint_reference(cloud, p, final_prim)
+        if ref is not None:
+            # the GPU's reverse sweep per iteration, every iteration taped (no re-runs)
+            gpu_rev = 1e-3 * out["tape_all"]["stats"]["adjoint_ms"] / iters
+            ref["gpu_seconds_per_reversed_iteration"] = gpu_rev
+            ref["gpu_over_cpu"] = ref["seconds_per_reversed_iteration"] / gpu_rev
+            out["cpu_adjoint"] = ref
     return out
 
 
@@ -560,7 +619,7 @@ def main():
         line["c2"] = run_c2()
         line["batch"] = run_batch(W.C5["iterations"], adjoint=False)
         if adj_iters > 0:
-            line["adjoint"] = run_adjoint(adj_iters, W.C3["snaps"])
+            line["adjoint"] = run_adjoint(adj_iters, W.C3["snaps"], cpu=not args.no_cpu)
     if not args.no_cpu:
         ref = cpu_baseline_reference(cloud, params, fp64["final_prim"])
         port = cpu_baseline_port(cloud, params, args.cpu_iters)
