@@ -227,19 +227,38 @@ __global__ __launch_bounds__(BLOCK) void k_lsq(Geom g, const V4<S>* __restrict__
       qmin[r] = group_min<LPP>(qmin[r]);
     }
     if (DELTA) min_delt = group_min<LPP>(min_delt);
-    if (lane != 0) return;
+    if (lane > 1) return;
   }
   if (halted) return;
   V4<S> ox, oy;
   lsq_solve(g, i, sxq, syq, ox, oy);
-  dq[2 * i] = ox;
-  dq[2 * i + 1] = oy;
-  qm[2 * i] = qmax;
-  qm[2 * i + 1] = qmin;
-  if (DELTA) delta[i] = min_delt;
-  if (qsum) {
-    qsum[2 * i] = V4<float>{(float)sxq[0], (float)sxq[1], (float)sxq[2], (float)sxq[3]};
-    qsum[2 * i + 1] = V4<float>{(float)syq[0], (float)syq[1], (float)syq[2], (float)syq[3]};
+  if constexpr (LPP == 1) {
+    dq[2 * i] = ox;
+    dq[2 * i + 1] = oy;
+    qm[2 * i] = qmax;
+    qm[2 * i + 1] = qmin;
+    if (DELTA) delta[i] = min_delt;
+    if (qsum) {
+      qsum[2 * i] = V4<float>{(float)sxq[0], (float)sxq[1], (float)sxq[2], (float)sxq[3]};
+      qsum[2 * i + 1] = V4<float>{(float)syq[0], (float)syq[1], (float)syq[2], (float)syq[3]};
+    }
+  } else {
+    // every lane of the group holds the sums: lane 0 stores the x (max) half of each
+    // pair, lane 1 the y (min) half, so one store covers a point's pair contiguously
+    // (selects per component: a select of whole records would put them in scratch)
+    const bool hy = lane == 1;
+    V4<S> dv, mv;
+    V4<float> sv;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      dv[r] = hy ? oy[r] : ox[r];
+      mv[r] = hy ? qmin[r] : qmax[r];
+      sv[r] = (float)(hy ? syq[r] : sxq[r]);
+    }
+    dq[2 * i + lane] = dv;
+    qm[2 * i + lane] = mv;
+    if (DELTA && !hy) delta[i] = min_delt;
+    if (qsum) qsum[2 * i + lane] = sv;
   }
 }
 
@@ -368,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
       sxq[r] = group_sum<LPP>(sxq[r]);
       syq[r] = group_sum<LPP>(syq[r]);
     }
-    if (lane != 0) return;
+    if (lane > 1) return;
   }
   if (halted) return;
   if constexpr (SUMS) {
@@ -382,8 +401,16 @@ __global__ __launch_bounds__(BLOCK) void k_inner(Geom g, const V4<S>* __restrict
   }
   V4<S> ox, oy;
   lsq_solve(g, i, sxq, syq, ox, oy);
-  dq_out[2 * i] = ox;
-  dq_out[2 * i + 1] = oy;
+  if constexpr (LPP == 1) {
+    dq_out[2 * i] = ox;
+    dq_out[2 * i + 1] = oy;
+  } else {
+    // lanes 0 and 1 hold the same sums: each stores one half of the point's pair
+    V4<S> dv;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dv[r] = lane == 1 ? oy[r] : ox[r];
+    dq_out[2 * i + lane] = dv;
+  }
 }
 
 template <class S, bool POWER0, int LPP>
