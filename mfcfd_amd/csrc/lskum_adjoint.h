@@ -15,6 +15,12 @@
 
 namespace mf {
 
+// The reverse routines contract multiply-adds into FMAs: the reverse is held to the
+// tangent and to finite differences (1e-10 and 1e-6), not to rounding parity with a
+// reference, and its recomputed forward values only steer the same branches as the
+// primal's (limiter thresholds, stencil halves from the taped connectivity bits).
+#pragma clang fp contract(fast)
+
 constexpr double TWO_BY_SQRT_PI = 1.1283791670955126;
 constexpr double INV_SQRT_PI = 0.5641895835477563;
 constexpr double INV_GAMMA_M1 = 1.0 / Consts<double>::GAMMA_M1;
@@ -510,5 +516,7 @@ __host__ __device__ inline void wave_speed_rev(const V4d& p, double cb, double p
   pb[3] += zb / p.x;
   pb[0] -= zb * z / p.x;
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace mf

@@ -19,6 +19,9 @@
 
 namespace mf {
 
+// reverse kernels: multiply-adds contracted into FMAs, as in lskum_adjoint.h
+#pragma clang fp contract(fast)
+
 __device__ inline void aadd(double* p, double v) {
   if (v != 0.0) unsafeAtomicAdd(p, v);
 }
@@ -918,6 +921,8 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
     p[j] = 0.0;
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace mf
 
