@@ -94,7 +94,9 @@ int mf_get_prim(mf_ctx *ctx, double *prim);
 
 /* Fields of the module state, original order, reference layouts. */
 #define MF_FIELD_Q 1         /* q(4,N)                                */
-#define MF_FIELD_DQ 2        /* dq(2,4,N)                             */
+#define MF_FIELD_DQ 2        /* dq(2,4,N); after a first-order run that of the last
+                              * iteration's first stage (first order does not use dq,
+                              * so mf_run computes it only where the time step needs it) */
 #define MF_FIELD_QM 3        /* qm(2,4,N)                             */
 #define MF_FIELD_FLUX_RES 4  /* flux_res(4,N) (last cal_flux_residual) */
 #define MF_FIELD_DELTA 5     /* delta(N)                              */

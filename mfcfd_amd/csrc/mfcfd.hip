@@ -192,7 +192,8 @@ struct mf_ctx {
   int hist_cap = 0;
   int32_t *sh_m = nullptr, *sh_id = nullptr;
   double *sgeo = nullptr, *cp = nullptr, *clcdcm = nullptr;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // captured iterations: [phase] one iteration, [2 + phase] kGraphIters iterations
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   // kept for the adjoint: internal ids of left/right (-1 for interior points), the
   // half-stencil sums, the shape points
   std::vector<int32_t> left_i, right_i, h_sh_m;
@@ -980,11 +981,16 @@ struct Timer {
 template <class S>
 static int record_iteration(mf_ctx* c, int phase, Timer* tm) {
   int buf = phase;
-  const int I = c->p.inner_iterations;
+  // first order: the fluxes take q at both ends, so dq, qm and the inner iterations are
+  // not needed; stage 1's LSQ kernel still runs for the time steps it fuses
+  const bool first = c->p.first_order != 0;
+  const int I = first ? 0 : c->p.inner_iterations;
   for (int rk = 1; rk <= c->p.rks; ++rk) {
-    if (tm) tm->begin(KK_LSQ);
-    launch_lsq<S>(c, buf, rk == 1, c->ctl, I > 0);
-    if (tm) tm->end(KK_LSQ);
+    if (rk == 1 || !first) {
+      if (tm) tm->begin(KK_LSQ);
+      launch_lsq<S>(c, buf, rk == 1, c->ctl, I > 0);
+      if (tm) tm->end(KK_LSQ);
+    }
     int d = 0;
     if (is_f32<S> && I > 0 && (c->lpp_inner & MF_LPP_RESIDENT)) {
       if (tm) tm->begin(KK_INNER);
@@ -1039,13 +1045,21 @@ static int ensure_history(mf_ctx* c, int iters) {
   return MF_OK;
 }
 
-static int build_graph(mf_ctx* c, int phase) {
-  if (c->graph[phase]) return MF_OK;
+// Iterations per graph launch on long runs: one launch replays this many captured
+// iterations back to back (the iteration counter, residue histories and halt flag are
+// device-side, so the replays need nothing from the host between iterations)
+static constexpr int kGraphIters = 16;
+
+// the graph of `n` iterations (1 or kGraphIters) from buffer phase `phase`
+static int build_graph(mf_ctx* c, int phase, int n) {
+  hipGraphExec_t& slot = c->graph[(n == 1 ? 0 : 2) + phase];
+  if (slot) return MF_OK;
   hipGraph_t gr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  record(c, phase, nullptr);
+  int ph = phase;
+  for (int j = 0; j < n; ++j) ph = record(c, ph, nullptr);
   HIPCHK(hipStreamEndCapture(c->stream, &gr));
-  hipError_t e = hipGraphInstantiate(&c->graph[phase], gr, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&slot, gr, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gr);
   if (e != hipSuccess) return set_err(MF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
   return MF_OK;
@@ -1061,12 +1075,18 @@ static int run_queue(mf_ctx* c, int32_t first_it, int32_t iters) {
                      (int)first_it);
   HIPCHK(hipGetLastError());
   int phase = c->cur;
-  for (int it = 0; it < iters; ++it) {
-    if ((rc = build_graph(c, phase))) return rc;
+  int it = 0;
+  for (; it + kGraphIters <= iters; it += kGraphIters) {
+    if ((rc = build_graph(c, phase, kGraphIters))) return rc;
+    HIPCHK(hipGraphLaunch(c->graph[2 + phase], c->stream));
+    phase ^= (kGraphIters * c->p.rks) & 1;
+  }
+  for (; it < iters; ++it) {
+    if ((rc = build_graph(c, phase, 1))) return rc;
     HIPCHK(hipGraphLaunch(c->graph[phase], c->stream));
     phase ^= (c->p.rks & 1);
   }
-  c->dq_cur = c->p.inner_iterations & 1;
+  c->dq_cur = c->p.first_order ? 0 : c->p.inner_iterations & 1;
   return MF_OK;
 }
 
@@ -1143,7 +1163,7 @@ int mf_profile_kernels(mf_ctx* c, int32_t iters, double* mean_ms, int32_t* launc
   Ctl ctl;
   HIPCHK(hipMemcpy(&ctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   c->cur = c->cur ^ ((ctl.it * c->p.rks) & 1);
-  c->dq_cur = c->p.inner_iterations & 1;
+  c->dq_cur = c->p.first_order ? 0 : c->p.inner_iterations & 1;
   return check_sweeps(c);
 }
 
