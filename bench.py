@@ -547,7 +547,7 @@ def main():
         return
     adj_iters = W.C3["iterations"] if args.adjoint_iters is None else args.adjoint_iters
     if args.adjoint:
-        print(json.dumps(run_adjoint(adj_iters, W.C3["snaps"])))
+        print(json.dumps(run_adjoint(adj_iters, W.C3["snaps"], cpu=not args.no_cpu)))
         return
     if args.batch:
         print(json.dumps(run_batch(args.batch_iters or W.C5["iterations"], adjoint=args.batch_adjoint)))

@@ -26,52 +26,93 @@ constexpr double INV_SQRT_PI = 0.5641895835477563;
 constexpr double INV_GAMMA_M1 = 1.0 / Consts<double>::GAMMA_M1;
 
 // ---------------------------------------------------------------------------------
-// q-tilde -> primitive -> flux preliminaries, with every intermediate kept
-struct KinF {
-  double qt[4];
-  double beta, tmp, u1, u2, T1, T2, rho, p, ut, un, b, sb, spb, S1, S2, ex1, ex2, B1, B2, e1,
-      e2, prr, usq;
+// q-tilde -> primitive -> flux preliminaries, with every intermediate kept.  S is the
+// arithmetic of the recomputation and of the reverse through the kinetic state: double
+// for the FP64 adjoint; float for the mixed-precision adjoint, whose fluxes and limiter
+// run in FP32 as the mixed primal's do (its sensitivities still accumulate in FP64).
+template <class S>
+struct KinFT {
+  S qt[4];
+  S beta, tmp, u1, u2, T1, T2, rho, p, ut, un, b, sb, spb, S1, S2, ex1, ex2, B1, B2, e1, e2, prr, usq;
 };
+using KinF = KinFT<double>;
 
-struct KinB {
-  double rho, ut, un, prr, usq, B1, B2, e1, e2;
+template <class S>
+struct KinBT {
+  S rho, ut, un, prr, usq, B1, B2, e1, e2;
 };
+using KinB = KinBT<double>;
 
-__host__ __device__ inline void kinb_zero(KinB& b) {
-  b.rho = b.ut = b.un = b.prr = b.usq = b.B1 = b.B2 = b.e1 = b.e2 = 0.0;
+template <class S>
+__host__ __device__ inline void kinb_zero(KinBT<S>& b) {
+  b.rho = b.ut = b.un = b.prr = b.usq = b.B1 = b.B2 = b.e1 = b.e2 = (S)0;
 }
 
-// Same operations, in the same order, as kin_of_qtilde<double>
-__host__ __device__ inline void kin_fwd(const double qt[4], double nx, double ny, KinF& k) {
+// The recomputation's elementary functions: the library's in FP64 (the primal's own
+// operations); in FP32 the hardware's, as the mixed primal (kin_fast) uses them.
+template <class S>
+__host__ __device__ inline S rv_rcp(S x) {
+  if constexpr (sizeof(S) == 4) return frcp(x);
+  else return 1.0 / x;
+}
+
+// Same operations, in the same order, as kin_of_qtilde<double> (FP64)
+template <class S>
+__host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) {
   for (int r = 0; r < 4; ++r) k.qt[r] = qt[r];
-  k.beta = -qt[3] * 0.5;
-  k.tmp = 0.5 / k.beta;
+  k.beta = -qt[3] * (S)0.5;
+  if constexpr (sizeof(S) == 4) k.tmp = 0.5f * frcp(k.beta);
+  else k.tmp = 0.5 / k.beta;
   k.u1 = qt[1] * k.tmp;
   k.u2 = qt[2] * k.tmp;
   k.T1 = qt[0] + k.beta * (k.u1 * k.u1 + k.u2 * k.u2);
-  k.T2 = k.T1 - (m_log(k.beta) / Consts<double>::GAMMA_M1);
-  k.rho = m_exp(k.T2);
+  if constexpr (sizeof(S) == 4) {
+    k.T2 = k.T1 - flog(k.beta) * (1.0f / Consts<float>::GAMMA_M1);
+    k.rho = fexp(k.T2);
+  } else {
+    k.T2 = k.T1 - (m_log(k.beta) / Consts<double>::GAMMA_M1);
+    k.rho = m_exp(k.T2);
+  }
   k.p = k.rho * k.tmp;
-  const double tx = ny, ty = -nx;
+  const S tx = ny, ty = -nx;
   k.ut = k.u1 * tx + k.u2 * ty;
   k.un = k.u1 * nx + k.u2 * ny;
-  k.b = 0.5 * k.rho / k.p;
-  k.sb = m_sqrt(k.b);
-  k.spb = m_sqrt(Consts<double>::PI * k.b);
+  if constexpr (sizeof(S) == 4) {
+    // b = rho / (2 p) = beta
+    k.b = k.beta;
+    k.sb = __builtin_amdgcn_sqrtf(k.b);
+    k.spb = k.sb * 1.7724538509f;  // sqrt(pi b)
+  } else {
+    k.b = 0.5 * k.rho / k.p;
+    k.sb = m_sqrt(k.b);
+    k.spb = m_sqrt(Consts<double>::PI * k.b);
+  }
   k.S1 = k.ut * k.sb;
   k.S2 = k.un * k.sb;
-  k.ex1 = m_exp(-k.S1 * k.S1);
-  k.ex2 = m_exp(-k.S2 * k.S2);
-  k.B1 = 0.5 * k.ex1 / k.spb;
-  k.B2 = 0.5 * k.ex2 / k.spb;
-  k.e1 = m_erf(k.S1);
-  k.e2 = m_erf(k.S2);
-  k.prr = k.p / k.rho;
+  if constexpr (sizeof(S) == 4) {
+    k.ex1 = fexp(-k.S1 * k.S1);
+    k.ex2 = fexp(-k.S2 * k.S2);
+    const float ispb = frcp(k.spb);
+    k.B1 = 0.5f * k.ex1 * ispb;
+    k.B2 = 0.5f * k.ex2 * ispb;
+    k.e1 = ferf_from_exp(k.S1, k.ex1);
+    k.e2 = ferf_from_exp(k.S2, k.ex2);
+    k.prr = k.tmp;  // p / rho = 1 / (2 beta)
+  } else {
+    k.ex1 = m_exp(-k.S1 * k.S1);
+    k.ex2 = m_exp(-k.S2 * k.S2);
+    k.B1 = 0.5 * k.ex1 / k.spb;
+    k.B2 = 0.5 * k.ex2 / k.spb;
+    k.e1 = m_erf(k.S1);
+    k.e2 = m_erf(k.S2);
+    k.prr = k.p / k.rho;
+  }
   k.usq = k.ut * k.ut + k.un * k.un;
 }
 
-__host__ __device__ inline Kin<double> kin_of(const KinF& f) {
-  Kin<double> k;
+template <class S>
+__host__ __device__ inline Kin<S> kin_of(const KinFT<S>& f) {
+  Kin<S> k;
   k.rho = f.rho;
   k.ut = f.ut;
   k.un = f.un;
@@ -85,85 +126,88 @@ __host__ __device__ inline Kin<double> kin_of(const KinF& f) {
 }
 
 // reverse of flux_gx (split_fluxes.F90:8-96)
-__host__ __device__ inline void flux_gx_rev(const KinF& k, bool plus, const double Gb[4], KinB& kb) {
-  const double s = plus ? 1.0 : -1.0;
-  const double A1 = 0.5 * (1.0 + s * k.e1);
-  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
-  const double F0 = k.ut * A1 + s * k.B1;
-  const double F1 = (k.prr + k.ut * k.ut) * A1 + s * k.ut * k.B1;
-  const double F2 = k.ut * k.un * A1 + s * k.un * k.B1;
-  const double F3 = 0.5 * k.ut * a7 * A1 + s * 0.5 * a6 * k.B1;
+template <class S>
+__host__ __device__ inline void flux_gx_rev(const KinFT<S>& k, bool plus, const S Gb[4], KinBT<S>& kb) {
+  const S s = plus ? (S)1 : (S)-1;
+  const S A1 = (S)0.5 * ((S)1 + s * k.e1);
+  const S a7 = (S)7 * k.prr + k.usq, a6 = (S)6 * k.prr + k.usq;
+  const S F0 = k.ut * A1 + s * k.B1;
+  const S F1 = (k.prr + k.ut * k.ut) * A1 + s * k.ut * k.B1;
+  const S F2 = k.ut * k.un * A1 + s * k.un * k.B1;
+  const S F3 = (S)0.5 * k.ut * a7 * A1 + s * (S)0.5 * a6 * k.B1;
   kb.rho += Gb[0] * F0 + Gb[1] * F1 + Gb[2] * F2 + Gb[3] * F3;
-  const double g0 = k.rho * Gb[0], g1 = k.rho * Gb[1], g2 = k.rho * Gb[2], g3 = k.rho * Gb[3];
-  const double A1b = g0 * k.ut + g1 * (k.prr + k.ut * k.ut) + g2 * k.ut * k.un + g3 * 0.5 * k.ut * a7;
-  kb.B1 += s * (g0 + g1 * k.ut + g2 * k.un + g3 * 0.5 * a6);
-  kb.ut += g0 * A1 + g1 * (2.0 * k.ut * A1 + s * k.B1) + g2 * k.un * A1 + g3 * 0.5 * a7 * A1;
+  const S g0 = k.rho * Gb[0], g1 = k.rho * Gb[1], g2 = k.rho * Gb[2], g3 = k.rho * Gb[3];
+  const S A1b = g0 * k.ut + g1 * (k.prr + k.ut * k.ut) + g2 * k.ut * k.un + g3 * (S)0.5 * k.ut * a7;
+  kb.B1 += s * (g0 + g1 * k.ut + g2 * k.un + g3 * (S)0.5 * a6);
+  kb.ut += g0 * A1 + g1 * ((S)2 * k.ut * A1 + s * k.B1) + g2 * k.un * A1 + g3 * (S)0.5 * a7 * A1;
   kb.un += g2 * (k.ut * A1 + s * k.B1);
-  kb.prr += g1 * A1 + g3 * (3.5 * k.ut * A1 + 3.0 * s * k.B1);
-  kb.usq += g3 * 0.5 * (k.ut * A1 + s * k.B1);
-  kb.e1 += 0.5 * s * A1b;
+  kb.prr += g1 * A1 + g3 * ((S)3.5 * k.ut * A1 + (S)3 * s * k.B1);
+  kb.usq += g3 * (S)0.5 * (k.ut * A1 + s * k.B1);
+  kb.e1 += (S)0.5 * s * A1b;
 }
 
 // reverse of flux_gy (split_fluxes.F90:99-190); outputs 1 and 2 swap roles
-__host__ __device__ inline void flux_gy_rev(const KinF& k, bool plus, const double Gb[4], KinB& kb) {
-  const double s = plus ? 1.0 : -1.0;
-  const double A2 = 0.5 * (1.0 + s * k.e2);
-  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
-  const double F0 = k.un * A2 + s * k.B2;
-  const double F1 = k.ut * k.un * A2 + s * k.ut * k.B2;
-  const double F2 = (k.prr + k.un * k.un) * A2 + s * k.un * k.B2;
-  const double F3 = 0.5 * k.un * a7 * A2 + s * 0.5 * a6 * k.B2;
+template <class S>
+__host__ __device__ inline void flux_gy_rev(const KinFT<S>& k, bool plus, const S Gb[4], KinBT<S>& kb) {
+  const S s = plus ? (S)1 : (S)-1;
+  const S A2 = (S)0.5 * ((S)1 + s * k.e2);
+  const S a7 = (S)7 * k.prr + k.usq, a6 = (S)6 * k.prr + k.usq;
+  const S F0 = k.un * A2 + s * k.B2;
+  const S F1 = k.ut * k.un * A2 + s * k.ut * k.B2;
+  const S F2 = (k.prr + k.un * k.un) * A2 + s * k.un * k.B2;
+  const S F3 = (S)0.5 * k.un * a7 * A2 + s * (S)0.5 * a6 * k.B2;
   kb.rho += Gb[0] * F0 + Gb[1] * F1 + Gb[2] * F2 + Gb[3] * F3;
-  const double h0 = k.rho * Gb[0], hT = k.rho * Gb[1], hN = k.rho * Gb[2], h3 = k.rho * Gb[3];
-  const double A2b = h0 * k.un + hN * (k.prr + k.un * k.un) + hT * k.ut * k.un + h3 * 0.5 * k.un * a7;
-  kb.B2 += s * (h0 + hN * k.un + hT * k.ut + h3 * 0.5 * a6);
-  kb.un += h0 * A2 + hN * (2.0 * k.un * A2 + s * k.B2) + hT * k.ut * A2 + h3 * 0.5 * a7 * A2;
+  const S h0 = k.rho * Gb[0], hT = k.rho * Gb[1], hN = k.rho * Gb[2], h3 = k.rho * Gb[3];
+  const S A2b = h0 * k.un + hN * (k.prr + k.un * k.un) + hT * k.ut * k.un + h3 * (S)0.5 * k.un * a7;
+  kb.B2 += s * (h0 + hN * k.un + hT * k.ut + h3 * (S)0.5 * a6);
+  kb.un += h0 * A2 + hN * ((S)2 * k.un * A2 + s * k.B2) + hT * k.ut * A2 + h3 * (S)0.5 * a7 * A2;
   kb.ut += hT * (k.un * A2 + s * k.B2);
-  kb.prr += hN * A2 + h3 * (3.5 * k.un * A2 + 3.0 * s * k.B2);
-  kb.usq += h3 * 0.5 * (k.un * A2 + s * k.B2);
-  kb.e2 += 0.5 * s * A2b;
+  kb.prr += hN * A2 + h3 * ((S)3.5 * k.un * A2 + (S)3 * s * k.B2);
+  kb.usq += h3 * (S)0.5 * (k.un * A2 + s * k.B2);
+  kb.e2 += (S)0.5 * s * A2b;
 }
 
 // reverse of flux_quad (quadrant_fluxes.F90:12-248)
-__host__ __device__ inline void flux_quad_rev(const KinF& k, bool a1p, bool a2p, const double Gb[4],
-                                              KinB& kb) {
-  const double s1 = a1p ? 1.0 : -1.0, s2 = a2p ? 1.0 : -1.0;
-  const double A1 = 0.5 * (1.0 + s1 * k.e1), A2 = 0.5 * (1.0 + s2 * k.e2);
-  const double a7 = 7.0 * k.prr + k.usq, a6 = 6.0 * k.prr + k.usq;
-  const double t1b = k.ut * A1 + s1 * k.B1;
-  const double P = (k.prr + k.ut * k.ut) * A1 + s1 * k.ut * k.B1;
-  const double Q = k.un * A2 + s2 * k.B2;
-  const double temp2 = 0.5 * k.ut * a7 * A1, temp3 = 0.5 * k.B1 * a6;
-  const double R = temp2 + s1 * temp3;
-  double rhob = Gb[0] * A2 * t1b + Gb[1] * A2 * P + Gb[2] * t1b * Q + Gb[3] * A2 * R;
-  double A2b = Gb[0] * k.rho * t1b + Gb[1] * k.rho * P + Gb[3] * k.rho * R;
-  double t1bb = Gb[0] * k.rho * A2 + Gb[2] * k.rho * Q;
-  const double Pb = Gb[1] * k.rho * A2;
-  const double Qb = Gb[2] * k.rho * t1b;
-  const double Rb = Gb[3] * k.rho * A2;
+template <class S>
+__host__ __device__ inline void flux_quad_rev(const KinFT<S>& k, bool a1p, bool a2p, const S Gb[4],
+                                              KinBT<S>& kb) {
+  const S s1 = a1p ? (S)1 : (S)-1, s2 = a2p ? (S)1 : (S)-1;
+  const S A1 = (S)0.5 * ((S)1 + s1 * k.e1), A2 = (S)0.5 * ((S)1 + s2 * k.e2);
+  const S a7 = (S)7 * k.prr + k.usq, a6 = (S)6 * k.prr + k.usq;
+  const S t1b = k.ut * A1 + s1 * k.B1;
+  const S P = (k.prr + k.ut * k.ut) * A1 + s1 * k.ut * k.B1;
+  const S Q = k.un * A2 + s2 * k.B2;
+  const S temp2 = (S)0.5 * k.ut * a7 * A1, temp3 = (S)0.5 * k.B1 * a6;
+  const S R = temp2 + s1 * temp3;
+  S rhob = Gb[0] * A2 * t1b + Gb[1] * A2 * P + Gb[2] * t1b * Q + Gb[3] * A2 * R;
+  S A2b = Gb[0] * k.rho * t1b + Gb[1] * k.rho * P + Gb[3] * k.rho * R;
+  S t1bb = Gb[0] * k.rho * A2 + Gb[2] * k.rho * Q;
+  const S Pb = Gb[1] * k.rho * A2;
+  const S Qb = Gb[2] * k.rho * t1b;
+  const S Rb = Gb[3] * k.rho * A2;
   // temp4 = 0.5 rho un B2 t1b enters G3 with sign s2
-  const double t4b = s2 * Gb[3];
-  rhob += t4b * 0.5 * k.un * k.B2 * t1b;
-  double unb = t4b * 0.5 * k.rho * k.B2 * t1b;
-  double B2b = t4b * 0.5 * k.rho * k.un * t1b;
-  t1bb += t4b * 0.5 * k.rho * k.un * k.B2;
-  double utb = 0.0, prrb = 0.0, usqb = 0.0, A1b = 0.0, B1b = 0.0;
+  const S t4b = s2 * Gb[3];
+  rhob += t4b * (S)0.5 * k.un * k.B2 * t1b;
+  S unb = t4b * (S)0.5 * k.rho * k.B2 * t1b;
+  S B2b = t4b * (S)0.5 * k.rho * k.un * t1b;
+  t1bb += t4b * (S)0.5 * k.rho * k.un * k.B2;
+  S utb = (S)0, prrb = (S)0, usqb = (S)0, A1b = (S)0, B1b = (S)0;
   // R = temp2 + s1 temp3
-  utb += Rb * 0.5 * a7 * A1;
-  prrb += Rb * 3.5 * k.ut * A1;
-  usqb += Rb * 0.5 * k.ut * A1;
-  A1b += Rb * 0.5 * k.ut * a7;
-  const double t3b = s1 * Rb;
-  B1b += t3b * 0.5 * a6;
-  prrb += t3b * 3.0 * k.B1;
-  usqb += t3b * 0.5 * k.B1;
+  utb += Rb * (S)0.5 * a7 * A1;
+  prrb += Rb * (S)3.5 * k.ut * A1;
+  usqb += Rb * (S)0.5 * k.ut * A1;
+  A1b += Rb * (S)0.5 * k.ut * a7;
+  const S t3b = s1 * Rb;
+  B1b += t3b * (S)0.5 * a6;
+  prrb += t3b * (S)3 * k.B1;
+  usqb += t3b * (S)0.5 * k.B1;
   // Q = un A2 + s2 B2
   unb += Qb * A2;
   A2b += Qb * k.un;
   B2b += Qb * s2;
   // P = (prr + ut^2) A1 + s1 ut B1
   prrb += Pb * A1;
-  utb += Pb * (2.0 * k.ut * A1 + s1 * k.B1);
+  utb += Pb * ((S)2 * k.ut * A1 + s1 * k.B1);
   A1b += Pb * (k.prr + k.ut * k.ut);
   B1b += Pb * s1 * k.ut;
   // t1b = ut A1 + s1 B1
@@ -177,88 +221,96 @@ __host__ __device__ inline void flux_quad_rev(const KinF& k, bool a1p, bool a2p,
   kb.usq += usqb;
   kb.B1 += B1b;
   kb.B2 += B2b;
-  kb.e1 += 0.5 * s1 * A1b;
-  kb.e2 += 0.5 * s2 * A2b;
+  kb.e1 += (S)0.5 * s1 * A1b;
+  kb.e2 += (S)0.5 * s2 * A2b;
 }
 
-// reverse of kin_fwd: adds d/d(qt) into qtb and d/d(nx, ny) into nxb, nyb
-__host__ __device__ inline void kin_rev(const KinF& k, const KinB& kb, double nx, double ny,
-                                        double qtb[4], double& nxb, double& nyb) {
+// reverse of kin_fwd: adds d/d(qt) into qtb and d/d(nx, ny) into nxb, nyb (FP64 sums)
+template <class S>
+__host__ __device__ inline void kin_rev(const KinFT<S>& k, const KinBT<S>& kb, S nx, S ny, S qtb[4], double& nxb,
+                                        double& nyb) {
   // the derivatives' divisions as products with three reciprocals: 1/rho and 1/sb
   // divided, 1/p = 2 beta / rho (p = rho tmp, tmp = 1/(2 beta)), 1/spb = (1/sb)/sqrt(pi)
   // and 1/beta = 2 tmp; the forward values above keep the primal's own operations
-  const double irho = 1.0 / k.rho, isb = 1.0 / k.sb;
-  const double ip = 2.0 * k.beta * irho, ispb = INV_SQRT_PI * isb, ibeta = 2.0 * k.tmp;
-  double utb = kb.ut, unb = kb.un, rhob = kb.rho;
+  const S irho = rv_rcp(k.rho), isb = rv_rcp(k.sb);
+  const S ip = (S)2 * k.beta * irho, ispb = (S)INV_SQRT_PI * isb, ibeta = (S)2 * k.tmp;
+  S utb = kb.ut, unb = kb.un, rhob = kb.rho;
   // usq = ut^2 + un^2
-  utb += 2.0 * k.ut * kb.usq;
-  unb += 2.0 * k.un * kb.usq;
+  utb += (S)2 * k.ut * kb.usq;
+  unb += (S)2 * k.un * kb.usq;
   // prr = p / rho
-  double pb = kb.prr * irho;
+  S pb = kb.prr * irho;
   rhob -= kb.prr * k.prr * irho;
   // e = erf(S), B = 0.5 exp(-S^2) / spb
-  double S1b = kb.e1 * TWO_BY_SQRT_PI * k.ex1 - 2.0 * k.S1 * k.B1 * kb.B1;
-  double S2b = kb.e2 * TWO_BY_SQRT_PI * k.ex2 - 2.0 * k.S2 * k.B2 * kb.B2;
-  const double spbb = -(kb.B1 * k.B1 + kb.B2 * k.B2) * ispb;
+  S S1b = kb.e1 * (S)TWO_BY_SQRT_PI * k.ex1 - (S)2 * k.S1 * k.B1 * kb.B1;
+  S S2b = kb.e2 * (S)TWO_BY_SQRT_PI * k.ex2 - (S)2 * k.S2 * k.B2 * kb.B2;
+  const S spbb = -(kb.B1 * k.B1 + kb.B2 * k.B2) * ispb;
   // S1 = ut sb, S2 = un sb
   utb += S1b * k.sb;
   unb += S2b * k.sb;
-  const double sbb = S1b * k.ut + S2b * k.un;
+  const S sbb = S1b * k.ut + S2b * k.un;
   // sb = sqrt(b), spb = sqrt(pi b)
-  const double bb = 0.5 * sbb * isb + 0.5 * Consts<double>::PI * spbb * ispb;
+  const S bb = (S)0.5 * sbb * isb + (S)0.5 * (S)Consts<double>::PI * spbb * ispb;
   // b = 0.5 rho / p
-  rhob += 0.5 * bb * ip;
+  rhob += (S)0.5 * bb * ip;
   pb -= bb * k.b * ip;
   // ut = u1 ny - u2 nx, un = u1 nx + u2 ny
-  double u1b = utb * ny + unb * nx;
-  double u2b = -utb * nx + unb * ny;
-  nxb += -utb * k.u2 + unb * k.u1;
-  nyb += utb * k.u1 + unb * k.u2;
+  S u1b = utb * ny + unb * nx;
+  S u2b = -utb * nx + unb * ny;
+  nxb += (double)(-utb * k.u2 + unb * k.u1);
+  nyb += (double)(utb * k.u1 + unb * k.u2);
   // p = rho tmp
   rhob += pb * k.tmp;
-  double tmpb = pb * k.rho;
+  S tmpb = pb * k.rho;
   // rho = exp(T2), T2 = T1 - log(beta)/(gamma-1)
-  const double T2b = rhob * k.rho;
-  const double T1b = T2b;
-  double betab = -T2b * ibeta * INV_GAMMA_M1;
+  const S T2b = rhob * k.rho;
+  const S T1b = T2b;
+  S betab = -T2b * ibeta * (S)INV_GAMMA_M1;
   // T1 = qt1 + beta (u1^2 + u2^2)
   qtb[0] += T1b;
   betab += T1b * (k.u1 * k.u1 + k.u2 * k.u2);
-  u1b += T1b * 2.0 * k.beta * k.u1;
-  u2b += T1b * 2.0 * k.beta * k.u2;
+  u1b += T1b * (S)2 * k.beta * k.u1;
+  u2b += T1b * (S)2 * k.beta * k.u2;
   // u1 = qt2 tmp, u2 = qt3 tmp
   qtb[1] += u1b * k.tmp;
   qtb[2] += u2b * k.tmp;
   tmpb += u1b * k.qt[1] + u2b * k.qt[2];
   // tmp = 0.5 / beta, beta = -0.5 qt4
   betab -= tmpb * k.tmp * ibeta;
-  qtb[3] += -0.5 * betab;
+  qtb[3] += (S)-0.5 * betab;
 }
 
-// reverse of venkat (limiters.F90:64-110), one component
-__host__ __device__ inline void venkat_rev(double qt0, double q, double qmax, double qmin,
-                                           double epsi, double phib, double& qt0b, double& qb,
+// reverse of venkat (limiters.F90:64-110), one component; the adjoints of the point
+// record accumulate in FP64
+template <class S>
+__host__ __device__ inline void venkat_rev(S qt0, S q, S qmax, S qmin, S epsi, S phib, S& qt0b, double& qb,
                                            double& qmaxb, double& qminb) {
-  const double dn = qt0 - q;
-  if (fabs(dn) <= Consts<double>::THRESH) return;
-  const bool use_max = dn > 0.0;
-  const double dp = (use_max ? qmax : qmin) - q;
-  const double e2 = epsi * epsi;
-  const double num = (dp * dp + e2) * dn + 2.0 * dn * dn * dp;
-  const double D = dp * dp + 2.0 * dn * dn + dn * dp + e2;
-  const double den = D * dn;
-  const double t = num / den;  // the primal's value, so the branch is the primal's
-  if (!(t < 1.0)) return;
-  const double iden = 1.0 / den;
-  const double numb = phib * iden;
-  const double denb = -phib * t * iden;
-  const double dnb = numb * ((dp * dp + e2) + 4.0 * dn * dp) + denb * (D + dn * (4.0 * dn + dp));
-  const double dpb = numb * (2.0 * dp * dn + 2.0 * dn * dn) + denb * (dn * (2.0 * dp + dn));
+  const S dn = qt0 - q;
+  if (m_abs(dn) <= Consts<S>::THRESH) return;
+  const bool use_max = dn > (S)0;
+  const S dp = (use_max ? qmax : qmin) - q;
+  const S e2 = epsi * epsi;
+  const S num = (dp * dp + e2) * dn + (S)2 * dn * dn * dp;
+  const S D = dp * dp + (S)2 * dn * dn + dn * dp + e2;
+  const S den = D * dn;
+  S t, iden;
+  if constexpr (sizeof(S) == 4) {
+    iden = frcp(den);
+    t = num * iden;
+  } else {
+    t = num / den;  // the primal's value, so the branch is the primal's
+    iden = 1.0 / den;
+  }
+  if (!(t < (S)1)) return;
+  const S numb = phib * iden;
+  const S denb = -phib * t * iden;
+  const S dnb = numb * ((dp * dp + e2) + (S)4 * dn * dp) + denb * (D + dn * ((S)4 * dn + dp));
+  const S dpb = numb * ((S)2 * dp * dn + (S)2 * dn * dn) + denb * (dn * ((S)2 * dp + dn));
   qt0b += dnb;
-  qb -= dnb;
-  if (use_max) qmaxb += dpb;
-  else qminb += dpb;
-  qb -= dpb;
+  qb -= (double)dnb;
+  if (use_max) qmaxb += (double)dpb;
+  else qminb += (double)dpb;
+  qb -= (double)dpb;
 }
 
 // Adjoints of one end's point record
@@ -281,25 +333,26 @@ __host__ __device__ inline void endd_zero(EndD& e) {
   for (int r = 0; r < 4; ++r) e.q[r] = e.d[r] = e.qmax[r] = e.qmin[r] = 0.0;
 }
 
-// reverse of reconstruct, keeping d's adjoint (see EndD); adds delx, dely adjoints
-__host__ __device__ inline void reconstruct_rev_d(const V4d& q, const V4d& dx, const V4d& dy,
-                                                  const V4d& qmax, const V4d& qmin, double eps,
-                                                  double delx, double dely, const double qtb[4],
-                                                  EndD& eb, double& delxb, double& delyb) {
+// reverse of reconstruct, keeping d's adjoint (see EndD); adds delx, dely adjoints.
+// The reconstruction's arithmetic in S, its adjoints accumulated in FP64.
+template <class S>
+__host__ __device__ inline void reconstruct_rev_d(const V4<S>& q, const V4<S>& dx, const V4<S>& dy,
+                                                  const V4<S>& qmax, const V4<S>& qmin, S eps, S delx, S dely,
+                                                  const S qtb[4], EndD& eb, double& delxb, double& delyb) {
   for (int r = 0; r < 4; ++r) {
-    const double d = delx * dx[r] + dely * dy[r];
-    const double qt0 = q[r] - 0.5 * d;
-    const double phi = venkat(qt0, q[r], qmax[r], qmin[r], eps);
-    eb.q[r] += qtb[r];
-    const double phib = -0.5 * d * qtb[r];
-    double db = -0.5 * phi * qtb[r];
-    double qt0b = 0.0;
-    venkat_rev(qt0, q[r], qmax[r], qmin[r], eps, phib, qt0b, eb.q[r], eb.qmax[r], eb.qmin[r]);
-    eb.q[r] += qt0b;
-    db += -0.5 * qt0b;
-    eb.d[r] += db;
-    delxb += db * dx[r];
-    delyb += db * dy[r];
+    const S d = delx * dx[r] + dely * dy[r];
+    const S qt0 = q[r] - (S)0.5 * d;
+    const S phi = venkat(qt0, q[r], qmax[r], qmin[r], eps);
+    eb.q[r] += (double)qtb[r];
+    const S phib = (S)-0.5 * d * qtb[r];
+    S db = (S)-0.5 * phi * qtb[r];
+    S qt0b = (S)0;
+    venkat_rev<S>(qt0, q[r], qmax[r], qmin[r], eps, phib, qt0b, eb.q[r], eb.qmax[r], eb.qmin[r]);
+    eb.q[r] += (double)qt0b;
+    db += (S)-0.5 * qt0b;
+    eb.d[r] += (double)db;
+    delxb += (double)(db * dx[r]);
+    delyb += (double)(db * dy[r]);
   }
 }
 
@@ -321,14 +374,14 @@ __host__ __device__ inline void reconstruct_rev(const V4d& q, const V4d& dx, con
 }
 
 // One end's split fluxes for an edge (as end_fluxes<CLS>) from its KinF
-template <int CLS>
-__host__ __device__ inline void end_fluxes_f(const KinF& f, bool xp, bool yp, double Gx[4], double Gy[4]) {
-  end_fluxes<CLS, double>(kin_of(f), xp, yp, Gx, Gy);
+template <int CLS, class S>
+__host__ __device__ inline void end_fluxes_f(const KinFT<S>& f, bool xp, bool yp, S Gx[4], S Gy[4]) {
+  end_fluxes<CLS, S>(kin_of(f), xp, yp, Gx, Gy);
 }
 
-template <int CLS>
-__host__ __device__ inline void end_fluxes_rev(const KinF& f, bool xp, bool yp, const double Gxb[4],
-                                               const double Gyb[4], KinB& kb) {
+template <int CLS, class S>
+__host__ __device__ inline void end_fluxes_rev(const KinFT<S>& f, bool xp, bool yp, const S Gxb[4],
+                                               const S Gyb[4], KinBT<S>& kb) {
   if (CLS == INTERIOR) {
     flux_gx_rev(f, xp, Gxb, kb);
     flux_gy_rev(f, yp, Gyb, kb);

@@ -607,6 +607,7 @@ __device__ inline float coef_y(const HalfCoefF& h, bool first, float dsw, float 
 // per slot of the lane, m the slot's half-stencil bits).
 template <class S>
 struct GlobalNbrs {
+  using scalar = S;
   const Geom& g;
   const State<S>& st;
   int i, nb, lane, lpp;
@@ -648,6 +649,7 @@ struct HaloRecs {
 
 template <class S, int P, int LPP>
 struct StagedNbrs {
+  using scalar = S;
   const Geom& g;
   const HaloRecs<S>& r;
   const uint16_t* s_cl;

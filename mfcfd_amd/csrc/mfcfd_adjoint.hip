@@ -85,10 +85,9 @@ __device__ inline void lsq_weight(double a, double b, double power, double& w, d
 
 // q-tilde of one end of an edge (the point's own q when first order); p indexes the
 // records of nbrs (a point id, or a halo position)
-template <class Nbrs>
-__device__ inline void end_qtilde(const Nbrs& nbrs, int p, double delx, double dely, int first_order,
-                                  double qt[4]) {
-  const V4d q = nbrs.q(p);
+template <class Nbrs, class S = typename Nbrs::scalar>
+__device__ inline void end_qtilde(const Nbrs& nbrs, int p, S delx, S dely, int first_order, S qt[4]) {
+  const V4<S> q = nbrs.q(p);
   if (first_order) {
     for (int r = 0; r < 4; ++r) qt[r] = q[r];
   } else {
@@ -105,6 +104,9 @@ __device__ inline void end_qtilde(const Nbrs& nbrs, int p, double delx, double d
 template <int CLS, int LPP, class Nbrs>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nbrs, int i, double delta,
                                const V4d& frbv, const AdjOut& o, int lane, double power) {
+  // the records' precision: the arithmetic of the ends' reconstruction, kinetic state
+  // and fluxes and of their reverse; coefficients and adjoint sums stay FP64
+  using S = typename Nbrs::scalar;
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -157,23 +159,23 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
       const int p = e ? k : nbrs.own();
-      double qt[4];
-      end_qtilde(nbrs, p, delx, dely, g.first_order, qt);
-      KinF f;
-      kin_fwd(qt, nx, ny, f);
+      S qt[4];
+      end_qtilde(nbrs, p, (S)delx, (S)dely, g.first_order, qt);
+      KinFT<S> f;
+      kin_fwd<S>(qt, (S)nx, (S)ny, f);
       const double sg = e ? 1.0 : -1.0;
-      KinB kb;
+      KinBT<S> kb;
       kinb_zero(kb);
       {
-        double Gx[4], Gy[4];
+        S Gx[4], Gy[4];
         end_fluxes_f<CLS>(f, xp, yp, Gx, Gy);
         double ax = 0.0, ay = 0.0, fd = 0.0;
         for (int r = 0; r < 4; ++r) {
-          ax += accb[r] * Gx[r];
-          ay += accb[r] * Gy[r];
-          fd += frbv[r] * (Gx[r] * cx + Gy[r] * cy);
-          Gx[r] = sg * accb[r] * cx;
-          Gy[r] = sg * accb[r] * cy;
+          ax += accb[r] * (double)Gx[r];
+          ay += accb[r] * (double)Gy[r];
+          fd += frbv[r] * ((double)Gx[r] * cx + (double)Gy[r] * cy);
+          Gx[r] = (S)(sg * accb[r] * cx);
+          Gy[r] = (S)(sg * accb[r] * cy);
         }
         cxb += sg * ax;
         cyb += sg * ay;
@@ -181,30 +183,30 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
         end_fluxes_rev<CLS>(f, xp, yp, Gx, Gy, kb);
       }
       if (tie) {
-        double Hx[4], Hy[4];
+        S Hx[4], Hy[4];
         end_fluxes_f<CLS>(f, false, false, Hx, Hy);
         double hx = 0.0, hy = 0.0, fd = 0.0;
         for (int r = 0; r < 4; ++r) {
-          hx += accb[r] * Hx[r];
-          hy += accb[r] * Hy[r];
-          fd += frbv[r] * (Hx[r] * cx2 + Hy[r] * cy2);
-          Hx[r] = sg * accb[r] * cx2;
-          Hy[r] = sg * accb[r] * cy2;
+          hx += accb[r] * (double)Hx[r];
+          hy += accb[r] * (double)Hy[r];
+          fd += frbv[r] * ((double)Hx[r] * cx2 + (double)Hy[r] * cy2);
+          Hx[r] = (S)(sg * accb[r] * cx2);
+          Hy[r] = (S)(sg * accb[r] * cy2);
         }
         cx2b += sg * hx;
         cy2b += sg * hy;
         dsum += sg * fd;
         end_fluxes_rev<CLS>(f, false, false, Hx, Hy, kb);
       }
-      double qtb[4] = {0, 0, 0, 0};
-      kin_rev(f, kb, nx, ny, qtb, nxb, nyb);
+      S qtb[4] = {(S)0, (S)0, (S)0, (S)0};
+      kin_rev<S>(f, kb, (S)nx, (S)ny, qtb, nxb, nyb);
       EndD d;
       endd_zero(d);
       if (g.first_order) {
-        for (int r = 0; r < 4; ++r) d.q[r] = qtb[r];
+        for (int r = 0; r < 4; ++r) d.q[r] = (double)qtb[r];
       } else {
-        reconstruct_rev_d(nbrs.q(p), nbrs.dqx(p), nbrs.dqy(p), nbrs.qmax(p), nbrs.qmin(p), nbrs.eps(p),
-                          delx, dely, qtb, d, delxb, delyb);
+        reconstruct_rev_d<S>(nbrs.q(p), nbrs.dqx(p), nbrs.dqy(p), nbrs.qmax(p), nbrs.qmin(p), nbrs.eps(p),
+                             (S)delx, (S)dely, qtb, d, delxb, delyb);
       }
       if (e) {
         ep[0] = D2{d.q[0], d.q[1]};
@@ -335,27 +337,29 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
   o.deltab[i] += (CLS == WALL) ? 2.0 * dsum : dsum;  // delta enters every stage
 }
 
-template <int LPP, bool STAGED>
-__global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<double> st,
+// S: the taped records' precision (double; float for a mixed-precision context, whose
+// records are the mixed primal's own FP32 ones)
+template <int LPP, bool STAGED, class S>
+__global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<S> st,
                                                        const double* __restrict__ delta,
                                                        const V4d* __restrict__ frb, AdjOut o, double power) {
   if constexpr (STAGED) {
     // the block's halo of the taped stage state in LDS (blocks of STAGE_PF points)
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    HaloRecs<double> r;
+    HaloRecs<S> r;
     uint16_t* s_cl;
-    const bool staged = stage_flux_halo<double, LPP>(g, st, b, smem, r, s_cl);
+    const bool staged = stage_flux_halo<S, LPP>(g, st, b, smem, r, s_cl);
     const int li = threadIdx.x / LPP, lane = threadIdx.x % LPP;
     const int i = b * STAGE_PF + li;
     if (i >= g.n) return;  // whole lane groups
     if (staged) {
-      const StagedNbrs<double, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
+      const StagedNbrs<S, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
       if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
       else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
       else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     } else {
-      const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
+      const GlobalNbrs<S> nb{g, st, i, g.nbhs[i], lane, LPP};
       if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
       else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
       else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, Stat
     const int t = point_id();
     const int i = t / LPP, lane = t % LPP;
     if (i >= g.n) return;  // whole lane groups
-    const GlobalNbrs<double> nb{g, st, i, g.nbhs[i], lane, LPP};
+    const GlobalNbrs<S> nb{g, st, i, g.nbhs[i], lane, LPP};
     if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
@@ -1076,16 +1080,16 @@ static int adj_record(mf_ctx* c, AdjState& A) {
 }
 
 // launchers of the reverse kernels at a given lanes-per-point shape
-static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lpp) {
+template <class S>
+static void launch_flux_rev_s(mf_ctx* c, AdjState& A, const State<S>& st, V4d* dqb_last, int lpp) {
   const HalfSums hs{A.hxx, A.hyy, A.hxy};
-  const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], c->eps64};
   const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
   if (lpp & LPP_STAGED) {
     // staged: blocks of STAGE_PF points, 2, 4 or 8 lanes each
     const dim3 gr((unsigned)((c->n + STAGE_PF - 1) / STAGE_PF));
-    const size_t lds = stage_lds_flux(c->g.hl[1].hcap, c->g.width, sizeof(double));
+    const size_t lds = stage_lds_flux(c->g.hl[1].hcap, c->g.width, sizeof(S));
     auto go = [&]<int L>() {
-      hipLaunchKernelGGL((k_flux_rev<L, true>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
+      hipLaunchKernelGGL((k_flux_rev<L, true, S>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
                          (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
     };
     switch (lpp & ~LPP_STAGED) {
@@ -1096,9 +1100,21 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
     return;
   }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL((k_flux_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
+    hipLaunchKernelGGL((k_flux_rev<L, false, S>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
                        (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
   });
+}
+
+// the flux reverse of stage s: on a mixed-precision context in FP32 from the taped
+// iteration's FP32 records (as the mixed primal evaluates the fluxes), else in FP64
+static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lpp) {
+  if (A.mixed) {
+    const State<float> st{A.fq[s], A.fdq[s * (A.I + 1) + A.I], A.fqm[s], (const float*)c->eps};
+    launch_flux_rev_s<float>(c, A, st, dqb_last, lpp);
+  } else {
+    const State<double> st{A.tq[s], A.tdq[s * (A.I + 1) + A.I], A.tqm[s], c->eps64};
+    launch_flux_rev_s<double>(c, A, st, dqb_last, lpp);
+  }
 }
 
 static void launch_gather_flux(mf_ctx* c, AdjState& A, V4d* dqb_last, int lpp) {

@@ -232,8 +232,9 @@ def test_adjoint_staged_reverse_overflow(c1_cloud, kernel):
 @pytest.mark.parametrize("snaps", [4, -1])
 def test_mixed_adjoint_close_to_fp64(small_cloud, snaps):
     """Mixed-precision adjoint: the primal sweep and the taped re-runs run the mixed
-    path's FP32 kernels, their records widened into the FP64 tape, and the reverse
-    kernels and every adjoint sum stay FP64.  Against the FP64 adjoint over the same
+    path's FP32 kernels, their records widened into the FP64 tape; the flux reverse runs
+    in FP32 on the FP32 records (as the mixed primal evaluates the fluxes), the other
+    reverse kernels in FP64, and every adjoint sum is FP64.  Against the FP64 adjoint over the same
     20 iterations: J within 1e-5 relative, the gradient within 1e-3 of its largest
     entry, and a directional derivative within 1e-3.  The binomial (4 snapshots) and
     tape-everything schedules give the same J bits (the re-runs repeat the primal's
