@@ -223,8 +223,10 @@ int mf_layout(mf_ctx *ctx, mf_layout_info *info);
  * context is left holding the final primal state.  Any power (the LSQ weights'
  * derivatives included, q_variables_b.F90).  On an
  * MF_MIXED context the primal sweep and the taped re-runs run the mixed path's FP32
- * kernels; their records are widened to FP64 and the reverse sweep and every adjoint
- * sum are FP64 (the derivative of the FP64 iteration along the mixed trajectory).
+ * kernels; the flux reverse runs in FP32 on their records, as the mixed primal's
+ * fluxes do, the other reverse kernels in FP64 on the records widened to FP64, and
+ * every adjoint sum is FP64 (the derivative of the iteration along the mixed
+ * trajectory).
  * snaps = MF_ADJ_TAPE_ALL tapes every iteration's stage states during the primal
  * sweep instead (about 1.3 kB per point per iteration at 2 inner iterations; refused
  * when that exceeds the free device memory), so the reverse sweep re-runs nothing. */
