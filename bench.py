@@ -74,8 +74,23 @@ def _timed(s, iters: int, first_it: int) -> float:
     return time.perf_counter() - t0
 
 
-def run_headline(cloud, params: dict, precision: str, warmup: int, steps: int, wl: dict,
-                 keep_prim: bool = False):
+def state_errors(prim: np.ndarray, prim_ref: np.ndarray, cp: np.ndarray, cp_ref: np.ndarray) -> dict:
+    """The mixed path's state against the FP64 path's at the same iteration: per
+    primitive component (rho, u1, u2, p) the largest and the root-mean-square point
+    difference, each over that component's largest magnitude; cp's largest difference
+    over the wall points."""
+    names = ("rho", "u1", "u2", "p")
+    d = np.abs(prim - prim_ref)
+    scale = np.abs(prim_ref).max(axis=0)
+    out = {"prim_max": {n: float(d[:, r].max() / scale[r]) for r, n in enumerate(names)},
+           "prim_rms": {n: float(np.sqrt(np.mean(d[:, r] ** 2)) / scale[r]) for r, n in enumerate(names)}}
+    out["prim_max_all"] = max(out["prim_max"].values())
+    out["cp_max_abs"] = float(np.abs(cp - cp_ref).max()) if len(cp) else None
+    out["cp_range"] = float(cp_ref.max() - cp_ref.min()) if len(cp) else None
+    return out
+
+
+def run_headline(cloud, params: dict, precision: str, warmup: int, steps: int, wl: dict):
     """W warm-up, autotune, K timed iterations (the headline), the kernels' steady
     launch times, then the workload's 100 + 500 schedule; Cl, Cd at the end."""
     from mfcfd_amd.solver import LskumSolver
@@ -96,10 +111,10 @@ def run_headline(cloud, params: dict, precision: str, warmup: int, steps: int, w
     done += wl["steps"]
     res, _, _ = s.q_lskum(1, first_it=done + 1)
     done += 1
-    cl, cdv, _, _ = s.compute_cl_cd_cm()
-    final_prim = s.prim if keep_prim else None
+    cl, cdv, _, cp = s.compute_cl_cd_cm()
+    final_prim = s.prim
     prof = s.profile_kernels(3)
-    out = dict(final_prim=final_prim, ms_per_step=1e3 * dt / steps, point_iters_per_s=cloud.n * steps / dt,
+    out = dict(final_prim=final_prim, cp=cp, ms_per_step=1e3 * dt / steps, point_iters_per_s=cloud.n * steps / dt,
                schedule={"iterations": f"{wl['warmup']} untimed + {wl['steps']} timed",
                          "ms_per_step": 1e3 * dt_sched / wl["steps"],
                          "point_iters_per_s": cloud.n * wl["steps"] / dt_sched},
@@ -119,12 +134,14 @@ def run_c2():
     cloud, p, n_it = W.cloud_of(W.C2), W.C2["params"], W.C2["iterations"]
     out = {"workload": W.DESCRIPTION["c2"] + f", {n_it} iterations from freestream",
            "points": cloud.n, "mean_nbhs": round(float(cloud.nbhs.mean()), 3)}
+    state = {}
     for prec in ("fp64", "mixed"):
         with LskumSolver(cloud, precision=prec, **p) as s:
             tune = s.autotune()
             dt = _timed(s, n_it - 1, 1)
             res, _, _ = s.q_lskum(1, first_it=n_it)
-            cl, cd, _, _ = s.compute_cl_cd_cm()
+            cl, cd, _, cp = s.compute_cl_cd_cm()
+            state[prec] = (s.prim, cp)
             steady = s.time_kernels(20)
         out[prec] = {"ms_per_step": 1e3 * dt / (n_it - 1), "point_iters_per_s": cloud.n * (n_it - 1) / dt,
                      "cl": float(cl[0]), "cd": float(cd[0]), "final_residue": float(res[-1]),
@@ -133,6 +150,8 @@ def run_c2():
     out["mixed_over_fp64"] = out["mixed"]["point_iters_per_s"] / out["fp64"]["point_iters_per_s"]
     out["dCl"] = abs(out["mixed"]["cl"] - out["fp64"]["cl"])
     out["dCd"] = abs(out["mixed"]["cd"] - out["fp64"]["cd"])
+    out["mixed_state_vs_fp64"] = state_errors(state["mixed"][0], state["fp64"][0], state["mixed"][1],
+                                              state["fp64"][1])
     return out
 
 
@@ -559,7 +578,7 @@ def main():
     wl = W.C4
     cloud, params = W.cloud_of(wl, verbose=False), wl["params"]
     mean_nbhs = float(cloud.nbhs.mean())
-    fp64 = run_headline(cloud, params, "fp64", args.warmup, args.steps, wl, keep_prim=True)
+    fp64 = run_headline(cloud, params, "fp64", args.warmup, args.steps, wl)
     mixed = run_headline(cloud, params, "mixed", args.warmup, args.steps, wl)
     dcl = abs(mixed["cl"] - fp64["cl"])
     dcd = abs(mixed["cd"] - fp64["cd"])
@@ -609,7 +628,8 @@ def main():
         "mixed_point_iters_per_s": mixed["point_iters_per_s"],
         "mixed_over_fp64": mixed["point_iters_per_s"] / fp64["point_iters_per_s"],
         "mixed_vs_fp64": {"iterations": mixed["iterations_at_gate"], "dCl": dcl, "dCd": dcd,
-                          "cl_fp64": fp64["cl"], "cd_fp64": fp64["cd"]},
+                          "cl_fp64": fp64["cl"], "cd_fp64": fp64["cd"],
+                          **state_errors(mixed["final_prim"], fp64["final_prim"], mixed["cp"], fp64["cp"])},
         "schedule_100_500": {"fp64": fp64["schedule"], "mixed": mixed["schedule"]},
         "kernel_ms": {"fp64": fp64["kernel_ms"], "mixed": mixed["kernel_ms"]},
         "steady_us": {"fp64": fp64["steady_us"], "mixed": mixed["steady_us"]},

@@ -34,6 +34,8 @@ MF_NOALLOC_MAX_SHAPES = 8
 # resident inner sweeps (MF_LPP_RESIDENT | 1, 2 or 4, mixed contexts)
 MF_LPP_STAGED = 16
 MF_LPP_RESIDENT = 32
+# the adjoint's inner reverse in its pull form (MF_LPP_PULL | 1, 2 or 4)
+MF_LPP_PULL = 64
 KERNEL_KINDS = ("lsq", "inner", "flux_update", "finalize", "forces")
 
 

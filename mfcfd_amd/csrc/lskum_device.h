@@ -37,9 +37,11 @@ constexpr int BLOCK = 256;
 // Staged stencil walks: blocks of consecutive points (STAGE_P for the LSQ and inner
 // kernels, STAGE_PF for the flux kernel, whose neighbour records are larger); a
 // lanes-per-point value with the LPP_STAGED bit set selects a kernel's staged form
+// (LPP_PULL: the adjoint inner reverse's pull form, also staged)
 constexpr int STAGE_P = 64;
 constexpr int STAGE_PF = 32;
 constexpr int LPP_STAGED = 16;
+constexpr int LPP_PULL = 64;
 
 // One staging layout (blocks of P points).  Block b's halo is ids[ptr[b] .. ptr[b+1]):
 // its own points first, then every other neighbour of its points once.  loc [W][stride]

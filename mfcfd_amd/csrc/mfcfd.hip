@@ -1225,6 +1225,13 @@ static bool valid_rev_staged(const mf_ctx* c, int kind, int lpp) {
   return lds <= kMaxStageLds;
 }
 
+// the pull form of the adjoint's inner reverse: MF_LPP_PULL | 1, 2 or 4 lanes (its halo
+// is built with the adjoint's buffers and sized to fit LDS)
+static bool valid_pull(int lpp) {
+  const int l = lpp & ~LPP_PULL;
+  return (lpp & LPP_PULL) && (l == 1 || l == 2 || l == 4);
+}
+
 // mean time (us) of `reps` launches of fn on the context's stream, after 2 untimed
 template <class F>
 static int time_launches(mf_ctx* c, int reps, F&& fn, double* us) {
@@ -1420,10 +1427,10 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
     if (rv[k] != 0 && !valid_lpp(rv[k]) && !(k == 0 && valid_shape(c, 2, rv[k])) &&
-        !((k == 1 || k == 2) && valid_rev_staged(c, k, rv[k])))
+        !((k == 1 || k == 2) && valid_rev_staged(c, k, rv[k])) && !(k == 2 && valid_pull(rv[k])))
       return set_err(MF_ERR_INVALID,
                      "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux, LSQ and inner reverses "
-                     "also their staged shapes)");
+                     "also their staged shapes, the inner reverse MF_LPP_PULL | 1, 2 or 4)");
   int rc;
   if ((t->lpp_inner & MF_LPP_RESIDENT) && (rc = ensure_sweep_bar(c))) return rc;
   c->lpp_lsq = t->lpp_lsq;

@@ -69,6 +69,20 @@ struct Trans {
   const int32_t* own;  // [E]
 };
 
+// Halo of the pull-form inner reverse in blocks of STAGE_P points: each block's own
+// points, then once each (ascending) every other point that is a neighbour of one of
+// them or names one of them as a neighbour.  loc8 [W][stride]: each slot's position in
+// its block's halo; tloc8 [E]: the position of each transposed entry's owner in the
+// halo of the block holding the entry's neighbour; 255 for a position past hcap.
+struct PullHalo {
+  const int32_t* ptr = nullptr;  // [nblk+1]
+  const int32_t* ids = nullptr;
+  const uint8_t* loc8 = nullptr;
+  const uint8_t* tloc8 = nullptr;
+  int32_t hcap = 0;    // entries held in LDS: at least STAGE_P, a multiple of 8, <= 248
+  int32_t stride = 0;  // N rounded up to a multiple of STAGE_P
+};
+
 // Least-squares weight w = dist^power of an edge with offset (a, b) (q_variables.F90:89-91,
 // interior_fluxes.F90:89-90) and f = power dist^(power - 2), so dw/da = f a and
 // dw/db = f b; power = 0 gives w = 1 and f = 0 exactly
@@ -409,6 +423,65 @@ __global__ __launch_bounds__(BLOCK) void k_update_rev(Geom g, const V4d* __restr
   nrmb[i].y += nyb;
 }
 
+// Reverse of one point's inner-iteration solve (q_variables.F90:197-220): from the
+// adjoints (bx, by) of its output dq = ((sum_x syy - sum_y sxy) obd, (sum_y sxx - sum_x
+// sxy) obd) and that output (tx, ty), the adjoints of the sums sum_x, sum_y per
+// component and of the geometry sums sxx, syy, sxy (also through obd = 1/det)
+struct InnerLam {
+  double sqb[4], syqb[4];
+  double sxxb, syyb, sxyb;
+};
+
+__device__ inline void inner_lam(const V4d& gg, const V4d& tx, const V4d& ty, const V4d& bx, const V4d& by,
+                                 InnerLam& L) {
+  const double sxx = gg.x, syy = gg.y, sxy = gg.z, obd = gg.w;
+  double sxxb = 0.0, syyb = 0.0, sxyb = 0.0, obdb = 0.0;
+  for (int r = 0; r < 4; ++r) {
+    const double sxq = sxx * tx[r] + sxy * ty[r];
+    const double syq = sxy * tx[r] + syy * ty[r];
+    L.sqb[r] = (bx[r] * syy - by[r] * sxy) * obd;
+    L.syqb[r] = (by[r] * sxx - bx[r] * sxy) * obd;
+    obdb += bx[r] * (sxq * syy - syq * sxy) + by[r] * (syq * sxx - sxq * sxy);
+    syyb += bx[r] * sxq * obd;
+    sxxb += by[r] * syq * obd;
+    sxyb += -(bx[r] * syq + by[r] * sxq) * obd;
+  }
+  const double detb = -obdb * obd * obd;
+  L.sxxb = sxxb + detb * syy;
+  L.syyb = syyb + detb * sxx;
+  L.sxyb = sxyb - 2.0 * detb * sxy;
+}
+
+// Reverse of one edge's terms of an inner iteration at its owner i, the neighbour k at
+// offset (dx, dy) = x_k - x_i (q_variables.F90:164-189): with the owner's solve adjoint
+// L, the adjoint Db of the edge's difference D = (q_k - dq_k.d/2) - (q_i - dq_i.d/2) per
+// component (the neighbour's q takes +Db, the owner's -Db; the dq records -/+ Db d/2)
+// and the adjoint (dxb, dyb) of the offset (the neighbour's coordinates take +, the
+// owner's -).  The owner (push form) and the neighbour (pull form) evaluate it alike.
+__device__ inline void inner_edge_rev(double dx, double dy, double power, const V4d& qi, const V4d& dxi,
+                                      const V4d& dyi, const V4d& qk, const V4d& dxk, const V4d& dyk,
+                                      const InnerLam& L, double Db[4], double& dxb, double& dyb) {
+  double w, wf;
+  lsq_weight(dx, dy, power, w, wf);
+  // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
+  dxb = w * (2.0 * L.sxxb * dx + L.sxyb * dy);
+  dyb = w * (2.0 * L.syyb * dy + L.sxyb * dx);
+  double wb = L.sxxb * dx * dx + L.syyb * dy * dy + L.sxyb * dx * dy;
+  for (int r = 0; r < 4; ++r) {
+    const double t1 = qi[r] - 0.5 * (dx * dxi[r] + dy * dyi[r]);
+    const double t2 = qk[r] - 0.5 * (dx * dxk[r] + dy * dyk[r]);
+    const double D = t2 - t1;
+    Db[r] = w * (L.sqb[r] * dx + L.syqb[r] * dy);
+    dxb += w * L.sqb[r] * D - 0.5 * Db[r] * (dxk[r] - dxi[r]);
+    dyb += w * L.syqb[r] * D - 0.5 * Db[r] * (dyk[r] - dyi[r]);
+    wb += D * (L.sqb[r] * dx + L.syqb[r] * dy);
+  }
+  if (wf != 0.0) {
+    dxb += wb * wf * dx;
+    dyb += wb * wf * dy;
+  }
+}
+
 // reverse of k_inner (one inner iteration): dq_out adjoint -> dq_in, q, xy adjoints.
 // Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
 // adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
@@ -425,26 +498,8 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   if (!STAGED && !pm.live) return;
   const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
   const D2 pi = g.xy[i];
-  const V4d gg = g.lsq_geo[i];
-  const double sxx = gg.x, syy = gg.y, sxy = gg.z, obd = gg.w;
-  const V4d tx = dq_out[2 * i], ty = dq_out[2 * i + 1];
-  const V4d bx = dqb_out[2 * i], by = dqb_out[2 * i + 1];
-  double sqb[4], syqb[4];
-  double sxxb = 0.0, syyb = 0.0, sxyb = 0.0, obdb = 0.0;
-  for (int r = 0; r < 4; ++r) {
-    const double sxq = sxx * tx[r] + sxy * ty[r];
-    const double syq = sxy * tx[r] + syy * ty[r];
-    sqb[r] = (bx[r] * syy - by[r] * sxy) * obd;
-    syqb[r] = (by[r] * sxx - bx[r] * sxy) * obd;
-    obdb += bx[r] * (sxq * syy - syq * sxy) + by[r] * (syq * sxx - sxq * sxy);
-    syyb += bx[r] * sxq * obd;
-    sxxb += by[r] * syq * obd;
-    sxyb += -(bx[r] * syq + by[r] * sxq) * obd;
-  }
-  const double detb = -obdb * obd * obd;
-  sxxb += detb * syy;
-  syyb += detb * sxx;
-  sxyb += -2.0 * detb * sxy;
+  InnerLam L;
+  inner_lam(g.lsq_geo[i], dq_out[2 * i], dq_out[2 * i + 1], dqb_out[2 * i], dqb_out[2 * i + 1], L);
   const V4d qi = q[i], dxi = dq_in[2 * i], dyi = dq_in[2 * i + 1];
   double qib[4] = {0, 0, 0, 0}, dxib[4] = {0, 0, 0, 0}, dyib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
@@ -452,28 +507,12 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   // edge in slot s to the neighbour with coordinates pk and records qk, dxk, dyk
   auto edge = [&](int s, const D2& pk, const V4d& qk, const V4d& dxk, const V4d& dyk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    double w, wf;
-    lsq_weight(dx, dy, power, w, wf);
-    // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
-    double dxb = w * (2.0 * sxxb * dx + sxyb * dy), dyb = w * (2.0 * syyb * dy + sxyb * dx);
-    double wb = sxxb * dx * dx + syyb * dy * dy + sxyb * dx * dy;
-    double Dbv[4];
+    double Dbv[4], dxb, dyb;
+    inner_edge_rev(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, L, Dbv, dxb, dyb);
     for (int r = 0; r < 4; ++r) {
-      const double t1 = qi[r] - 0.5 * (dx * dxi[r] + dy * dyi[r]);
-      const double t2 = qk[r] - 0.5 * (dx * dxk[r] + dy * dyk[r]);
-      const double D = t2 - t1;
-      const double Db = w * (sqb[r] * dx + syqb[r] * dy);
-      dxb += w * sqb[r] * D - 0.5 * Db * (dxk[r] - dxi[r]);
-      dyb += w * syqb[r] * D - 0.5 * Db * (dyk[r] - dyi[r]);
-      wb += D * (sqb[r] * dx + syqb[r] * dy);
-      Dbv[r] = Db;
-      qib[r] -= Db;
-      dxib[r] += 0.5 * dx * Db;
-      dyib[r] += 0.5 * dy * Db;
-    }
-    if (wf != 0.0) {
-      dxb += wb * wf * dx;
-      dyb += wb * wf * dy;
+      qib[r] -= Dbv[r];
+      dxib[r] += 0.5 * dx * Dbv[r];
+      dyib[r] += 0.5 * dy * Dbv[r];
     }
     D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
     ep[0] = D2{Dbv[0], Dbv[1]};
@@ -524,6 +563,199 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   dqb_in[2 * i + 1] = V4d{dyib[0], dyib[1], dyib[2], dyib[3]};
   const D2 pxy = xyb[i];
   xyb[i] = D2{pxy.x + xib, pxy.y + yib};
+}
+
+// LDS of the pull-form inner reverse per halo entry: q, dq_x, dq_y, the solve adjoint's
+// sqb and syqb (V4d each), xy, and sxxb, syyb, sxyb; then the block's slot words
+inline size_t stage_lds_inner_pull(int hcap, int width) {
+  return (size_t)hcap * (5 * sizeof(V4d) + sizeof(D2) + 3 * sizeof(double)) + (size_t)width * STAGE_P;
+}
+
+// A point's solve adjoint (InnerLam) as three records: sqb, syqb, (sxxb, syyb, sxyb, 0)
+__device__ inline void lam_store(V4d* __restrict__ lam, int i, const InnerLam& L) {
+  lam[3 * i] = V4d{L.sqb[0], L.sqb[1], L.sqb[2], L.sqb[3]};
+  lam[3 * i + 1] = V4d{L.syqb[0], L.syqb[1], L.syqb[2], L.syqb[3]};
+  lam[3 * i + 2] = V4d{L.sxxb, L.syyb, L.sxyb, 0.0};
+}
+
+// Solve adjoints of every point for the first reversed inner iteration of a stage, from
+// the adjoint of that iteration's output (dqb_out) and the output itself (dq_out)
+__global__ __launch_bounds__(BLOCK) void k_inner_lam(Geom g, const V4d* __restrict__ dq_out,
+                                                     const V4d* __restrict__ dqb_out, V4d* __restrict__ lam) {
+  const int i = point_id();
+  if (i >= g.n) return;
+  InnerLam L;
+  inner_lam(g.lsq_geo[i], dq_out[2 * i], dq_out[2 * i + 1], dqb_out[2 * i], dqb_out[2 * i + 1], L);
+  lam_store(lam, i, L);
+}
+
+// Pull form of k_inner_rev + k_gather_lsq<true> in one launch, without the edge
+// buffer.  The inner iteration is linear in dq, and the adjoint a point receives over an
+// edge it is the neighbour of depends only on the owner's solve adjoint (lam), the two
+// ends' records and the offset, so each point gathers it over the transposed
+// connectivity instead of the owner writing it out.  Blocks of STAGE_P points stage
+// their PullHalo (own points, neighbours and owners: q, dq_in, xy and lam) in LDS; then
+// every point walks its own slots (owner terms) and its transposed entries (neighbour
+// terms), both in the push form's orders, and its first lane writes dqb_in, adds into
+// qb and xyb and, for the next reversed inner iteration (lam_next non-null), writes its
+// solve adjoint from that dqb_in (dq_in being the next iteration's output).  Entries past
+// the LDS capacity (255) are read from global memory.
+template <int LPP>
+__global__ __launch_bounds__(BLOCK, 3) void k_inner_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
+                                                      const V4d* __restrict__ dq_in, const V4d* __restrict__ lam,
+                                                      V4d* __restrict__ dqb_in, V4d* __restrict__ lam_next,
+                                                      V4d* __restrict__ qb, D2* __restrict__ xyb, double power) {
+  const PointMap<LPP, true> pm(g.n);
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int H = ph.hcap;
+  V4d* s_q = (V4d*)smem;
+  V4d* s_dx = s_q + H;
+  V4d* s_dy = s_dx + H;
+  V4d* s_sqb = s_dy + H;
+  V4d* s_syqb = s_sqb + H;
+  D2* s_xy = (D2*)(s_syqb + H);
+  double* s_gxx = (double*)(s_xy + H);
+  double* s_gyy = s_gxx + H;
+  double* s_gxy = s_gyy + H;
+  uint8_t* s_cl = (uint8_t*)(s_gxy + H);
+  const int b = pm.b;
+  const int h0 = ph.ptr[b], hn = ph.ptr[b + 1] - h0;
+  const int hl = hn < H ? hn : H;
+  for (int j = threadIdx.x; j < hl; j += STAGE_P * LPP) {
+    const int k = ph.ids[h0 + j];
+    s_q[j] = q[k];
+    s_dx[j] = dq_in[2 * k];
+    s_dy[j] = dq_in[2 * k + 1];
+    s_xy[j] = g.xy[k];
+    s_sqb[j] = lam[3 * k];
+    s_syqb[j] = lam[3 * k + 1];
+    const V4d gb = lam[3 * k + 2];
+    s_gxx[j] = gb.x;
+    s_gyy[j] = gb.y;
+    s_gxy[j] = gb.z;
+  }
+  // the block's slot words: per slot a STAGE_P-byte row segment in 16-byte loads
+  constexpr int V = STAGE_P / 16;
+  const uint4* src = (const uint4*)(ph.loc8 + (size_t)b * STAGE_P);
+  for (int j = threadIdx.x; j < g.width * V; j += STAGE_P * LPP) {
+    const int s = j / V;
+    ((uint4*)s_cl)[j] = src[(size_t)s * (ph.stride / 16) + (j - s * V)];
+  }
+  __syncthreads();
+  if (!pm.live) return;
+  const int li = (int)threadIdx.x / LPP;  // the block's own points lead its halo
+  const D2 pi = s_xy[li];
+  const V4d qi = s_q[li], dxi = s_dx[li], dyi = s_dy[li];
+  auto lam_lds = [&](int j, InnerLam& L) {
+    const V4d a = s_sqb[j], c = s_syqb[j];
+    for (int r = 0; r < 4; ++r) {
+      L.sqb[r] = a[r];
+      L.syqb[r] = c[r];
+    }
+    L.sxxb = s_gxx[j];
+    L.syyb = s_gyy[j];
+    L.sxyb = s_gxy[j];
+  };
+  auto lam_glb = [&](int k, InnerLam& L) {
+    const V4d a = lam[3 * k], c = lam[3 * k + 1], gb = lam[3 * k + 2];
+    for (int r = 0; r < 4; ++r) {
+      L.sqb[r] = a[r];
+      L.syqb[r] = c[r];
+    }
+    L.sxxb = gb.x;
+    L.syyb = gb.y;
+    L.sxyb = gb.z;
+  };
+  double aq[4] = {0, 0, 0, 0}, ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0};
+  double xb = 0.0, yb = 0.0;
+  // owner terms: the point's own slots (k_inner_rev's walk)
+  {
+    InnerLam Li;
+    lam_lds(li, Li);
+    const int nb = g.nbhs[i];
+    for (int s = lane; s < nb; s += LPP) {
+      const int j = s_cl[s * STAGE_P + li];
+      D2 pk;
+      V4d qk, dxk, dyk;
+      if (j < H) {
+        pk = s_xy[j];
+        qk = s_q[j];
+        dxk = s_dx[j];
+        dyk = s_dy[j];
+      } else {
+        const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
+        pk = g.xy[k];
+        qk = q[k];
+        dxk = dq_in[2 * k];
+        dyk = dq_in[2 * k + 1];
+      }
+      const double dx = pk.x - pi.x, dy = pk.y - pi.y;
+      double Db[4], dxb, dyb;
+      inner_edge_rev(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, Li, Db, dxb, dyb);
+      for (int r = 0; r < 4; ++r) {
+        aq[r] -= Db[r];
+        ax[r] += 0.5 * dx * Db[r];
+        ay[r] += 0.5 * dy * Db[r];
+      }
+      xb -= dxb;
+      yb -= dyb;
+    }
+  }
+  // neighbour terms: the edges that name the point (k_gather_lsq's walk)
+  const int e1 = tr.ptr[i + 1];
+  for (int e = tr.ptr[i] + lane; e < e1; e += LPP) {
+    const int j = ph.tloc8[e];
+    D2 po;
+    V4d qo, dxo, dyo;
+    InnerLam Lo;
+    if (j < H) {
+      po = s_xy[j];
+      qo = s_q[j];
+      dxo = s_dx[j];
+      dyo = s_dy[j];
+      lam_lds(j, Lo);
+    } else {
+      const int o = tr.own[e];
+      po = g.xy[o];
+      qo = q[o];
+      dxo = dq_in[2 * o];
+      dyo = dq_in[2 * o + 1];
+      lam_glb(o, Lo);
+    }
+    const double dx = pi.x - po.x, dy = pi.y - po.y;
+    double Db[4], dxb, dyb;
+    inner_edge_rev(dx, dy, power, qo, dxo, dyo, qi, dxi, dyi, Lo, Db, dxb, dyb);
+    for (int r = 0; r < 4; ++r) {
+      aq[r] += Db[r];
+      ax[r] += -0.5 * dx * Db[r];
+      ay[r] += -0.5 * dy * Db[r];
+    }
+    xb += dxb;
+    yb += dyb;
+  }
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) {
+      aq[r] = group_sum<LPP>(aq[r]);
+      ax[r] = group_sum<LPP>(ax[r]);
+      ay[r] = group_sum<LPP>(ay[r]);
+    }
+    xb = group_sum<LPP>(xb);
+    yb = group_sum<LPP>(yb);
+    if (lane != 0) return;
+  }
+  const V4d a = qb[i];
+  qb[i] = V4d{a.x + aq[0], a.y + aq[1], a.z + aq[2], a.w + aq[3]};
+  const V4d bx{ax[0], ax[1], ax[2], ax[3]}, by{ay[0], ay[1], ay[2], ay[3]};
+  dqb_in[2 * i] = bx;
+  dqb_in[2 * i + 1] = by;
+  const D2 pxy = xyb[i];
+  xyb[i] = D2{pxy.x + xb, pxy.y + yb};
+  if (lam_next) {
+    InnerLam L;
+    inner_lam(g.lsq_geo[i], dxi, dyi, bx, by, L);
+    lam_store(lam_next, i, L);
+  }
 }
 
 // reverse of k_lsq (eval_q_derivatives): dq and qm adjoints -> q, xy adjoints.  Own
@@ -961,6 +1193,10 @@ struct AdjState {
   // transposed connectivity
   double* eb = nullptr;
   int32_t *epos = nullptr, *t_ptr = nullptr, *t_own = nullptr;
+  // halo of the pull-form inner reverse (built with the transposed connectivity) and
+  // its solve adjoints, 3 records per point, double-buffered over the inner iterations
+  PullHalo ph;
+  V4d* lam[2] = {nullptr, nullptr};
   // checkpoints
   std::vector<V4d*> snap;       // prim snapshots
   std::vector<int> snap_t;      // iteration index held by each slot (-1 free)
@@ -1145,6 +1381,30 @@ static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
   });
 }
 
+// inner iteration j of stage s in the pull form (MF_LPP_PULL | lanes): one launch, no
+// edge buffer and no gather
+// inner iteration j of stage s in the pull form (MF_LPP_PULL | lanes): one launch, no
+// edge buffer and no gather.  It reads the solve adjoints of iteration j (A.lam[j & 1]:
+// k_inner_lam's for j = I, else the previous pull launch's) and, for j > 1, writes those
+// of iteration j - 1.
+static void launch_inner_pull(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
+  const int cur = j & 1, prv = (j - 1) & 1;
+  const V4d* q = (const V4d*)A.tq[s];
+  const V4d* din = (const V4d*)A.tdq[s * (A.I + 1) + j - 1];
+  if (j == A.I) {
+    const V4d* dout = (const V4d*)A.tdq[s * (A.I + 1) + j];
+    hipLaunchKernelGGL(k_inner_lam, dim3(nblocks(c)), dim3(BLOCK), 0, c->stream, c->g, dout,
+                       (const V4d*)A.dqb[cur], A.lam[cur]);
+  }
+  const Trans tr{A.t_ptr, A.t_own};
+  const size_t lds = stage_lds_inner_pull(A.ph.hcap, c->g.width);
+  V4d* lam_next = j > 1 ? A.lam[prv] : nullptr;
+  with_lpp_staged(lpp & ~LPP_PULL, [&]<int L>() {
+    hipLaunchKernelGGL((k_inner_pull<L>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
+                       din, (const V4d*)A.lam[cur], A.dqb[prv], lam_next, A.qb, A.xyb, c->p.power);
+  });
+}
+
 template <bool WITH_DQ>
 static void launch_gather_lsq(mf_ctx* c, AdjState& A, V4d* dqb, int lpp) {
   const Trans tr{A.t_ptr, A.t_own};
@@ -1189,8 +1449,12 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
     launch_gather_flux(c, A, dqb_last, A.lpp_gath);
     // inner iterations, last first: dq^(j) adjoint -> dq^(j-1)
     for (int j = A.I; j >= 1; --j) {
-      launch_inner_rev(c, A, s, j, A.lpp_irev);
-      launch_gather_lsq<true>(c, A, A.dqb[(j - 1) & 1], A.lpp_gath);
+      if (A.lpp_irev & LPP_PULL) {
+        launch_inner_pull(c, A, s, j, A.lpp_irev);
+      } else {
+        launch_inner_rev(c, A, s, j, A.lpp_irev);
+        launch_gather_lsq<true>(c, A, A.dqb[(j - 1) & 1], A.lpp_gath);
+      }
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
     launch_lsq_rev(c, A, s, A.lpp_lrev);
@@ -1264,6 +1528,23 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
     for (int k = 0; k < 4; ++k)
       if (us[k] < best[k]) best[k] = us[k], pick[k] = lpp;
   }
+  // the inner reverse: a stage's I reversed inner iterations in the push form (kernel
+  // and gather per iteration) against the pull form (its solve-adjoint kernel, then one
+  // launch per iteration)
+  if (A.I > 0) {
+    double us_g = 0;
+    if ((rc = time_launches(c, reps, [&] { launch_gather_lsq<true>(c, A, A.dqb[0], pick[3]); }, &us_g))) return rc;
+    double push = A.I * (best[2] + us_g);
+    for (int l : {1, 2, 4}) {
+      double us = 0;
+      if (A.ph.hcap == 0) break;
+      auto stage = [&] {
+        for (int j = A.I; j >= 1; --j) launch_inner_pull(c, A, 0, j, LPP_PULL | l);
+      };
+      if ((rc = time_launches(c, reps, stage, &us))) return rc;
+      if (us < push) push = us, pick[2] = LPP_PULL | l;
+    }
+  }
   A.lpp_frev = pick[0];
   A.lpp_lrev = pick[1];
   A.lpp_irev = pick[2];
@@ -1290,6 +1571,65 @@ static int binomial_mid(int t0, int t1, int free_slots) {
   while (eta(free_slots, r) < m) ++r;
   const int64_t right = std::min<int64_t>(eta(free_slots - 1, r), (int64_t)m - 1);
   return t1 - (int)right;
+}
+
+// The pull-form inner reverse's halo (PullHalo) from the connectivity (hc [W][n], hn)
+// and its transpose (ptr, own).  LDS holds the 99th percentile of the blocks' halo
+// sizes, at most 248 entries and what kMaxStageLds allows, at least the STAGE_P own
+// points; entries past it are marked 255 and read from global memory.
+static int build_pull_halo(mf_ctx* c, AdjState& A, const std::vector<uint32_t>& hc, const std::vector<int32_t>& hn,
+                           const std::vector<int32_t>& ptr, const std::vector<int32_t>& own) {
+  const int n = c->n, W = c->W, P = STAGE_P;
+  const int nblk = (n + P - 1) / P, stride = nblk * P;
+  std::vector<int32_t> hptr(nblk + 1, 0), ids, other, sizes;
+  ids.reserve((size_t)n * 4);
+  sizes.reserve(nblk);
+  std::vector<uint16_t> loc((size_t)W * stride, 0), tloc(std::max<size_t>(ptr[n], 1), 0);
+  for (int b = 0; b < nblk; ++b) {
+    const int p0 = b * P, p1 = std::min(n, p0 + P);
+    other.clear();
+    for (int i = p0; i < p1; ++i) {
+      for (int s = 0; s < hn[i]; ++s) {
+        const int k = (int)(hc[(size_t)s * n + i] >> 4);
+        if (k < p0 || k >= p1) other.push_back(k);
+      }
+      for (int e = ptr[i]; e < ptr[i + 1]; ++e)
+        if (own[e] < p0 || own[e] >= p1) other.push_back(own[e]);
+    }
+    std::sort(other.begin(), other.end());
+    other.erase(std::unique(other.begin(), other.end()), other.end());
+    const int nown = p1 - p0;
+    auto pos_of = [&](int k) {
+      return (k >= p0 && k < p1) ? k - p0
+                                 : nown + (int)(std::lower_bound(other.begin(), other.end(), k) - other.begin());
+    };
+    sizes.push_back(nown + (int)other.size());
+    for (int i = p0; i < p1; ++i) ids.push_back(i);
+    ids.insert(ids.end(), other.begin(), other.end());
+    for (int i = p0; i < p1; ++i) {
+      for (int s = 0; s < hn[i]; ++s)
+        loc[(size_t)s * stride + i] = (uint16_t)std::min(pos_of((int)(hc[(size_t)s * n + i] >> 4)), 65535);
+      for (int e = ptr[i]; e < ptr[i + 1]; ++e) tloc[e] = (uint16_t)std::min(pos_of(own[e]), 65535);
+    }
+    hptr[b + 1] = (int32_t)ids.size();
+  }
+  std::sort(sizes.begin(), sizes.end());
+  int cap = sizes[std::min(nblk - 1, (int)((int64_t)nblk * 99 / 100))];
+  const int lds_cap = (int)((kMaxStageLds - std::min(kMaxStageLds, (size_t)W * P)) / stage_lds_inner_pull(1, 0));
+  if (lds_cap < P) return MF_OK;  // stencils too wide: the pull form is not offered (hcap 0)
+  cap = std::max(P, std::min({cap, 248, lds_cap}));
+  const int hcap = std::min((cap + 7) / 8 * 8, std::max(P, lds_cap / 8 * 8));
+  std::vector<uint8_t> loc8(loc.size()), tloc8(tloc.size());
+  for (size_t e = 0; e < loc.size(); ++e) loc8[e] = (uint8_t)(loc[e] < hcap ? loc[e] : 255);
+  for (size_t e = 0; e < tloc.size(); ++e) tloc8[e] = (uint8_t)(tloc[e] < hcap ? tloc[e] : 255);
+  int32_t *d_ptr, *d_ids;
+  uint8_t *d_loc8, *d_tloc8;
+  int rc;
+  if ((rc = upload(c, &d_ptr, hptr)) || (rc = upload(c, &d_ids, ids)) || (rc = upload(c, &d_loc8, loc8)) ||
+      (rc = upload(c, &d_tloc8, tloc8)))
+    return rc;
+  A.ph = PullHalo{d_ptr, d_ids, d_loc8, d_tloc8, hcap, stride};
+  return MF_OK;
 }
 
 static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
@@ -1396,6 +1736,9 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
     if ((rc = upload(c, &A.t_ptr, ptr)) || (rc = upload(c, &A.t_own, own)) || (rc = upload(c, &A.epos, pos)))
       return rc;
     if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1)))) return rc;
+    if (I > 0 && ((rc = build_pull_halo(c, A, hc, hn, ptr, own)) || (rc = dev_alloc(c, &A.lam[0], 3 * (size_t)n)) ||
+                  (rc = dev_alloc(c, &A.lam[1], 3 * (size_t)n))))
+      return rc;
   }
   A.snap.assign(snaps, nullptr);
   for (auto& p : A.snap) if ((rc = dev_alloc(c, &p, snap_v4))) return rc;
@@ -1587,6 +1930,8 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   if (c->lpp_rev[1]) A.lpp_lrev = c->lpp_rev[1];
   if (c->lpp_rev[2]) A.lpp_irev = c->lpp_rev[2];
   if (c->lpp_rev[3]) A.lpp_gath = c->lpp_rev[3];
+  if ((A.lpp_irev & LPP_PULL) && A.I > 0 && A.ph.hcap == 0)
+    return set_err(MF_ERR_INVALID, "the pull-form inner reverse does not fit in LDS at this stencil width");
   A.first_it = first_it;
   A.iters = iters;
   A.fwd_steps = A.taped = 0;

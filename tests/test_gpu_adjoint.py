@@ -9,8 +9,9 @@ differences of the FP64 oracle.
   a 1e-7 step (stencil split held fixed, normals recomputed), to 1e-5 relative.
 * The checkpoint schedule changes the recomputation, not the result.
 
-The Tapenade adjoint itself (pointb%x/y of src_mpi_adjoint) cannot be built here
-(PETSc); that comparison is unpinned.
+The reference's Tapenade reverse routines are pinned routine by routine in
+tests/test_tapenade.py; its whole-iteration pointb%x/y (state_update_b, fpi_solver_b,
+PETSc) cannot be built here and stays unpinned.
 """
 import numpy as np
 import pytest
@@ -223,6 +224,39 @@ def test_adjoint_staged_reverse_overflow(c1_cloud, kernel):
     with _gpu(c1_cloud) as a, _gpu(c1_cloud) as b:
         a.tuning = {**pin, key: 2}
         b.tuning = {**pin, key: 16 | 2}
+        Ja, xa, ya, _ = a.adjoint(6, snaps=2)
+        Jb, xb, yb, _ = b.adjoint(6, snaps=2)
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("lpp", [1, 2, 4])
+@pytest.mark.parametrize("power", [0.0, -2.0])
+def test_adjoint_pull_inner_reverse(small_cloud, lpp, power):
+    """The pull form of the inner-iteration reverse (MF_LPP_PULL | lanes: each point
+    gathers the terms it receives as a neighbour from its owners' records and solve
+    adjoints staged in LDS, with no edge buffer and no gather kernel) gives the push
+    form's gradient up to summation order, with and without distance weights."""
+    pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_gather": 2}
+    with _gpu(small_cloud, power=power) as a, _gpu(small_cloud, power=power) as b:
+        a.tuning = {**pin, "lpp_inner_rev": 2}
+        b.tuning = {**pin, "lpp_inner_rev": 64 | lpp}
+        Ja, xa, ya, _ = a.adjoint(10, snaps=3)
+        Jb, xb, yb, sb = b.adjoint(10, snaps=3)
+    assert sb["lpp_inner_rev"] == 64 | lpp
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_adjoint_pull_inner_reverse_overflow(c1_cloud):
+    """On the 9,600-point cloud some 64-point blocks' pull halos (neighbours and owners)
+    exceed the LDS capacity: those entries, as slots and as transposed entries, are read
+    from global memory.  Same gradient as the push form, 3 inner iterations (so the
+    pull launches hand their solve adjoints on twice)."""
+    pin = {"lpp_flux_rev": 2, "lpp_lsq_rev": 2, "lpp_gather": 2}
+    with _gpu(c1_cloud, inner_iterations=3) as a, _gpu(c1_cloud, inner_iterations=3) as b:
+        a.tuning = {**pin, "lpp_inner_rev": 2}
+        b.tuning = {**pin, "lpp_inner_rev": 64 | 4}
         Ja, xa, ya, _ = a.adjoint(6, snaps=2)
         Jb, xb, yb, _ = b.adjoint(6, snaps=2)
     ref = np.concatenate([xa, ya])
