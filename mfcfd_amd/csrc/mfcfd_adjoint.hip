@@ -38,10 +38,11 @@ __device__ inline void aadd2(D2* p, double x, double y) {
 // contiguous, and the edges are stored in the order of the transposed connectivity
 // (grouped by the neighbour that receives them), so a gather reads one contiguous
 // block per point; epos[s * N + i] is the position of the edge in slot s of point i.
-// The flux reverse writes 18 fields (q(4), d(4), qmax(4), qmin(4), delx, dely; d being
-// the reconstruction increment delx dq_x + dely dq_y), the LSQ reverses 6 (q(4), delx,
-// dely).
-constexpr int EF_FLUX = 18;
+// The flux reverse writes 14 fields (q(4), d(4), m(4), delx, dely; d being the
+// reconstruction increment delx dq_x + dely dq_y, m the adjoint of the one of qmax or
+// qmin the limiter read for that component, which esel's bit r names: 1 qmax, 0 qmin),
+// the LSQ reverses 6 (q(4), delx, dely).
+constexpr int EF_FLUX = 14;
 constexpr int EF_LSQ = 6;
 
 struct AdjOut {
@@ -53,6 +54,7 @@ struct AdjOut {
   D2* nrmb;     // adjoint of the normals
   double* eb;            // edge buffer
   const int32_t* epos;   // edge positions [W * N]
+  uint8_t* esel;         // per edge: bit r set when field m(r) is qmax's adjoint [E]
 };
 
 // Half-stencil geometry sums (sum ds^2, dn^2, ds dn per half), kept for the adjoint
@@ -169,7 +171,8 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     // the intermediates and the reconstruction
     double cxb = 0.0, cyb = 0.0, cx2b = 0.0, cy2b = 0.0;
     double delxb = 0.0, delyb = 0.0;
-    D2* ep = reinterpret_cast<D2*>(o.eb + (size_t)o.epos[(size_t)s * g.n + i] * EF_FLUX);
+    const size_t epj = (size_t)o.epos[(size_t)s * g.n + i];
+    D2* ep = reinterpret_cast<D2*>(o.eb + epj * EF_FLUX);
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
       const int p = e ? k : nbrs.own();
@@ -227,10 +230,18 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
         ep[1] = D2{d.q[2], d.q[3]};
         ep[2] = D2{d.d[0], d.d[1]};
         ep[3] = D2{d.d[2], d.d[3]};
-        ep[4] = D2{d.qmax[0], d.qmax[1]};
-        ep[5] = D2{d.qmax[2], d.qmax[3]};
-        ep[6] = D2{d.qmin[0], d.qmin[1]};
-        ep[7] = D2{d.qmin[2], d.qmin[3]};
+        // the limiter reads qmax or qmin per component (venkat_rev), so at most one
+        // of the two adjoints is nonzero
+        double m[4];
+        unsigned sel = 0u;
+        for (int r = 0; r < 4; ++r) {
+          const bool mx = d.qmax[r] != 0.0;
+          m[r] = mx ? d.qmax[r] : d.qmin[r];
+          sel |= mx ? 1u << r : 0u;
+        }
+        ep[4] = D2{m[0], m[1]};
+        ep[5] = D2{m[2], m[3]};
+        o.esel[epj] = (uint8_t)sel;
       } else {
         for (int r = 0; r < 4; ++r) {
           own.q[r] += d.q[r];
@@ -260,7 +271,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     nyb += delsb * delx + delnb * dely;
     xib -= delxb;
     yib -= delyb;
-    ep[8] = D2{delxb, delyb};
+    ep[6] = D2{delxb, delyb};
   }
   // half-stencil geometry: c1 = sum dn^2 (x halves) or sum ds^2 (y halves), sxy = sum
   // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2); every lane needs the point's totals
@@ -316,8 +327,8 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     xib -= delxb;
     yib -= delyb;
     D2* ep = reinterpret_cast<D2*>(o.eb + (size_t)o.epos[(size_t)s * g.n + i] * EF_FLUX);
-    const D2 t = ep[8];
-    ep[8] = D2{t.x + delxb, t.y + delyb};
+    const D2 t = ep[6];
+    ep[6] = D2{t.x + delxb, t.y + delyb};
   }
   // own contributions, summed over the point's lanes and written (q, dq, qm) or
   // added (xy, normals, delta) by its first lane alone; the edges' contributions to
@@ -601,7 +612,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_lam(Geom g, const V4d* __restri
 // solve adjoint from that dqb_in (dq_in being the next iteration's output).  Entries past
 // the LDS capacity (255) are read from global memory.
 template <int LPP>
-__global__ __launch_bounds__(BLOCK, 3) void k_inner_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
+__global__ __launch_bounds__(BLOCK) void k_inner_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
                                                       const V4d* __restrict__ dq_in, const V4d* __restrict__ lam,
                                                       V4d* __restrict__ dqb_in, V4d* __restrict__ lam_next,
                                                       V4d* __restrict__ qb, D2* __restrict__ xyb, double power) {
@@ -674,8 +685,11 @@ __global__ __launch_bounds__(BLOCK, 3) void k_inner_pull(Geom g, PullHalo ph, Tr
     InnerLam Li;
     lam_lds(li, Li);
     const int nb = g.nbhs[i];
+    // the next slot's halo position is read while this edge is evaluated
+    int jn = lane < nb ? s_cl[lane * STAGE_P + li] : 0;
     for (int s = lane; s < nb; s += LPP) {
-      const int j = s_cl[s * STAGE_P + li];
+      const int j = jn;
+      if (s + LPP < nb) jn = s_cl[(s + LPP) * STAGE_P + li];
       D2 pk;
       V4d qk, dxk, dyk;
       if (j < H) {
@@ -703,9 +717,12 @@ __global__ __launch_bounds__(BLOCK, 3) void k_inner_pull(Geom g, PullHalo ph, Tr
     }
   }
   // neighbour terms: the edges that name the point (k_gather_lsq's walk)
-  const int e1 = tr.ptr[i + 1];
-  for (int e = tr.ptr[i] + lane; e < e1; e += LPP) {
-    const int j = ph.tloc8[e];
+  const int e0 = tr.ptr[i], e1 = tr.ptr[i + 1];
+  // the next entry's halo position (a global byte) is loaded while this edge is evaluated
+  int jn = e0 + lane < e1 ? ph.tloc8[e0 + lane] : 0;
+  for (int e = e0 + lane; e < e1; e += LPP) {
+    const int j = jn;
+    if (e + LPP < e1) jn = ph.tloc8[e + LPP];
     D2 po;
     V4d qo, dxo, dyo;
     InnerLam Lo;
@@ -910,6 +927,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
 // connectivity (each point reads the edges that name it; no atomics)
 template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const double* __restrict__ eb,
+                                                       const uint8_t* __restrict__ esel,
                                                        V4d* __restrict__ qb, V4d* __restrict__ dqb,
                                                        V4d* __restrict__ qmb, D2* __restrict__ xyb) {
   const int t = point_id();
@@ -922,7 +940,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
   const int j1 = tr.ptr[k + 1];
   for (int j = tr.ptr[k] + lane; j < j1; j += LPP) {
     const D2* ep = reinterpret_cast<const D2*>(eb + (size_t)j * EF_FLUX);
-    const D2 q01 = ep[0], q23 = ep[1], w = ep[8];
+    const D2 q01 = ep[0], q23 = ep[1], w = ep[6];
     aq[0] += q01.x;
     aq[1] += q01.y;
     aq[2] += q23.x;
@@ -938,15 +956,14 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
         ax[r] += d[r] * delx;
         ay[r] += d[r] * dely;
       }
-      const D2 x01 = ep[4], x23 = ep[5], n01 = ep[6], n23 = ep[7];
-      amx[0] += x01.x;
-      amx[1] += x01.y;
-      amx[2] += x23.x;
-      amx[3] += x23.y;
-      amn[0] += n01.x;
-      amn[1] += n01.y;
-      amn[2] += n23.x;
-      amn[3] += n23.y;
+      const D2 m01 = ep[4], m23 = ep[5];
+      const double m[4] = {m01.x, m01.y, m23.x, m23.y};
+      const unsigned sel = esel[j];
+      for (int r = 0; r < 4; ++r) {
+        const bool mx = (sel >> r) & 1u;
+        amx[r] += mx ? m[r] : 0.0;
+        amn[r] += mx ? 0.0 : m[r];
+      }
     }
   }
   if (LPP > 1) {
@@ -1192,6 +1209,7 @@ struct AdjState {
   // edge buffer (EF_FLUX doubles per edge, transposed order), edge positions and the
   // transposed connectivity
   double* eb = nullptr;
+  uint8_t* esel = nullptr;
   int32_t *epos = nullptr, *t_ptr = nullptr, *t_own = nullptr;
   // halo of the pull-form inner reverse (built with the transposed connectivity) and
   // its solve adjoints, 3 records per point, double-buffered over the inner iterations
@@ -1319,7 +1337,7 @@ static int adj_record(mf_ctx* c, AdjState& A) {
 template <class S>
 static void launch_flux_rev_s(mf_ctx* c, AdjState& A, const State<S>& st, V4d* dqb_last, int lpp) {
   const HalfSums hs{A.hxx, A.hyy, A.hxy};
-  const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos};
+  const AdjOut o{A.qb, dqb_last, A.qmb, A.deltab, A.xyb, A.nrmb, A.eb, A.epos, A.esel};
   if (lpp & LPP_STAGED) {
     // staged: blocks of STAGE_PF points, 2, 4 or 8 lanes each
     const dim3 gr((unsigned)((c->n + STAGE_PF - 1) / STAGE_PF));
@@ -1357,7 +1375,7 @@ static void launch_gather_flux(mf_ctx* c, AdjState& A, V4d* dqb_last, int lpp) {
   const Trans tr{A.t_ptr, A.t_own};
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL(k_gather_flux<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, tr,
-                       (const double*)A.eb, A.qb, dqb_last, A.qmb, A.xyb);
+                       (const double*)A.eb, (const uint8_t*)A.esel, A.qb, dqb_last, A.qmb, A.xyb);
   });
 }
 
@@ -1735,7 +1753,9 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
       }
     if ((rc = upload(c, &A.t_ptr, ptr)) || (rc = upload(c, &A.t_own, own)) || (rc = upload(c, &A.epos, pos)))
       return rc;
-    if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1)))) return rc;
+    if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1))) ||
+        (rc = dev_alloc(c, &A.esel, (size_t)std::max(ptr[n], 1))))
+      return rc;
     if (I > 0 && ((rc = build_pull_halo(c, A, hc, hn, ptr, own)) || (rc = dev_alloc(c, &A.lam[0], 3 * (size_t)n)) ||
                   (rc = dev_alloc(c, &A.lam[1], 3 * (size_t)n))))
       return rc;
