@@ -67,6 +67,12 @@ struct alignas(4 * sizeof(T)) V4 {
 };
 using V4d = V4<double>;
 
+// componentwise difference
+template <class T>
+__host__ __device__ inline V4<T> operator-(const V4<T>& a, const V4<T>& b) {
+  return V4<T>{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
+}
+
 struct alignas(16) D2 {
   double x, y;
 };
@@ -306,6 +312,23 @@ __host__ __device__ inline S venkat(S qt, S qk, S qmax, S qmin, S epsi) {
   return (t < (S)1.0) ? t : (S)1.0;
 }
 
+// venkat with del_pos's two candidates qmax - qk, qmin - qk (pmax, pmin) and epsi^2
+// (e2) given, as the stencil kernels precompute them once per point: the same
+// operations in the same order, so the same bits
+template <class S>
+__host__ __device__ inline S venkat_pre(S qt, S qk, S pmax, S pmin, S e2) {
+  S del_neg = qt - qk;
+  if (m_abs(del_neg) <= Consts<S>::THRESH) return (S)1.0;
+  S del_pos = (del_neg > (S)0) ? pmax : pmin;
+  S num = (del_pos * del_pos) + e2;
+  num = num * del_neg + (S)2.0 * del_neg * del_neg * del_pos;
+  S den = del_pos * del_pos + (S)2.0 * del_neg * del_neg;
+  den = den + del_neg * del_pos + e2;
+  den = den * del_neg;
+  S t = num / den;
+  return (t < (S)1.0) ? t : (S)1.0;
+}
+
 // State at one end of an edge after the limited reconstruction, in the point's
 // (t, n) frame, with the shared pieces of every split flux (split_fluxes.F90:19-31,
 // quadrant_fluxes.F90:219-234).
@@ -486,11 +509,17 @@ __host__ __device__ inline void reconstruct(const V4<S>& q, const V4<S>& dx, con
   }
 }
 
-// reconstruct for the stencil kernels (see kin_eval)
-__host__ __device__ inline void reconstruct_eval(const V4d& q, const V4d& dx, const V4d& dy, const V4d& qmax,
-                                                 const V4d& qmin, double eps, double delx, double dely,
+// reconstruct for the stencil kernels (see kin_eval), from the end's pmax = qmax - q,
+// pmin = qmin - q and e2 = eps^2 (precomputed per point; same bits as reconstruct)
+__host__ __device__ inline void reconstruct_eval(const V4d& q, const V4d& dx, const V4d& dy, const V4d& pmax,
+                                                 const V4d& pmin, double e2, double delx, double dely,
                                                  double qt[4]) {
-  reconstruct(q, dx, dy, qmax, qmin, eps, delx, dely, qt);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double d = delx * dx[r] + dely * dy[r];
+    const double phi = venkat_pre(q[r] - 0.5 * d, q[r], pmax[r], pmin[r], e2);
+    qt[r] = q[r] - 0.5 * phi * d;
+  }
 }
 // The mixed path's limited reconstruction.  The reference's limiter ratio has the
 // factor del_neg in both numerator and denominator (limiters.F90:91-97); past the
@@ -500,14 +529,13 @@ __host__ __device__ inline void reconstruct_eval(const V4d& q, const V4d& dx, co
 // -d/2 exactly, so num = a - d p, den = a + del_neg (p - d) and qtilde = q + phi del_neg;
 // a reciprocal replaces the division.
 __device__ inline void reconstruct_eval(const V4<float>& q, const V4<float>& dx, const V4<float>& dy,
-                                        const V4<float>& qmax, const V4<float>& qmin, float eps, float delx,
+                                        const V4<float>& pmax, const V4<float>& pmin, float e2, float delx,
                                         float dely, float qt[4]) {
-  const float e2 = eps * eps;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const float d = delx * dx[r] + dely * dy[r];
     const float dn = -0.5f * d;
-    const float p = (dn > 0.0f ? qmax[r] : qmin[r]) - q[r];
+    const float p = dn > 0.0f ? pmax[r] : pmin[r];
     const float a = p * p + e2;
     const float num = a - d * p;
     const float den = a + dn * (p - d);
@@ -620,6 +648,13 @@ struct GlobalNbrs {
   __device__ V4<S> qmax(int k) const { return st.qm[2 * k]; }
   __device__ V4<S> qmin(int k) const { return st.qm[2 * k + 1]; }
   __device__ S eps(int k) const { return st.eps[k]; }
+  // the limiter's inputs of a record: qmax - q, qmin - q, eps^2
+  __device__ V4<S> pmax(int k) const { return st.qm[2 * k] - st.q[k]; }
+  __device__ V4<S> pmin(int k) const { return st.qm[2 * k + 1] - st.q[k]; }
+  __device__ S e2(int k) const {
+    const S e = st.eps[k];
+    return e * e;
+  }
   // the point's own record index; slot s's neighbour index and half-stencil bits
   __device__ int own() const { return i; }
   __device__ void slot(int s, int& k, unsigned& m) const {
@@ -641,7 +676,9 @@ struct GlobalNbrs {
   }
 };
 
-// A block's staged halo in LDS (structure of arrays, one entry per halo position)
+// A block's staged halo in LDS (structure of arrays, one entry per halo position).
+// Staged with PRE (the forward flux kernels), qmax, qmin and eps hold the limiter's
+// inputs qmax - q, qmin - q and eps^2 instead.
 template <class S>
 struct HaloRecs {
   D2* xy;
@@ -649,7 +686,7 @@ struct HaloRecs {
   S* eps;
 };
 
-template <class S, int P, int LPP>
+template <class S, int P, int LPP, bool PRE = false>
 struct StagedNbrs {
   using scalar = S;
   const Geom& g;
@@ -660,9 +697,30 @@ struct StagedNbrs {
   __device__ V4<S> q(int k) const { return r.q[k]; }
   __device__ V4<S> dqx(int k) const { return r.dqx[k]; }
   __device__ V4<S> dqy(int k) const { return r.dqy[k]; }
-  __device__ V4<S> qmax(int k) const { return r.qmax[k]; }
-  __device__ V4<S> qmin(int k) const { return r.qmin[k]; }
-  __device__ S eps(int k) const { return r.eps[k]; }
+  __device__ V4<S> qmax(int k) const {
+    static_assert(!PRE, "a PRE halo holds qmax - q");
+    return r.qmax[k];
+  }
+  __device__ V4<S> qmin(int k) const {
+    static_assert(!PRE, "a PRE halo holds qmin - q");
+    return r.qmin[k];
+  }
+  __device__ S eps(int k) const {
+    static_assert(!PRE, "a PRE halo holds eps^2");
+    return r.eps[k];
+  }
+  __device__ V4<S> pmax(int k) const {
+    if constexpr (PRE) return r.qmax[k];
+    else return r.qmax[k] - r.q[k];
+  }
+  __device__ V4<S> pmin(int k) const {
+    if constexpr (PRE) return r.qmin[k];
+    else return r.qmin[k] - r.q[k];
+  }
+  __device__ S e2(int k) const {
+    if constexpr (PRE) return r.eps[k];
+    else return r.eps[k] * r.eps[k];
+  }
   __device__ int own() const { return li; }  // a block's own points lead its halo
   __device__ void slot(int s, int& k, unsigned& m) const {
     const unsigned e = s_cl[s * P + li];
@@ -685,7 +743,7 @@ struct StagedNbrs {
 // few blocks of wall and outer points): its points then walk the global records with
 // the same lanes and slots (GlobalNbrs), which gives the same bits.  Every thread
 // reaches the barrier either way.
-template <class S, int LPP>
+template <class S, int LPP, bool PRE = false>
 __device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b, unsigned char* smem,
                                        HaloRecs<S>& r, uint16_t*& s_cl) {
   const Halo& h = g.hl[1];
@@ -705,13 +763,21 @@ __device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b,
   }
   for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
     const int k = h.ids[h0 + j];
-    r.q[j] = st.q[k];
+    const V4<S> qk = st.q[k];
+    r.q[j] = qk;
     r.dqx[j] = st.dq[2 * k];
     r.dqy[j] = st.dq[2 * k + 1];
-    r.qmax[j] = st.qm[2 * k];
-    r.qmin[j] = st.qm[2 * k + 1];
+    if constexpr (PRE) {
+      r.qmax[j] = st.qm[2 * k] - qk;
+      r.qmin[j] = st.qm[2 * k + 1] - qk;
+      const S e = st.eps[k];
+      r.eps[j] = e * e;
+    } else {
+      r.qmax[j] = st.qm[2 * k];
+      r.qmin[j] = st.qm[2 * k + 1];
+      r.eps[j] = st.eps[k];
+    }
     r.xy[j] = g.xy[k];
-    r.eps[j] = st.eps[k];
   }
   constexpr int V = STAGE_PF * 2 / 16;
   const uint4* src = (const uint4*)(h.loc + (size_t)b * STAGE_PF);
@@ -732,8 +798,10 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
   const S nxs = (S)nx, nys = (S)ny;
   const V4<S> qi = st.q[i];
   const V4<S> dxi = st.dq[2 * i], dyi = st.dq[2 * i + 1];
-  const V4<S> qmaxi = st.qm[2 * i], qmini = st.qm[2 * i + 1];
+  // the own end's limiter inputs, once per point
+  const V4<S> pmaxi = st.qm[2 * i] - qi, pmini = st.qm[2 * i + 1] - qi;
   const S epsi = st.eps[i];
+  const S e2i = epsi * epsi;
   // FP64: the reference's coefficient arithmetic; the mixed path: FP32 edge geometry
   // and coefficients (HalfCoefF)
   constexpr bool F32 = sizeof(S) == 4;
@@ -789,11 +857,10 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
         qtk[r] = qk[r];
       }
     } else {
-      reconstruct_eval(qi, dxi, dyi, qmaxi, qmini, epsi, dx, dy, qti);
+      reconstruct_eval(qi, dxi, dyi, pmaxi, pmini, e2i, dx, dy, qti);
       const V4<S> qk = nbrs.q(k);
       const V4<S> dxk = nbrs.dqx(k), dyk = nbrs.dqy(k);
-      const V4<S> qmaxk = nbrs.qmax(k), qmink = nbrs.qmin(k);
-      reconstruct_eval(qk, dxk, dyk, qmaxk, qmink, nbrs.eps(k), dx, dy, qtk);
+      reconstruct_eval(qk, dxk, dyk, nbrs.pmax(k), nbrs.pmin(k), nbrs.e2(k), dx, dy, qtk);
     }
     S gxi[4], gyi[4], gxk[4], gyk[4];
     if constexpr (sizeof(S) == 4) {

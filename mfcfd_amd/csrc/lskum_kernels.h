@@ -490,11 +490,11 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
     extern __shared__ __align__(16) unsigned char smem[];
     HaloRecs<S> r;
     uint16_t* s_cl;
-    const bool staged = stage_flux_halo<S, LPP>(g, st, pm.b, smem, r, s_cl);
+    const bool staged = stage_flux_halo<S, LPP, true>(g, st, pm.b, smem, r, s_cl);
     if (pm.live) {
       const double dl = delta[i];
       if (staged) {
-        const StagedNbrs<S, STAGE_PF, LPP> nbrs{g, r, s_cl, (int)threadIdx.x / LPP, g.nbhs[i], lane};
+        const StagedNbrs<S, STAGE_PF, LPP, true> nbrs{g, r, s_cl, (int)threadIdx.x / LPP, g.nbhs[i], lane};
         if (i < g.nw) flux_point_src<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
         else if (i < g.nw + g.no) flux_point_src<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
         else flux_point_src<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
