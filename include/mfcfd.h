@@ -190,19 +190,21 @@ int mf_batch_cases(mf_ctx *ctx, int32_t *ncases, int32_t *first);
  * the same lanes.  us_inner is per sweep. */
 #define MF_LPP_STAGED 16
 #define MF_LPP_RESIDENT 32
-/* Adjoint inner-iteration reverse only, lpp_inner_rev = MF_LPP_PULL | lanes (1, 2 or 4):
- * the pull form, one launch per reversed inner iteration.  Each point gathers what it
- * receives as a neighbour over the transposed connectivity (from its owners' records
- * and solve adjoints staged in LDS), instead of the owners writing it to an edge buffer
- * that a second kernel gathers; same terms, summed in the push form's orders. */
+/* Adjoint LSQ and inner-iteration reverses, lpp_lsq_rev / lpp_inner_rev = MF_LPP_PULL |
+ * lanes (1, 2 or 4): the pull form, one launch per reversed step.  Each point gathers
+ * what it receives as a neighbour over the transposed connectivity (from its owners'
+ * records and solve adjoints staged in LDS), instead of the owners writing it to an edge
+ * buffer that a second kernel gathers; same terms, summed in the push form's orders.
+ * The LSQ reverse's qm adjoints (at most 8 per point) reach the selected neighbours by
+ * atomic adds, so the gradient's last bits can vary between runs. */
 #define MF_LPP_PULL 64
 typedef struct mf_tuning {
   int32_t lpp_lsq, lpp_inner, lpp_flux;
   int32_t barrier_polls;
   double us_lsq, us_inner, us_flux;  /* mf_autotune: mean launch time of the chosen shape */
   /* reverse kernels of the adjoint (flux, LSQ, inner iteration, edge gathers): 0 lets
-   * the first mf_adjoint_forward time the shapes (the inner reverse: its push form with
-   * the gather against its pull form) and keep the fastest */
+   * the first mf_adjoint_forward time the shapes (the LSQ and inner reverses: their push
+   * forms with the gather against their pull forms) and keep the fastest */
   int32_t lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather;
 } mf_tuning;
 /* Times every shape on the context's current state (`reps` launches each, default 10),

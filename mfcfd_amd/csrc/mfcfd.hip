@@ -1225,8 +1225,8 @@ static bool valid_rev_staged(const mf_ctx* c, int kind, int lpp) {
   return lds <= kMaxStageLds;
 }
 
-// the pull form of the adjoint's inner reverse: MF_LPP_PULL | 1, 2 or 4 lanes (its halo
-// is built with the adjoint's buffers and sized to fit LDS)
+// the pull forms of the adjoint's LSQ and inner reverses: MF_LPP_PULL | 1, 2 or 4 lanes
+// (their halo is built with the adjoint's buffers and sized to fit LDS)
 static bool valid_pull(int lpp) {
   const int l = lpp & ~LPP_PULL;
   return (lpp & LPP_PULL) && (l == 1 || l == 2 || l == 4);
@@ -1427,10 +1427,10 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   const int32_t rv[4] = {t->lpp_flux_rev, t->lpp_lsq_rev, t->lpp_inner_rev, t->lpp_gather};
   for (int k = 0; k < 4; ++k)
     if (rv[k] != 0 && !valid_lpp(rv[k]) && !(k == 0 && valid_shape(c, 2, rv[k])) &&
-        !((k == 1 || k == 2) && valid_rev_staged(c, k, rv[k])) && !(k == 2 && valid_pull(rv[k])))
+        !((k == 1 || k == 2) && valid_rev_staged(c, k, rv[k])) && !((k == 1 || k == 2) && valid_pull(rv[k])))
       return set_err(MF_ERR_INVALID,
                      "reverse lanes per point must be 0 (timed), 1, 2, 4 or 8 (the flux, LSQ and inner reverses "
-                     "also their staged shapes, the inner reverse MF_LPP_PULL | 1, 2 or 4)");
+                     "also their staged shapes, the LSQ and inner reverses MF_LPP_PULL | 1, 2 or 4)");
   int rc;
   if ((t->lpp_inner & MF_LPP_RESIDENT) && (rc = ensure_sweep_bar(c))) return rc;
   c->lpp_lsq = t->lpp_lsq;

@@ -493,6 +493,32 @@ __device__ inline void inner_edge_rev(double dx, double dy, double power, const 
   }
 }
 
+// Reverse of one edge's terms of the LSQ derivatives at its owner i, the neighbour k at
+// offset (dx, dy) (q_variables.F90:89-103): with the owner's solve adjoint L, the
+// adjoint Db of D = q_k - q_i per component (+ to the neighbour's q, - to the owner's)
+// and the adjoint (dxb, dyb) of the offset.  The owner (push form) and the neighbour
+// (pull form) evaluate it alike.
+__device__ inline void lsq_edge_rev(double dx, double dy, double power, const V4d& qi, const V4d& qk,
+                                    const InnerLam& L, double Db[4], double& dxb, double& dyb) {
+  double w, wf;
+  lsq_weight(dx, dy, power, w, wf);
+  // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
+  dxb = w * (2.0 * L.sxxb * dx + L.sxyb * dy);
+  dyb = w * (2.0 * L.syyb * dy + L.sxyb * dx);
+  double wb = L.sxxb * dx * dx + L.syyb * dy * dy + L.sxyb * dx * dy;
+  for (int r = 0; r < 4; ++r) {
+    const double D = qk[r] - qi[r];
+    Db[r] = w * (L.sqb[r] * dx + L.syqb[r] * dy);
+    dxb += w * L.sqb[r] * D;
+    dyb += w * L.syqb[r] * D;
+    wb += D * (L.sqb[r] * dx + L.syqb[r] * dy);
+  }
+  if (wf != 0.0) {
+    dxb += wb * wf * dx;
+    dyb += wb * wf * dy;
+  }
+}
+
 // reverse of k_inner (one inner iteration): dq_out adjoint -> dq_in, q, xy adjoints.
 // Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
 // adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
@@ -819,15 +845,10 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   double qib[4] = {0, 0, 0, 0};
   double xib = 0.0, yib = 0.0;
   const int nb = pm.live ? g.nbhs[i] : 0;
+  const InnerLam L{{sqb[0], sqb[1], sqb[2], sqb[3]}, {syqb[0], syqb[1], syqb[2], syqb[3]}, sxxb, syyb, sxyb};
   // edge in slot s to the neighbour with coordinates pk and q qk
   auto edge = [&](int s, const D2& pk, const V4d& qk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
-    double w, wf;
-    lsq_weight(dx, dy, power, w, wf);
-    // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
-    double dxb = w * (2.0 * sxxb * dx + sxyb * dy), dyb = w * (2.0 * syyb * dy + sxyb * dx);
-    double wb = sxxb * dx * dx + syyb * dy * dy + sxyb * dx * dy;
-    double Dbv[4];
     for (int r = 0; r < 4; ++r) {
       if (qk[r] > qmax[r]) {
         qmax[r] = qk[r];
@@ -837,18 +858,10 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
         qmin[r] = qk[r];
         smin[r] = s;
       }
-      const double D = qk[r] - qi[r];
-      const double Db = w * (sqb[r] * dx + syqb[r] * dy);
-      dxb += w * sqb[r] * D;
-      dyb += w * syqb[r] * D;
-      wb += D * (sqb[r] * dx + syqb[r] * dy);
-      Dbv[r] = Db;
-      qib[r] -= Db;
     }
-    if (wf != 0.0) {
-      dxb += wb * wf * dx;
-      dyb += wb * wf * dy;
-    }
+    double Dbv[4], dxb, dyb;
+    lsq_edge_rev(dx, dy, power, qi, qk, L, Dbv, dxb, dyb);
+    for (int r = 0; r < 4; ++r) qib[r] -= Dbv[r];
     D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
     ep[0] = D2{Dbv[0], Dbv[1]};
     ep[1] = D2{Dbv[2], Dbv[3]};
@@ -921,6 +934,194 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
   qb[i] = V4d{a.x + qib[0], a.y + qib[1], a.z + qib[2], a.w + qib[3]};
   const D2 pxy = xyb[i];
   xyb[i] = D2{pxy.x + xib, pxy.y + yib};
+}
+
+// LDS of the pull-form LSQ reverse per halo entry: q, sqb, syqb (V4d each), xy, and
+// sxxb, syyb, sxyb; then the block's slot words
+inline size_t stage_lds_lsq_pull(int hcap, int width) {
+  return (size_t)hcap * (3 * sizeof(V4d) + sizeof(D2) + 3 * sizeof(double)) + (size_t)width * STAGE_P;
+}
+
+// Pull form of k_lsq_rev + k_gather_lsq<false>: as k_inner_pull, each point adds the
+// owner terms of its own slots and gathers the neighbour terms of the edges that name it
+// (D = q_k - q_i, so no dq records), from the PullHalo staged with q, xy and the solve
+// adjoints (lam).  The qm adjoints (q_variables.F90:68-82: the max and min over the
+// point and its stencil, strict > and <, the adjoint going to the selected point) are
+// resolved by the owner's walk as in k_lsq_rev; those of a selected neighbour, at most 8
+// per point, are added atomically into qbx (summed into the stage's q adjoint and
+// cleared by k_qvar_rev), the others into the point's own q adjoint.
+template <int LPP>
+__global__ __launch_bounds__(BLOCK) void k_lsq_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
+                                                    const V4d* __restrict__ lam, const V4d* __restrict__ qmb,
+                                                    V4d* __restrict__ qb, V4d* __restrict__ qbx,
+                                                    D2* __restrict__ xyb, double power) {
+  const PointMap<LPP, true> pm(g.n);
+  const int i = pm.live ? pm.i : g.n - 1, lane = pm.lane;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int H = ph.hcap;
+  V4d* s_q = (V4d*)smem;
+  V4d* s_sqb = s_q + H;
+  V4d* s_syqb = s_sqb + H;
+  D2* s_xy = (D2*)(s_syqb + H);
+  double* s_gxx = (double*)(s_xy + H);
+  double* s_gyy = s_gxx + H;
+  double* s_gxy = s_gyy + H;
+  uint8_t* s_cl = (uint8_t*)(s_gxy + H);
+  const int b = pm.b;
+  const int h0 = ph.ptr[b], hn = ph.ptr[b + 1] - h0;
+  const int hl = hn < H ? hn : H;
+  for (int j = threadIdx.x; j < hl; j += STAGE_P * LPP) {
+    const int k = ph.ids[h0 + j];
+    s_q[j] = q[k];
+    s_xy[j] = g.xy[k];
+    s_sqb[j] = lam[3 * k];
+    s_syqb[j] = lam[3 * k + 1];
+    const V4d gb = lam[3 * k + 2];
+    s_gxx[j] = gb.x;
+    s_gyy[j] = gb.y;
+    s_gxy[j] = gb.z;
+  }
+  constexpr int V = STAGE_P / 16;
+  const uint4* src = (const uint4*)(ph.loc8 + (size_t)b * STAGE_P);
+  for (int j = threadIdx.x; j < g.width * V; j += STAGE_P * LPP) {
+    const int s = j / V;
+    ((uint4*)s_cl)[j] = src[(size_t)s * (ph.stride / 16) + (j - s * V)];
+  }
+  __syncthreads();
+  if (!pm.live) return;
+  const int li = (int)threadIdx.x / LPP;  // the block's own points lead its halo
+  const D2 pi = s_xy[li];
+  const V4d qi = s_q[li];
+  auto lam_lds = [&](int j, InnerLam& L) {
+    const V4d a = s_sqb[j], c = s_syqb[j];
+    for (int r = 0; r < 4; ++r) {
+      L.sqb[r] = a[r];
+      L.syqb[r] = c[r];
+    }
+    L.sxxb = s_gxx[j];
+    L.syyb = s_gyy[j];
+    L.sxyb = s_gxy[j];
+  };
+  auto lam_glb = [&](int k, InnerLam& L) {
+    const V4d a = lam[3 * k], c = lam[3 * k + 1], gb = lam[3 * k + 2];
+    for (int r = 0; r < 4; ++r) {
+      L.sqb[r] = a[r];
+      L.syqb[r] = c[r];
+    }
+    L.sxxb = gb.x;
+    L.syyb = gb.y;
+    L.sxyb = gb.z;
+  };
+  double aq[4] = {0, 0, 0, 0};
+  double xb = 0.0, yb = 0.0;
+  V4d qmax = qi, qmin = qi;
+  int smax[4] = {-1, -1, -1, -1}, smin[4] = {-1, -1, -1, -1};
+  const int nb = g.nbhs[i];
+  // owner terms and the qm extremes: the point's own slots (k_lsq_rev's walk)
+  {
+    InnerLam Li;
+    lam_lds(li, Li);
+    int jn = lane < nb ? s_cl[lane * STAGE_P + li] : 0;
+    for (int s = lane; s < nb; s += LPP) {
+      const int j = jn;
+      if (s + LPP < nb) jn = s_cl[(s + LPP) * STAGE_P + li];
+      D2 pk;
+      V4d qk;
+      if (j < H) {
+        pk = s_xy[j];
+        qk = s_q[j];
+      } else {
+        const int k = (int)(g.conn[(size_t)s * g.n + i] >> 4);
+        pk = g.xy[k];
+        qk = q[k];
+      }
+      for (int r = 0; r < 4; ++r) {
+        if (qk[r] > qmax[r]) {
+          qmax[r] = qk[r];
+          smax[r] = s;
+        }
+        if (qk[r] < qmin[r]) {
+          qmin[r] = qk[r];
+          smin[r] = s;
+        }
+      }
+      double Db[4], dxb, dyb;
+      lsq_edge_rev(pk.x - pi.x, pk.y - pi.y, power, qi, qk, Li, Db, dxb, dyb);
+      for (int r = 0; r < 4; ++r) aq[r] -= Db[r];
+      xb -= dxb;
+      yb -= dyb;
+    }
+  }
+  // neighbour terms: the edges that name the point (k_gather_lsq's walk)
+  const int e0 = tr.ptr[i], e1 = tr.ptr[i + 1];
+  int jn = e0 + lane < e1 ? ph.tloc8[e0 + lane] : 0;
+  for (int e = e0 + lane; e < e1; e += LPP) {
+    const int j = jn;
+    if (e + LPP < e1) jn = ph.tloc8[e + LPP];
+    D2 po;
+    V4d qo;
+    InnerLam Lo;
+    if (j < H) {
+      po = s_xy[j];
+      qo = s_q[j];
+      lam_lds(j, Lo);
+    } else {
+      const int o = tr.own[e];
+      po = g.xy[o];
+      qo = q[o];
+      lam_glb(o, Lo);
+    }
+    double Db[4], dxb, dyb;
+    lsq_edge_rev(pi.x - po.x, pi.y - po.y, power, qo, qi, Lo, Db, dxb, dyb);
+    for (int r = 0; r < 4; ++r) aq[r] += Db[r];
+    xb += dxb;
+    yb += dyb;
+  }
+  if (LPP > 1) {
+    // the point's extremes over its lanes: the larger (smaller) value, on ties the
+    // lower slot, the point itself (slot -1) first, as the sequential strict > and <
+    for (int r = 0; r < 4; ++r) {
+      for (int off = LPP / 2; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(qmax[r], off, 64);
+        const int os = __shfl_xor(smax[r], off, 64);
+        if (ov > qmax[r] || (ov == qmax[r] && os < smax[r])) {
+          qmax[r] = ov;
+          smax[r] = os;
+        }
+        const double nv = __shfl_xor(qmin[r], off, 64);
+        const int ns = __shfl_xor(smin[r], off, 64);
+        if (nv < qmin[r] || (nv == qmin[r] && ns < smin[r])) {
+          qmin[r] = nv;
+          smin[r] = ns;
+        }
+      }
+    }
+  }
+  // qm adjoints: to the point itself (its first lane), or by the lane that walked the
+  // selected slot to that neighbour's qbx
+  const V4d mxb = qmb[2 * i], mnb = qmb[2 * i + 1];
+  for (int r = 0; r < 4; ++r) {
+    if (smax[r] < 0) {
+      if (lane == 0) aq[r] += mxb[r];
+    } else if (smax[r] % LPP == lane) {
+      aadd(&qbx[(int)(g.conn[(size_t)smax[r] * g.n + i] >> 4)].x + r, mxb[r]);
+    }
+    if (smin[r] < 0) {
+      if (lane == 0) aq[r] += mnb[r];
+    } else if (smin[r] % LPP == lane) {
+      aadd(&qbx[(int)(g.conn[(size_t)smin[r] * g.n + i] >> 4)].x + r, mnb[r]);
+    }
+  }
+  if (LPP > 1) {
+    for (int r = 0; r < 4; ++r) aq[r] = group_sum<LPP>(aq[r]);
+    xb = group_sum<LPP>(xb);
+    yb = group_sum<LPP>(yb);
+    if (lane != 0) return;
+  }
+  const V4d a = qb[i];
+  qb[i] = V4d{a.x + aq[0], a.y + aq[1], a.z + aq[2], a.w + aq[3]};
+  const D2 pxy = xyb[i];
+  xyb[i] = D2{pxy.x + xb, pxy.y + yb};
 }
 
 // Gathers of the edge buffer into the neighbour's adjoints, over the transposed
@@ -1047,11 +1248,19 @@ __global__ __launch_bounds__(BLOCK) void k_gather_lsq(Geom g, Trans tr, const do
 }
 
 // reverse of eval_q_variables: pb += dq/dprim^T qb (own point)
+// qbx (non-null after the pull-form LSQ reverse): the qm adjoints it delivered, added
+// to qb here and cleared for the next stage
 __global__ __launch_bounds__(BLOCK) void k_qvar_rev(int n, const V4d* __restrict__ prim,
-                                                    const V4d* __restrict__ qb, V4d* __restrict__ pb) {
+                                                    const V4d* __restrict__ qb, V4d* __restrict__ qbx,
+                                                    V4d* __restrict__ pb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const V4d b = qb[i];
+  V4d b = qb[i];
+  if (qbx) {
+    const V4d x = qbx[i];
+    b = V4d{b.x + x.x, b.y + x.y, b.z + x.z, b.w + x.w};
+    qbx[i] = V4d{0.0, 0.0, 0.0, 0.0};
+  }
   double pv[4] = {0, 0, 0, 0};
   const double qbv[4] = {b.x, b.y, b.z, b.w};
   q_of_prim_rev(prim[i], qbv, pv);
@@ -1215,6 +1424,8 @@ struct AdjState {
   // its solve adjoints, 3 records per point, double-buffered over the inner iterations
   PullHalo ph;
   V4d* lam[2] = {nullptr, nullptr};
+  // qm adjoints the pull-form LSQ reverse delivers to selected neighbours (atomics)
+  V4d* qbx = nullptr;
   // checkpoints
   std::vector<V4d*> snap;       // prim snapshots
   std::vector<int> snap_t;      // iteration index held by each slot (-1 free)
@@ -1416,7 +1627,9 @@ static void launch_inner_pull(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
   }
   const Trans tr{A.t_ptr, A.t_own};
   const size_t lds = stage_lds_inner_pull(A.ph.hcap, c->g.width);
-  V4d* lam_next = j > 1 ? A.lam[prv] : nullptr;
+  // the next reversed step's solve adjoints: inner iteration j - 1, or the LSQ
+  // derivatives (same solve) when their reverse is in the pull form too
+  V4d* lam_next = (j > 1 || (A.lpp_lrev & LPP_PULL)) ? A.lam[prv] : nullptr;
   with_lpp_staged(lpp & ~LPP_PULL, [&]<int L>() {
     hipLaunchKernelGGL((k_inner_pull<L>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
                        din, (const V4d*)A.lam[cur], A.dqb[prv], lam_next, A.qb, A.xyb, c->p.power);
@@ -1429,6 +1642,22 @@ static void launch_gather_lsq(mf_ctx* c, AdjState& A, V4d* dqb, int lpp) {
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL((k_gather_lsq<WITH_DQ, L>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, tr,
                        (const double*)A.eb, A.qb, dqb, A.xyb);
+  });
+}
+
+// the LSQ reverse of stage s in the pull form (MF_LPP_PULL | lanes).  Its solve adjoints
+// (A.lam[0]) come from the last pull-form inner reverse, else (need_lam) from k_inner_lam
+// here.
+static void launch_lsq_pull(mf_ctx* c, AdjState& A, int s, int lpp, bool need_lam) {
+  const V4d* q = (const V4d*)A.tq[s];
+  if (need_lam)
+    hipLaunchKernelGGL(k_inner_lam, dim3(nblocks(c)), dim3(BLOCK), 0, c->stream, c->g,
+                       (const V4d*)A.tdq[s * (A.I + 1)], (const V4d*)A.dqb[0], A.lam[0]);
+  const Trans tr{A.t_ptr, A.t_own};
+  const size_t lds = stage_lds_lsq_pull(A.ph.hcap, c->g.width);
+  with_lpp_staged(lpp & ~LPP_PULL, [&]<int L>() {
+    hipLaunchKernelGGL((k_lsq_pull<L>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
+                       (const V4d*)A.lam[0], (const V4d*)A.qmb, A.qb, A.qbx, A.xyb, c->p.power);
   });
 }
 
@@ -1475,11 +1704,16 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
       }
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
-    launch_lsq_rev(c, A, s, A.lpp_lrev);
-    launch_gather_lsq<false>(c, A, (V4d*)nullptr, A.lpp_gath);
+    const bool lsq_pull = (A.lpp_lrev & LPP_PULL) != 0;
+    if (lsq_pull) {
+      launch_lsq_pull(c, A, s, A.lpp_lrev, !(A.I > 0 && (A.lpp_irev & LPP_PULL)));
+    } else {
+      launch_lsq_rev(c, A, s, A.lpp_lrev);
+      launch_gather_lsq<false>(c, A, (V4d*)nullptr, A.lpp_gath);
+    }
     // q = q(prim_s): into the stage-input adjoint
     hipLaunchKernelGGL(k_qvar_rev, dim3(nblocks(c)), bl, 0, c->stream, n, (const V4d*)A.tprim[s],
-                       (const V4d*)A.qb, A.Pstage);
+                       (const V4d*)A.qb, lsq_pull ? A.qbx : (V4d*)nullptr, A.Pstage);
     std::swap(A.Pb, A.Pstage);
   }
   // func_delta on the iteration input, then prim_old = prim (fpi_solver.F90:20-24)
@@ -1562,6 +1796,23 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
       if ((rc = time_launches(c, reps, stage, &us))) return rc;
       if (us < push) push = us, pick[2] = LPP_PULL | l;
     }
+  }
+  // the LSQ reverse: its push form's kernel and gather against the pull form (with the
+  // solve-adjoint kernel, which it needs unless a pull-form inner reverse precedes it)
+  {
+    double us_g = 0;
+    if ((rc = time_launches(c, reps, [&] { launch_gather_lsq<false>(c, A, (V4d*)nullptr, pick[3]); }, &us_g)))
+      return rc;
+    double push = best[1] + us_g;
+    for (int l : {1, 2, 4}) {
+      double us = 0;
+      if (A.ph.hcap == 0) break;
+      auto step = [&] { launch_lsq_pull(c, A, 0, LPP_PULL | l, true); };
+      if ((rc = time_launches(c, reps, step, &us))) return rc;
+      if (us < push) push = us, pick[1] = LPP_PULL | l;
+    }
+    // the timed pull launches delivered qm adjoints into qbx
+    if ((rc = adj_zero(c, A.qbx, 4 * (int64_t)n))) return rc;
   }
   A.lpp_frev = pick[0];
   A.lpp_lrev = pick[1];
@@ -1756,8 +2007,9 @@ static int adj_alloc(mf_ctx* c, AdjState& A, int snaps) {
     if ((rc = dev_alloc(c, &A.eb, (size_t)EF_FLUX * std::max(ptr[n], 1))) ||
         (rc = dev_alloc(c, &A.esel, (size_t)std::max(ptr[n], 1))))
       return rc;
-    if (I > 0 && ((rc = build_pull_halo(c, A, hc, hn, ptr, own)) || (rc = dev_alloc(c, &A.lam[0], 3 * (size_t)n)) ||
-                  (rc = dev_alloc(c, &A.lam[1], 3 * (size_t)n))))
+    if ((rc = build_pull_halo(c, A, hc, hn, ptr, own)) || (rc = dev_alloc(c, &A.lam[0], 3 * (size_t)n)) ||
+        (rc = dev_alloc(c, &A.lam[1], 3 * (size_t)n)) || (rc = dev_alloc(c, &A.qbx, (size_t)n)) ||
+        (rc = adj_zero(c, A.qbx, 4 * (int64_t)n)))
       return rc;
   }
   A.snap.assign(snaps, nullptr);
@@ -1950,8 +2202,8 @@ int mf_adjoint_forward(mf_ctx* c, int32_t first_it, int32_t iters, int32_t snaps
   if (c->lpp_rev[1]) A.lpp_lrev = c->lpp_rev[1];
   if (c->lpp_rev[2]) A.lpp_irev = c->lpp_rev[2];
   if (c->lpp_rev[3]) A.lpp_gath = c->lpp_rev[3];
-  if ((A.lpp_irev & LPP_PULL) && A.I > 0 && A.ph.hcap == 0)
-    return set_err(MF_ERR_INVALID, "the pull-form inner reverse does not fit in LDS at this stencil width");
+  if (((A.lpp_irev & LPP_PULL) && A.I > 0 && A.ph.hcap == 0) || ((A.lpp_lrev & LPP_PULL) && A.ph.hcap == 0))
+    return set_err(MF_ERR_INVALID, "the pull-form reverses do not fit in LDS at this stencil width");
   A.first_it = first_it;
   A.iters = iters;
   A.fwd_steps = A.taped = 0;

@@ -263,6 +263,44 @@ def test_adjoint_pull_inner_reverse_overflow(c1_cloud):
     assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("lpp", [1, 2, 4])
+@pytest.mark.parametrize("power,inner", [(0.0, 2), (-2.0, 2), (0.0, 0)])
+def test_adjoint_pull_lsq_reverse(small_cloud, lpp, power, inner):
+    """The pull form of the LSQ reverse (MF_LPP_PULL | lanes; the qm adjoints of a
+    selected neighbour delivered by atomic adds) gives the push form's gradient up to
+    summation order: after the push-form inner reverse, after the pull-form one (which
+    hands over the solve adjoints) and with no inner iterations."""
+    pin = {"lpp_flux_rev": 2, "lpp_gather": 2}
+    with _gpu(small_cloud, power=power, inner_iterations=inner) as a, \
+            _gpu(small_cloud, power=power, inner_iterations=inner) as b, \
+            _gpu(small_cloud, power=power, inner_iterations=inner) as c:
+        a.tuning = {**pin, "lpp_lsq_rev": 2, "lpp_inner_rev": 2}
+        b.tuning = {**pin, "lpp_lsq_rev": 64 | lpp, "lpp_inner_rev": 2}
+        c.tuning = {**pin, "lpp_lsq_rev": 64 | lpp, "lpp_inner_rev": 64 | 4}
+        Ja, xa, ya, _ = a.adjoint(10, snaps=3)
+        Jb, xb, yb, sb = b.adjoint(10, snaps=3)
+        Jc, xc, yc, sc = c.adjoint(10, snaps=3)
+    assert sb["lpp_lsq_rev"] == 64 | lpp and sc["lpp_inner_rev"] == 64 | 4
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb == Jc
+    assert np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(np.concatenate([xc, yc]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_adjoint_pull_lsq_reverse_overflow(c1_cloud):
+    """The pull-form LSQ reverse on the 9,600-point cloud, whose largest pull halos
+    exceed the LDS capacity (entries read from global memory): the push form's
+    gradient."""
+    pin = {"lpp_flux_rev": 2, "lpp_inner_rev": 2, "lpp_gather": 2}
+    with _gpu(c1_cloud) as a, _gpu(c1_cloud) as b:
+        a.tuning = {**pin, "lpp_lsq_rev": 2}
+        b.tuning = {**pin, "lpp_lsq_rev": 64 | 4}
+        Ja, xa, ya, _ = a.adjoint(6, snaps=2)
+        Jb, xb, yb, _ = b.adjoint(6, snaps=2)
+    ref = np.concatenate([xa, ya])
+    assert Ja == Jb and np.abs(np.concatenate([xb, yb]) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("snaps", [4, -1])
 def test_mixed_adjoint_close_to_fp64(small_cloud, snaps):
     """Mixed-precision adjoint: the primal sweep and the taped re-runs run the mixed
