@@ -1127,10 +1127,13 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_pull(Geom g, PullHalo ph, Trans t
 // Gathers of the edge buffer into the neighbour's adjoints, over the transposed
 // connectivity (each point reads the edges that name it; no atomics)
 template <int LPP>
+// With lam non-null it also writes each point's solve adjoint (k_inner_lam's) from its
+// completed dq adjoint and the stage's last dq (dq_last), for a pull-form reverse next.
 __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const double* __restrict__ eb,
                                                        const uint8_t* __restrict__ esel,
                                                        V4d* __restrict__ qb, V4d* __restrict__ dqb,
-                                                       V4d* __restrict__ qmb, D2* __restrict__ xyb) {
+                                                       V4d* __restrict__ qmb, D2* __restrict__ xyb,
+                                                       const V4d* __restrict__ dq_last, V4d* __restrict__ lam) {
   const int t = point_id();
   const int k = t / LPP, lane = t % LPP;
   if (k >= g.n) return;
@@ -1181,18 +1184,24 @@ __global__ __launch_bounds__(BLOCK) void k_gather_flux(Geom g, Trans tr, const d
   }
   V4d t4 = qb[k];
   qb[k] = V4d{t4.x + aq[0], t4.y + aq[1], t4.z + aq[2], t4.w + aq[3]};
+  V4d bx = dqb[2 * k], by = dqb[2 * k + 1];
   if (!g.first_order) {
-    V4d t = dqb[2 * k];
-    dqb[2 * k] = V4d{t.x + ax[0], t.y + ax[1], t.z + ax[2], t.w + ax[3]};
-    t = dqb[2 * k + 1];
-    dqb[2 * k + 1] = V4d{t.x + ay[0], t.y + ay[1], t.z + ay[2], t.w + ay[3]};
-    t = qmb[2 * k];
+    bx = V4d{bx.x + ax[0], bx.y + ax[1], bx.z + ax[2], bx.w + ax[3]};
+    by = V4d{by.x + ay[0], by.y + ay[1], by.z + ay[2], by.w + ay[3]};
+    dqb[2 * k] = bx;
+    dqb[2 * k + 1] = by;
+    V4d t = qmb[2 * k];
     qmb[2 * k] = V4d{t.x + amx[0], t.y + amx[1], t.z + amx[2], t.w + amx[3]};
     t = qmb[2 * k + 1];
     qmb[2 * k + 1] = V4d{t.x + amn[0], t.y + amn[1], t.z + amn[2], t.w + amn[3]};
   }
   const D2 p = xyb[k];
   xyb[k] = D2{p.x + xb, p.y + yb};
+  if (lam) {
+    InnerLam L;
+    inner_lam(g.lsq_geo[k], dq_last[2 * k], dq_last[2 * k + 1], bx, by, L);
+    lam_store(lam, k, L);
+  }
 }
 
 // WITH_DQ: the inner-loop payload (dq adjoint = -0.5 (delx, dely) q adjoint)
@@ -1582,11 +1591,15 @@ static void launch_flux_rev(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lp
   }
 }
 
-static void launch_gather_flux(mf_ctx* c, AdjState& A, V4d* dqb_last, int lpp) {
+// the flux gather of stage s; with_lam: also the solve adjoints of the stage's last dq
+// (into A.lam[I & 1]) for the pull-form reverse that follows
+static void launch_gather_flux(mf_ctx* c, AdjState& A, int s, V4d* dqb_last, int lpp, bool with_lam) {
   const Trans tr{A.t_ptr, A.t_own};
+  const V4d* dq_last = (const V4d*)A.tdq[s * (A.I + 1) + A.I];
+  V4d* lam = with_lam ? A.lam[A.I & 1] : nullptr;
   with_lpp(lpp, [&]<int L>() {
     hipLaunchKernelGGL(k_gather_flux<L>, lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, tr,
-                       (const double*)A.eb, (const uint8_t*)A.esel, A.qb, dqb_last, A.qmb, A.xyb);
+                       (const double*)A.eb, (const uint8_t*)A.esel, A.qb, dqb_last, A.qmb, A.xyb, dq_last, lam);
   });
 }
 
@@ -1611,16 +1624,14 @@ static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
 }
 
 // inner iteration j of stage s in the pull form (MF_LPP_PULL | lanes): one launch, no
-// edge buffer and no gather
-// inner iteration j of stage s in the pull form (MF_LPP_PULL | lanes): one launch, no
 // edge buffer and no gather.  It reads the solve adjoints of iteration j (A.lam[j & 1]:
-// k_inner_lam's for j = I, else the previous pull launch's) and, for j > 1, writes those
-// of iteration j - 1.
-static void launch_inner_pull(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
+// for j = I the flux gather's, or k_inner_lam's here when need_lam; else the previous
+// pull launch's) and, for j > 1, writes those of iteration j - 1.
+static void launch_inner_pull(mf_ctx* c, AdjState& A, int s, int j, int lpp, bool need_lam) {
   const int cur = j & 1, prv = (j - 1) & 1;
   const V4d* q = (const V4d*)A.tq[s];
   const V4d* din = (const V4d*)A.tdq[s * (A.I + 1) + j - 1];
-  if (j == A.I) {
+  if (j == A.I && need_lam) {
     const V4d* dout = (const V4d*)A.tdq[s * (A.I + 1) + j];
     hipLaunchKernelGGL(k_inner_lam, dim3(nblocks(c)), dim3(BLOCK), 0, c->stream, c->g, dout,
                        (const V4d*)A.dqb[cur], A.lam[cur]);
@@ -1693,20 +1704,22 @@ static int adj_reverse_iteration(mf_ctx* c, AdjState& A) {
     V4d* dqb_last = A.dqb[A.I & 1];
     // flux: frb -> q, dq, qm (written), delta, xy, nrm; neighbours' parts gathered
     launch_flux_rev(c, A, s, dqb_last, A.lpp_frev);
-    launch_gather_flux(c, A, dqb_last, A.lpp_gath);
+    // the pull-form reverse that follows the gather takes its solve adjoints from it
+    const bool inner_pull = A.I > 0 && (A.lpp_irev & LPP_PULL);
+    const bool lsq_pull = (A.lpp_lrev & LPP_PULL) != 0;
+    launch_gather_flux(c, A, s, dqb_last, A.lpp_gath, inner_pull || (A.I == 0 && lsq_pull));
     // inner iterations, last first: dq^(j) adjoint -> dq^(j-1)
     for (int j = A.I; j >= 1; --j) {
       if (A.lpp_irev & LPP_PULL) {
-        launch_inner_pull(c, A, s, j, A.lpp_irev);
+        launch_inner_pull(c, A, s, j, A.lpp_irev, false);
       } else {
         launch_inner_rev(c, A, s, j, A.lpp_irev);
         launch_gather_lsq<true>(c, A, A.dqb[(j - 1) & 1], A.lpp_gath);
       }
     }
     // LSQ: dq^(0), qm adjoints -> q, xy
-    const bool lsq_pull = (A.lpp_lrev & LPP_PULL) != 0;
     if (lsq_pull) {
-      launch_lsq_pull(c, A, s, A.lpp_lrev, !(A.I > 0 && (A.lpp_irev & LPP_PULL)));
+      launch_lsq_pull(c, A, s, A.lpp_lrev, A.I > 0 && !inner_pull);
     } else {
       launch_lsq_rev(c, A, s, A.lpp_lrev);
       launch_gather_lsq<false>(c, A, (V4d*)nullptr, A.lpp_gath);
@@ -1774,7 +1787,7 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
     double us[4] = {0, 0, 0, 0};
     if ((rc = time_launches(c, reps, [&] { launch_flux_rev(c, A, 0, A.dqb[A.I & 1], lpp); }, &us[0])) ||
         (rc = time_launches(c, reps, [&] { launch_lsq_rev(c, A, 0, lpp); }, &us[1])) ||
-        (rc = time_launches(c, reps, [&] { launch_gather_flux(c, A, A.dqb[A.I & 1], lpp); }, &us[3])))
+        (rc = time_launches(c, reps, [&] { launch_gather_flux(c, A, 0, A.dqb[A.I & 1], lpp, false); }, &us[3])))
       return rc;
     if (A.I > 0 && (rc = time_launches(c, reps, [&] { launch_inner_rev(c, A, 0, 1, lpp); }, &us[2]))) return rc;
     for (int k = 0; k < 4; ++k)
@@ -1791,7 +1804,7 @@ static int adj_tune(mf_ctx* c, AdjState& A) {
       double us = 0;
       if (A.ph.hcap == 0) break;
       auto stage = [&] {
-        for (int j = A.I; j >= 1; --j) launch_inner_pull(c, A, 0, j, LPP_PULL | l);
+        for (int j = A.I; j >= 1; --j) launch_inner_pull(c, A, 0, j, LPP_PULL | l, true);
       };
       if ((rc = time_launches(c, reps, stage, &us))) return rc;
       if (us < push) push = us, pick[2] = LPP_PULL | l;
