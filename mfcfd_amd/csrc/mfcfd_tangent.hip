@@ -4,11 +4,14 @@
 // pointd%x/y, fpi_solver_d.F90:27-85 carries the tangents through every routine of an
 // iteration, objective_function_d the cost's).  Every quantity is a dual number Dd
 // (value, directional derivative along the seeded coordinate direction (vx, vy)); the
-// kernels follow the reference routines one point per thread, with the device
-// header's generic point and edge routines (reconstruction and limiter, q-tilde ->
-// primitive, the kinetic split fluxes, the conserved states) instantiated for Dd.  The
-// stencil split is the context's (fixed, as the reference's generate_connectivity is
-// not differentiated); min_dist and so the limiter's epsilon are constants.
+// kernels follow the reference routines, with the device header's generic point and
+// edge routines (reconstruction and limiter, q-tilde -> primitive, the kinetic split
+// fluxes, the conserved states) instantiated for Dd.  The stencil kernels (inner
+// iteration, flux residual) split each point's edges over lanes, and the flux residual
+// takes the primal kernels' linear form (each edge's two ends evaluated once, its flux
+// differences times its half-stencils' solve coefficients, here with their tangents).
+// The stencil split is the context's (fixed, as the reference's generate_connectivity
+// is not differentiated); min_dist and so the limiter's epsilon are constants.
 //
 // Included by mfcfd.hip (it uses the context).  FP64 arithmetic throughout; a mixed
 // context's state is widened at the start.
@@ -181,15 +184,24 @@ __global__ __launch_bounds__(BLOCK) void k_tn_lsq(TanArgs a, const V4t* __restri
   qm[2 * i + 1] = mn;
 }
 
-// q_variables.F90:115-227: one Jacobi sweep of the defect correction, din -> dout
+// sum of a dual number over a point's LPP lanes
+template <int LPP>
+__device__ inline Dd group_sum_d(Dd v) {
+  return Dd(group_sum<LPP>(v.v), group_sum<LPP>(v.d));
+}
+
+// q_variables.F90:115-227: one Jacobi sweep of the defect correction, din -> dout,
+// LPP lanes per point (their sums combined before the solve)
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __restrict__ q,
                                                     const V4t* __restrict__ din, V4t* __restrict__ dout) {
-  const int i = tan_point();
-  if (i >= a.n) return;
+  const int t = tan_point();
+  const int i = t / LPP, lane = t % LPP;
+  if (i >= a.n) return;  // whole lane groups
   const D2t pi = a.xy[i];
   const V4t qi = q[i], dxi = din[2 * i], dyi = din[2 * i + 1];
   Dd sxx, syy, sxy, sxq[4], syq[4];
-  for (int s = 0; s < a.nbhs[i]; ++s) {
+  for (int s = lane; s < a.nbhs[i]; s += LPP) {
     const int k = (int)(a.conn[(size_t)s * a.n + i] >> 4);
     const V4t qk = q[k], dxk = din[2 * k], dyk = din[2 * k + 1];
     const Dd delx = a.xy[k].x - pi.x, dely = a.xy[k].y - pi.y;
@@ -204,6 +216,16 @@ __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __rest
       syq[r] = syq[r] + (w * dely * (t2 - t1));
     }
   }
+  if (LPP > 1) {
+    sxx = group_sum_d<LPP>(sxx);
+    syy = group_sum_d<LPP>(syy);
+    sxy = group_sum_d<LPP>(sxy);
+    for (int r = 0; r < 4; ++r) {
+      sxq[r] = group_sum_d<LPP>(sxq[r]);
+      syq[r] = group_sum_d<LPP>(syq[r]);
+    }
+    if (lane != 0) return;
+  }
   const Dd obd = 1.0 / (sxx * syy - sxy * sxy);
   V4t tx, ty;
   for (int r = 0; r < 4; ++r) {
@@ -214,97 +236,122 @@ __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __rest
   dout[2 * i + 1] = ty;
 }
 
-// The kinetic split flux `kind` of one end (0 Gxp, 1 Gxn, 2 Gyp, 3 Gyn, 4 GxI, 5 GxII,
-// 6 GxIII, 7 GxIV: split_fluxes.F90, quadrant_fluxes.F90)
-__device__ inline void tn_flux(int kind, const Kin<Dd>& k, Dd G[4]) {
-  switch (kind) {
-    case 0: flux_gx(k, true, G); break;
-    case 1: flux_gx(k, false, G); break;
-    case 2: flux_gy(k, true, G); break;
-    case 3: flux_gy(k, false, G); break;
-    case 4: flux_quad(k, false, false, G); break;
-    case 5: flux_quad(k, true, false, G); break;
-    case 6: flux_quad(k, true, true, G); break;
-    default: flux_quad(k, false, true, G); break;
-  }
-}
-
-// One half-stencil's least-squares flux derivative (wall_fluxes.F90:20-292,
-// outer_fluxes.F90:18-300, interior_fluxes.F90:19-404): the half's neighbours are the
-// slots with bit `hbit`, in connectivity order; both ends reconstructed with their
-// own limiter, the flux `kind` at both ends, the t-derivative (solve_y false) or the
-// n-derivative of the 2x2 weighted least squares
-__device__ inline void tn_dG(const TanArgs& a, int i, int hbit, int kind, bool solve_y, const V4t* __restrict__ q,
-                             const V4t* __restrict__ dq, const V4t* __restrict__ qm, Dd G[4]) {
+// Flux residual of one point (flux_residual.F90:14-73 over wall_fluxes.F90:20-292,
+// outer_fluxes.F90:18-300, interior_fluxes.F90:19-404), this lane's share of its edges.
+// The reference solves each half-stencil's weighted 2x2 least squares after summing
+// its edges; the solve is linear in the flux sums, so, as in the primal kernels, each
+// edge adds its end-flux difference times its own coefficient, here carrying the
+// coefficients' tangents: a first walk sums each half's geometry, then each edge's two
+// ends are evaluated once (limited reconstruction, q-tilde -> primitive) with the x and
+// y split fluxes of the halves the neighbour is in (both halves on a tie).
+template <int CLS, int LPP>
+__device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4t* __restrict__ q,
+                                     const V4t* __restrict__ dq, const V4t* __restrict__ qm, Dd acc[4]) {
   const D2t pi = a.xy[i], ni = a.nrm[i];
-  const Dd tx = ni.y, ty = -ni.x;
+  const Dd nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
+  const int nb = a.nbhs[i];
+  auto offset = [&](int k, Dd& delx, Dd& dely, Dd& dels, Dd& deln, Dd& dsw, Dd& dnw) {
+    delx = a.xy[k].x - pi.x;
+    dely = a.xy[k].y - pi.y;
+    dels = delx * tx + dely * ty;
+    deln = delx * nx + dely * ny;
+    const Dd w = m_pow(m_sqrt(dels * dels + deln * deln), a.power);
+    dsw = dels * w;
+    dnw = deln * w;
+  };
+  // per half (xpos, xneg, ypos, yneg): sum ds^2 w, dn^2 w, ds dn w
+  Dd hxx[4], hyy[4], hxy[4];
+  for (int s = lane; s < nb; s += LPP) {
+    const uint32_t word = a.conn[(size_t)s * a.n + i];
+    Dd delx, dely, dels, deln, dsw, dnw;
+    offset((int)(word >> 4), delx, dely, dels, deln, dsw, dnw);
+    for (int d = 0; d < 4; ++d) {
+      if (!(word & (1u << d))) continue;
+      hxx[d] = hxx[d] + dels * dsw;
+      hyy[d] = hyy[d] + deln * dnw;
+      hxy[d] = hxy[d] + dels * dnw;
+    }
+  }
+  Dd c1[4], obd[4];
+  for (int d = 0; d < 4; ++d) {
+    if (LPP > 1) {
+      hxx[d] = group_sum_d<LPP>(hxx[d]);
+      hyy[d] = group_sum_d<LPP>(hyy[d]);
+      hxy[d] = group_sum_d<LPP>(hxy[d]);
+    }
+    c1[d] = d < 2 ? hyy[d] : hxx[d];
+    obd[d] = 1.0 / (hxx[d] * hyy[d] - hxy[d] * hxy[d]);
+  }
+  // half d's t-derivative coefficient (x halves) or n-derivative coefficient (y halves)
+  auto coef = [&](int d, const Dd& dsw, const Dd& dnw) {
+    return d < 2 ? (dsw * c1[d] - dnw * hxy[d]) * obd[d] : (dnw * c1[d] - dsw * hxy[d]) * obd[d];
+  };
   const V4t qi = q[i], dxi = dq[2 * i], dyi = dq[2 * i + 1], mxi = qm[2 * i], mni = qm[2 * i + 1];
   const Dd ei(a.eps[i]);
-  Dd sxx, syy, sxy, sxf[4], syf[4];
-  for (int s = 0; s < a.nbhs[i]; ++s) {
+  for (int s = lane; s < nb; s += LPP) {
     const uint32_t word = a.conn[(size_t)s * a.n + i];
-    if (!(word & (uint32_t)hbit)) continue;
     const int k = (int)(word >> 4);
-    const Dd delx = a.xy[k].x - pi.x, dely = a.xy[k].y - pi.y;
-    const Dd dels = delx * tx + dely * ty;
-    const Dd deln = delx * ni.x + dely * ni.y;
-    const Dd w = m_pow(m_sqrt(dels * dels + deln * deln), a.power);
-    const Dd dsw = dels * w, dnw = deln * w;
-    sxx = sxx + dels * dsw;
-    syy = syy + deln * dnw;
-    sxy = sxy + dels * dnw;
-    Dd qti[4], qtk[4];
-    const V4t qk = q[k];
-    if (a.first_order) {
-      for (int r = 0; r < 4; ++r) {
-        qti[r] = qi[r];
-        qtk[r] = qk[r];
+    const unsigned m = word & 15u;
+    Dd delx, dely, dels, deln, dsw, dnw;
+    offset(k, delx, dely, dels, deln, dsw, dnw);
+    const bool xp = (m & XPOS) != 0u;
+    const bool tie_x = xp && (m & XNEG) != 0u;
+    // interior: y half from the mask; wall: yneg (every neighbour); outer: ypos
+    const bool yp = (CLS == INTERIOR) ? (m & YPOS) != 0u : (CLS == OUTER);
+    const bool tie_y = (CLS == INTERIOR) && yp && (m & YNEG) != 0u;
+    const Dd cx = coef(xp ? 0 : 1, dsw, dnw), cy = coef(yp ? 2 : 3, dsw, dnw);
+    const Dd cx2 = tie_x ? coef(1, dsw, dnw) : Dd(), cy2 = tie_y ? coef(3, dsw, dnw) : Dd();
+    // the ends one after the other (own: -, neighbour: +)
+#pragma unroll 1
+    for (int e = 0; e < 2; ++e) {
+      Dd qt[4];
+      if (a.first_order) {
+        const V4t qe = e ? q[k] : qi;
+        for (int r = 0; r < 4; ++r) qt[r] = qe[r];
+      } else if (e) {
+        reconstruct<Dd>(q[k], dq[2 * k], dq[2 * k + 1], qm[2 * k], qm[2 * k + 1], Dd(a.eps[k]), delx, dely, qt);
+      } else {
+        reconstruct<Dd>(qi, dxi, dyi, mxi, mni, ei, delx, dely, qt);
       }
-    } else {
-      reconstruct<Dd>(qi, dxi, dyi, mxi, mni, ei, delx, dely, qti);
-      reconstruct<Dd>(qk, dq[2 * k], dq[2 * k + 1], qm[2 * k], qm[2 * k + 1], Dd(a.eps[k]), delx, dely, qtk);
-    }
-    Dd Gi[4], Gk[4];
-    tn_flux(kind, kin_of_qtilde<Dd>(qti, ni.x, ni.y), Gi);
-    tn_flux(kind, kin_of_qtilde<Dd>(qtk, ni.x, ni.y), Gk);
-    for (int r = 0; r < 4; ++r) {
-      sxf[r] = sxf[r] + (Gk[r] - Gi[r]) * dsw;
-      syf[r] = syf[r] + (Gk[r] - Gi[r]) * dnw;
+      const Kin<Dd> kn = kin_of_qtilde<Dd>(qt, nx, ny);
+      const double sg = e ? 1.0 : -1.0;
+      Dd Gx[4], Gy[4];
+      end_fluxes<CLS, Dd>(kn, xp, yp, Gx, Gy);
+      for (int r = 0; r < 4; ++r) acc[r] = acc[r] + sg * (Gx[r] * cx + Gy[r] * cy);
+      if (tie_x || tie_y) {
+        // the neighbour also belongs to the xneg (yneg) half, whose flux has the
+        // opposite split
+        end_fluxes<CLS, Dd>(kn, false, false, Gx, Gy);
+        for (int r = 0; r < 4; ++r) acc[r] = acc[r] + sg * (Gx[r] * cx2 + Gy[r] * cy2);
+      }
     }
   }
-  const Dd obd = 1.0 / (sxx * syy - sxy * sxy);
-  for (int r = 0; r < 4; ++r)
-    G[r] = solve_y ? (syf[r] * sxx - sxf[r] * sxy) * obd : (sxf[r] * syy - syf[r] * sxy) * obd;
+  if (LPP > 1)
+    for (int r = 0; r < 4; ++r) acc[r] = group_sum_d<LPP>(acc[r]);
 }
 
-// flux_residual.F90:14-73 then state_update.F90:10-141 (stage rk) of every point; the
-// new prim replaces the old in place (no point reads another's prim here)
+// flux_residual.F90:14-73 then state_update.F90:10-141 (stage rk) of every point, LPP
+// lanes per point; the new prim replaces the old in place (no point reads another's
+// prim here)
+template <int LPP>
 __global__ __launch_bounds__(BLOCK) void k_tn_flux_update(TanArgs a, const V4t* __restrict__ q,
                                                           const V4t* __restrict__ dq, const V4t* __restrict__ qm,
                                                           const Dd* __restrict__ delta, V4t* __restrict__ prim,
                                                           const V4t* __restrict__ pold, int rk, double euler,
                                                           Freestream fs) {
-  const int i = tan_point();
-  if (i >= a.n) return;
+  const int t = tan_point();
+  const int i = t / LPP, lane = t % LPP;
+  if (i >= a.n) return;  // whole lane groups
   const int cls = i < a.nw ? WALL : (i < a.nw + a.no ? OUTER : INTERIOR);
-  Dd fr[4], G1[4], G2[4], G3[4], G4[4];
-  if (cls == WALL) {
-    tn_dG(a, i, 1, 5, false, q, dq, qm, G1);  // xpos: GxII
-    tn_dG(a, i, 2, 4, false, q, dq, qm, G2);  // xneg: GxI
-    tn_dG(a, i, 8, 3, true, q, dq, qm, G4);   // yneg (every neighbour): Gyn
-    for (int r = 0; r < 4; ++r) fr[r] = 2.0 * delta[i] * (G1[r] + G2[r] + G4[r]);
-  } else if (cls == OUTER) {
-    tn_dG(a, i, 1, 6, false, q, dq, qm, G1);  // xpos: GxIII
-    tn_dG(a, i, 2, 7, false, q, dq, qm, G2);  // xneg: GxIV
-    tn_dG(a, i, 4, 2, true, q, dq, qm, G3);   // ypos (every neighbour): Gyp
-    for (int r = 0; r < 4; ++r) fr[r] = delta[i] * (G1[r] + G2[r] + G3[r]);
-  } else {
-    tn_dG(a, i, 1, 0, false, q, dq, qm, G1);
-    tn_dG(a, i, 2, 1, false, q, dq, qm, G2);
-    tn_dG(a, i, 4, 2, true, q, dq, qm, G3);
-    tn_dG(a, i, 8, 3, true, q, dq, qm, G4);
-    for (int r = 0; r < 4; ++r) fr[r] = delta[i] * (G1[r] + G2[r] + G3[r] + G4[r]);
-  }
+  Dd acc[4];
+  if (cls == WALL) tn_flux_point<WALL, LPP>(a, i, lane, q, dq, qm, acc);
+  else if (cls == OUTER) tn_flux_point<OUTER, LPP>(a, i, lane, q, dq, qm, acc);
+  else tn_flux_point<INTERIOR, LPP>(a, i, lane, q, dq, qm, acc);
+  if (lane != 0) return;
+  // flux_residual.F90:36 (wall: 2 delta), :50 and :68 (delta)
+  const Dd scale = cls == WALL ? 2.0 * delta[i] : delta[i];
+  Dd fr[4];
+  for (int r = 0; r < 4; ++r) fr[r] = scale * acc[r];
   const Dd nx = a.nrm[i].x, ny = a.nrm[i].y;
   Dd U[4], Uo[4];
   if (cls == OUTER) {
@@ -325,10 +372,10 @@ __global__ __launch_bounds__(BLOCK) void k_tn_flux_update(TanArgs a, const V4t* 
   U[2] = u3r * ny - u2r * nx;
   V4t out;
   out.x = U[0];
-  const Dd t = 1.0 / U[0];
-  out.y = U[1] * t;
-  out.z = U[2] * t;
-  out.w = 0.4 * (U[3] - (0.5 * t) * (U[1] * U[1] + U[2] * U[2]));
+  const Dd tt = 1.0 / U[0];
+  out.y = U[1] * tt;
+  out.z = U[2] * tt;
+  out.w = 0.4 * (U[3] - (0.5 * tt) * (U[1] * U[1] + U[2] * U[2]));
   prim[i] = out;
 }
 
@@ -439,7 +486,10 @@ extern "C" int mf_tangent(mf_ctx* c, int32_t iters, const double* vx, const doub
     HIPCHK(hipMemcpy(T.xy, txy.data(), n * sizeof(D2t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(T.nrm, tn.data(), n * sizeof(D2t), hipMemcpyHostToDevice));
   }
+  // the stencil kernels split each point's edges over kTanLanes lanes
+  constexpr int kTanLanes = 4;
   const dim3 bl(BLOCK), gr((n + BLOCK - 1) / BLOCK);
+  const dim3 gr_l((unsigned)(((int64_t)n * kTanLanes + BLOCK - 1) / BLOCK));
   if (T.nbd > 0)
     hipLaunchKernelGGL(k_tn_normals, dim3((T.nbd + BLOCK - 1) / BLOCK), bl, 0, c->stream, T.nbd, T.bd_m, T.bd_l,
                        T.bd_r, (const D2t*)T.xy, T.nrm);
@@ -459,9 +509,11 @@ extern "C" int mf_tangent(mf_ctx* c, int32_t iters, const double* vx, const doub
       hipLaunchKernelGGL(k_tn_lsq, gr, bl, 0, c->stream, a, (const V4t*)T.q, T.dq[0], T.qm);
       int d = 0;
       for (int j = 0; j < I; ++j, d ^= 1)
-        hipLaunchKernelGGL(k_tn_inner, gr, bl, 0, c->stream, a, (const V4t*)T.q, (const V4t*)T.dq[d], T.dq[d ^ 1]);
-      hipLaunchKernelGGL(k_tn_flux_update, gr, bl, 0, c->stream, a, (const V4t*)T.q, (const V4t*)T.dq[d],
-                         (const V4t*)T.qm, (const Dd*)T.delta, T.prim, (const V4t*)T.pold, rk, c->p.euler, c->fs);
+        hipLaunchKernelGGL(k_tn_inner<kTanLanes>, gr_l, bl, 0, c->stream, a, (const V4t*)T.q, (const V4t*)T.dq[d],
+                           T.dq[d ^ 1]);
+      hipLaunchKernelGGL(k_tn_flux_update<kTanLanes>, gr_l, bl, 0, c->stream, a, (const V4t*)T.q,
+                         (const V4t*)T.dq[d], (const V4t*)T.qm, (const Dd*)T.delta, T.prim, (const V4t*)T.pold, rk,
+                         c->p.euler, c->fs);
     }
     HIPCHK(hipGetLastError());
   }
