@@ -145,25 +145,51 @@ __global__ __launch_bounds__(BLOCK) void k_tn_delta(TanArgs a, const V4t* __rest
   delta[i] = min_delt;
 }
 
+// LSQ weight dist^power of an edge (q_variables.F90:89-91); POWER0: power = 0, weight 1
+// with no pow evaluation (the library's pow is long and register-hungry)
+template <bool POWER0>
+__device__ inline Dd tn_weight(const Dd& r2, double power) {
+  if constexpr (POWER0) return Dd(1.0);
+  else return m_pow(m_sqrt(r2), power);
+}
+
+// sum of a dual number over a point's LPP lanes
+template <int LPP>
+__device__ inline Dd group_sum_d(Dd v) {
+  return Dd(group_sum<LPP>(v.v), group_sum<LPP>(v.d));
+}
+
 // q_variables.F90:41-112 eval_q_derivatives: qm over {i} + stencil (strict > and <),
-// distance-weighted least squares, Cramer's rule
+// distance-weighted least squares, Cramer's rule; LPP lanes per point, their extremes
+// combined as the sequential walk selects them (the larger value; on ties the lower
+// slot, the point itself first), so each qm carries the selected point's tangent
+template <int LPP, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_tn_lsq(TanArgs a, const V4t* __restrict__ q, V4t* __restrict__ dq,
                                                   V4t* __restrict__ qm) {
-  const int i = tan_point();
-  if (i >= a.n) return;
+  const int t = tan_point();
+  const int i = t / LPP, lane = t % LPP;
+  if (i >= a.n) return;  // whole lane groups
   const D2t pi = a.xy[i];
   const V4t qi = q[i];
   V4t mx = qi, mn = qi;
+  int smx[4] = {-1, -1, -1, -1}, smn[4] = {-1, -1, -1, -1};
   Dd sxx, syy, sxy, sxq[4], syq[4];
-  for (int s = 0; s < a.nbhs[i]; ++s) {
+#pragma unroll 1
+  for (int s = lane; s < a.nbhs[i]; s += LPP) {
     const int k = (int)(a.conn[(size_t)s * a.n + i] >> 4);
     const V4t qk = q[k];
     for (int r = 0; r < 4; ++r) {
-      if (qk[r] > mx[r]) mx[r] = qk[r];
-      if (qk[r] < mn[r]) mn[r] = qk[r];
+      if (qk[r] > mx[r]) {
+        mx[r] = qk[r];
+        smx[r] = s;
+      }
+      if (qk[r] < mn[r]) {
+        mn[r] = qk[r];
+        smn[r] = s;
+      }
     }
     const Dd delx = a.xy[k].x - pi.x, dely = a.xy[k].y - pi.y;
-    const Dd w = m_pow(m_sqrt(delx * delx + dely * dely), a.power);
+    const Dd w = tn_weight<POWER0>(delx * delx + dely * dely, a.power);
     sxx = sxx + delx * delx * w;
     syy = syy + dely * dely * w;
     sxy = sxy + delx * dely * w;
@@ -171,6 +197,30 @@ __global__ __launch_bounds__(BLOCK) void k_tn_lsq(TanArgs a, const V4t* __restri
       sxq[r] = sxq[r] + w * delx * (qk[r] - qi[r]);
       syq[r] = syq[r] + w * dely * (qk[r] - qi[r]);
     }
+  }
+  if (LPP > 1) {
+    sxx = group_sum_d<LPP>(sxx);
+    syy = group_sum_d<LPP>(syy);
+    sxy = group_sum_d<LPP>(sxy);
+    for (int r = 0; r < 4; ++r) {
+      sxq[r] = group_sum_d<LPP>(sxq[r]);
+      syq[r] = group_sum_d<LPP>(syq[r]);
+      for (int off = LPP / 2; off > 0; off >>= 1) {
+        const Dd ov(__shfl_xor(mx[r].v, off, 64), __shfl_xor(mx[r].d, off, 64));
+        const int os = __shfl_xor(smx[r], off, 64);
+        if (ov.v > mx[r].v || (ov.v == mx[r].v && os < smx[r])) {
+          mx[r] = ov;
+          smx[r] = os;
+        }
+        const Dd nv(__shfl_xor(mn[r].v, off, 64), __shfl_xor(mn[r].d, off, 64));
+        const int ns = __shfl_xor(smn[r], off, 64);
+        if (nv.v < mn[r].v || (nv.v == mn[r].v && ns < smn[r])) {
+          mn[r] = nv;
+          smn[r] = ns;
+        }
+      }
+    }
+    if (lane != 0) return;
   }
   const Dd obd = 1.0 / (sxx * syy - sxy * sxy);
   V4t tx, ty;
@@ -184,15 +234,9 @@ __global__ __launch_bounds__(BLOCK) void k_tn_lsq(TanArgs a, const V4t* __restri
   qm[2 * i + 1] = mn;
 }
 
-// sum of a dual number over a point's LPP lanes
-template <int LPP>
-__device__ inline Dd group_sum_d(Dd v) {
-  return Dd(group_sum<LPP>(v.v), group_sum<LPP>(v.d));
-}
-
 // q_variables.F90:115-227: one Jacobi sweep of the defect correction, din -> dout,
 // LPP lanes per point (their sums combined before the solve)
-template <int LPP>
+template <int LPP, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __restrict__ q,
                                                     const V4t* __restrict__ din, V4t* __restrict__ dout) {
   const int t = tan_point();
@@ -201,11 +245,13 @@ __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __rest
   const D2t pi = a.xy[i];
   const V4t qi = q[i], dxi = din[2 * i], dyi = din[2 * i + 1];
   Dd sxx, syy, sxy, sxq[4], syq[4];
+  // one edge at a time: its neighbour records (24 doubles) live only for that edge
+#pragma unroll 1
   for (int s = lane; s < a.nbhs[i]; s += LPP) {
     const int k = (int)(a.conn[(size_t)s * a.n + i] >> 4);
     const V4t qk = q[k], dxk = din[2 * k], dyk = din[2 * k + 1];
     const Dd delx = a.xy[k].x - pi.x, dely = a.xy[k].y - pi.y;
-    const Dd w = m_pow(m_sqrt(delx * delx + dely * dely), a.power);
+    const Dd w = tn_weight<POWER0>(delx * delx + dely * dely, a.power);
     sxx = sxx + delx * delx * w;
     syy = syy + dely * dely * w;
     sxy = sxy + delx * dely * w;
@@ -244,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_tn_inner(TanArgs a, const V4t* __rest
 // coefficients' tangents: a first walk sums each half's geometry, then each edge's two
 // ends are evaluated once (limited reconstruction, q-tilde -> primitive) with the x and
 // y split fluxes of the halves the neighbour is in (both halves on a tie).
-template <int CLS, int LPP>
+template <int CLS, int LPP, bool POWER0>
 __device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4t* __restrict__ q,
                                      const V4t* __restrict__ dq, const V4t* __restrict__ qm, Dd acc[4]) {
   const D2t pi = a.xy[i], ni = a.nrm[i];
@@ -255,7 +301,7 @@ __device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4
     dely = a.xy[k].y - pi.y;
     dels = delx * tx + dely * ty;
     deln = delx * nx + dely * ny;
-    const Dd w = m_pow(m_sqrt(dels * dels + deln * deln), a.power);
+    const Dd w = tn_weight<POWER0>(dels * dels + deln * deln, a.power);
     dsw = dels * w;
     dnw = deln * w;
   };
@@ -282,9 +328,14 @@ __device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4
     c1[d] = d < 2 ? hyy[d] : hxx[d];
     obd[d] = 1.0 / (hxx[d] * hyy[d] - hxy[d] * hxy[d]);
   }
-  // half d's t-derivative coefficient (x halves) or n-derivative coefficient (y halves)
-  auto coef = [&](int d, const Dd& dsw, const Dd& dnw) {
-    return d < 2 ? (dsw * c1[d] - dnw * hxy[d]) * obd[d] : (dnw * c1[d] - dsw * hxy[d]) * obd[d];
+  // the t-derivative coefficient of x half 0 or 1 (first: xpos) and the n-derivative
+  // coefficient of y half 2 or 3 (first: ypos); selects, not a runtime array index, so
+  // the half sums stay in registers
+  auto coef_x = [&](bool first, const Dd& dsw, const Dd& dnw) {
+    return (dsw * (first ? c1[0] : c1[1]) - dnw * (first ? hxy[0] : hxy[1])) * (first ? obd[0] : obd[1]);
+  };
+  auto coef_y = [&](bool first, const Dd& dsw, const Dd& dnw) {
+    return (dnw * (first ? c1[2] : c1[3]) - dsw * (first ? hxy[2] : hxy[3])) * (first ? obd[2] : obd[3]);
   };
   const V4t qi = q[i], dxi = dq[2 * i], dyi = dq[2 * i + 1], mxi = qm[2 * i], mni = qm[2 * i + 1];
   const Dd ei(a.eps[i]);
@@ -299,8 +350,8 @@ __device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4
     // interior: y half from the mask; wall: yneg (every neighbour); outer: ypos
     const bool yp = (CLS == INTERIOR) ? (m & YPOS) != 0u : (CLS == OUTER);
     const bool tie_y = (CLS == INTERIOR) && yp && (m & YNEG) != 0u;
-    const Dd cx = coef(xp ? 0 : 1, dsw, dnw), cy = coef(yp ? 2 : 3, dsw, dnw);
-    const Dd cx2 = tie_x ? coef(1, dsw, dnw) : Dd(), cy2 = tie_y ? coef(3, dsw, dnw) : Dd();
+    const Dd cx = coef_x(xp, dsw, dnw), cy = coef_y(yp, dsw, dnw);
+    const Dd cx2 = tie_x ? coef_x(false, dsw, dnw) : Dd(), cy2 = tie_y ? coef_y(false, dsw, dnw) : Dd();
     // the ends one after the other (own: -, neighbour: +)
 #pragma unroll 1
     for (int e = 0; e < 2; ++e) {
@@ -333,7 +384,7 @@ __device__ inline void tn_flux_point(const TanArgs& a, int i, int lane, const V4
 // flux_residual.F90:14-73 then state_update.F90:10-141 (stage rk) of every point, LPP
 // lanes per point; the new prim replaces the old in place (no point reads another's
 // prim here)
-template <int LPP>
+template <int LPP, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_tn_flux_update(TanArgs a, const V4t* __restrict__ q,
                                                           const V4t* __restrict__ dq, const V4t* __restrict__ qm,
                                                           const Dd* __restrict__ delta, V4t* __restrict__ prim,
@@ -344,9 +395,9 @@ __global__ __launch_bounds__(BLOCK) void k_tn_flux_update(TanArgs a, const V4t* 
   if (i >= a.n) return;  // whole lane groups
   const int cls = i < a.nw ? WALL : (i < a.nw + a.no ? OUTER : INTERIOR);
   Dd acc[4];
-  if (cls == WALL) tn_flux_point<WALL, LPP>(a, i, lane, q, dq, qm, acc);
-  else if (cls == OUTER) tn_flux_point<OUTER, LPP>(a, i, lane, q, dq, qm, acc);
-  else tn_flux_point<INTERIOR, LPP>(a, i, lane, q, dq, qm, acc);
+  if (cls == WALL) tn_flux_point<WALL, LPP, POWER0>(a, i, lane, q, dq, qm, acc);
+  else if (cls == OUTER) tn_flux_point<OUTER, LPP, POWER0>(a, i, lane, q, dq, qm, acc);
+  else tn_flux_point<INTERIOR, LPP, POWER0>(a, i, lane, q, dq, qm, acc);
   if (lane != 0) return;
   // flux_residual.F90:36 (wall: 2 delta), :50 and :68 (delta)
   const Dd scale = cls == WALL ? 2.0 * delta[i] : delta[i];
@@ -506,14 +557,18 @@ extern "C" int mf_tangent(mf_ctx* c, int32_t iters, const double* vx, const doub
     hipLaunchKernelGGL(k_tn_delta, gr, bl, 0, c->stream, a, (const V4t*)T.prim, c->p.cfl, T.delta);
     for (int rk = 1; rk <= S; ++rk) {
       hipLaunchKernelGGL(k_tn_qvar, gr, bl, 0, c->stream, n, (const V4t*)T.prim, T.q);
-      hipLaunchKernelGGL(k_tn_lsq, gr, bl, 0, c->stream, a, (const V4t*)T.q, T.dq[0], T.qm);
       int d = 0;
-      for (int j = 0; j < I; ++j, d ^= 1)
-        hipLaunchKernelGGL(k_tn_inner<kTanLanes>, gr_l, bl, 0, c->stream, a, (const V4t*)T.q, (const V4t*)T.dq[d],
-                           T.dq[d ^ 1]);
-      hipLaunchKernelGGL(k_tn_flux_update<kTanLanes>, gr_l, bl, 0, c->stream, a, (const V4t*)T.q,
-                         (const V4t*)T.dq[d], (const V4t*)T.qm, (const Dd*)T.delta, T.prim, (const V4t*)T.pold, rk,
-                         c->p.euler, c->fs);
+      auto stencils = [&]<bool P0>() {
+        hipLaunchKernelGGL((k_tn_lsq<kTanLanes, P0>), gr_l, bl, 0, c->stream, a, (const V4t*)T.q, T.dq[0], T.qm);
+        for (int j = 0; j < I; ++j, d ^= 1)
+          hipLaunchKernelGGL((k_tn_inner<kTanLanes, P0>), gr_l, bl, 0, c->stream, a, (const V4t*)T.q,
+                             (const V4t*)T.dq[d], T.dq[d ^ 1]);
+        hipLaunchKernelGGL((k_tn_flux_update<kTanLanes, P0>), gr_l, bl, 0, c->stream, a, (const V4t*)T.q,
+                           (const V4t*)T.dq[d], (const V4t*)T.qm, (const Dd*)T.delta, T.prim, (const V4t*)T.pold, rk,
+                           c->p.euler, c->fs);
+      };
+      if (c->p.power == 0.0) stencils.template operator()<true>();
+      else stencils.template operator()<false>();
     }
     HIPCHK(hipGetLastError());
   }
