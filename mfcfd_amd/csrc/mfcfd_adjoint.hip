@@ -88,8 +88,11 @@ struct PullHalo {
 // Least-squares weight w = dist^power of an edge with offset (a, b) (q_variables.F90:89-91,
 // interior_fluxes.F90:89-90) and f = power dist^(power - 2), so dw/da = f a and
 // dw/db = f b; power = 0 gives w = 1 and f = 0 exactly
+// POWER0: the kernels' unweighted specialisation (power = 0), which carries no pow
+// evaluation (long and register-hungry in FP64)
+template <bool POWER0>
 __device__ inline void lsq_weight(double a, double b, double power, double& w, double& f) {
-  if (power == 0.0) {
+  if (POWER0 || power == 0.0) {
     w = 1.0;
     f = 0.0;
     return;
@@ -117,7 +120,7 @@ __device__ inline void end_qtilde(const Nbrs& nbrs, int p, S delx, S dely, int f
 // edge's coefficients and of delta, and the end's flux adjoints are reversed into its
 // record.  nbrs gives the records of both ends (the taped state in global memory, or a
 // block's halo staged in LDS) and the point's slots.
-template <int CLS, int LPP, class Nbrs>
+template <int CLS, int LPP, bool POWER0, class Nbrs>
 __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nbrs, int i, double delta,
                                const V4d& frbv, const AdjOut& o, int lane, double power) {
   // the records' precision: the arithmetic of the ends' reconstruction, kinetic state
@@ -155,7 +158,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
     double w, wf;
-    lsq_weight(dels, deln, power, w, wf);
+    lsq_weight<POWER0>(dels, deln, power, w, wf);
     const double dsw = dels * w, dnw = deln * w;
     const bool xp = (m & XPOS) != 0u;
     const bool tie_x = xp && (m & XNEG) != 0u;
@@ -307,7 +310,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     const double dels = delx * tx + dely * ty;
     const double deln = delx * nx + dely * ny;
     double w, wf;
-    lsq_weight(dels, deln, power, w, wf);
+    lsq_weight<POWER0>(dels, deln, power, w, wf);
     // the half sums dels^2 w, deln^2 w, dels deln w
     double delsb = 0.0, delnb = 0.0, wb = 0.0;
     for (int d = 0; d < 4; ++d) {
@@ -364,7 +367,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
 
 // S: the taped records' precision (double; float for a mixed-precision context, whose
 // records are the mixed primal's own FP32 ones)
-template <int LPP, bool STAGED, class S>
+template <int LPP, bool STAGED, class S, bool POWER0>
 __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, State<S> st,
                                                        const double* __restrict__ delta,
                                                        const V4d* __restrict__ frb, AdjOut o, double power) {
@@ -380,23 +383,23 @@ __global__ __launch_bounds__(BLOCK, 1) void k_flux_rev(Geom g, HalfSums hs, Stat
     if (i >= g.n) return;  // whole lane groups
     if (staged) {
       const StagedNbrs<S, STAGE_PF, LPP> nb{g, r, s_cl, li, g.nbhs[i], lane};
-      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      if (i < g.nw) flux_point_rev<WALL, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else flux_point_rev<INTERIOR, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     } else {
       const GlobalNbrs<S> nb{g, st, i, g.nbhs[i], lane, LPP};
-      if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-      else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      if (i < g.nw) flux_point_rev<WALL, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+      else flux_point_rev<INTERIOR, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
     }
   } else {
     const int t = point_id();
     const int i = t / LPP, lane = t % LPP;
     if (i >= g.n) return;  // whole lane groups
     const GlobalNbrs<S> nb{g, st, i, g.nbhs[i], lane, LPP};
-    if (i < g.nw) flux_point_rev<WALL, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
-    else flux_point_rev<INTERIOR, LPP>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+    if (i < g.nw) flux_point_rev<WALL, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+    else if (i < g.nw + g.no) flux_point_rev<OUTER, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
+    else flux_point_rev<INTERIOR, LPP, POWER0>(g, hs, nb, i, delta[i], frb[i], o, lane, power);
   }
 }
 
@@ -469,11 +472,12 @@ __device__ inline void inner_lam(const V4d& gg, const V4d& tx, const V4d& ty, co
 // component (the neighbour's q takes +Db, the owner's -Db; the dq records -/+ Db d/2)
 // and the adjoint (dxb, dyb) of the offset (the neighbour's coordinates take +, the
 // owner's -).  The owner (push form) and the neighbour (pull form) evaluate it alike.
+template <bool POWER0>
 __device__ inline void inner_edge_rev(double dx, double dy, double power, const V4d& qi, const V4d& dxi,
                                       const V4d& dyi, const V4d& qk, const V4d& dxk, const V4d& dyk,
                                       const InnerLam& L, double Db[4], double& dxb, double& dyb) {
   double w, wf;
-  lsq_weight(dx, dy, power, w, wf);
+  lsq_weight<POWER0>(dx, dy, power, w, wf);
   // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
   dxb = w * (2.0 * L.sxxb * dx + L.sxyb * dy);
   dyb = w * (2.0 * L.syyb * dy + L.sxyb * dx);
@@ -498,10 +502,11 @@ __device__ inline void inner_edge_rev(double dx, double dy, double power, const 
 // adjoint Db of D = q_k - q_i per component (+ to the neighbour's q, - to the owner's)
 // and the adjoint (dxb, dyb) of the offset.  The owner (push form) and the neighbour
 // (pull form) evaluate it alike.
+template <bool POWER0>
 __device__ inline void lsq_edge_rev(double dx, double dy, double power, const V4d& qi, const V4d& qk,
                                     const InnerLam& L, double Db[4], double& dxb, double& dyb) {
   double w, wf;
-  lsq_weight(dx, dy, power, w, wf);
+  lsq_weight<POWER0>(dx, dy, power, w, wf);
   // the sums of w dx^2, w dy^2, w dx dy and w (dx, dy) D
   dxb = w * (2.0 * L.sxxb * dx + L.sxyb * dy);
   dyb = w * (2.0 * L.syyb * dy + L.sxyb * dx);
@@ -523,7 +528,7 @@ __device__ inline void lsq_edge_rev(double dx, double dy, double power, const V4
 // Own contributions: dqb_in written, qb and xyb added; per edge, the neighbour's q
 // adjoint and the edge's (delx, dely) adjoint go to the edge buffer (fields 0-5), the
 // neighbour's dq adjoint being -0.5 (delx, dely) times its q adjoint (k_gather_lsq).
-template <int LPP, bool STAGED>
+template <int LPP, bool STAGED, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restrict__ q,
                                                      const V4d* __restrict__ dq_in,
                                                      const V4d* __restrict__ dq_out,
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_rev(Geom g, const V4d* __restri
   auto edge = [&](int s, const D2& pk, const V4d& qk, const V4d& dxk, const V4d& dyk) {
     const double dx = pk.x - pi.x, dy = pk.y - pi.y;
     double Dbv[4], dxb, dyb;
-    inner_edge_rev(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, L, Dbv, dxb, dyb);
+    inner_edge_rev<POWER0>(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, L, Dbv, dxb, dyb);
     for (int r = 0; r < 4; ++r) {
       qib[r] -= Dbv[r];
       dxib[r] += 0.5 * dx * Dbv[r];
@@ -637,7 +642,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_lam(Geom g, const V4d* __restri
 // qb and xyb and, for the next reversed inner iteration (lam_next non-null), writes its
 // solve adjoint from that dqb_in (dq_in being the next iteration's output).  Entries past
 // the LDS capacity (255) are read from global memory.
-template <int LPP>
+template <int LPP, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_inner_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
                                                       const V4d* __restrict__ dq_in, const V4d* __restrict__ lam,
                                                       V4d* __restrict__ dqb_in, V4d* __restrict__ lam_next,
@@ -732,7 +737,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_pull(Geom g, PullHalo ph, Trans
       }
       const double dx = pk.x - pi.x, dy = pk.y - pi.y;
       double Db[4], dxb, dyb;
-      inner_edge_rev(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, Li, Db, dxb, dyb);
+      inner_edge_rev<POWER0>(dx, dy, power, qi, dxi, dyi, qk, dxk, dyk, Li, Db, dxb, dyb);
       for (int r = 0; r < 4; ++r) {
         aq[r] -= Db[r];
         ax[r] += 0.5 * dx * Db[r];
@@ -768,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_pull(Geom g, PullHalo ph, Trans
     }
     const double dx = pi.x - po.x, dy = pi.y - po.y;
     double Db[4], dxb, dyb;
-    inner_edge_rev(dx, dy, power, qo, dxo, dyo, qi, dxi, dyi, Lo, Db, dxb, dyb);
+    inner_edge_rev<POWER0>(dx, dy, power, qo, dxo, dyo, qi, dxi, dyi, Lo, Db, dxb, dyb);
     for (int r = 0; r < 4; ++r) {
       aq[r] += Db[r];
       ax[r] += -0.5 * dx * Db[r];
@@ -805,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) void k_inner_pull(Geom g, PullHalo ph, Trans
 // contributions added to qb, xyb; per edge, the neighbour's q adjoint (with the qm
 // adjoint of the components it maximises or minimises) and the edge's (delx, dely)
 // adjoint go to the edge buffer (fields 0-5, k_gather_lsq)
-template <int LPP, bool STAGED>
+template <int LPP, bool STAGED, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict__ q,
                                                    const V4d* __restrict__ dq,
                                                    const V4d* __restrict__ dqb,
@@ -860,7 +865,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_rev(Geom g, const V4d* __restrict
       }
     }
     double Dbv[4], dxb, dyb;
-    lsq_edge_rev(dx, dy, power, qi, qk, L, Dbv, dxb, dyb);
+    lsq_edge_rev<POWER0>(dx, dy, power, qi, qk, L, Dbv, dxb, dyb);
     for (int r = 0; r < 4; ++r) qib[r] -= Dbv[r];
     D2* ep = reinterpret_cast<D2*>(eb + (size_t)epos[(size_t)s * g.n + i] * EF_LSQ);
     ep[0] = D2{Dbv[0], Dbv[1]};
@@ -950,7 +955,7 @@ inline size_t stage_lds_lsq_pull(int hcap, int width) {
 // resolved by the owner's walk as in k_lsq_rev; those of a selected neighbour, at most 8
 // per point, are added atomically into qbx (summed into the stage's q adjoint and
 // cleared by k_qvar_rev), the others into the point's own q adjoint.
-template <int LPP>
+template <int LPP, bool POWER0>
 __global__ __launch_bounds__(BLOCK) void k_lsq_pull(Geom g, PullHalo ph, Trans tr, const V4d* __restrict__ q,
                                                     const V4d* __restrict__ lam, const V4d* __restrict__ qmb,
                                                     V4d* __restrict__ qb, V4d* __restrict__ qbx,
@@ -1046,7 +1051,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_pull(Geom g, PullHalo ph, Trans t
         }
       }
       double Db[4], dxb, dyb;
-      lsq_edge_rev(pk.x - pi.x, pk.y - pi.y, power, qi, qk, Li, Db, dxb, dyb);
+      lsq_edge_rev<POWER0>(pk.x - pi.x, pk.y - pi.y, power, qi, qk, Li, Db, dxb, dyb);
       for (int r = 0; r < 4; ++r) aq[r] -= Db[r];
       xb -= dxb;
       yb -= dyb;
@@ -1072,7 +1077,7 @@ __global__ __launch_bounds__(BLOCK) void k_lsq_pull(Geom g, PullHalo ph, Trans t
       lam_glb(o, Lo);
     }
     double Db[4], dxb, dyb;
-    lsq_edge_rev(pi.x - po.x, pi.y - po.y, power, qo, qi, Lo, Db, dxb, dyb);
+    lsq_edge_rev<POWER0>(pi.x - po.x, pi.y - po.y, power, qo, qi, Lo, Db, dxb, dyb);
     for (int r = 0; r < 4; ++r) aq[r] += Db[r];
     xb += dxb;
     yb += dyb;
@@ -1553,6 +1558,14 @@ static int adj_record(mf_ctx* c, AdjState& A) {
   return MF_OK;
 }
 
+// f.template operator()<P0>() with P0 = (power == 0): the reverse kernels' unweighted
+// specialisation
+template <class F>
+static void with_power0(const mf_ctx* c, F&& f) {
+  if (c->p.power == 0.0) f.template operator()<true>();
+  else f.template operator()<false>();
+}
+
 // launchers of the reverse kernels at a given lanes-per-point shape
 template <class S>
 static void launch_flux_rev_s(mf_ctx* c, AdjState& A, const State<S>& st, V4d* dqb_last, int lpp) {
@@ -1563,8 +1576,10 @@ static void launch_flux_rev_s(mf_ctx* c, AdjState& A, const State<S>& st, V4d* d
     const dim3 gr((unsigned)((c->n + STAGE_PF - 1) / STAGE_PF));
     const size_t lds = stage_lds_flux(c->g.hl[1].hcap, c->g.width, sizeof(S));
     auto go = [&]<int L>() {
-      hipLaunchKernelGGL((k_flux_rev<L, true, S>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
-                         (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
+      with_power0(c, [&]<bool P0>() {
+        hipLaunchKernelGGL((k_flux_rev<L, true, S, P0>), gr, dim3(STAGE_PF * L), lds, c->stream, c->g, hs, st,
+                           (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
+      });
     };
     switch (lpp & ~LPP_STAGED) {
       case 2: go.template operator()<2>(); break;
@@ -1574,8 +1589,10 @@ static void launch_flux_rev_s(mf_ctx* c, AdjState& A, const State<S>& st, V4d* d
     return;
   }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL((k_flux_rev<L, false, S>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs, st,
-                       (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
+    with_power0(c, [&]<bool P0>() {
+      hipLaunchKernelGGL((k_flux_rev<L, false, S, P0>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, hs,
+                         st, (const double*)A.tdelta, (const V4d*)A.frb, o, c->p.power);
+    });
   });
 }
 
@@ -1612,14 +1629,20 @@ static void launch_inner_rev(mf_ctx* c, AdjState& A, int s, int j, int lpp) {
   if (lpp & LPP_STAGED) {
     const size_t lds = stage_lds_inner_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
-      hipLaunchKernelGGL((k_inner_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, din,
-                         dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
+      with_power0(c, [&]<bool P0>() {
+        hipLaunchKernelGGL((k_inner_rev<L, true, P0>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q,
+                           din, dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos,
+                           c->p.power);
+      });
     });
     return;
   }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL((k_inner_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, din, dout,
-                       (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
+    with_power0(c, [&]<bool P0>() {
+      hipLaunchKernelGGL((k_inner_rev<L, false, P0>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, din,
+                         dout, (const V4d*)A.dqb[cur], A.dqb[prv], A.qb, A.xyb, A.eb, (const int32_t*)A.epos,
+                         c->p.power);
+    });
   });
 }
 
@@ -1642,8 +1665,10 @@ static void launch_inner_pull(mf_ctx* c, AdjState& A, int s, int j, int lpp, boo
   // derivatives (same solve) when their reverse is in the pull form too
   V4d* lam_next = (j > 1 || (A.lpp_lrev & LPP_PULL)) ? A.lam[prv] : nullptr;
   with_lpp_staged(lpp & ~LPP_PULL, [&]<int L>() {
-    hipLaunchKernelGGL((k_inner_pull<L>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
-                       din, (const V4d*)A.lam[cur], A.dqb[prv], lam_next, A.qb, A.xyb, c->p.power);
+    with_power0(c, [&]<bool P0>() {
+      hipLaunchKernelGGL((k_inner_pull<L, P0>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr,
+                         q, din, (const V4d*)A.lam[cur], A.dqb[prv], lam_next, A.qb, A.xyb, c->p.power);
+    });
   });
 }
 
@@ -1667,8 +1692,10 @@ static void launch_lsq_pull(mf_ctx* c, AdjState& A, int s, int lpp, bool need_la
   const Trans tr{A.t_ptr, A.t_own};
   const size_t lds = stage_lds_lsq_pull(A.ph.hcap, c->g.width);
   with_lpp_staged(lpp & ~LPP_PULL, [&]<int L>() {
-    hipLaunchKernelGGL((k_lsq_pull<L>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
-                       (const V4d*)A.lam[0], (const V4d*)A.qmb, A.qb, A.qbx, A.xyb, c->p.power);
+    with_power0(c, [&]<bool P0>() {
+      hipLaunchKernelGGL((k_lsq_pull<L, P0>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, A.ph, tr, q,
+                         (const V4d*)A.lam[0], (const V4d*)A.qmb, A.qb, A.qbx, A.xyb, c->p.power);
+    });
   });
 }
 
@@ -1678,14 +1705,20 @@ static void launch_lsq_rev(mf_ctx* c, AdjState& A, int s, int lpp) {
   if (lpp & LPP_STAGED) {
     const size_t lds = stage_lds_lsq_rev(c->g.hl[0].hcap, c->g.width);
     with_lpp_staged(lpp & ~LPP_STAGED, [&]<int L>() {
-      hipLaunchKernelGGL((k_lsq_rev<L, true>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q, dq0,
-                         (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
+      with_power0(c, [&]<bool P0>() {
+        hipLaunchKernelGGL((k_lsq_rev<L, true, P0>), stage_grid(c->n), dim3(STAGE_P * L), lds, c->stream, c->g, q,
+                           dq0, (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos,
+                           c->p.power);
+      });
     });
     return;
   }
   with_lpp(lpp, [&]<int L>() {
-    hipLaunchKernelGGL((k_lsq_rev<L, false>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, dq0,
-                       (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos, c->p.power);
+    with_power0(c, [&]<bool P0>() {
+      hipLaunchKernelGGL((k_lsq_rev<L, false, P0>), lane_blocks(c, L), dim3(BLOCK), 0, c->stream, c->g, q, dq0,
+                         (const V4d*)A.dqb[0], (const V4d*)A.qmb, A.qb, A.xyb, A.eb, (const int32_t*)A.epos,
+                         c->p.power);
+    });
   });
 }
 
