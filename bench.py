@@ -444,6 +444,8 @@ def run_batch(iters: int, adjoint: bool = True, warmup: int = 20):
         out["adjoint"] = {"iterations": iters + warmup, "snapshots": snaps}
         for prec in ("fp64", "mixed"):
             with LskumBatch(cases, precision=prec, **p) as b:
+                # the forward shapes the primal run's autotuner kept
+                b.tuning = out[prec]["tuning"]
                 b.run(0)
                 t0 = time.perf_counter()
                 J, dx, dy, st = b.adjoint(iters + warmup, snaps=snaps)
