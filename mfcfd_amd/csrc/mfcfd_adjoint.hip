@@ -1393,6 +1393,24 @@ __global__ void k_widen(const V4<float>* __restrict__ a, V4d* __restrict__ b, in
   }
 }
 
+// Several arrays widened in one launch (blockIdx.y picks the array): the taped
+// iteration's records are about 20 arrays, each too short to fill the GPU alone
+constexpr int kWidenMax = 32;
+struct WidenJobs {
+  const V4<float>* src[kWidenMax];
+  V4d* dst[kWidenMax];
+  int64_t n[kWidenMax];
+};
+__global__ void k_widen_jobs(WidenJobs w) {
+  const int k = blockIdx.y;
+  const V4<float>* __restrict__ a = w.src[k];
+  V4d* __restrict__ b = w.dst[k];
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < w.n[k]; j += (int64_t)gridDim.x * blockDim.x) {
+    const V4<float> v = a[j];
+    b[j] = V4d{(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+  }
+}
+
 __global__ void k_zero(double* __restrict__ p, int64_t n) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
     p[j] = 0.0;
@@ -1498,16 +1516,33 @@ static void adj_widen_n(mf_ctx* c, const V4f* a, V4d* b, int64_t n) {
                      c->stream, a, b, n);
 }
 
-// the FP32 records of the taped iteration into the FP64 tape
+// the FP32 records of the taped iteration into the FP64 tape, as few launches of
+// k_widen_jobs as the records need.  The qm records are not widened: the flux reverse
+// reads the FP32 ones, and the LSQ reverses recompute the extremes from q.
 static int adj_widen(mf_ctx* c, AdjState& A) {
   const int64_t n = c->n;
-  for (int s = 0; s <= A.S; ++s) adj_widen_n(c, A.fprim[s], A.tprim[s], n);
-  for (int s = 0; s < A.S; ++s) {
-    adj_widen_n(c, A.fq[s], A.tq[s], n);
-    adj_widen_n(c, A.fqm[s], A.tqm[s], 2 * n);
-  }
-  for (size_t k = 0; k < A.fdq.size(); ++k) adj_widen_n(c, A.fdq[k], A.tdq[k], 2 * n);
-  adj_widen_n(c, A.fpold, A.tpold, n);
+  WidenJobs w{};
+  int cnt = 0;
+  int64_t nmax = 0;
+  auto flush = [&] {
+    if (cnt == 0) return;
+    const dim3 gr((unsigned)std::min<int64_t>((nmax + BLOCK - 1) / BLOCK, 1024), (unsigned)cnt);
+    hipLaunchKernelGGL(k_widen_jobs, gr, dim3(BLOCK), 0, c->stream, w);
+    cnt = 0;
+    nmax = 0;
+  };
+  auto add = [&](const V4f* a, V4d* b, int64_t m) {
+    w.src[cnt] = a;
+    w.dst[cnt] = b;
+    w.n[cnt] = m;
+    nmax = std::max(nmax, m);
+    if (++cnt == kWidenMax) flush();
+  };
+  for (int s = 0; s <= A.S; ++s) add(A.fprim[s], A.tprim[s], n);
+  for (int s = 0; s < A.S; ++s) add(A.fq[s], A.tq[s], n);
+  for (size_t k = 0; k < A.fdq.size(); ++k) add(A.fdq[k], A.tdq[k], 2 * n);
+  add(A.fpold, A.tpold, n);
+  flush();
   HIPCHK(hipGetLastError());
   return MF_OK;
 }
