@@ -56,7 +56,11 @@ __host__ __device__ inline S rv_rcp(S x) {
   else return 1.0 / x;
 }
 
-// Same operations, in the same order, as kin_of_qtilde<double> (FP64)
+// kin_of_qtilde's quantities with every intermediate kept.  The reverse only consumes
+// their values (no branch depends on them), so both precisions use the identities
+// b = rho / (2 p) = beta, p / rho = tmp = 1 / (2 beta) and sqrt(pi b) = sqrt(pi) sqrt(b),
+// and one reciprocal for both B's: 2 divisions and 1 square root per end in FP64
+// instead of the primal routine's 6 and 2 (the values agree to rounding).
 template <class S>
 __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) {
   for (int r = 0; r < 4; ++r) k.qt[r] = qt[r];
@@ -70,7 +74,7 @@ __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) 
     k.T2 = k.T1 - flog(k.beta) * (1.0f / Consts<float>::GAMMA_M1);
     k.rho = fexp(k.T2);
   } else {
-    k.T2 = k.T1 - (m_log(k.beta) / Consts<double>::GAMMA_M1);
+    k.T2 = k.T1 - m_log(k.beta) * INV_GAMMA_M1;
     k.rho = m_exp(k.T2);
   }
   k.p = k.rho * k.tmp;
@@ -83,9 +87,9 @@ __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) 
     k.sb = __builtin_amdgcn_sqrtf(k.b);
     k.spb = k.sb * 1.7724538509f;  // sqrt(pi b)
   } else {
-    k.b = 0.5 * k.rho / k.p;
+    k.b = k.beta;
     k.sb = m_sqrt(k.b);
-    k.spb = m_sqrt(Consts<double>::PI * k.b);
+    k.spb = k.sb * 1.7724538509055160;  // sqrt(pi b)
   }
   k.S1 = k.ut * k.sb;
   k.S2 = k.un * k.sb;
@@ -101,11 +105,12 @@ __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) 
   } else {
     k.ex1 = m_exp(-k.S1 * k.S1);
     k.ex2 = m_exp(-k.S2 * k.S2);
-    k.B1 = 0.5 * k.ex1 / k.spb;
-    k.B2 = 0.5 * k.ex2 / k.spb;
+    const double hispb = 0.5 / k.spb;
+    k.B1 = k.ex1 * hispb;
+    k.B2 = k.ex2 * hispb;
     k.e1 = m_erf(k.S1);
     k.e2 = m_erf(k.S2);
-    k.prr = k.p / k.rho;
+    k.prr = k.tmp;  // p / rho
   }
   k.usq = k.ut * k.ut + k.un * k.un;
 }
