@@ -259,7 +259,14 @@ __device__ __forceinline__ void stage_and_walk(const Geom& g, const Halo& h, int
   ready();
   const int li = threadIdx.x / LPP;
   if (hn <= h.hcap) {
-    for (int s = threadIdx.x % LPP; s < nb; s += LPP) use((int)s_cl[s * P + li], s);
+    // the next slot's halo position is read while this edge is evaluated
+    int s = threadIdx.x % LPP;
+    int jn = s < nb ? s_cl[s * P + li] : 0;
+    for (; s < nb; s += LPP) {
+      const int j = jn;
+      if (s + LPP < nb) jn = s_cl[(s + LPP) * P + li];
+      use(j, s);
+    }
   } else {
     const int i = b * P + li;
     for (int s = threadIdx.x % LPP; s < nb; s += LPP) {
@@ -729,8 +736,11 @@ struct StagedNbrs {
   }
   template <class F>
   __device__ void walk(F&& f) const {
+    // the next slot's word is read while this edge is evaluated
+    unsigned en = lane < nb ? s_cl[lane * P + li] : 0u;
     for (int s = lane; s < nb; s += LPP) {
-      const unsigned e = s_cl[s * P + li];
+      const unsigned e = en;
+      if (s + LPP < nb) en = s_cl[(s + LPP) * P + li];
       f((int)(e & 0xFFFu), e >> 12);
     }
   }
