@@ -867,7 +867,15 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
         qtk[r] = qk[r];
       }
     } else {
-      reconstruct_eval(qi, dxi, dyi, pmaxi, pmini, e2i, dx, dy, qti);
+      if constexpr (F32) {
+        // the own end's records read again per edge (the staged halo holds the point
+        // at its own position), not held across the walk
+        int o = nbrs.own();
+        asm volatile("" : "+v"(o));
+        reconstruct_eval(nbrs.q(o), nbrs.dqx(o), nbrs.dqy(o), nbrs.pmax(o), nbrs.pmin(o), nbrs.e2(o), dx, dy, qti);
+      } else {
+        reconstruct_eval(qi, dxi, dyi, pmaxi, pmini, e2i, dx, dy, qti);
+      }
       const V4<S> qk = nbrs.q(k);
       const V4<S> dxk = nbrs.dqx(k), dyk = nbrs.dqy(k);
       reconstruct_eval(qk, dxk, dyk, nbrs.pmax(k), nbrs.pmin(k), nbrs.e2(k), dx, dy, qtk);
