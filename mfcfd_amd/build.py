@@ -27,12 +27,15 @@ HIPCC_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-# The library's translation units and their contraction mode: mfcfd.hip (host side,
+# The library's translation units and their floating-point flags: mfcfd.hip (host side,
 # FP64 kernels, adjoint) without contraction; mfcfd_f32.hip (the mixed path's FP32
 # kernels, held to a tolerance against FP64, not to rounding parity) with FMA
-# contraction.
-UNITS = [("mfcfd.hip", "-ffp-contract=off"), ("mfcfd_f32.hip", "-ffp-contract=fast"),
-         ("mfcfd_noalloc.hip", "-ffp-contract=off")]
+# contraction; mfcfd_lsq32.hip (the mixed path's LSQ kernel) also without the IEEE mode
+# bit and with NaN operands assumed absent, so that its stencil extremes are single
+# max/min instructions.
+UNITS = [("mfcfd.hip", ["-ffp-contract=off"]), ("mfcfd_f32.hip", ["-ffp-contract=fast"]),
+         ("mfcfd_lsq32.hip", ["-ffp-contract=fast", "-fno-honor-nans", "-mno-amdgpu-ieee"]),
+         ("mfcfd_noalloc.hip", ["-ffp-contract=off"])]
 
 
 def hipcc() -> str:
@@ -81,15 +84,15 @@ def build_host(verbose: bool = False) -> str | None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile csrc/mfcfd.hip and csrc/mfcfd_f32.hip and link them into
-    mfcfd_amd/libmfcfd_hip.so; returns the path."""
+    """Compile the units of UNITS and link them into mfcfd_amd/libmfcfd_hip.so;
+    returns the path."""
     import tempfile
     common = [f for f in HIPCC_FLAGS if f not in ("-shared", "-ffp-contract=off")]
     with tempfile.TemporaryDirectory() as tmpd:
         objs = []
-        for src, contract in UNITS:
+        for src, flags in UNITS:
             obj = os.path.join(tmpd, src.replace(".hip", ".o"))
-            cmd = [hipcc()] + common + [contract, "-c", "-o", obj, os.path.join(CSRC, src)]
+            cmd = [hipcc()] + common + flags + ["-c", "-o", obj, os.path.join(CSRC, src)]
             if verbose:
                 print(" ".join(cmd), flush=True)
             subprocess.check_call(cmd)

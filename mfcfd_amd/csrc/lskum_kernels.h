@@ -803,7 +803,7 @@ void launch_update_env(const LaunchEnv& e, V4<S>* prim, const V4<S>* pold, const
                      flux_res, partial, rk, e.euler, e.fs);
 }
 
-// The mixed path's launchers, defined in mfcfd_f32.hip
+// The mixed path's launchers, defined in mfcfd_f32.hip (launch_lsq_f32 in mfcfd_lsq32.hip)
 using V4f = V4<float>;
 void launch_lsq_f32(const LaunchEnv& e, const V4f* q, V4f* dq, V4f* qm, const double* cspeed, double* delta,
                     bool with_delta, const Ctl* ctl, int lpp, V4f* qsum);

@@ -1,13 +1,9 @@
 // The mixed path (FP32 state, FP64 sums) of the LSKUM stencil kernels: this unit
-// instantiates the launchers of lskum_kernels.h for S = float and is compiled with FMA
-// contraction (the FP64 path, in mfcfd.hip, is not).
+// instantiates the launchers of lskum_kernels.h for S = float (the LSQ kernel's in
+// mfcfd_lsq32.hip) and is compiled with FMA contraction (the FP64 path, in mfcfd.hip,
+// is not).
 #include "lskum_kernels.h"
 #include "lskum_sweeps.h"
-
-void launch_lsq_f32(const LaunchEnv& e, const V4f* q, V4f* dq, V4f* qm, const double* cspeed, double* delta,
-                    bool with_delta, const Ctl* ctl, int lpp, V4f* qsum) {
-  launch_lsq_env<float>(e, q, dq, qm, cspeed, delta, with_delta, ctl, lpp, qsum);
-}
 
 void launch_inner_f32(const LaunchEnv& e, const V4f* q, const V4f* a, V4f* b, const Ctl* ctl, int lpp,
                       const V4f* qsum) {
