@@ -1065,6 +1065,19 @@ static int build_graph(mf_ctx* c, int phase, int n) {
   return MF_OK;
 }
 
+// The graphs the next runs replay (one and kGraphIters iterations, from the current
+// buffer phase and, when an iteration flips the phase, from the other), built when the
+// launch shapes are chosen so that the first run after a tuning does not capture and
+// instantiate them (about 2 ms for a 16-iteration graph at 2.56M points)
+static int prebuild_graphs(mf_ctx* c) {
+  int rc;
+  if ((rc = ensure_history(c, kGraphIters))) return rc;
+  for (int f = 0; f < ((c->p.rks & 1) ? 2 : 1); ++f)
+    for (int n : {1, kGraphIters})
+      if ((rc = build_graph(c, c->cur ^ f, n))) return rc;
+  return MF_OK;
+}
+
 static int run_queue(mf_ctx* c, int32_t first_it, int32_t iters) {
   int rc;
   if ((rc = ensure_history(c, iters))) return rc;
@@ -1376,6 +1389,7 @@ static int autotune_impl(mf_ctx* c, int reps, mf_tuning* t) {
   c->lpp_inner = pick[1];
   c->lpp_flux = pick[2];
   drop_graphs(c);
+  if ((rc = prebuild_graphs(c))) return rc;
   if (t)
     *t = mf_tuning{pick[0], pick[1], pick[2], (int32_t)c->barrier_polls, best[0], best[1], best[2],
                    c->lpp_rev[0], c->lpp_rev[1], c->lpp_rev[2], c->lpp_rev[3]};
@@ -1440,7 +1454,7 @@ int mf_set_tuning(mf_ctx* c, const mf_tuning* t) {
   for (int k = 0; k < 4; ++k) c->lpp_rev[k] = rv[k];
   adj_forget_tuning(c);
   drop_graphs(c);
-  return MF_OK;
+  return prebuild_graphs(c);
 }
 
 int mf_batch_run(mf_ctx* c, int32_t first_it, int32_t iters, double* residue, double* res_new, double* cd) {
