@@ -397,30 +397,42 @@ __host__ __device__ inline void end_fluxes_rev(const KinFT<S>& f, bool xp, bool 
   }
 }
 
-// Adjoints of the per-point half-stencil coefficients (c1, sxy, obd of each half)
-struct HalfCoefB {
-  double c1[4], sxy[4], obd[4];
+// The half coefficients as products per half (0 xpos, 1 xneg, 2 ypos, 3 yneg): A = c1 obd,
+// B = sxy obd, so an edge's x coefficient is dsw A - dnw B of its x half and its y
+// coefficient dnw A - dsw B of its y half (the flux reverse's form of coef_x / coef_y;
+// its adjoints have the same shape)
+struct HalfAB {
+  double A[4], B[4];
 };
 
-// reverse of coef_x / coef_y for half `d` (0 xpos, 1 xneg, 2 ypos, 3 yneg)
-__host__ __device__ inline void coef_rev(const HalfCoef& h, int d, bool ydir, double dsw, double dnw,
-                                         double cb, HalfCoefB& hb, double& dswb, double& dnwb) {
-  double c1 = 0, sxy = 0, obd = 0;
+__host__ __device__ inline double coef_x(const HalfAB& h, bool first, double dsw, double dnw) {
+  const double a = first ? h.A[0] : h.A[1];
+  const double b = first ? h.B[0] : h.B[1];
+  return dsw * a - dnw * b;
+}
+__host__ __device__ inline double coef_y(const HalfAB& h, bool first, double dsw, double dnw) {
+  const double a = first ? h.A[2] : h.A[3];
+  const double b = first ? h.B[2] : h.B[3];
+  return dnw * a - dsw * b;
+}
+
+// reverse of coef_x / coef_y (HalfAB form) for half d with the coefficient's adjoint cb
+__host__ __device__ inline void coef_rev_ab(const HalfAB& h, int d, bool ydir, double dsw, double dnw, double cb,
+                                            HalfAB& hb, double& dswb, double& dnwb) {
+  double A = 0, B = 0;
   for (int j = 0; j < 4; ++j)
     if (j == d) {
-      c1 = h.c1[j];
-      sxy = h.sxy[j];
-      obd = h.obd[j];
+      A = h.A[j];
+      B = h.B[j];
     }
+  // c = a A - bq B
   const double a = ydir ? dnw : dsw, bq = ydir ? dsw : dnw;
-  // c = (a c1 - bq sxy) obd
-  const double ab = cb * c1 * obd, bqb = -cb * sxy * obd;
   for (int j = 0; j < 4; ++j)
     if (j == d) {
-      hb.c1[j] += cb * a * obd;
-      hb.sxy[j] += -cb * bq * obd;
-      hb.obd[j] += cb * (a * c1 - bq * sxy);
+      hb.A[j] += cb * a;
+      hb.B[j] += -cb * bq;
     }
+  const double ab = cb * A, bqb = -cb * B;
   if (ydir) {
     dnwb += ab;
     dswb += bqb;

@@ -129,21 +129,21 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
-  HalfCoef hc;
+  // the half coefficients as the products A = c1 obd, B = sxy obd, and their adjoints:
+  // 8 values each live across the walk instead of 12
+  HalfAB hc, hcb;
   {
     const V4d a = g.dg_c1[i], b = g.dg_sxy[i], c = g.dg_obd[i];
     for (int d = 0; d < 4; ++d) {
-      hc.c1[d] = a[d];
-      hc.sxy[d] = b[d];
-      hc.obd[d] = c[d];
+      hc.A[d] = a[d] * c[d];
+      hc.B[d] = b[d] * c[d];
+      hcb.A[d] = hcb.B[d] = 0.0;
     }
   }
   const double scale = (CLS == WALL) ? 2.0 * delta : delta;
   double accb[4];
   for (int r = 0; r < 4; ++r) accb[r] = scale * frbv[r];
-  double dsum = 0.0;  // frb . (flux_res / scale): delta's adjoint
-  HalfCoefB hcb;
-  for (int d = 0; d < 4; ++d) hcb.c1[d] = hcb.sxy[d] = hcb.obd[d] = 0.0;
+  double dsum = 0.0;  // accb . (flux_res / scale) = scale times delta's adjoint
   EndD own;
   endd_zero(own);
   double odx[4] = {0, 0, 0, 0}, ody[4] = {0, 0, 0, 0};  // own dq adjoints
@@ -193,7 +193,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
         for (int r = 0; r < 4; ++r) {
           ax += accb[r] * (double)Gx[r];
           ay += accb[r] * (double)Gy[r];
-          fd += frbv[r] * ((double)Gx[r] * cx + (double)Gy[r] * cy);
+          fd += accb[r] * ((double)Gx[r] * cx + (double)Gy[r] * cy);
           Gx[r] = (S)(sg * accb[r] * cx);
           Gy[r] = (S)(sg * accb[r] * cy);
         }
@@ -209,7 +209,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
         for (int r = 0; r < 4; ++r) {
           hx += accb[r] * (double)Hx[r];
           hy += accb[r] * (double)Hy[r];
-          fd += frbv[r] * ((double)Hx[r] * cx2 + (double)Hy[r] * cy2);
+          fd += accb[r] * ((double)Hx[r] * cx2 + (double)Hy[r] * cy2);
           Hx[r] = (S)(sg * accb[r] * cx2);
           Hy[r] = (S)(sg * accb[r] * cy2);
         }
@@ -257,10 +257,10 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
       }
     }
     double dswb = 0.0, dnwb = 0.0;
-    coef_rev(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
-    coef_rev(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
-    if (tie_x) coef_rev(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
-    if (tie_y) coef_rev(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
+    coef_rev_ab(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
+    coef_rev_ab(hc, yp ? 2 : 3, true, dsw, dnw, cyb, hcb, dswb, dnwb);
+    if (tie_x) coef_rev_ab(hc, 1, false, dsw, dnw, cx2b, hcb, dswb, dnwb);
+    if (tie_y) coef_rev_ab(hc, 3, true, dsw, dnw, cy2b, hcb, dswb, dnwb);
     // dsw = dels w, dnw = deln w; dels = delx ny - dely nx, deln = delx nx + dely ny
     double delsb = dswb * w, delnb = dnwb * w;
     if (wf != 0.0) {
@@ -280,21 +280,25 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
   // ds dn, obd = 1/(sum ds^2 sum dn^2 - sxy^2); every lane needs the point's totals
   if (LPP > 1) {
     for (int d = 0; d < 4; ++d) {
-      hcb.c1[d] = group_sum<LPP>(hcb.c1[d]);
-      hcb.sxy[d] = group_sum<LPP>(hcb.sxy[d]);
-      hcb.obd[d] = group_sum<LPP>(hcb.obd[d]);
+      hcb.A[d] = group_sum<LPP>(hcb.A[d]);
+      hcb.B[d] = group_sum<LPP>(hcb.B[d]);
     }
   }
   double hxxb[4], hyyb[4], hxyb[4];
   {
     const V4d hxx = hs.hxx[i], hyy = hs.hyy[i], hxy = hs.hxy[i];
+    const V4d c1v = g.dg_c1[i], sxyv = g.dg_sxy[i], obdv = g.dg_obd[i];
     for (int d = 0; d < 4; ++d) {
-      hxxb[d] = (d >= 2) ? hcb.c1[d] : 0.0;
-      hyyb[d] = (d < 2) ? hcb.c1[d] : 0.0;
-      hxyb[d] = hcb.sxy[d];
-      const double obd = hc.obd[d];
-      const double detb = -hcb.obd[d] * obd * obd;
-      if (hcb.obd[d] != 0.0) {
+      // A = c1 obd, B = sxy obd; a half without edges (obd infinite) has no adjoint
+      const double obd = obdv[d];
+      const bool used = hcb.A[d] != 0.0 || hcb.B[d] != 0.0;
+      const double c1b = used ? hcb.A[d] * obd : 0.0;
+      const double obdb = used ? hcb.A[d] * c1v[d] + hcb.B[d] * sxyv[d] : 0.0;
+      hxxb[d] = (d >= 2) ? c1b : 0.0;
+      hyyb[d] = (d < 2) ? c1b : 0.0;
+      hxyb[d] = used ? hcb.B[d] * obd : 0.0;
+      const double detb = -obdb * obd * obd;
+      if (obdb != 0.0) {
         hxxb[d] += detb * hyy[d];
         hyyb[d] += detb * hxx[d];
         hxyb[d] += -2.0 * detb * hxy[d];
@@ -362,7 +366,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
   }
   o.nrmb[i].x += nxb;
   o.nrmb[i].y += nyb;
-  o.deltab[i] += (CLS == WALL) ? 2.0 * dsum : dsum;  // delta enters every stage
+  o.deltab[i] += dsum / delta;  // delta enters every stage (scale = 2 delta at a wall point)
 }
 
 // S: the taped records' precision (double; float for a mixed-precision context, whose
