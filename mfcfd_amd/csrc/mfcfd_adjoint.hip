@@ -176,8 +176,7 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
     double delxb = 0.0, delyb = 0.0;
     const size_t epj = (size_t)o.epos[(size_t)s * g.n + i];
     D2* ep = reinterpret_cast<D2*>(o.eb + epj * EF_FLUX);
-#pragma unroll 1
-    for (int e = 0; e < 2; ++e) {
+    auto end_pass = [&](const int e) {
       const int p = e ? k : nbrs.own();
       S qt[4];
       end_qtilde(nbrs, p, (S)delx, (S)dely, g.first_order, qt);
@@ -255,6 +254,16 @@ __device__ void flux_point_rev(const Geom& g, const HalfSums& hs, const Nbrs& nb
           ody[r] += d.d[r] * dely;
         }
       }
+    };
+    if constexpr (sizeof(S) == 8) {
+      // FP64: both ends in straight-line code, so that the one wave a SIMD holds can
+      // overlap one end's library transcendentals with the other's
+      end_pass(0);
+      end_pass(1);
+    } else {
+      // FP32: one end at a time, which keeps the kernel at 2 waves per SIMD
+#pragma unroll 1
+      for (int e = 0; e < 2; ++e) end_pass(e);
     }
     double dswb = 0.0, dnwb = 0.0;
     coef_rev_ab(hc, xp ? 0 : 1, false, dsw, dnw, cxb, hcb, dswb, dnwb);
