@@ -33,7 +33,7 @@ constexpr double INV_GAMMA_M1 = 1.0 / Consts<double>::GAMMA_M1;
 template <class S>
 struct KinFT {
   S qt[4];
-  S beta, tmp, u1, u2, T1, T2, rho, p, ut, un, b, sb, spb, S1, S2, ex1, ex2, B1, B2, e1, e2, prr, usq;
+  S beta, tmp, u1, u2, T1, rho, p, ut, un, b, sb, spb, S1, S2, ex1, ex2, B1, B2, e1, e2, prr, usq;
 };
 using KinF = KinFT<double>;
 
@@ -60,7 +60,10 @@ __host__ __device__ inline S rv_rcp(S x) {
 // their values (no branch depends on them), so both precisions use the identities
 // b = rho / (2 p) = beta, p / rho = tmp = 1 / (2 beta) and sqrt(pi b) = sqrt(pi) sqrt(b),
 // and one reciprocal for both B's: 2 divisions and 1 square root per end in FP64
-// instead of the primal routine's 6 and 2 (the values agree to rounding).
+// instead of the primal routine's 6 and 2 (the values agree to rounding).  In FP64,
+// rho = exp(T1 - log(beta) / (gamma - 1)) is formed as exp(T1) beta^-2.5 with
+// beta^-2.5 = (1 / beta)^3 sqrt(beta), from the reciprocal and the square root above:
+// no logarithm (about 90 VALU instructions in FP64).
 template <class S>
 __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) {
   for (int r = 0; r < 4; ++r) k.qt[r] = qt[r];
@@ -70,27 +73,21 @@ __host__ __device__ inline void kin_fwd(const S qt[4], S nx, S ny, KinFT<S>& k) 
   k.u1 = qt[1] * k.tmp;
   k.u2 = qt[2] * k.tmp;
   k.T1 = qt[0] + k.beta * (k.u1 * k.u1 + k.u2 * k.u2);
+  k.b = k.beta;  // b = rho / (2 p) = beta
   if constexpr (sizeof(S) == 4) {
-    k.T2 = k.T1 - flog(k.beta) * (1.0f / Consts<float>::GAMMA_M1);
-    k.rho = fexp(k.T2);
+    k.rho = fexp(k.T1 - flog(k.beta) * (1.0f / Consts<float>::GAMMA_M1));
+    k.sb = __builtin_amdgcn_sqrtf(k.b);
+    k.spb = k.sb * 1.7724538509f;  // sqrt(pi b)
   } else {
-    k.T2 = k.T1 - m_log(k.beta) * INV_GAMMA_M1;
-    k.rho = m_exp(k.T2);
+    k.sb = m_sqrt(k.b);
+    k.spb = k.sb * 1.7724538509055160;  // sqrt(pi b)
+    const double ib = 2.0 * k.tmp;      // 1 / beta
+    k.rho = m_exp(k.T1) * ((ib * ib) * (ib * k.sb));
   }
   k.p = k.rho * k.tmp;
   const S tx = ny, ty = -nx;
   k.ut = k.u1 * tx + k.u2 * ty;
   k.un = k.u1 * nx + k.u2 * ny;
-  if constexpr (sizeof(S) == 4) {
-    // b = rho / (2 p) = beta
-    k.b = k.beta;
-    k.sb = __builtin_amdgcn_sqrtf(k.b);
-    k.spb = k.sb * 1.7724538509f;  // sqrt(pi b)
-  } else {
-    k.b = k.beta;
-    k.sb = m_sqrt(k.b);
-    k.spb = k.sb * 1.7724538509055160;  // sqrt(pi b)
-  }
   k.S1 = k.ut * k.sb;
   k.S2 = k.un * k.sb;
   if constexpr (sizeof(S) == 4) {
@@ -267,7 +264,7 @@ __host__ __device__ inline void kin_rev(const KinFT<S>& k, const KinBT<S>& kb, S
   // p = rho tmp
   rhob += pb * k.tmp;
   S tmpb = pb * k.rho;
-  // rho = exp(T2), T2 = T1 - log(beta)/(gamma-1)
+  // rho = exp(T2), T2 = T1 - log(beta)/(gamma-1): d rho = rho (dT1 - dbeta / (beta (gamma-1)))
   const S T2b = rhob * k.rho;
   const S T1b = T2b;
   S betab = -T2b * ibeta * (S)INV_GAMMA_M1;
