@@ -152,6 +152,20 @@ def test_iterations_fp64(small_cloud, inner, tscheme):
         assert abs(cd_g[-1] - cd[0]) <= 1e-14 * max(abs(cd[0]), 1e-300)
 
 
+@pytest.mark.parametrize("tscheme", ["ssprk43", "first"])
+def test_graphs_built_at_tuning(small_cloud, tscheme):
+    """mf_set_tuning builds the replayed graphs (16 iterations and 1, from both buffer
+    phases when a stage count is odd) before the next run: 17 and then 3 iterations
+    after a tuning give the bits of a context whose graphs are built inside its runs."""
+    with _gpu(small_cloud, tscheme=tscheme) as a, _gpu(small_cloud, tscheme=tscheme) as b:
+        a.tuning = b.tuning  # the same shapes, set on a only
+        ra1, _, _ = a.q_lskum(17, first_it=1)
+        ra2, _, _ = a.q_lskum(3, first_it=18)
+        rb, _, _ = b.q_lskum(20, first_it=1)
+        assert np.array_equal(a.prim, b.prim)
+        assert np.array_equal(np.concatenate([ra1, ra2]), rb)
+
+
 def test_split_runs_continue(small_cloud):
     """Two mf_run calls (10 + 10 iterations) equal one of 20, residue included."""
     with _gpu(small_cloud) as a, _gpu(small_cloud) as b:
