@@ -128,6 +128,9 @@ struct Geom {
   // mixed path: the half-stencil coefficients in FP32, [2i] c1 * obd and [2i+1] sxy * obd
   // of the four halves (HalfCoefF)
   const V4<float>* dg_f = nullptr;
+  // 1 when every interior normal is exactly (0, 1) (point_normals.F90:91-96, the
+  // default): an interior point's (t, n) frame is then the (x, y) frame
+  int32_t axis_interior = 0;
 };
 
 // halt codes of Ctl
@@ -402,6 +405,7 @@ __device__ inline float ferf_from_exp(float s, float ex, float p = 0.3275911f) {
 // p / rho = 1 / (2 beta) = rsq(beta)^2 / 2 and ut^2 + un^2 = u1^2 + u2^2 used instead of
 // recomputing them.  The exponentials run in base 2: S1, S2 are scaled by sqrt(log2 e),
 // so exp(-s^2) = exp2(-S^2) and erf's t = 1 / (1 + p s) takes p / sqrt(log2 e).
+template <bool AXIS = false>
 __device__ inline Kin<float> kin_fast(const float qt[4], float nx, float ny) {
   constexpr float LOG2E = 1.44269504f, SQRT_LOG2E = 1.20112241f;
   const float beta = -qt[3] * 0.5f;
@@ -413,8 +417,14 @@ __device__ inline Kin<float> kin_fast(const float qt[4], float nx, float ny) {
   // rho = exp(temp1 - log(beta) / (gamma - 1)), 1 / (gamma - 1) = 2.5
   const float rho = __builtin_amdgcn_exp2f(temp1 * LOG2E - 2.5f * __builtin_amdgcn_logf(beta));
   Kin<float> k;
-  k.ut = u1 * ny - u2 * nx;
-  k.un = u1 * nx + u2 * ny;
+  if constexpr (AXIS) {
+    // normal (0, 1): ut = u1 ny - u2 nx = u1 and un = u2 exactly (up to the sign of a zero)
+    k.ut = u1;
+    k.un = u2;
+  } else {
+    k.ut = u1 * ny - u2 * nx;
+    k.un = u1 * nx + u2 * ny;
+  }
   const float sbl = (beta * rs) * SQRT_LOG2E;  // sqrt(beta log2 e)
   const float S1 = k.ut * sbl, S2 = k.un * sbl;
   const float ex1 = __builtin_amdgcn_exp2f(-S1 * S1), ex2 = __builtin_amdgcn_exp2f(-S2 * S2);
@@ -799,9 +809,15 @@ __device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b,
   return true;
 }
 
-template <int CLS, class S, bool POWER0, int LPP, class Nbrs>
+// AXIS (mixed path, interior points, Geom::axis_interior): the point's normal is (0, 1),
+// so the edge offsets and the ends' velocities need no rotation into its frame (12 of an
+// edge's ~350 VALU instructions).  The frame values equal the rotation's (x 1 and x 0
+// are exact, up to the sign of a zero); the fluxes computed from them can differ in the
+// last bit, as FMA contraction pairs their products by the shape of the code.
+template <int CLS, class S, bool POWER0, int LPP, bool AXIS = false, class Nbrs>
 __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, double delta, double power,
                                       double fr[4], const Nbrs& nbrs) {
+  static_assert(!AXIS || (CLS == INTERIOR && sizeof(S) == 4), "the axis frame is the mixed interior form");
   const D2 pi = g.xy[i];
   const D2 ni = g.nrm[i];
   const double nx = ni.x, ny = ni.y, tx = ny, ty = -nx;
@@ -834,14 +850,24 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
     }
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // an end's kinetic state in the point's frame
+  auto kin = [&](const S* qt) {
+    if constexpr (F32) return kin_fast<AXIS>(qt, nxs, nys);
+    else return kin_eval(qt, nxs, nys);
+  };
   nbrs.walk([&](int k, unsigned m) {
     const D2 pkk = nbrs.xy(k);
     const double delx = pkk.x - pi.x, dely = pkk.y - pi.y;
     R dels, deln;
     if constexpr (F32) {
       const float fx = (float)delx, fy = (float)dely;
-      dels = fx * nys - fy * nxs;
-      deln = fx * nxs + fy * nys;
+      if constexpr (AXIS) {
+        dels = fx;
+        deln = fy;
+      } else {
+        dels = fx * nys - fy * nxs;
+        deln = fx * nxs + fy * nys;
+      }
     } else {
       dels = delx * tx + dely * ty;
       deln = delx * nx + dely * ny;
@@ -884,8 +910,8 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
     if constexpr (sizeof(S) == 4) {
       // the mixed path's short sequences: both ends in straight-line code (a loop
       // over the ends moves its inputs and results between iterations)
-      end_fluxes<CLS>(kin_eval(qti, nxs, nys), xp, yp, gxi, gyi);
-      end_fluxes<CLS>(kin_eval(qtk, nxs, nys), xp, yp, gxk, gyk);
+      end_fluxes<CLS>(kin(qti), xp, yp, gxi, gyi);
+      end_fluxes<CLS>(kin(qtk), xp, yp, gxk, gyk);
     } else {
     // FP64: the two ends one after the other, so one end's library-transcendental
     // temporaries live at a time
@@ -895,7 +921,7 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
 #pragma unroll
       for (int r = 0; r < 4; ++r) qt[r] = e ? qtk[r] : qti[r];
       S Gx[4], Gy[4];
-      end_fluxes<CLS>(kin_eval(qt, nxs, nys), xp, yp, Gx, Gy);
+      end_fluxes<CLS>(kin(qt), xp, yp, Gx, Gy);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (e) {
@@ -923,9 +949,9 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
       const double cx2 = tie_x ? coef_x(hc, false, dsw, dnw) : 0.0;
       const double cy2 = tie_y ? coef_y(hc, false, dsw, dnw) : 0.0;
       S Gx[4], Gy[4];
-      end_fluxes<CLS>(kin_eval(qti, nxs, nys), false, false, Gx, Gy);
+      end_fluxes<CLS>(kin(qti), false, false, Gx, Gy);
       S Hx[4], Hy[4];
-      end_fluxes<CLS>(kin_eval(qtk, nxs, nys), false, false, Hx, Hy);
+      end_fluxes<CLS>(kin(qtk), false, false, Hx, Hy);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (tie_x) acc[r] = acc[r] + (double)(Hx[r] - Gx[r]) * cx2;
@@ -939,12 +965,29 @@ __device__ inline void flux_point_src(const Geom& g, const State<S>& st, int i, 
   for (int r = 0; r < 4; ++r) fr[r] = scale * acc[r];
 }
 
+// flux_point_src with the mixed path's interior points in the axis frame when
+// Geom::axis_interior says their normals are (0, 1).  Every walk, staged or global, goes
+// through it: the two frames' code contracts its products into FMAs differently, so a
+// walk's bits depend on the frame form it runs, and the walks keep the same bits only if
+// they run the same form.
+template <int CLS, class S, bool POWER0, int LPP, class Nbrs>
+__device__ inline void flux_point_frame(const Geom& g, const State<S>& st, int i, double delta, double power,
+                                        double fr[4], const Nbrs& nbrs) {
+  if constexpr (CLS == INTERIOR && sizeof(S) == 4) {
+    if (g.axis_interior) {
+      flux_point_src<CLS, S, POWER0, LPP, true>(g, st, i, delta, power, fr, nbrs);
+      return;
+    }
+  }
+  flux_point_src<CLS, S, POWER0, LPP>(g, st, i, delta, power, fr, nbrs);
+}
+
 // A point's flux residual from the global walk
 template <int CLS, class S, bool POWER0, int LPP = 1>
 __device__ inline void flux_point(const Geom& g, const State<S>& st, int i, double delta, double power,
                                   double fr[4], int lane = 0) {
-  flux_point_src<CLS, S, POWER0, LPP>(g, st, i, delta, power, fr,
-                                      GlobalNbrs<S>{g, st, i, g.nbhs[i], lane, LPP});
+  flux_point_frame<CLS, S, POWER0, LPP>(g, st, i, delta, power, fr,
+                                        GlobalNbrs<S>{g, st, i, g.nbhs[i], lane, LPP});
 }
 
 // state_update.F90:144-163 primitive_to_conserved (rotated frame)

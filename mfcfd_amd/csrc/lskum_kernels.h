@@ -495,14 +495,14 @@ __global__ __launch_bounds__(BLOCK, 2) void k_flux_update(
       const double dl = delta[i];
       if (staged) {
         const StagedNbrs<S, STAGE_PF, LPP, true> nbrs{g, r, s_cl, (int)threadIdx.x / LPP, g.nbhs[i], lane};
-        if (i < g.nw) flux_point_src<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
-        else if (i < g.nw + g.no) flux_point_src<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
-        else flux_point_src<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        if (i < g.nw) flux_point_frame<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        else if (i < g.nw + g.no) flux_point_frame<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        else flux_point_frame<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
       } else {
         const GlobalNbrs<S> nbrs{g, st, i, g.nbhs[i], lane, LPP};
-        if (i < g.nw) flux_point_src<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
-        else if (i < g.nw + g.no) flux_point_src<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
-        else flux_point_src<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        if (i < g.nw) flux_point_frame<WALL, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        else if (i < g.nw + g.no) flux_point_frame<OUTER, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
+        else flux_point_frame<INTERIOR, S, POWER0, LPP>(g, st, i, dl, power, fr, nbrs);
       }
     }
     if (LPP > 1) {

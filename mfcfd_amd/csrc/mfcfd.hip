@@ -748,6 +748,12 @@ static int create_impl(const mf_geometry* geo, const mf_params* prm, mf_ctx* c, 
       (rc = upload(c, &d_sxy, sxyv)) || (rc = upload(c, &d_obd, obd)))
     return rc;
   c->g = Geom{d_xy, d_nrm, d_nbhs, d_conn, d_lsq, d_c1, d_sxy, d_obd, n, c->nw, c->no, prm->first_order ? 1 : 0};
+  c->g.axis_interior = 1;
+  for (int j = c->nw + c->no; j < n; ++j)
+    if (!(nrm[j].x == 0.0 && nrm[j].y == 1.0)) {
+      c->g.axis_interior = 0;
+      break;
+    }
   if ((rc = build_halos(c, conn_d, nbh_d))) return rc;
   if (c->ssz == 4) {
     // the mixed path's solve coefficients (sum dy^2, sum dx dy, sum dx^2) / det in FP32
