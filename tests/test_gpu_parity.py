@@ -392,6 +392,30 @@ def test_interior_normal_options(small_cloud, case):
         assert abs(cd[0] - o.field("Cd")[0]) <= 1e-10 * abs(o.field("Cd")[0])
 
 
+@pytest.mark.parametrize("case", ["normal_flag_1", "format_2"])
+def test_mixed_rotated_interior_frame(small_cloud, case):
+    """The mixed path's interior points in a rotated frame (normals (1, 0), or the file's
+    normals): the general form of the flux kernel, not the (x, y) frame it takes when
+    every interior normal is (0, 1).  Against the FP64 path after 60 iterations:
+    |dCl|, |dCd| <= 1e-4 and every prim within 1e-4 relative."""
+    c = small_cloud
+    kw = {}
+    if case == "normal_flag_1":
+        kw["interior_points_normal_flag"] = 1
+    else:
+        rng = np.random.default_rng(7)
+        ang = rng.uniform(0.0, 2.0 * np.pi, c.n)
+        c = _relabelled(c, nx=np.cos(ang), ny=np.sin(ang))
+        kw["format"] = 2
+    with _gpu(c, **kw) as a, _gpu(c, precision="mixed", **kw) as b:
+        a.q_lskum(60)
+        b.q_lskum(60)
+        cla, cda, _, _ = a.compute_cl_cd_cm()
+        clb, cdb, _, _ = b.compute_cl_cd_cm()
+        assert abs(cla[0] - clb[0]) <= 1e-4 and abs(cda[0] - cdb[0]) <= 1e-4
+        assert rel_err(b.prim, a.prim) < 1e-4
+
+
 def test_two_shapes(small_cloud):
     """-shapes 2 with the lower surface's wall points relabelled flag_2 = 2: per-shape
     Cl, Cd, Cm (compute_force_coeffs.F90:11-102) against the oracle, and the two shapes'
