@@ -698,9 +698,12 @@ struct GlobalNbrs {
 // inputs qmax - q, qmin - q and eps^2 instead.
 template <class S>
 struct HaloRecs {
-  D2* xy;
-  V4<S> *q, *dqx, *dqy, *qmax, *qmin;
-  S* eps;
+  // per halo entry its five 4-vectors side by side: q, dq_x, dq_y, then qmax and qmin (a
+  // PRE halo: qmax - q and qmin - q), so that an entry's fields sit at fixed offsets
+  // from one address and the LDS reads take them as immediate offsets
+  V4<S>* rec;  // [H][5]
+  D2* xy;      // [H]
+  S* eps;      // [H], epsilon (a PRE halo: epsilon^2)
 };
 
 template <class S, int P, int LPP, bool PRE = false>
@@ -711,28 +714,28 @@ struct StagedNbrs {
   const uint16_t* s_cl;
   int li, nb, lane;
   __device__ D2 xy(int k) const { return r.xy[k]; }
-  __device__ V4<S> q(int k) const { return r.q[k]; }
-  __device__ V4<S> dqx(int k) const { return r.dqx[k]; }
-  __device__ V4<S> dqy(int k) const { return r.dqy[k]; }
+  __device__ V4<S> q(int k) const { return r.rec[5 * k]; }
+  __device__ V4<S> dqx(int k) const { return r.rec[5 * k + 1]; }
+  __device__ V4<S> dqy(int k) const { return r.rec[5 * k + 2]; }
   __device__ V4<S> qmax(int k) const {
     static_assert(!PRE, "a PRE halo holds qmax - q");
-    return r.qmax[k];
+    return r.rec[5 * k + 3];
   }
   __device__ V4<S> qmin(int k) const {
     static_assert(!PRE, "a PRE halo holds qmin - q");
-    return r.qmin[k];
+    return r.rec[5 * k + 4];
   }
   __device__ S eps(int k) const {
     static_assert(!PRE, "a PRE halo holds eps^2");
     return r.eps[k];
   }
   __device__ V4<S> pmax(int k) const {
-    if constexpr (PRE) return r.qmax[k];
-    else return r.qmax[k] - r.q[k];
+    if constexpr (PRE) return r.rec[5 * k + 3];
+    else return r.rec[5 * k + 3] - r.rec[5 * k];
   }
   __device__ V4<S> pmin(int k) const {
-    if constexpr (PRE) return r.qmin[k];
-    else return r.qmin[k] - r.q[k];
+    if constexpr (PRE) return r.rec[5 * k + 4];
+    else return r.rec[5 * k + 4] - r.rec[5 * k];
   }
   __device__ S e2(int k) const {
     if constexpr (PRE) return r.eps[k];
@@ -768,12 +771,8 @@ __device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b,
                                        HaloRecs<S>& r, uint16_t*& s_cl) {
   const Halo& h = g.hl[1];
   const int H = h.hcap;
-  r.q = (V4<S>*)smem;
-  r.dqx = r.q + H;
-  r.dqy = r.dqx + H;
-  r.qmax = r.dqy + H;
-  r.qmin = r.qmax + H;
-  r.xy = (D2*)(r.qmin + H);
+  r.rec = (V4<S>*)smem;
+  r.xy = (D2*)(r.rec + 5 * H);
   r.eps = (S*)(r.xy + H);
   s_cl = (uint16_t*)(r.eps + H);
   const int h0 = h.ptr[b], hn = h.ptr[b + 1] - h0;
@@ -784,17 +783,18 @@ __device__ inline bool stage_flux_halo(const Geom& g, const State<S>& st, int b,
   for (int j = threadIdx.x; j < hn; j += STAGE_PF * LPP) {
     const int k = h.ids[h0 + j];
     const V4<S> qk = st.q[k];
-    r.q[j] = qk;
-    r.dqx[j] = st.dq[2 * k];
-    r.dqy[j] = st.dq[2 * k + 1];
+    V4<S>* e = r.rec + 5 * j;
+    e[0] = qk;
+    e[1] = st.dq[2 * k];
+    e[2] = st.dq[2 * k + 1];
     if constexpr (PRE) {
-      r.qmax[j] = st.qm[2 * k] - qk;
-      r.qmin[j] = st.qm[2 * k + 1] - qk;
-      const S e = st.eps[k];
-      r.eps[j] = e * e;
+      e[3] = st.qm[2 * k] - qk;
+      e[4] = st.qm[2 * k + 1] - qk;
+      const S ep = st.eps[k];
+      r.eps[j] = ep * ep;
     } else {
-      r.qmax[j] = st.qm[2 * k];
-      r.qmin[j] = st.qm[2 * k + 1];
+      e[3] = st.qm[2 * k];
+      e[4] = st.qm[2 * k + 1];
       r.eps[j] = st.eps[k];
     }
     r.xy[j] = g.xy[k];
