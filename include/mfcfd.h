@@ -208,7 +208,10 @@ typedef struct mf_tuning {
   int32_t lpp_flux_rev, lpp_lsq_rev, lpp_inner_rev, lpp_gather;
 } mf_tuning;
 /* Times every shape on the context's current state (`reps` launches each, default 10),
- * keeps the fastest of each kernel and reports them.  Leaves prim untouched. */
+ * keeps the fastest of each kernel and reports them.  Leaves prim untouched.  Like
+ * mf_set_tuning, it ends by capturing and instantiating the HIP graphs that mf_run
+ * replays for the chosen shapes (a few ms), so the next run starts without that cost;
+ * neither may be called while the context's stream is being captured. */
 int mf_autotune(mf_ctx *ctx, int32_t reps, mf_tuning *out);
 int mf_get_tuning(mf_ctx *ctx, mf_tuning *out);
 int mf_set_tuning(mf_ctx *ctx, const mf_tuning *t);
